@@ -1,0 +1,108 @@
+/*
+ * larmor_sd.h -- C-ABI of liblarmor_sd.so, the MI355X (gfx950) per-voxel spherical
+ * deconvolution of larmor (larmor-dmri/larmor 2.2.0).
+ *
+ * The entry points replace the nanobind module larmor._core for the spherical-deconvolution
+ * path; each declaration names the reference binding it replaces (paths relative to the
+ * reference repository).  Conventions:
+ *  - matrices are row-major float64, C-contiguous;
+ *  - "host" pointers are read during the call; "device" pointers are HIP device buffers, and
+ *    the work is enqueued on `stream` (a hipStream_t, or NULL for the default stream) -- the
+ *    call returns before the device work ends;
+ *  - every function returns 0 on success and a non-zero LSD_ERR_* code otherwise, with the
+ *    message (the reference's ValueError text where it has one) from lsd_last_error(), kept per
+ *    calling thread.
+ * Results are bit-identical for any grid, any stream and any batch composition: each voxel's
+ * arithmetic depends on its own signal only.
+ */
+#ifndef LARMOR_SD_H
+#define LARMOR_SD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSD_ABI_VERSION 1
+#define LSD_ERR_INVALID 1 /* invalid argument (the reference raises ValueError) */
+#define LSD_ERR_HIP 2     /* a HIP runtime call failed */
+
+/* Per-voxel status bits written by the fits. */
+#define LSD_STATUS_NOT_CONVERGED 1 /* an NNLS stopped at its 3n inner-step limit */
+#define LSD_STATUS_QR_FALLBACK 4   /* a Cholesky pivot was refused; pivoted QR finished */
+
+typedef struct lsd_solver lsd_solver;
+typedef struct lsd_tissue_kernel lsd_tissue_kernel;
+
+/* The last error message of this thread. */
+const char* lsd_last_error(void);
+int lsd_abi_version(void);
+
+/*
+ * An L2-regularised NNLS FOD solver for kernel K (host, m x n): its device copy, the Gram matrix
+ * K^T K + ridge^2 I, and whether fits use the normal equations (ridge 0, or
+ * (||K||_F^2 + ridge^2) / ridge^2 <= 1e8).
+ * Replaces: FodSolver.l2nnls(k, reg, tol_x)  cpp/src/python/sd.cpp:89-99 (l2nnls_solver),
+ *           make_ridge_design cpp/src/sd/nnls.cpp:280-292.
+ * Errors: "reg must be at least 0", "tol_x must be positive".
+ */
+int lsd_solver_create_l2nnls(const double* kernel, int64_t m, int64_t n, double ridge,
+                             double tol_x, lsd_solver** out);
+void lsd_solver_destroy(lsd_solver* solver);
+/* FodSolver.n_dirs / n_meas  cpp/src/python/sd.cpp:399-443 */
+int64_t lsd_solver_n_dirs(const lsd_solver* solver);
+int64_t lsd_solver_n_meas(const lsd_solver* solver);
+int lsd_solver_normal_equations(const lsd_solver* solver);
+
+/*
+ * One resolution of a multi-tissue kernel (host arrays): the unweighted anisotropic columns
+ * (m x ca), the unweighted isotropic columns (m x ci), the weighted isotropic columns (m x ci) and
+ * the anisotropic blocks as (start, stop) pairs (n_blocks x 2).
+ * Replaces: DeconvKernel(k_aniso, k_iso, w_k_iso, aniso_blocks)  cpp/src/python/sd.cpp:240-277
+ *           (construct_kernel), TissueKernel cpp/src/sd/fractions.hpp:27-38.
+ * Errors: "aniso_blocks must be non-empty ranges within 0..ca, got [...]"; at most 4 fraction
+ * columns (blocks + isotropic tissues).
+ */
+int lsd_tissue_kernel_create(const double* k_aniso, const double* k_iso, const double* w_k_iso,
+                             int64_t m, int64_t ca, int64_t ci, const int64_t* blocks,
+                             int64_t n_blocks, lsd_tissue_kernel** out);
+void lsd_tissue_kernel_destroy(lsd_tissue_kernel* kernel);
+
+/*
+ * The FOD of each voxel with one solver: signals (device, n_voxels x m) -> fods (device,
+ * n_voxels x n), status (device int32, n_voxels).
+ * Replaces: fit_single_tissue(signals, solver, n_threads)  cpp/src/python/sd.cpp:145-166,
+ *           L2NnlsSolver::solve_block cpp/src/sd/solver.cpp:14-21,
+ *           solve_ridge_nnls cpp/src/sd/nnls.cpp:294-350.
+ * Errors: "signals (r, c) do not have the solver's N measurements".
+ */
+int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
+                          const lsd_solver* solver, double* fods, int32_t* status, void* stream);
+
+/*
+ * The multi-tissue fit: signals (device, n_voxels x m, unweighted), weights (device, m);
+ * low-resolution rounds of FOD and lumped-fraction fits (at most max_rounds, a voxel leaving when
+ * no fraction moves by fraction_tol), then the high-resolution FOD and the per-block fractions
+ * (NNLS tolerance final_tol) normalised to sum to 1.
+ * Outputs (device): fods (n_voxels x hr.ca, may be NULL), fractions (n_voxels x
+ * (hr.n_blocks + ci)), rounds (int32, n_voxels: low-resolution rounds run), status (int32).
+ * Replaces: fit_multi_tissue(signals, weights, lr, lr_solver, hr, hr_solver, max_outer,
+ *           fraction_tol, final_tol_x, n_threads)  cpp/src/python/sd.cpp:279-330,
+ *           fit_multi_tissue_block cpp/src/sd/multi_tissue.cpp:28-100,
+ *           compute_fractions cpp/src/sd/fractions.cpp:45-76.
+ * Errors: the reference's shape and kernel/solver mismatch messages; "max_outer must be at
+ * least 0".
+ */
+int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m,
+                         const double* weights, const lsd_tissue_kernel* lr,
+                         const lsd_solver* lr_solver, const lsd_tissue_kernel* hr,
+                         const lsd_solver* hr_solver, int64_t max_rounds, double fraction_tol,
+                         double final_tol, double* fods, double* fractions, int32_t* rounds,
+                         int32_t* status, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LARMOR_SD_H */

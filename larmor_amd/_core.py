@@ -1,0 +1,248 @@
+"""The compiled core of the spherical-deconvolution path, on the GPU.
+
+Mirrors the spherical-deconvolution part of larmor's nanobind module `larmor._core`
+(cpp/src/python/sd.cpp:398-543, stub src/larmor/_core.pyi): the same names, arguments, result
+layout and ValueError/TypeError messages, so larmor's Python fit code can call it unchanged.  The
+work runs in liblarmor_sd.so's HIP kernels; device memory and streams come from torch.
+
+Arrays may be numpy (host; copied to the device and the results copied back, like the
+reference's copies in and out of C++) or torch CUDA tensors (device; results stay on the
+device).  `n_threads` is validated as the reference does and otherwise unused: the GPU grid is
+sized to the device.
+"""
+
+import ctypes
+
+import numpy as np
+
+from larmor_amd import _lib
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise RuntimeError("larmor_amd needs a HIP GPU (torch.cuda.is_available() is False).")
+    return torch
+
+
+def _check_f64(array, name: str) -> None:
+    """The reference binds with .noconvert(): another dtype is a TypeError, not a copy."""
+    dtype = getattr(array, "dtype", None)
+    if dtype == np.float64:
+        return
+    try:
+        import torch
+
+        if dtype == torch.float64:
+            return
+    except ImportError:  # pragma: no cover
+        pass
+    raise TypeError(f"{name} must be a float64 array, got {dtype}")
+
+
+def _is_c_contiguous(array) -> bool:
+    if isinstance(array, np.ndarray):
+        return array.flags.c_contiguous
+    return array.is_contiguous()
+
+
+def _require_threads(n_threads: int) -> None:
+    if n_threads < 1:
+        raise ValueError("n_threads must be at least 1")
+
+
+def _host_matrix(a, name: str) -> np.ndarray:
+    """A design matrix copied to a C-contiguous float64 host array (the reference copies them)."""
+    _check_f64(a, name)
+    if not isinstance(a, np.ndarray):
+        a = a.detach().cpu().numpy()
+    if a.ndim != 2:
+        raise ValueError(f"{name} must be 2-D, got shape {a.shape}")
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _stream_ptr(torch):
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _release(obj, destroy: str) -> None:
+    """Frees a device object; skipped during interpreter shutdown, when modules are gone."""
+    handle = getattr(obj, "_handle", None)
+    lib = getattr(_lib, "_lib", None) if _lib is not None else None
+    if handle and lib is not None:
+        getattr(lib, destroy)(handle)
+        obj._handle = None
+
+
+class FodSolver:
+    """An FOD solver with its kernel and Gram matrix resident on the device.
+
+    Only the L2NNLS solver is part of this path (FodSolver.l2nnls, sd.cpp:89-99); the reference's
+    `drl` and `lasso` solvers belong to other methods.
+    """
+
+    def __init__(self, handle, m: int, n: int):
+        self._handle = handle
+        self._m = m
+        self._n = n
+
+    @staticmethod
+    def l2nnls(k, reg: float, tol_x: float) -> "FodSolver":
+        kernel = _host_matrix(k, "k")
+        lib = _lib.load()
+        handle = ctypes.c_void_p()
+        _lib.check(
+            lib.lsd_solver_create_l2nnls(
+                _ptr(kernel), kernel.shape[0], kernel.shape[1], float(reg), float(tol_x),
+                ctypes.byref(handle),
+            )
+        )
+        return FodSolver(handle, kernel.shape[0], kernel.shape[1])
+
+    @property
+    def n_dirs(self) -> int:
+        return self._n
+
+    @property
+    def n_meas(self) -> int:
+        return self._m
+
+    @property
+    def n_unconverged(self) -> int:
+        """Always 0 for L2NNLS, as in the reference (only the elastic net counts)."""
+        return 0
+
+    @property
+    def normal_equations(self) -> bool:
+        return bool(_lib.load().lsd_solver_normal_equations(self._handle))
+
+    def __del__(self):
+        _release(self, "lsd_solver_destroy")
+
+
+class DeconvKernel:
+    """One resolution of a multi-tissue kernel, resident on the device (sd.cpp:240-277)."""
+
+    def __init__(self, k_aniso, k_iso, w_k_iso, aniso_blocks):
+        ka = _host_matrix(k_aniso, "k_aniso")
+        ki = _host_matrix(k_iso, "k_iso")
+        wki = _host_matrix(w_k_iso, "w_k_iso")
+        m = ka.shape[0]
+        if ki.shape[0] != m or wki.shape[0] != m or wki.shape[1] != ki.shape[1]:
+            raise ValueError("the kernel matrices' shapes do not match")
+        blocks = np.ascontiguousarray(np.asarray(list(aniso_blocks), dtype=np.int64).reshape(-1, 2))
+        lib = _lib.load()
+        handle = ctypes.c_void_p()
+        _lib.check(
+            lib.lsd_tissue_kernel_create(
+                _ptr(ka), _ptr(ki), _ptr(wki), m, ka.shape[1], ki.shape[1],
+                blocks.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(blocks),
+                ctypes.byref(handle),
+            )
+        )
+        self._handle = handle
+        self.m = m
+        self.n_anisotropic = ka.shape[1]
+        self.n_isotropic = ki.shape[1]
+        self.n_blocks = len(blocks)
+
+    def __del__(self):
+        _release(self, "lsd_tissue_kernel_destroy")
+
+
+def _device_signals(signals, torch):
+    """Returns (device tensor, was_host) for a float64 C-contiguous 2-D signal array."""
+    _check_f64(signals, "signals")
+    if signals.ndim != 2:
+        raise ValueError(f"signals must be 2-D, got shape {tuple(signals.shape)}")
+    if not _is_c_contiguous(signals):
+        raise ValueError("the array must be C-contiguous")
+    if isinstance(signals, np.ndarray):
+        return torch.from_numpy(signals).to("cuda"), True
+    if signals.device.type != "cuda":
+        return signals.to("cuda"), True
+    return signals, False
+
+
+def _out(tensor, host: bool):
+    return tensor.cpu().numpy() if host else tensor
+
+
+def fit_single_tissue(signals, solver: FodSolver, n_threads: int, *, return_status: bool = False):
+    """Fits the FOD (n, c) of every row of signals (n, m) with one solver (sd.cpp:145-166)."""
+    torch = _torch()
+    dev, host = _device_signals(signals, torch)
+    n, m = dev.shape
+    if m != solver.n_meas:
+        raise ValueError(
+            f"signals ({n}, {m}) do not have the solver's {solver.n_meas} measurements"
+        )
+    _require_threads(n_threads)
+    fods = torch.empty((n, solver.n_dirs), dtype=torch.float64, device="cuda")
+    status = torch.zeros(n, dtype=torch.int32, device="cuda")
+    lib = _lib.load()
+    _lib.check(
+        lib.lsd_fit_single_tissue(
+            dev.data_ptr(), n, m, solver._handle, fods.data_ptr(), status.data_ptr(),
+            _stream_ptr(torch),
+        )
+    )
+    if return_status:
+        return _out(fods, host), _out(status, host)
+    return _out(fods, host)
+
+
+def fit_multi_tissue(
+    signals,
+    weights,
+    lr: DeconvKernel,
+    lr_solver: FodSolver,
+    hr: DeconvKernel,
+    hr_solver: FodSolver,
+    max_outer: int,
+    fraction_tol: float,
+    final_tol_x: float,
+    n_threads: int,
+    *,
+    return_diagnostics: bool = False,
+):
+    """Fits the FODs (n, ca_hr) and fractions (n, n_blocks + ci) of every row of signals.
+
+    Replaces `_core.fit_multi_tissue` (sd.cpp:279-330).  With return_diagnostics, also returns
+    the low-resolution rounds run per voxel and the status bits (include/larmor_sd.h).
+    """
+    torch = _torch()
+    dev, host = _device_signals(signals, torch)
+    _check_f64(weights, "weights")
+    n, m = dev.shape
+    w = torch.as_tensor(np.ascontiguousarray(weights) if isinstance(weights, np.ndarray) else weights)
+    w = w.to(device="cuda", dtype=torch.float64).contiguous()
+    if w.numel() != m or hr.m != m or lr.m != m:
+        raise ValueError(
+            f"signals ({n}, {m}), weights {w.numel()} and the kernels' {hr.m} measurements "
+            "do not match"
+        )
+    _require_threads(n_threads)
+    nf = hr.n_blocks + hr.n_isotropic
+    fods = torch.empty((n, hr.n_anisotropic), dtype=torch.float64, device="cuda")
+    fractions = torch.empty((n, nf), dtype=torch.float64, device="cuda")
+    rounds = torch.zeros(n, dtype=torch.int32, device="cuda")
+    status = torch.zeros(n, dtype=torch.int32, device="cuda")
+    lib = _lib.load()
+    _lib.check(
+        lib.lsd_fit_multi_tissue(
+            dev.data_ptr(), n, m, w.data_ptr(), lr._handle, lr_solver._handle, hr._handle,
+            hr_solver._handle, int(max_outer), float(fraction_tol), float(final_tol_x),
+            fods.data_ptr(), fractions.data_ptr(), rounds.data_ptr(), status.data_ptr(),
+            _stream_ptr(torch),
+        )
+    )
+    out = (_out(fods, host), _out(fractions, host))
+    if return_diagnostics:
+        out += (_out(rounds, host), _out(status, host))
+    return out

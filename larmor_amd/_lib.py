@@ -1,0 +1,86 @@
+"""Loads liblarmor_sd.so (the HIP kernels and their C-ABI) and declares its entry points.
+
+The library is built in-tree by `larmor_amd.build.build_library()`.  There is no other compute
+path: when the library or a GPU is missing, every fit raises.
+"""
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "liblarmor_sd.so"
+HEADER_PATH = Path(__file__).resolve().parents[1] / "include" / "larmor_sd.h"
+
+_lib = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+# name: (restype, argtypes)
+SIGNATURES = {
+    "lsd_last_error": (ctypes.c_char_p, []),
+    "lsd_abi_version": (ctypes.c_int, []),
+    "lsd_solver_create_l2nnls": (
+        ctypes.c_int,
+        [_d, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+         ctypes.POINTER(_vp)],
+    ),
+    "lsd_solver_destroy": (None, [_vp]),
+    "lsd_solver_n_dirs": (ctypes.c_int64, [_vp]),
+    "lsd_solver_n_meas": (ctypes.c_int64, [_vp]),
+    "lsd_solver_normal_equations": (ctypes.c_int, [_vp]),
+    "lsd_tissue_kernel_create": (
+        ctypes.c_int,
+        [_d, _d, _d, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _i64, ctypes.c_int64,
+         ctypes.POINTER(_vp)],
+    ),
+    "lsd_tissue_kernel_destroy": (None, [_vp]),
+    "lsd_fit_single_tissue": (
+        ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp]
+    ),
+    "lsd_fit_multi_tissue": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64,
+         ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp, _vp],
+    ),
+}
+
+
+class LibraryMissing(RuntimeError):
+    """liblarmor_sd.so is not built (run __graft_entry__.build() or larmor_amd.build)."""
+
+
+def load() -> ctypes.CDLL:
+    """Returns the loaded library, declaring every entry point's signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise LibraryMissing(
+            f"{LIB_PATH} is not built; build it with `python -m larmor_amd.build`."
+        )
+    lib = ctypes.CDLL(str(LIB_PATH))
+    for name, (restype, argtypes) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    """Raises ValueError (invalid argument) or RuntimeError (HIP failure) for a non-zero status."""
+    if status == 0:
+        return
+    message = load().lsd_last_error().decode()
+    if status == 1:
+        raise ValueError(message)
+    raise RuntimeError(message)
+
+
+def declared_symbols() -> list[str]:
+    """The function names include/larmor_sd.h declares."""
+    import re
+
+    text = HEADER_PATH.read_text()
+    return sorted(set(re.findall(r"\b(lsd_[a-z0-9_]+)\s*\(", text)))

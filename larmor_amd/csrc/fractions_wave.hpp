@@ -1,0 +1,188 @@
+// Tissue fractions of one voxel on one wavefront: the median threshold of the FOD, the group
+// signal columns and the NNLS fraction fit.
+//
+// Restates larmor's median (cpp/src/sd/fractions.cpp:31-43; NaN sorted last, the mean of the
+// two middle values for an even count) and compute_fractions (fractions.cpp:45-76): FOD values
+// below median * (1 - 1e-9) are zeroed, each group's signal K_group f is scaled to a largest
+// value of 1 (when positive), and [groups, isotropic kernels] are fitted to the unweighted
+// signal by the generic NNLS with the given tolerance or the default one.
+#pragma once
+
+#include "qr_wave.hpp"
+
+namespace lsd {
+
+constexpr double MEDIAN_TIE_TOLERANCE = 1e-9;
+
+__device__ __forceinline__ bool less_nan_last(double a, double b) {
+    return isnan(b) ? !isnan(a) : a < b;
+}
+
+constexpr int pow2_ceil(int v) {
+    int p = 1;
+    while (p < v) {
+        p *= 2;
+    }
+    return p;
+}
+
+// The median of the n values spread as columns (lane c % 64, slot c / 64), by a bitonic sort of
+// the wave's 64 * NP slots with NaN padding.
+template <int NCPL>
+__device__ __forceinline__ double median_cols(const double (&v)[NCPL], int n) {
+    constexpr int NP = pow2_ceil(NCPL);
+    constexpr int TOTAL = NP * WAVE;
+    const int l = lane_id();
+    dvec<NP> a;
+#pragma unroll
+    for (int t = 0; t < NP; ++t) {
+        double x = __builtin_nan("");
+        if (t < NCPL) {
+            x = (l + WAVE * t) < n ? v[t < NCPL ? t : 0] : x;
+        }
+        a[t] = x;
+    }
+#pragma unroll
+    for (int kk = 2; kk <= TOTAL; kk <<= 1) {
+#pragma unroll
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            if (j >= WAVE) {
+                const int js = j / WAVE;
+#pragma unroll
+                for (int t = 0; t < NP; ++t) {
+                    const int tp = t ^ js;
+                    if (tp > t) {
+                        const int idx = t * WAVE + l;
+                        const bool ascending = (idx & kk) == 0;
+                        const double lo = a[t];
+                        const double hi = a[tp];
+                        const bool swap = ascending ? less_nan_last(hi, lo) : less_nan_last(lo, hi);
+                        a[t] = swap ? hi : lo;
+                        a[tp] = swap ? lo : hi;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < NP; ++t) {
+                    const double other = __shfl_xor(a[t], j);
+                    const int idx = t * WAVE + l;
+                    const bool ascending = (idx & kk) == 0;
+                    const bool lower = (l & j) == 0;
+                    const bool keep_min = lower == ascending;
+                    const bool other_less = less_nan_last(other, a[t]);
+                    const bool other_greater = less_nan_last(a[t], other);
+                    a[t] = keep_min ? (other_less ? other : a[t]) : (other_greater ? other : a[t]);
+                }
+            }
+        }
+    }
+    if (n == 0) {
+        return __builtin_nan("");
+    }
+    auto at = [&](int i) { return readlane(static_cast<double>(a[i / WAVE]), i % WAVE); };
+    const int mid = n / 2;
+    if (n % 2 == 1) {
+        return at(mid);
+    }
+    return (at(mid - 1) + at(mid)) / 2.0;
+}
+
+// Shared matrices of one resolution's fraction fit.
+struct FractionKernelDev {
+    int m;
+    int ca;                 // anisotropic columns
+    int ci;                 // isotropic tissues
+    const double* aniso_t;  // (ca, m): the unweighted anisotropic kernel, transposed
+    const double* iso_t;    // (ci, m): the unweighted isotropic kernel, transposed
+    const int* blocks;      // (n_blocks, 2) column ranges
+    int n_blocks;
+};
+
+// compute_fractions of one voxel: fod in column slots, s the unweighted signal.  lumped takes
+// all anisotropic columns as one group.  fr receives the fractions, groups first.  Returns false
+// when the NNLS hit its inner-step limit.
+template <int NCPL, int RPL, int NQ>
+__device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const double (&fod)[NCPL],
+                               const RowVec<RPL>& s, bool lumped, bool has_tol, double tol_in,
+                               double (&fr)[NQ]) {
+    const int l = lane_id();
+    const double threshold = median_cols<NCPL>(fod, K.ca) * (1.0 - MEDIAN_TIE_TOLERANCE);
+    double thr[NCPL];
+#pragma unroll
+    for (int t = 0; t < NCPL; ++t) {
+        thr[t] = fod[t] < threshold ? 0.0 : fod[t];
+    }
+    const int n_groups = lumped ? 1 : K.n_blocks;
+    RegCols<NQ, RPL> C;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        C.col[q] = 0.0;
+    }
+    for (int grp = 0; grp < n_groups; ++grp) {
+        const int begin = lumped ? 0 : K.blocks[2 * grp];
+        const int end = lumped ? K.ca : K.blocks[2 * grp + 1];
+        double gs[RPL];
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            gs[t] = 0.0;
+        }
+        // Zero terms add exactly nothing, so only the non-zero thresholded values are visited,
+        // in column order.
+#pragma unroll
+        for (int t2 = 0; t2 < NCPL; ++t2) {
+            const int c0 = l + WAVE * t2;
+            unsigned long long mask = ballot(c0 >= begin && c0 < end && thr[t2] != 0.0);
+            while (mask) {
+                const int b = __ffsll(static_cast<long long>(mask)) - 1;
+                mask &= mask - 1;
+                const int c = WAVE * t2 + b;
+                const double f = readlane(thr[t2], b);
+                const double* krow = K.aniso_t + static_cast<long>(c) * K.m;
+#pragma unroll
+                for (int t = 0; t < RPL; ++t) {
+                    const int r = l + WAVE * t;
+                    if (r < K.m) {
+                        gs[t] = gs[t] + krow[r] * f;
+                    }
+                }
+            }
+        }
+        if (K.m > 0) {
+            double largest = __builtin_nan("");
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                const int r = l + WAVE * t;
+                if (r < K.m) {
+                    largest = fmax(largest, gs[t]);
+                }
+            }
+#pragma unroll
+            for (int o = WAVE / 2; o >= 1; o >>= 1) {
+                largest = fmax(largest, __shfl_xor(largest, o));
+            }
+            if (largest > 0.0) {
+#pragma unroll
+                for (int t = 0; t < RPL; ++t) {
+                    gs[t] = gs[t] / largest;
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            C.set(grp, t, gs[t]);
+        }
+    }
+    for (int tt = 0; tt < K.ci; ++tt) {
+        const double* irow = K.iso_t + static_cast<long>(tt) * K.m;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            const int r = l + WAVE * t;
+            C.set(n_groups + tt, t, r < K.m ? irow[r] : 0.0);
+        }
+    }
+    const int ncols = n_groups + K.ci;
+    const double tol = has_tol ? tol_in : default_tolerance_wave<NQ, RPL>(C, K.m, ncols);
+    return nnls_small<NQ, RPL>(C, K.m, ncols, s, tol, 3 * ncols, fr);
+}
+
+} // namespace lsd

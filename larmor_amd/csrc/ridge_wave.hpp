@@ -1,0 +1,510 @@
+// The ridge-regularised Lawson-Hanson NNLS of one voxel on one wavefront.
+//
+// Restates larmor's solve_ridge_nnls (cpp/src/sd/nnls.cpp:294-350) with its Lawson-Hanson loop
+// (nnls.cpp:120-153) and ActiveCholesky (nnls.cpp:169-233):
+//  - the gradient is C^T (d - C x) = K^T s - l1 - (K^T K + r^2 I) x, summed over the non-zero
+//    coefficients in column order (nnls.cpp:305-313);
+//  - the entering column is the inactive one of largest gradient, the first on a tie; the loop
+//    ends when it does not exceed the tolerance;
+//  - least-squares fits on the active set come from the Cholesky factor L of the Gram matrix on
+//    the active columns, in the order they joined: a joining column appends the row
+//    L w = G_A,j, sqrt(G_jj - w.w), refused when that pivot is not above 1e-12 G_jj
+//    (nnls.cpp:211-227); a leaving column is removed by shifting the rows below it up and
+//    clearing the entry above the diagonal with Givens rotations (remove_cholesky_column,
+//    nnls.cpp:237-256);
+//  - once a pivot is refused, and always when the design is not well conditioned enough for the
+//    normal equations, the fits are pivoted-QR least squares on [K_A; r I] (nnls.cpp:317-338);
+//  - inner steps move x towards the candidate until the first active coefficient reaches zero
+//    (smallest x / (x - candidate), the first on a tie), drop it and any |x| below the
+//    tolerance, and refit; more than 3 n of them end the solve with the last candidate.
+//
+// Layout: solver columns are spread over lanes as LaneVec slots (column c in lane c % 64, slot
+// c / 64: NCPL slots); factor positions likewise (NSK slots).  L lives column-major with an odd
+// stride KCAP + 1 (conflict-free for both its row and column walks), in LDS for the fast path or
+// in a global scratch slab for the rerun of a voxel whose active set outgrows the LDS factor.
+#pragma once
+
+#include "qr_wave.hpp"
+
+namespace lsd {
+
+// Status bits of a voxel solve.
+constexpr int ST_NOT_CONVERGED = 1; // the inner-step limit ended an NNLS (result kept)
+constexpr int ST_OVERFLOW = 2;      // the active set outgrew the factor capacity: rerun
+constexpr int ST_QR_FALLBACK = 4;   // a Cholesky pivot was refused; the QR path finished
+constexpr int ST_OUTER_GUARD = 8;   // more entering steps than any finite solve takes: stopped
+
+// Entering steps allowed before a solve is declared cycling.  Each entering step adds a column
+// and the reference has no limit; a solve that has not ended after this many can only be
+// cycling on rounding, which would never end, so it stops with the current solution.
+__device__ __forceinline__ int outer_guard(int n) { return 8 * n + 64; }
+
+constexpr double MIN_RELATIVE_PIVOT = 1e-12;
+
+// The shared matrices of one ridge design.
+struct RidgeDesignDev {
+    int m;                // measurements
+    int n;                // columns
+    const double* kernel; // (m, n) row-major, the weighted anisotropic kernel
+    const double* gram;   // (n, n) row-major, K^T K + ridge^2 I, bitwise symmetric
+    double ridge;
+    int normal_equations; // 1: Cholesky path (with QR fallback); 0: QR only
+};
+
+// Rows per lane of the QR fallback's stacked matrix [K_A; r I].
+template <int KCAP>
+constexpr int qr_rows_per_lane(int max_m) {
+    return (max_m + KCAP + WAVE - 1) / WAVE;
+}
+
+template <int NCPL, int KCAP, int RPLQ>
+struct RidgeNnlsWave {
+    static constexpr int NSK = (KCAP + WAVE - 1) / WAVE; // factor-position slots
+    static constexpr int S = KCAP + 1;                   // factor column stride
+
+    const RidgeDesignDev& D;
+    double tol;
+    int max_inner;
+    double l1;
+    double* L;   // factor storage, S * KCAP doubles
+    double* qrs; // QR fallback slab: (RPLQ * 64) * KCAP doubles, then RPLQ * 64 for the data
+
+    // Per-lane state.
+    double x[NCPL];
+    double g[NCPL];
+    double cand[NCPL];
+    bool act[NCPL];
+    int pos[NCPL];     // factor position of the column, -1 when not in the factor
+    LaneVec<NSK, int> fcol; // column at each factor position
+    int k;             // factor size (uniform)
+    bool factor_failed;
+
+    __device__ RidgeNnlsWave(const RidgeDesignDev& design, double tol_, int max_inner_, double l1_,
+                             double* L_, double* qrs_)
+        : D(design), tol(tol_), max_inner(max_inner_), l1(l1_), L(L_), qrs(qrs_) {}
+
+    __device__ __forceinline__ int col(int t) const { return lane_id() + WAVE * t; }
+    __device__ __forceinline__ double& Lat(int i, int j) { return L[j * S + i]; }
+    __device__ __forceinline__ double Luni(int i, int j) const {
+        // A uniform factor entry, read as a vector access of the lane's neighbour + readlane.
+        const int base = j * S + (i / WAVE) * WAVE;
+        return readlane(L[base + lane_id()], i % WAVE);
+    }
+
+    // L w = b on positions [0, kk), column-oriented; returns sum of w_j^2 in order.
+    __device__ __forceinline__ double forward(LaneVec<NSK>& b, int kk) {
+        const int l = lane_id();
+        double sq = 0.0;
+        for (int j = 0; j < kk; ++j) {
+            const double bj = b.get(j);
+            const double ljj = Luni(j, j);
+            const double wj = bj != 0.0 ? bj / ljj : bj;
+            sq = sq + wj * wj;
+#pragma unroll
+            for (int u = 0; u < NSK; ++u) {
+                const int i = l + WAVE * u;
+                if (i == j) {
+                    b.v[u] = wj;
+                } else if (i > j && i < kk) {
+                    b.v[u] = b.v[u] - Lat(i, j) * wj;
+                }
+            }
+        }
+        return sq;
+    }
+
+    // L^T z = w on positions [0, kk), column-oriented.
+    __device__ __forceinline__ void backward(LaneVec<NSK>& b, int kk) {
+        const int l = lane_id();
+        for (int j = kk - 1; j >= 0; --j) {
+            const double wj = b.get(j);
+            const double ljj = Luni(j, j);
+            const double zj = wj != 0.0 ? wj / ljj : wj;
+#pragma unroll
+            for (int u = 0; u < NSK; ++u) {
+                const int i = l + WAVE * u;
+                if (i == j) {
+                    b.v[u] = zj;
+                } else if (i < j) {
+                    b.v[u] = b.v[u] - Lat(j, i) * zj;
+                }
+            }
+        }
+    }
+
+    // The value of a column-spread vector at column c for each lane's factor position.
+    __device__ __forceinline__ double gather_cols(const double (&v)[NCPL], int c) const {
+        const int src = c % WAVE;
+        const int slot = c / WAVE;
+        double out = 0.0;
+#pragma unroll
+        for (int t = 0; t < NCPL; ++t) {
+            const double w = gather(v[t], src);
+            out = slot == t ? w : out;
+        }
+        return out;
+    }
+
+    // Appends column c to the factor; false when its pivot is refused.
+    __device__ __forceinline__ bool append(int c) {
+        const int l = lane_id();
+        LaneVec<NSK> b;
+#pragma unroll
+        for (int u = 0; u < NSK; ++u) {
+            const int p = l + WAVE * u;
+            b.v[u] = p < k ? D.gram[static_cast<long>(c) * D.n + fcol.v[u]] : 0.0;
+        }
+        const double sq = forward(b, k);
+        const double gcc = D.gram[static_cast<long>(c) * D.n + c];
+        const double pivot = gcc - sq;
+        if (!(pivot > MIN_RELATIVE_PIVOT * gcc)) {
+            return false;
+        }
+#pragma unroll
+        for (int u = 0; u < NSK; ++u) {
+            const int p = l + WAVE * u;
+            if (p < k) {
+                Lat(k, p) = b.v[u];
+            }
+        }
+        if (l == k % WAVE) {
+            Lat(k, k) = sqrt(pivot);
+        }
+        fcol.set(k, c);
+#pragma unroll
+        for (int t = 0; t < NCPL; ++t) {
+            pos[t] = col(t) == c ? k : pos[t];
+        }
+        k += 1;
+        return true;
+    }
+
+    // Removes the factor position p (remove_cholesky_column).
+    __device__ __forceinline__ void remove(int p) {
+        const int l = lane_id();
+        // Shift rows p..k-2 up by one; only entries on or left of the superdiagonal are read
+        // again.  Slots run in increasing order so each row is read before it is overwritten.
+        for (int c = 0; c < k; ++c) {
+#pragma unroll
+            for (int u = 0; u < NSK; ++u) {
+                const int i = l + WAVE * u;
+                const bool move = i >= p && i < k - 1 && c <= i + 1;
+                const double v = move ? Lat(i + 1, c) : 0.0;
+                if (move) {
+                    Lat(i, c) = v;
+                }
+            }
+        }
+        for (int column = p; column < k - 1; ++column) {
+            const double diagonal = Luni(column, column);
+            const double above = Luni(column, column + 1);
+            const double length = hypot_fixed(diagonal, above);
+            const double cosine = diagonal / length;
+            const double sine = above / length;
+#pragma unroll
+            for (int u = 0; u < NSK; ++u) {
+                const int i = l + WAVE * u;
+                if (i >= column && i < k - 1) {
+                    const double left = Lat(i, column);
+                    const double right = Lat(i, column + 1);
+                    Lat(i, column) = cosine * left + sine * right;
+                    Lat(i, column + 1) = cosine * right - sine * left;
+                }
+            }
+        }
+        // Positions above p move down one.
+        LaneVec<NSK, int> shifted;
+#pragma unroll
+        for (int u = 0; u < NSK; ++u) {
+            const int i = l + WAVE * u;
+            // the column at position i + 1
+            const int src_lane = (i + 1) % WAVE;
+            const int src_slot = (i + 1) / WAVE;
+            int from = 0;
+#pragma unroll
+            for (int s2 = 0; s2 < NSK; ++s2) {
+                const int w = gather(fcol.v[s2], src_lane);
+                from = src_slot == s2 ? w : from;
+            }
+            shifted.v[u] = (i >= p && i < k - 1) ? from : fcol.v[u];
+        }
+        fcol = shifted;
+#pragma unroll
+        for (int t = 0; t < NCPL; ++t) {
+            pos[t] = pos[t] == p ? -1 : (pos[t] > p ? pos[t] - 1 : pos[t]);
+        }
+        k -= 1;
+    }
+
+    // G_AA z = corr_A from the factor, scattered to the columns (zero elsewhere).
+    __device__ __forceinline__ void solve_factor(const double (&corr)[NCPL]) {
+        const int l = lane_id();
+        LaneVec<NSK> b;
+#pragma unroll
+        for (int u = 0; u < NSK; ++u) {
+            const int p = l + WAVE * u;
+            const double v = gather_cols(corr, p < k ? fcol.v[u] : 0);
+            b.v[u] = p < k ? v : 0.0;
+        }
+        forward(b, k);
+        backward(b, k);
+#pragma unroll
+        for (int t = 0; t < NCPL; ++t) {
+            const int p = pos[t] < 0 ? 0 : pos[t];
+            double z = 0.0;
+#pragma unroll
+            for (int u = 0; u < NSK; ++u) {
+                const double w = gather(b.v[u], p % WAVE);
+                z = (p / WAVE) == u ? w : z;
+            }
+            cand[t] = pos[t] >= 0 ? z : 0.0;
+        }
+    }
+
+    // Least squares on [K_A; r I] by pivoted QR, the fallback fit (nnls.cpp fit_by_qr).
+    // Returns false on overflow (more active columns than the slab holds).
+    template <int RPL>
+    __device__ __forceinline__ bool solve_qr(const RowVec<RPL>& signal) {
+        const int l = lane_id();
+        // Active columns in increasing order.
+        LaneVec<NSK, int> idx;
+        idx.fill(0);
+        int ka = 0;
+#pragma unroll
+        for (int t = 0; t < NCPL; ++t) {
+            unsigned long long mask = ballot(col(t) < D.n && act[t]);
+            while (mask) {
+                const int b = __ffsll(static_cast<long long>(mask)) - 1;
+                mask &= mask - 1;
+                if (ka >= KCAP) {
+                    return false;
+                }
+                idx.set(ka, WAVE * t + b);
+                ++ka;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NCPL; ++t) {
+            cand[t] = 0.0;
+        }
+        if (ka == 0) {
+            return true;
+        }
+        const bool has_ridge = D.ridge > 0.0;
+        const int rows = D.m + (has_ridge ? ka : 0);
+        MemCols<RPLQ> A{qrs};
+        RowVec<RPLQ> d;
+        for (int p = 0; p < ka; ++p) {
+            const int c = idx.get(p);
+#pragma unroll
+            for (int t = 0; t < RPLQ; ++t) {
+                const int r = l + WAVE * t;
+                double v = 0.0;
+                if (r < D.m) {
+                    v = D.kernel[static_cast<long>(r) * D.n + c];
+                } else if (has_ridge && r == D.m + p) {
+                    v = D.ridge;
+                }
+                A.set(p, t, v);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < RPLQ; ++t) {
+            const int r = l + WAVE * t;
+            double v = 0.0;
+            if (r < D.m) {
+#pragma unroll
+                for (int s2 = 0; s2 < RPL; ++s2) {
+                    v = (s2 == t) ? signal.v[s2] : v;
+                }
+            } else if (has_ridge && r < rows) {
+                v = -l1 / D.ridge;
+            }
+            d.v[t] = v;
+        }
+        LaneVec<NSK> coef;
+        least_squares_wave<NSK>(A, rows, ka, d, coef);
+        for (int p = 0; p < ka; ++p) {
+            const int c = idx.get(p);
+            const double v = coef.get(p);
+#pragma unroll
+            for (int t = 0; t < NCPL; ++t) {
+                cand[t] = col(t) == c ? v : cand[t];
+            }
+        }
+        return true;
+    }
+
+    // fit_on: brings the factor to the active set and solves; false on overflow.
+    template <int RPL>
+    __device__ __forceinline__ bool fit_on(const double (&corr)[NCPL], const RowVec<RPL>& signal, int& status) {
+        const int l = lane_id();
+        if (!factor_failed) {
+            // Remove the positions whose column left, last to first.
+            int actbits = 0;
+#pragma unroll
+            for (int t = 0; t < NCPL; ++t) {
+                actbits |= act[t] ? (1 << t) : 0;
+            }
+            for (int u = NSK - 1; u >= 0; --u) {
+                const int p0 = l + WAVE * u;
+                const int c = p0 < k ? fcol.v[u] : 0;
+                const int bits = gather(actbits, c % WAVE);
+                const bool leaves = p0 < k && !((bits >> (c / WAVE)) & 1);
+                unsigned long long mask = ballot(leaves);
+                while (mask) {
+                    const int b = 63 - __clzll(static_cast<long long>(mask));
+                    mask &= ~(1ull << b);
+                    remove(WAVE * u + b);
+                }
+            }
+            // Append the active columns not in the factor, in increasing order.
+            bool failed = false;
+#pragma unroll
+            for (int t = 0; t < NCPL; ++t) {
+                unsigned long long mask = ballot(col(t) < D.n && act[t] && pos[t] < 0);
+                while (mask && !failed) {
+                    const int b = __ffsll(static_cast<long long>(mask)) - 1;
+                    mask &= mask - 1;
+                    if (k >= KCAP) {
+                        status |= ST_OVERFLOW;
+                        return false;
+                    }
+                    if (!append(WAVE * t + b)) {
+                        failed = true;
+                    }
+                }
+            }
+            if (failed) {
+                factor_failed = true;
+                status |= ST_QR_FALLBACK;
+            }
+        }
+        if (factor_failed) {
+            if (!this->template solve_qr<RPL>(signal)) {
+                status |= ST_OVERFLOW;
+                return false;
+            }
+            return true;
+        }
+        solve_factor(corr);
+        return true;
+    }
+
+    // Runs the solve; corr = K^T s - l1 per column, signal = s (for the QR fallback).
+    // Returns the status bits; x holds the solution.
+    template <int RPL>
+    __device__ __forceinline__ int run(const double (&corr)[NCPL], const RowVec<RPL>& signal) {
+        const int l = lane_id();
+        int status = 0;
+        k = 0;
+        factor_failed = D.normal_equations == 0;
+#pragma unroll
+        for (int t = 0; t < NCPL; ++t) {
+            x[t] = 0.0;
+            g[t] = corr[t];
+            act[t] = false;
+            pos[t] = -1;
+            cand[t] = 0.0;
+        }
+        fcol.fill(0);
+        int n_inner = 0;
+        int n_outer = 0;
+        while (true) {
+            if (++n_outer > outer_guard(D.n)) {
+                status |= ST_OUTER_GUARD | ST_NOT_CONVERGED;
+                break;
+            }
+            double bv = 0.0;
+            int bi = -1;
+#pragma unroll
+            for (int t = 0; t < NCPL; ++t) {
+                if (col(t) < D.n && !act[t] && (bi < 0 || g[t] > bv)) {
+                    bv = g[t];
+                    bi = col(t);
+                }
+            }
+            wave_argmax_first(bv, bi);
+            if (bi < 0 || bv <= tol) {
+                break;
+            }
+#pragma unroll
+            for (int t = 0; t < NCPL; ++t) {
+                act[t] = act[t] || col(t) == bi;
+            }
+            if (!this->template fit_on<RPL>(corr, signal, status)) {
+                return status;
+            }
+            while (true) {
+                bool mine = false;
+#pragma unroll
+                for (int t = 0; t < NCPL; ++t) {
+                    mine = mine || (col(t) < D.n && act[t] && cand[t] <= 0.0);
+                }
+                if (!ballot(mine)) {
+                    break;
+                }
+                if (++n_inner > max_inner) {
+#pragma unroll
+                    for (int t = 0; t < NCPL; ++t) {
+                        x[t] = cand[t];
+                    }
+                    return status | ST_NOT_CONVERGED;
+                }
+                double sv = 0.0;
+                int si = -1;
+#pragma unroll
+                for (int t = 0; t < NCPL; ++t) {
+                    if (col(t) < D.n && act[t] && cand[t] <= 0.0) {
+                        const double distance = x[t] - cand[t];
+                        const double s = distance > 0.0 ? x[t] / distance : 0.0;
+                        if (si < 0 || s < sv) {
+                            sv = s;
+                            si = col(t);
+                        }
+                    }
+                }
+                wave_argmin_first(sv, si);
+#pragma unroll
+                for (int t = 0; t < NCPL; ++t) {
+                    x[t] = x[t] + sv * (cand[t] - x[t]);
+                    if (col(t) == si) {
+                        x[t] = 0.0;
+                        act[t] = false;
+                    }
+                    act[t] = act[t] && fabs(x[t]) >= tol;
+                }
+                if (!this->template fit_on<RPL>(corr, signal, status)) {
+                    return status;
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < NCPL; ++t) {
+                x[t] = cand[t];
+                g[t] = corr[t];
+            }
+            // Gradient: corr - sum over non-zero x_c, in column order, of x_c G[:, c].
+#pragma unroll
+            for (int t2 = 0; t2 < NCPL; ++t2) {
+                unsigned long long mask = ballot(col(t2) < D.n && x[t2] != 0.0);
+                while (mask) {
+                    const int b = __ffsll(static_cast<long long>(mask)) - 1;
+                    mask &= mask - 1;
+                    const int c = WAVE * t2 + b;
+                    const double xc = readlane(x[t2], b);
+                    const double* grow = D.gram + static_cast<long>(c) * D.n;
+#pragma unroll
+                    for (int t = 0; t < NCPL; ++t) {
+                        if (col(t) < D.n) {
+                            g[t] = g[t] - xc * grow[col(t)];
+                        }
+                    }
+                }
+            }
+        }
+        (void)l;
+        return status;
+    }
+};
+
+} // namespace lsd

@@ -1,0 +1,171 @@
+// Wavefront-level building blocks for the CDNA4 (gfx950) spherical-deconvolution kernels.
+//
+// One 64-lane wavefront works on one voxel.  Two layouts spread a voxel's vectors over the
+// lanes:
+//  - RowVec<RPL>: a vector indexed by measurement (row) r lives in lane r % 64, register slot
+//    r / 64, so a voxel's m measurements take RPL = ceil(m / 64) registers per lane;
+//  - LaneVec<NS>: a vector indexed by solver column or factor position p lives in lane p % 64,
+//    slot p / 64.
+// A uniform (wave-wide) index into either is resolved with readlane from an ext_vector_type
+// register vector, so no register array is ever indexed dynamically.
+//
+// Sums over rows use one fixed order, the "wave sum": each lane adds its rows in increasing order
+// from 0, then the 64 partials are combined by an XOR butterfly p = p + p[lane ^ o] for
+// o = 32, 16, 8, 4, 2, 1.  The CPU oracle reproduces this order, which makes the GPU results
+// bit-identical to it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lsd {
+
+constexpr int WAVE = 64;
+
+__device__ __forceinline__ int lane_id() { return static_cast<int>(__lane_id()); }
+
+// The value of v in lane src (uniform src).
+__device__ __forceinline__ double readlane(double v, int src) {
+    const auto bits = static_cast<unsigned long long>(__double_as_longlong(v));
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits & 0xffffffffull), src);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), src);
+    const unsigned long long out =
+        (static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+        static_cast<unsigned>(lo);
+    return __longlong_as_double(static_cast<long long>(out));
+}
+
+__device__ __forceinline__ int readlane(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
+
+// The value of v in lane src, src per lane.
+__device__ __forceinline__ double gather(double v, int src) { return __shfl(v, src); }
+__device__ __forceinline__ int gather(int v, int src) { return __shfl(v, src); }
+
+__device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
+
+// The canonical butterfly of 64 lane partials; every lane gets the same sum.
+__device__ __forceinline__ double butterfly_sum(double p) {
+#pragma unroll
+    for (int o = WAVE / 2; o >= 1; o >>= 1) {
+        p = p + __shfl_xor(p, o);
+    }
+    return p;
+}
+
+// max over lanes (exact, order-free).
+__device__ __forceinline__ double wave_max(double p) {
+#pragma unroll
+    for (int o = WAVE / 2; o >= 1; o >>= 1) {
+        const double q = __shfl_xor(p, o);
+        p = q > p ? q : p;
+    }
+    return p;
+}
+
+// The largest value and the smallest index holding it, over lanes; idx < 0 means "none".
+__device__ __forceinline__ void wave_argmax_first(double& value, int& idx) {
+#pragma unroll
+    for (int o = WAVE / 2; o >= 1; o >>= 1) {
+        const double ov = __shfl_xor(value, o);
+        const int oi = __shfl_xor(idx, o);
+        const bool take = oi >= 0 && (idx < 0 || ov > value || (ov == value && oi < idx));
+        value = take ? ov : value;
+        idx = take ? oi : idx;
+    }
+}
+
+// The smallest value and the smallest index holding it, over lanes; idx < 0 means "none".
+__device__ __forceinline__ void wave_argmin_first(double& value, int& idx) {
+#pragma unroll
+    for (int o = WAVE / 2; o >= 1; o >>= 1) {
+        const double ov = __shfl_xor(value, o);
+        const int oi = __shfl_xor(idx, o);
+        const bool take = oi >= 0 && (idx < 0 || ov < value || (ov == value && oi < idx));
+        value = take ? ov : value;
+        idx = take ? oi : idx;
+    }
+}
+
+// Register vectors.  Element selection by a run-time (uniform) index stays in registers for
+// ext_vector_type values, where a plain array would be moved to scratch memory.
+template <int N>
+using dvec = double __attribute__((ext_vector_type(N)));
+template <class T, int N>
+using vec_t = T __attribute__((ext_vector_type(N)));
+
+// A vector over rows: row r in lane r % 64, slot r / 64.
+template <int RPL>
+struct RowVec {
+    dvec<RPL> v;
+
+    __device__ __forceinline__ void fill(double x) {
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+            v[t] = x;
+        }
+    }
+    // Row r (uniform r) as a wave-wide value.
+    __device__ __forceinline__ double get(int r) const {
+        const double x = v[r / WAVE];
+        return readlane(x, r % WAVE);
+    }
+    // Sets row r (uniform r) to x (uniform x).
+    __device__ __forceinline__ void set(int r, double x) {
+        if (lane_id() == r % WAVE) {
+            v[r / WAVE] = x;
+        }
+    }
+};
+
+// The wave sum over rows [begin, end) of the per-row terms held in slots.
+template <int RPL>
+__device__ __forceinline__ double wave_sum_rows(const double (&terms)[RPL], int begin, int end) {
+    const int l = lane_id();
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+        const int r = l + WAVE * t;
+        if (r >= begin && r < end) {
+            s = s + terms[t];
+        }
+    }
+    return butterfly_sum(s);
+}
+
+// A vector over columns or positions: index p in lane p % 64, slot p / 64.
+template <int NS, class T = double>
+struct LaneVec {
+    vec_t<T, NS> v;
+
+    __device__ __forceinline__ void fill(T x) {
+#pragma unroll
+        for (int t = 0; t < NS; ++t) {
+            v[t] = x;
+        }
+    }
+    __device__ __forceinline__ T get(int p) const {
+        const T x = v[p / WAVE];
+        return readlane(x, p % WAVE);
+    }
+    __device__ __forceinline__ void set(int p, T x) {
+        if (lane_id() == p % WAVE) {
+            v[p / WAVE] = x;
+        }
+    }
+};
+
+// hypot as m * sqrt(1 + (s / m)^2), m the larger magnitude; the oracle uses the same formula.
+__device__ __forceinline__ double hypot_fixed(double a, double b) {
+    const double x = fabs(a);
+    const double y = fabs(b);
+    const double big = x > y ? x : y;
+    const double small = x > y ? y : x;
+    if (big == 0.0) {
+        return 0.0;
+    }
+    const double r = small / big;
+    return big * sqrt(1.0 + r * r);
+}
+
+} // namespace lsd

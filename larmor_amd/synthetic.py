@@ -1,0 +1,149 @@
+"""Seeded synthetic diffusion MRI volumes for tests and benchmarks.
+
+Acquisition: b0 rows first, then per shell a golden-spiral set of near-uniform full-sphere
+directions, rotated by a random rotation per shell.  Tissue: 1-3 fibres along uniform random
+directions with Dirichlet(1) weights, each the axially symmetric tensor response
+exp(-b (l_perp + (l_par - l_perp)(g.u)^2)); multi-tissue voxels mix WM / GM / CSF with
+Dirichlet fractions, GM and CSF being exp(-b D).  Rician noise |S + n1 + i n2|, n ~ N(0, 1/SNR)
+relative to b0 = 1.  Masks are ellipsoids centred in the volume, scaled to a target voxel share.
+"""
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+def spiral_directions(n: int) -> np.ndarray:
+    """n near-uniform unit vectors over the full sphere (golden-angle spiral)."""
+    k = np.arange(n) + 0.5
+    z = 1.0 - 2.0 * k / n
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    phi = np.pi * (1.0 + np.sqrt(5.0)) * k
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], axis=1)
+
+
+def random_rotation(rng: np.random.Generator) -> np.ndarray:
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def acquisition(n_b0: int, shells: list[tuple[float, int]], rng: np.random.Generator):
+    """Returns (bvals, bvecs) with n_b0 b0 rows first, then each (b, n_dirs) shell."""
+    bvals = [np.zeros(n_b0)]
+    bvecs = [np.zeros((n_b0, 3))]
+    for b, n in shells:
+        dirs = spiral_directions(n) @ random_rotation(rng).T
+        bvals.append(np.full(n, float(b)))
+        bvecs.append(dirs)
+    return np.concatenate(bvals), np.concatenate(bvecs)
+
+
+def ellipsoid_mask(shape: tuple[int, int, int], share: float) -> np.ndarray:
+    """An ellipsoid centred in the grid holding about `share` of the voxels."""
+    scale = (share / (np.pi / 6.0)) ** (1.0 / 3.0)
+    axes = [np.linspace(-1.0, 1.0, s) for s in shape]
+    grid = np.meshgrid(*axes, indexing="ij")
+    radius = sum(g**2 for g in grid)
+    return radius <= scale**2
+
+
+def random_unit(rng: np.random.Generator, n: int) -> np.ndarray:
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+@dataclass
+class Phantom:
+    data: np.ndarray  # (X, Y, Z, N) float32 DWI
+    bvals: np.ndarray
+    bvecs: np.ndarray
+    mask: np.ndarray  # (X, Y, Z) bool
+    fractions: np.ndarray | None  # (n_mask, 3) true WM/GM/CSF fractions, or None
+
+
+def wm_signal(bvals, bvecs, dirs, weights, l_par=1.7e-3, l_perp=0.3e-3):
+    """Sum over fibres of w_i exp(-b (l_perp + (l_par - l_perp)(g.u_i)^2)), shape (n_vox, m)."""
+    cos = np.einsum("mj,vfj->vmf", bvecs, dirs)
+    att = np.exp(-bvals[None, :, None] * (l_perp + (l_par - l_perp) * cos**2))
+    return np.einsum("vmf,vf->vm", att, weights)
+
+
+def phantom(
+    shape: tuple[int, int, int],
+    n_b0: int,
+    shells: list[tuple[float, int]],
+    *,
+    seed: int,
+    snr: float,
+    mask_share: float | None = None,
+    tissues: bool = True,
+    tissue_alpha=(2.0, 1.0, 0.5),
+    gm_d: float = 0.8e-3,
+    csf_d: float = 3.0e-3,
+) -> Phantom:
+    """Builds a seeded phantom; only mask voxels get tissue, the others hold pure noise on S0=1."""
+    rng = np.random.default_rng(seed)
+    bvals, bvecs = acquisition(n_b0, shells, rng)
+    mask = np.ones(shape, dtype=bool) if mask_share is None else ellipsoid_mask(shape, mask_share)
+    nv = int(mask.sum())
+    m = len(bvals)
+    n_fib = rng.integers(1, 4, size=nv)
+    dirs = random_unit(rng, nv * 3).reshape(nv, 3, 3)
+    weights = np.zeros((nv, 3))
+    for k in (1, 2, 3):
+        sel = n_fib == k
+        weights[sel, :k] = rng.dirichlet(np.ones(k), size=int(sel.sum()))
+    sig = np.empty((nv, m))
+    chunk = 65536
+    for s in range(0, nv, chunk):
+        e = min(nv, s + chunk)
+        sig[s:e] = wm_signal(bvals, bvecs, dirs[s:e], weights[s:e])
+    fractions = None
+    if tissues:
+        fractions = rng.dirichlet(np.asarray(tissue_alpha), size=nv)
+        gm = np.exp(-bvals * gm_d)
+        csf = np.exp(-bvals * csf_d)
+        sig = fractions[:, :1] * sig + fractions[:, 1:2] * gm + fractions[:, 2:3] * csf
+    sigma = 1.0 / snr
+    data = np.ones(shape + (m,), dtype=np.float32)
+    noisy = np.empty((nv, m), dtype=np.float32)
+    for s in range(0, nv, chunk):
+        e = min(nv, s + chunk)
+        re = sig[s:e] + rng.normal(scale=sigma, size=(e - s, m))
+        im = rng.normal(scale=sigma, size=(e - s, m))
+        noisy[s:e] = np.sqrt(re * re + im * im)
+    data[mask] = noisy
+    outside = ~mask
+    n_out = int(outside.sum())
+    if n_out:
+        # Background: the S0 = 1 signal with no tissue contrast, plus Rician noise.
+        re = 1.0 + rng.normal(scale=sigma, size=(n_out, m))
+        im = rng.normal(scale=sigma, size=(n_out, m))
+        data[outside] = np.sqrt(re * re + im * im).astype(np.float32)
+    return Phantom(data=data, bvals=bvals, bvecs=bvecs, mask=mask, fractions=fractions)
+
+
+# The named workloads of the benchmark and the parity tests.
+CSD_CONFIG = dict(shape=(32, 32, 20), n_b0=5, shells=[(1000.0, 60)], seed=1, snr=20.0,
+                  mask_share=None, tissues=False)
+MFOD_CONFIG = dict(shape=(96, 96, 60), n_b0=10, shells=[(1000.0, 64), (2000.0, 64), (3000.0, 64)],
+                   seed=2, snr=25.0, mask_share=0.40, tissues=True)
+PRECISION_CONFIG = dict(shape=(140, 140, 96), n_b0=12,
+                        shells=[(700.0, 90), (1000.0, 90), (2000.0, 90), (3000.0, 90)], seed=3,
+                        snr=25.0, mask_share=0.45, tissues=True)
+
+
+def normalized_signals(data: np.ndarray, bvals: np.ndarray, mask: np.ndarray) -> np.ndarray:
+    """Mask voxels' signals divided by their mean b0 and capped at 1, float64 (n_mask, m).
+
+    The simple path of larmor's SDModel.fit (model.py:369-372) for voxels with plausible b0s.
+    """
+    rows = data[mask].astype(np.float64)
+    b0 = bvals <= 50.0
+    s0 = rows[:, b0].mean(axis=1)
+    out = rows / s0[:, None]
+    np.minimum(out, 1.0, out=out)
+    return out

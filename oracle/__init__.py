@@ -1,0 +1,194 @@
+"""Test infrastructure: ctypes binding of the CPU FP64 oracle (liblarmor_oracle.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module, as the
+checker or the timed CPU baseline.  The product package never imports it.
+"""
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+_DIR = Path(__file__).resolve().parent
+_LIB_PATH = _DIR / "liblarmor_oracle.so"
+_lib = None
+
+_d = ctypes.POINTER(ctypes.c_double)
+_i64 = ctypes.POINTER(ctypes.c_int64)
+
+
+def build() -> Path:
+    """Compiles the oracle with its Makefile (g++ -O3 -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", str(_DIR)], check=True)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        build()
+    lib = ctypes.CDLL(str(_LIB_PATH))
+    lib.oracle_last_error.restype = ctypes.c_char_p
+    lib.oracle_default_gradient_tolerance.restype = ctypes.c_double
+    lib.oracle_default_gradient_tolerance.argtypes = [_d, ctypes.c_int64, ctypes.c_int64]
+    lib.oracle_nnls.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, _d, ctypes.c_double,
+                                ctypes.c_int64, _d]
+    lib.oracle_ridge_nnls.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, _d, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, _d, _i64]
+    lib.oracle_ridge_normal_equations.argtypes = [_d, ctypes.c_int64, ctypes.c_int64,
+                                                  ctypes.c_double]
+    lib.oracle_ridge_gram.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, _d]
+    lib.oracle_remove_cholesky_column.argtypes = [_d, ctypes.c_int64, ctypes.c_int64]
+    lib.oracle_median.restype = ctypes.c_double
+    lib.oracle_median.argtypes = [_d, ctypes.c_int64]
+    lib.oracle_least_squares.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, _d, _d]
+    lib.oracle_compute_fractions.argtypes = [_d, _d, _d, ctypes.c_int64, ctypes.c_int64,
+                                             ctypes.c_int64, _i64, ctypes.c_int64, _d,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_double, _d]
+    lib.oracle_fit_single_tissue.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, _d,
+                                             ctypes.c_int64, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int, _d]
+    lib.oracle_fit_multi_tissue.argtypes = (
+        [_d, ctypes.c_int64, ctypes.c_int64, _d]
+        + [_d, _d, _d, _d, ctypes.c_int64, _i64, ctypes.c_int64, ctypes.c_double, ctypes.c_double]
+        + [_d, _d, _d, _d, ctypes.c_int64, _i64, ctypes.c_int64, ctypes.c_double, ctypes.c_double]
+        + [ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int, _d, _d,
+           _i64]
+    )
+    _lib = lib
+    return lib
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _p(a):
+    return a.ctypes.data_as(_d)
+
+
+def _check(status):
+    if status != 0:
+        raise ValueError(_load().oracle_last_error().decode())
+
+
+def default_gradient_tolerance(design):
+    d = _c(design)
+    return _load().oracle_default_gradient_tolerance(_p(d), d.shape[0], d.shape[1])
+
+
+def nnls(design, data, tol=None, max_inner=None):
+    """Generic Lawson-Hanson NNLS; returns (x, converged)."""
+    c = _c(design)
+    d = _c(data)
+    if tol is None:
+        tol = default_gradient_tolerance(c)
+    if max_inner is None:
+        max_inner = 3 * c.shape[1]
+    x = np.zeros(c.shape[1])
+    status = _load().oracle_nnls(_p(c), c.shape[0], c.shape[1], _p(d), tol, max_inner, _p(x))
+    if status < 0:
+        raise ValueError(_load().oracle_last_error().decode())
+    return x, bool(status)
+
+
+def ridge_nnls(kernel, signal, ridge, l1=0.0, tol=1e-6, trace=False):
+    """Ridge NNLS of one signal; with trace, also (outer, inner, max_active, qr_fallback, final)."""
+    k = _c(kernel)
+    s = _c(signal)
+    x = np.zeros(k.shape[1])
+    t = np.zeros(5, dtype=np.int64)
+    _check(_load().oracle_ridge_nnls(_p(k), k.shape[0], k.shape[1], _p(s), ridge, l1, tol, _p(x),
+                                     t.ctypes.data_as(_i64)))
+    return (x, t) if trace else x
+
+
+def ridge_normal_equations(kernel, ridge):
+    k = _c(kernel)
+    return bool(_load().oracle_ridge_normal_equations(_p(k), k.shape[0], k.shape[1], ridge))
+
+
+def ridge_gram(kernel, ridge):
+    k = _c(kernel)
+    g = np.zeros((k.shape[1], k.shape[1]))
+    _load().oracle_ridge_gram(_p(k), k.shape[0], k.shape[1], ridge, _p(g))
+    return g
+
+
+def remove_cholesky_column(factor, position):
+    f = _c(factor).copy()
+    _load().oracle_remove_cholesky_column(_p(f), f.shape[0], position)
+    return f
+
+
+def median(values):
+    v = _c(values)
+    return _load().oracle_median(_p(v), v.size)
+
+
+def least_squares(design, data):
+    c = _c(design)
+    d = _c(data)
+    x = np.zeros(c.shape[1])
+    _check(_load().oracle_least_squares(_p(c), c.shape[0], c.shape[1], _p(d), _p(x)))
+    return x
+
+
+def _blocks(blocks):
+    return _c(np.asarray([(s.start, s.stop) if isinstance(s, slice) else s for s in blocks],
+                         dtype=np.int64).reshape(-1, 2), np.int64)
+
+
+def compute_fractions(fod, k_aniso, k_iso, blocks, signal, lumped, tol=None):
+    ka = _c(k_aniso)
+    ki = _c(k_iso)
+    b = _blocks(blocks)
+    n_groups = 1 if lumped else len(b)
+    out = np.zeros(n_groups + ki.shape[1])
+    _check(_load().oracle_compute_fractions(
+        _p(_c(fod)), _p(ka), _p(ki), ka.shape[0], ka.shape[1], ki.shape[1],
+        b.ctypes.data_as(_i64), len(b), _p(_c(signal)), int(lumped), int(tol is not None),
+        0.0 if tol is None else tol, _p(out)))
+    return out
+
+
+def fit_single_tissue(signals, kernel, ridge, tol, n_threads=None):
+    s = _c(signals)
+    k = _c(kernel)
+    out = np.zeros((s.shape[0], k.shape[1]))
+    n_threads = n_threads or os.cpu_count() or 1
+    _check(_load().oracle_fit_single_tissue(_p(s), s.shape[0], s.shape[1], _p(k), k.shape[1],
+                                            ridge, tol, n_threads, _p(out)))
+    return out
+
+
+def fit_multi_tissue(signals, weights, lr, hr, lr_ridge, hr_ridge, tol_x, max_rounds=50,
+                     fraction_tol=1e-7, final_tol=1e-2, n_threads=None, trace=False):
+    """Multi-tissue fit with kernels given as objects with k_aniso, k_iso, w_k_iso, w_k_aniso
+    and aniso_blocks (larmor_amd.sd.kernel.Kernel).  Returns (fods, fractions[, trace])."""
+    s = _c(signals)
+    nv, m = s.shape
+    w = _c(weights)
+    mats = []
+    for k in (lr, hr):
+        mats.append([_c(k.k_aniso), _c(k.k_iso), _c(k.w_k_iso), _c(k.w_k_aniso),
+                     _blocks(k.aniso_blocks)])
+    ci = mats[0][1].shape[1]
+    ca_hr = mats[1][0].shape[1]
+    nf = len(mats[1][4]) + ci
+    fods = np.zeros((nv, ca_hr))
+    fractions = np.zeros((nv, nf))
+    tr = np.zeros((nv, 10), dtype=np.int64)
+    n_threads = n_threads or os.cpu_count() or 1
+    args = [_p(s), nv, m, _p(w)]
+    for (ka, ki, wki, wka, b), ridge in zip(mats, (lr_ridge, hr_ridge)):
+        args += [_p(ka), _p(ki), _p(wki), _p(wka), ka.shape[1], b.ctypes.data_as(_i64), len(b),
+                 ridge, tol_x]
+    args += [ci, max_rounds, fraction_tol, final_tol, n_threads, _p(fods), _p(fractions),
+             tr.ctypes.data_as(_i64) if trace else None]
+    _check(_load().oracle_fit_multi_tissue(*args))
+    return (fods, fractions, tr) if trace else (fods, fractions)

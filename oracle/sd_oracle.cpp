@@ -1,0 +1,1117 @@
+// Test infrastructure only: the CPU FP64 oracle of larmor's per-voxel spherical deconvolution.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg load this library, and
+// only as the checker or the timed CPU baseline; the product path (larmor_amd) never does.
+//
+// Each function restates the reference algorithm step by step, with the reference's step order
+// and tie rules, written from its source (paths relative to the reference repository):
+//   cpp/src/sd/nnls.cpp        Lawson-Hanson NNLS, pivoted-QR least squares, ridge NNLS with a
+//                              growing / downdated Cholesky factor and the QR fallback
+//   cpp/src/sd/fractions.cpp   median, compute_fractions
+//   cpp/src/sd/multi_tissue.cpp fit_multi_tissue_block
+//   cpp/src/sd/solver.cpp      L2NnlsSolver::solve_block
+//   cpp/src/common/parallel.hpp parallel_for_blocks (blocks of 128 voxels, dynamic)
+// Eigen's ColPivHouseholderQR (Eigen 5.0.1, the reference's pinned dependency) is restated from
+// its published algorithm (LAPACK xGEQP3 column pivoting with the LAWN 176 norm downdate).
+//
+// Rounding order.  The reference's sums run in Eigen's vectorised order, which no restatement
+// can reproduce, so bit-level agreement with the reference is not claimed.  This oracle fixes
+// its own order, the one the HIP kernels use, so that the oracle and the kernels agree to the bit
+// on the same inputs:
+//  - a sum over the rows (measurements) of a design -- the norms, dot products and gradients of
+//    the pivoted QR and of the generic NNLS, and the column sums of the default tolerance -- is a
+//    "wave sum": row r belongs to lane r mod 64, each lane adds its rows in increasing order
+//    starting from 0, and the 64 lane partials are combined by an XOR butterfly,
+//    p[l] = p[l] + p[l ^ o] for o = 32, 16, 8, 4, 2, 1;
+//  - every other sum (a correlation K^T s over measurements, a sum over columns or over factor
+//    positions) is sequential in index order from 0;
+//  - every triangular solve is column-oriented: a value is final once the terms of all earlier
+//    (later, for L^T) positions are subtracted, one at a time, in the order the solve meets them.
+// hypot is evaluated as m * sqrt(1 + (s / m)^2), m the larger magnitude, here and on the GPU.
+//
+// Compiled with -O3 -ffp-contract=off (no fused multiply-add), as the reference is
+// (cpp/CMakeLists.txt sets -ffp-contract=off).
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <limits>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using Index = std::int64_t;
+constexpr double EPS = std::numeric_limits<double>::epsilon();
+
+// A Cholesky pivot below this share of its diagonal entry counts as a dependent column
+// (nnls.cpp MIN_RELATIVE_PIVOT).
+constexpr double MIN_RELATIVE_PIVOT = 1e-12;
+// The largest (||K||_F^2 + r^2) / r^2 for which the ridge fits use the normal equations
+// (nnls.cpp MAX_NORMAL_EQUATIONS_CONDITION).
+constexpr double MAX_NORMAL_EQUATIONS_CONDITION = 1e8;
+// FOD values this close below the median count as equal to it (fractions.cpp).
+constexpr double MEDIAN_TIE_TOLERANCE = 1e-9;
+// Keeps the normalisation of the final fractions finite (multi_tissue.cpp).
+constexpr double FRACTION_SUM_OFFSET = 1e-16;
+
+thread_local std::string g_error;
+
+// Row-major dense matrix.
+struct Mat {
+    Index rows = 0;
+    Index cols = 0;
+    std::vector<double> v;
+    Mat() = default;
+    Mat(Index r, Index c, double fill = 0.0) : rows(r), cols(c), v(static_cast<size_t>(r * c), fill) {}
+    double& operator()(Index r, Index c) { return v[static_cast<size_t>(r * cols + c)]; }
+    double operator()(Index r, Index c) const { return v[static_cast<size_t>(r * cols + c)]; }
+};
+using Vec = std::vector<double>;
+
+Mat from_ptr(const double* p, Index rows, Index cols) {
+    Mat m(rows, cols);
+    if (rows * cols > 0) {
+        std::memcpy(m.v.data(), p, sizeof(double) * static_cast<size_t>(rows * cols));
+    }
+    return m;
+}
+
+double hypot_fixed(double a, double b) {
+    const double x = std::fabs(a);
+    const double y = std::fabs(b);
+    const double big = x > y ? x : y;
+    const double small = x > y ? y : x;
+    if (big == 0.0) {
+        return 0.0;
+    }
+    const double r = small / big;
+    return big * std::sqrt(1.0 + r * r);
+}
+
+// The wave sum of term(r) over rows r in [begin, end) (see the header).
+constexpr int WAVE = 64;
+template <class Term>
+double wave_sum(Index begin, Index end, Term term) {
+    double partial[WAVE];
+    for (int l = 0; l < WAVE; ++l) {
+        partial[l] = 0.0;
+    }
+    for (Index r = begin; r < end; ++r) {
+        partial[r % WAVE] += term(r);
+    }
+    for (int o = WAVE / 2; o >= 1; o /= 2) {
+        double next[WAVE];
+        for (int l = 0; l < WAVE; ++l) {
+            next[l] = partial[l] + partial[l ^ o];
+        }
+        std::memcpy(partial, next, sizeof(partial));
+    }
+    return partial[0];
+}
+
+// Per-voxel trace counters of one NNLS solve.
+struct NnlsTrace {
+    Index outer = 0;      // entering columns accepted
+    Index inner = 0;      // inner (feasibility) steps
+    Index max_active = 0; // largest active-set size seen
+    Index qr_fallback = 0;
+    Index final_active = 0;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Column-pivoted Householder QR (Eigen ColPivHouseholderQR, restated).
+// ---------------------------------------------------------------------------------------------
+struct PivotedQR {
+    Mat qr;
+    Vec hcoeffs;
+    std::vector<Index> perm; // perm[p]: the original column in pivot position p
+    Index nonzero_pivots = 0;
+    double maxpivot = 0.0;
+
+    void compute(Mat a) {
+        qr = std::move(a);
+        const Index rows = qr.rows;
+        const Index cols = qr.cols;
+        const Index size = std::min(rows, cols);
+        hcoeffs.assign(static_cast<size_t>(size), 0.0);
+        Vec norms_updated(static_cast<size_t>(cols));
+        Vec norms_direct(static_cast<size_t>(cols));
+        for (Index c = 0; c < cols; ++c) {
+            const double s = wave_sum(0, rows, [&](Index r) { return qr(r, c) * qr(r, c); });
+            norms_direct[c] = std::sqrt(s);
+            norms_updated[c] = norms_direct[c];
+        }
+        double max_norm = 0.0;
+        for (Index c = 0; c < cols; ++c) {
+            if (c == 0 || norms_updated[c] > max_norm) {
+                max_norm = norms_updated[c];
+            }
+        }
+        const double scaled = max_norm * EPS;
+        const double threshold_helper = (scaled * scaled) / static_cast<double>(rows);
+        const double norm_downdate_threshold = std::sqrt(EPS);
+        nonzero_pivots = size;
+        maxpivot = 0.0;
+        std::vector<Index> transpositions(static_cast<size_t>(size));
+
+        for (Index k = 0; k < size; ++k) {
+            Index biggest = k;
+            for (Index c = k + 1; c < cols; ++c) {
+                if (norms_updated[c] > norms_updated[biggest]) {
+                    biggest = c;
+                }
+            }
+            const double biggest_sq = norms_updated[biggest] * norms_updated[biggest];
+            if (nonzero_pivots == size &&
+                biggest_sq < threshold_helper * static_cast<double>(rows - k)) {
+                nonzero_pivots = k;
+            }
+            transpositions[k] = biggest;
+            if (k != biggest) {
+                for (Index r = 0; r < rows; ++r) {
+                    std::swap(qr(r, k), qr(r, biggest));
+                }
+                std::swap(norms_updated[k], norms_updated[biggest]);
+                std::swap(norms_direct[k], norms_direct[biggest]);
+            }
+
+            // Householder vector of column k, rows k..rows-1, essential part stored below.
+            const double c0 = qr(k, k);
+            const double tail_sq =
+                wave_sum(k + 1, rows, [&](Index r) { return qr(r, k) * qr(r, k); });
+            double tau;
+            double beta;
+            if (tail_sq <= std::numeric_limits<double>::min()) {
+                tau = 0.0;
+                beta = c0;
+                for (Index r = k + 1; r < rows; ++r) {
+                    qr(r, k) = 0.0;
+                }
+            } else {
+                beta = std::sqrt(c0 * c0 + tail_sq);
+                if (c0 >= 0.0) {
+                    beta = -beta;
+                }
+                const double denom = c0 - beta;
+                for (Index r = k + 1; r < rows; ++r) {
+                    qr(r, k) = qr(r, k) / denom;
+                }
+                tau = (beta - c0) / beta;
+            }
+            hcoeffs[k] = tau;
+            qr(k, k) = beta;
+            if (std::fabs(beta) > maxpivot) {
+                maxpivot = std::fabs(beta);
+            }
+
+            // Apply H_k to the columns right of k.
+            if (rows - k == 1) {
+                for (Index c = k + 1; c < cols; ++c) {
+                    qr(k, c) *= (1.0 - tau);
+                }
+            } else if (tau != 0.0) {
+                for (Index c = k + 1; c < cols; ++c) {
+                    double tmp =
+                        wave_sum(k + 1, rows, [&](Index r) { return qr(r, k) * qr(r, c); });
+                    tmp += qr(k, c);
+                    qr(k, c) -= tau * tmp;
+                    for (Index r = k + 1; r < rows; ++r) {
+                        qr(r, c) -= (tau * qr(r, k)) * tmp;
+                    }
+                }
+            }
+
+            // Norm downdate (LAWN 176).
+            for (Index c = k + 1; c < cols; ++c) {
+                if (norms_updated[c] != 0.0) {
+                    double temp = std::fabs(qr(k, c)) / norms_updated[c];
+                    temp = (1.0 + temp) * (1.0 - temp);
+                    temp = temp < 0.0 ? 0.0 : temp;
+                    const double ratio = norms_updated[c] / norms_direct[c];
+                    const double temp2 = temp * (ratio * ratio);
+                    if (temp2 <= norm_downdate_threshold) {
+                        const double s =
+                            wave_sum(k + 1, rows, [&](Index r) { return qr(r, c) * qr(r, c); });
+                        norms_direct[c] = std::sqrt(s);
+                        norms_updated[c] = norms_direct[c];
+                    } else {
+                        norms_updated[c] *= std::sqrt(temp);
+                    }
+                }
+            }
+        }
+        perm.resize(static_cast<size_t>(cols));
+        for (Index c = 0; c < cols; ++c) {
+            perm[c] = c;
+        }
+        for (Index k = 0; k < size; ++k) {
+            std::swap(perm[k], perm[transpositions[k]]);
+        }
+    }
+
+    Index rank(double threshold) const {
+        const double premultiplied = std::fabs(maxpivot) * threshold;
+        Index result = 0;
+        for (Index i = 0; i < nonzero_pivots; ++i) {
+            result += std::fabs(qr(i, i)) > premultiplied ? 1 : 0;
+        }
+        return result;
+    }
+
+    // Q^T d restricted to the first `length` reflectors (H_0 applied first).
+    void apply_qt(Vec& d, Index length) const {
+        const Index rows = qr.rows;
+        for (Index k = 0; k < length; ++k) {
+            const double tau = hcoeffs[k];
+            if (rows - k == 1) {
+                d[k] *= (1.0 - tau);
+            } else if (tau != 0.0) {
+                double tmp = wave_sum(k + 1, rows, [&](Index r) { return qr(r, k) * d[r]; });
+                tmp += d[k];
+                d[k] -= tau * tmp;
+                for (Index r = k + 1; r < rows; ++r) {
+                    d[r] -= (tau * qr(r, k)) * tmp;
+                }
+            }
+        }
+    }
+};
+
+// least_squares (nnls.cpp): coefficients of data on the columns of `design`; columns past the
+// numerical rank of the pivoted QR get zero.
+Vec least_squares(const Mat& design, const Vec& data) {
+    PivotedQR qr;
+    qr.compute(design);
+    const double threshold =
+        static_cast<double>(std::max(design.rows, design.cols)) * EPS;
+    const Index rank = qr.rank(threshold);
+    Vec rotated = data;
+    qr.apply_qt(rotated, rank);
+    // Back substitution on R11, column-oriented (one term subtracted at a time).
+    for (Index i = rank - 1; i >= 0; --i) {
+        if (rotated[i] != 0.0) {
+            rotated[i] = rotated[i] / qr.qr(i, i);
+        }
+        for (Index r = 0; r < i; ++r) {
+            rotated[r] -= qr.qr(r, i) * rotated[i];
+        }
+    }
+    Vec coefficients(static_cast<size_t>(design.cols), 0.0);
+    for (Index p = 0; p < rank; ++p) {
+        coefficients[qr.perm[p]] = rotated[p];
+    }
+    return coefficients;
+}
+
+double default_gradient_tolerance(const Mat& design) {
+    double norm1 = 0.0;
+    if (design.rows * design.cols > 0) {
+        for (Index c = 0; c < design.cols; ++c) {
+            const double s =
+                wave_sum(0, design.rows, [&](Index r) { return std::fabs(design(r, c)); });
+            if (c == 0 || s > norm1) {
+                norm1 = s;
+            }
+        }
+    }
+    return 10.0 * EPS * norm1 * static_cast<double>(std::max(design.rows, design.cols));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Lawson-Hanson (nnls.cpp lawson_hanson).
+// ---------------------------------------------------------------------------------------------
+struct NnlsResult {
+    Vec solution;
+    bool converged = true;
+};
+
+template <class GradientAt, class FitOn>
+NnlsResult lawson_hanson(Index n, double tol, Index max_inner, GradientAt gradient_at,
+                         FitOn fit_on, NnlsTrace* trace) {
+    std::vector<char> active(static_cast<size_t>(n), 0);
+    Vec x(static_cast<size_t>(n), 0.0);
+    Vec g = gradient_at(x);
+    Index n_inner = 0;
+    auto count_active = [&] {
+        Index k = 0;
+        for (Index c = 0; c < n; ++c) {
+            k += active[c] ? 1 : 0;
+        }
+        return k;
+    };
+    while (true) {
+        Index entering = -1;
+        for (Index c = 0; c < n; ++c) {
+            if (!active[c] && (entering < 0 || g[c] > g[entering])) {
+                entering = c;
+            }
+        }
+        if (entering < 0 || g[entering] <= tol) {
+            break;
+        }
+        active[entering] = 1;
+        if (trace) {
+            trace->outer += 1;
+            trace->max_active = std::max(trace->max_active, count_active());
+        }
+        Vec cand = fit_on(active);
+        while (true) {
+            bool any_nonpositive = false;
+            for (Index c = 0; c < n; ++c) {
+                if (active[c] && cand[c] <= 0.0) {
+                    any_nonpositive = true;
+                    break;
+                }
+            }
+            if (!any_nonpositive) {
+                break;
+            }
+            if (++n_inner > max_inner) {
+                if (trace) {
+                    trace->inner = n_inner;
+                    trace->final_active = count_active();
+                }
+                return {cand, false};
+            }
+            double step = std::numeric_limits<double>::infinity();
+            Index blocking = 0;
+            for (Index c = 0; c < n; ++c) {
+                if (active[c] && cand[c] <= 0.0) {
+                    const double distance = x[c] - cand[c];
+                    const double s = distance > 0.0 ? x[c] / distance : 0.0;
+                    if (s < step) {
+                        step = s;
+                        blocking = c;
+                    }
+                }
+            }
+            for (Index c = 0; c < n; ++c) {
+                x[c] = x[c] + step * (cand[c] - x[c]);
+            }
+            x[blocking] = 0.0;
+            active[blocking] = 0;
+            for (Index c = 0; c < n; ++c) {
+                active[c] = (active[c] && std::fabs(x[c]) >= tol) ? 1 : 0;
+            }
+            cand = fit_on(active);
+        }
+        x = cand;
+        g = gradient_at(x);
+    }
+    if (trace) {
+        trace->inner = n_inner;
+        trace->final_active = count_active();
+    }
+    return {x, true};
+}
+
+std::vector<Index> active_indices(const std::vector<char>& active) {
+    std::vector<Index> idx;
+    for (Index c = 0; c < static_cast<Index>(active.size()); ++c) {
+        if (active[c]) {
+            idx.push_back(c);
+        }
+    }
+    return idx;
+}
+
+// solve_nnls_with_limit (nnls.cpp): the generic NNLS, a pivoted QR per least-squares fit.
+NnlsResult solve_nnls_with_limit(const Mat& design, const Vec& data, double tol,
+                                 Index max_inner, NnlsTrace* trace) {
+    const Index n = design.cols;
+    const Index m = design.rows;
+    auto gradient_at = [&](const Vec& x) {
+        Vec r(static_cast<size_t>(m));
+        for (Index i = 0; i < m; ++i) {
+            double s = 0.0;
+            for (Index c = 0; c < n; ++c) {
+                s += design(i, c) * x[c];
+            }
+            r[i] = data[i] - s;
+        }
+        Vec g(static_cast<size_t>(n));
+        for (Index c = 0; c < n; ++c) {
+            g[c] = wave_sum(0, m, [&](Index i) { return design(i, c) * r[i]; });
+        }
+        return g;
+    };
+    auto fit_on = [&](const std::vector<char>& active) {
+        const std::vector<Index> idx = active_indices(active);
+        Vec full(static_cast<size_t>(n), 0.0);
+        if (idx.empty()) {
+            return full;
+        }
+        Mat sub(m, static_cast<Index>(idx.size()));
+        for (Index i = 0; i < m; ++i) {
+            for (size_t p = 0; p < idx.size(); ++p) {
+                sub(i, static_cast<Index>(p)) = design(i, idx[p]);
+            }
+        }
+        const Vec coef = least_squares(sub, data);
+        for (size_t p = 0; p < idx.size(); ++p) {
+            full[idx[p]] = coef[p];
+        }
+        return full;
+    };
+    return lawson_hanson(n, tol, max_inner, gradient_at, fit_on, trace);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Ridge NNLS (nnls.cpp make_ridge_design, solve_ridge_nnls, ActiveCholesky,
+// remove_cholesky_column).
+// ---------------------------------------------------------------------------------------------
+struct RidgeDesign {
+    Mat kernel;
+    Mat gram;
+    double ridge = 0.0;
+    bool normal_equations = false;
+};
+
+RidgeDesign make_ridge_design(const Mat& kernel, double ridge) {
+    RidgeDesign d;
+    d.kernel = kernel;
+    const Index m = kernel.rows;
+    const Index n = kernel.cols;
+    d.gram = Mat(n, n);
+    for (Index a = 0; a < n; ++a) {
+        for (Index b = 0; b < n; ++b) {
+            double s = 0.0;
+            for (Index i = 0; i < m; ++i) {
+                s += kernel(i, a) * kernel(i, b);
+            }
+            d.gram(a, b) = s;
+        }
+    }
+    const double r2 = ridge * ridge;
+    for (Index a = 0; a < n; ++a) {
+        d.gram(a, a) += r2;
+    }
+    double fro = 0.0;
+    for (double value : kernel.v) {
+        fro += value * value;
+    }
+    d.ridge = ridge;
+    d.normal_equations = r2 == 0.0 || fro + r2 <= MAX_NORMAL_EQUATIONS_CONDITION * r2;
+    return d;
+}
+
+// Removes the column at `position` from the lower factor of n_columns columns (nnls.cpp).
+void remove_cholesky_column(Mat& factor, Index n_columns, Index position) {
+    for (Index row = position; row < n_columns - 1; ++row) {
+        for (Index c = 0; c < n_columns; ++c) {
+            factor(row, c) = factor(row + 1, c);
+        }
+    }
+    for (Index column = position; column < n_columns - 1; ++column) {
+        const double diagonal = factor(column, column);
+        const double above = factor(column, column + 1);
+        const double length = hypot_fixed(diagonal, above);
+        const double cosine = diagonal / length;
+        const double sine = above / length;
+        for (Index row = column; row < n_columns - 1; ++row) {
+            const double left = factor(row, column);
+            const double right = factor(row, column + 1);
+            factor(row, column) = cosine * left + sine * right;
+            factor(row, column + 1) = cosine * right - sine * left;
+        }
+    }
+}
+
+class ActiveCholesky {
+public:
+    explicit ActiveCholesky(const Mat& g)
+        : gram(g), factor(g.rows, g.rows), in_factor(static_cast<size_t>(g.rows), 0) {}
+
+    bool update(const std::vector<char>& active) {
+        for (Index pos = static_cast<Index>(columns.size()) - 1; pos >= 0; --pos) {
+            const Index column = columns[pos];
+            if (!active[column]) {
+                remove_cholesky_column(factor, static_cast<Index>(columns.size()), pos);
+                columns.erase(columns.begin() + pos);
+                in_factor[column] = 0;
+            }
+        }
+        for (Index column = 0; column < static_cast<Index>(active.size()); ++column) {
+            if (active[column] && !in_factor[column] && !append(column)) {
+                return false;
+            }
+        }
+        return true;
+    }
+
+    Vec solve(const Vec& rhs) const {
+        const Index k = static_cast<Index>(columns.size());
+        Vec w(static_cast<size_t>(k));
+        for (Index p = 0; p < k; ++p) {
+            w[p] = rhs[columns[p]];
+        }
+        forward(w, k);
+        // L^T z = w, column-oriented: z_j final, then w_i -= L(j, i) z_j for i < j.
+        for (Index j = k - 1; j >= 0; --j) {
+            if (w[j] != 0.0) {
+                w[j] = w[j] / factor(j, j);
+            }
+            for (Index i = 0; i < j; ++i) {
+                w[i] -= factor(j, i) * w[j];
+            }
+        }
+        Vec full(rhs.size(), 0.0);
+        for (Index p = 0; p < k; ++p) {
+            full[columns[p]] = w[p];
+        }
+        return full;
+    }
+
+    Index size() const { return static_cast<Index>(columns.size()); }
+
+private:
+    // L w = b in place on the first k rows, column-oriented.
+    void forward(Vec& w, Index k) const {
+        for (Index j = 0; j < k; ++j) {
+            if (w[j] != 0.0) {
+                w[j] = w[j] / factor(j, j);
+            }
+            for (Index i = j + 1; i < k; ++i) {
+                w[i] -= factor(i, j) * w[j];
+            }
+        }
+    }
+
+    bool append(Index column) {
+        const Index pos = static_cast<Index>(columns.size());
+        Vec cross(static_cast<size_t>(pos));
+        for (Index r = 0; r < pos; ++r) {
+            cross[r] = gram(columns[r], column);
+        }
+        forward(cross, pos);
+        double sq = 0.0;
+        for (Index r = 0; r < pos; ++r) {
+            sq += cross[r] * cross[r];
+        }
+        const double pivot = gram(column, column) - sq;
+        if (!(pivot > MIN_RELATIVE_PIVOT * gram(column, column))) {
+            return false;
+        }
+        for (Index r = 0; r < pos; ++r) {
+            factor(pos, r) = cross[r];
+        }
+        factor(pos, pos) = std::sqrt(pivot);
+        columns.push_back(column);
+        in_factor[column] = 1;
+        return true;
+    }
+
+    const Mat& gram;
+    Mat factor;
+    std::vector<Index> columns;
+    std::vector<char> in_factor;
+};
+
+Vec solve_ridge_nnls(const RidgeDesign& design, const Vec& signal, double l1, double tol,
+                     NnlsTrace* trace) {
+    if (l1 > 0 && design.ridge == 0) {
+        throw std::invalid_argument("l1 > 0 needs ridge > 0.");
+    }
+    const Index m = design.kernel.rows;
+    const Index n = design.kernel.cols;
+    Vec correlations(static_cast<size_t>(n));
+    for (Index c = 0; c < n; ++c) {
+        double s = 0.0;
+        for (Index i = 0; i < m; ++i) {
+            s += design.kernel(i, c) * signal[i];
+        }
+        correlations[c] = s - l1;
+    }
+    auto gradient_at = [&](const Vec& x) {
+        Vec g = correlations;
+        for (Index c = 0; c < n; ++c) {
+            if (x[c] != 0.0) {
+                const double xc = x[c];
+                for (Index i = 0; i < n; ++i) {
+                    g[i] -= xc * design.gram(i, c);
+                }
+            }
+        }
+        return g;
+    };
+    auto fit_by_qr = [&](const std::vector<char>& active) {
+        const std::vector<Index> idx = active_indices(active);
+        const Index k = static_cast<Index>(idx.size());
+        Vec full(static_cast<size_t>(n), 0.0);
+        if (k == 0) {
+            return full;
+        }
+        const bool has_ridge = design.ridge > 0.0;
+        Mat sub(m + (has_ridge ? k : 0), k);
+        for (Index i = 0; i < m; ++i) {
+            for (Index p = 0; p < k; ++p) {
+                sub(i, p) = design.kernel(i, idx[p]);
+            }
+        }
+        Vec data(static_cast<size_t>(sub.rows), 0.0);
+        for (Index i = 0; i < m; ++i) {
+            data[i] = signal[i];
+        }
+        if (has_ridge) {
+            for (Index p = 0; p < k; ++p) {
+                sub(m + p, p) = design.ridge;
+                data[m + p] = -l1 / design.ridge;
+            }
+        }
+        const Vec coef = least_squares(sub, data);
+        for (Index p = 0; p < k; ++p) {
+            full[idx[p]] = coef[p];
+        }
+        return full;
+    };
+    const Index max_inner = 3 * n;
+    if (!design.normal_equations) {
+        return lawson_hanson(n, tol, max_inner, gradient_at, fit_by_qr, trace).solution;
+    }
+    ActiveCholesky cholesky(design.gram);
+    bool factor_failed = false;
+    auto fit_on = [&](const std::vector<char>& active) {
+        factor_failed = factor_failed || !cholesky.update(active);
+        if (factor_failed && trace) {
+            trace->qr_fallback = 1;
+        }
+        return factor_failed ? fit_by_qr(active) : cholesky.solve(correlations);
+    };
+    return lawson_hanson(n, tol, max_inner, gradient_at, fit_on, trace).solution;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Fractions (fractions.cpp) and the multi-tissue loop (multi_tissue.cpp).
+// ---------------------------------------------------------------------------------------------
+bool less_nan_last(double left, double right) {
+    return std::isnan(right) ? !std::isnan(left) : left < right;
+}
+
+double median(const Vec& values) {
+    Vec sorted = values;
+    if (sorted.empty()) {
+        return std::numeric_limits<double>::quiet_NaN();
+    }
+    std::sort(sorted.begin(), sorted.end(), less_nan_last);
+    const size_t mid = sorted.size() / 2;
+    if (sorted.size() % 2 == 1) {
+        return sorted[mid];
+    }
+    return (sorted[mid - 1] + sorted[mid]) / 2.0;
+}
+
+struct TissueKernel {
+    Mat anisotropic;        // (m, ca)
+    Mat isotropic;          // (m, ci)
+    Mat weighted_isotropic; // (m, ci)
+    std::vector<std::pair<Index, Index>> blocks;
+};
+
+// compute_fractions (fractions.cpp). groups: one range per group column.
+Vec compute_fractions(const Vec& fod, const TissueKernel& kernel, const Vec& signal,
+                      bool lumped, bool has_tolerance, double tolerance, NnlsTrace* trace) {
+    const double threshold = median(fod) * (1.0 - MEDIAN_TIE_TOLERANCE);
+    Vec thr(fod.size());
+    for (size_t c = 0; c < fod.size(); ++c) {
+        thr[c] = fod[c] < threshold ? 0.0 : fod[c];
+    }
+    std::vector<std::pair<Index, Index>> groups;
+    if (lumped) {
+        groups.push_back({0, kernel.anisotropic.cols});
+    } else {
+        groups = kernel.blocks;
+    }
+    const Index m = kernel.anisotropic.rows;
+    const Index ng = static_cast<Index>(groups.size());
+    const Index ci = kernel.isotropic.cols;
+    Mat columns(m, ng + ci);
+    for (Index g = 0; g < ng; ++g) {
+        const auto [begin, end] = groups[g];
+        Vec gs(static_cast<size_t>(m));
+        for (Index i = 0; i < m; ++i) {
+            double s = 0.0;
+            for (Index c = begin; c < end; ++c) {
+                s += kernel.anisotropic(i, c) * thr[c];
+            }
+            gs[i] = s;
+        }
+        if (m > 0) {
+            // maxCoeff<PropagateNumbers>: the largest non-NaN value.
+            double largest = std::numeric_limits<double>::quiet_NaN();
+            for (Index i = 0; i < m; ++i) {
+                if (std::isnan(largest) || gs[i] > largest) {
+                    if (!std::isnan(gs[i])) {
+                        largest = gs[i];
+                    }
+                }
+            }
+            if (largest > 0.0) {
+                for (Index i = 0; i < m; ++i) {
+                    gs[i] = gs[i] / largest;
+                }
+            }
+        }
+        for (Index i = 0; i < m; ++i) {
+            columns(i, g) = gs[i];
+        }
+    }
+    for (Index i = 0; i < m; ++i) {
+        for (Index t = 0; t < ci; ++t) {
+            columns(i, ng + t) = kernel.isotropic(i, t);
+        }
+    }
+    const double tol = has_tolerance ? tolerance : default_gradient_tolerance(columns);
+    return solve_nnls_with_limit(columns, signal, tol, 3 * columns.cols, trace).solution;
+}
+
+struct Solver {
+    RidgeDesign design;
+    double tol = 0.0;
+};
+
+struct MtTrace {
+    Index rounds = 0;
+    NnlsTrace lr_sum;      // summed over rounds
+    Index lr_max_active = 0;
+    Index lr_qr_fallbacks = 0;
+    NnlsTrace hr;
+    Index frac_inner = 0;
+};
+
+void fit_multi_tissue_block(const Mat& signals, const Vec& weights, const TissueKernel& lr,
+                            const Solver& lr_solver, const TissueKernel& hr,
+                            const Solver& hr_solver, Index max_rounds, double fraction_tol,
+                            double final_tol, double* fods_out, double* fractions_out,
+                            MtTrace* traces) {
+    const Index nv = signals.rows;
+    const Index m = signals.cols;
+    const Index ci = lr.isotropic.cols;
+    const Index nlf = 1 + ci;
+    Mat weighted(nv, m);
+    for (Index v = 0; v < nv; ++v) {
+        for (Index i = 0; i < m; ++i) {
+            weighted(v, i) = signals(v, i) * weights[i];
+        }
+    }
+    Mat iso(nv, ci, 0.0);
+    Mat previous(nv, nlf, std::numeric_limits<double>::infinity());
+    std::vector<Index> looping;
+    for (Index v = 0; v < nv; ++v) {
+        looping.push_back(v);
+    }
+    std::vector<Index> still;
+    auto aniso_signal = [&](Index v, const Mat& w_iso) {
+        Vec a(static_cast<size_t>(m));
+        for (Index i = 0; i < m; ++i) {
+            double s = 0.0;
+            for (Index t = 0; t < ci; ++t) {
+                s += w_iso(i, t) * iso(v, t);
+            }
+            a[i] = std::fmax(weighted(v, i) - s, 0.0);
+        }
+        return a;
+    };
+    auto row = [&](const Mat& mat, Index v) {
+        return Vec(mat.v.begin() + v * mat.cols, mat.v.begin() + (v + 1) * mat.cols);
+    };
+    for (Index round = 0; round < max_rounds && !looping.empty(); ++round) {
+        still.clear();
+        for (Index v : looping) {
+            const Vec a = aniso_signal(v, lr.weighted_isotropic);
+            NnlsTrace tr;
+            const Vec fod = solve_ridge_nnls(lr_solver.design, a, 0.0, lr_solver.tol,
+                                             traces ? &tr : nullptr);
+            NnlsTrace ftr;
+            const Vec fr = compute_fractions(fod, lr, row(signals, v), true, false, 0.0,
+                                             traces ? &ftr : nullptr);
+            if (traces) {
+                MtTrace& t = traces[v];
+                t.rounds += 1;
+                t.lr_sum.outer += tr.outer;
+                t.lr_sum.inner += tr.inner;
+                t.lr_sum.max_active += tr.final_active;
+                t.lr_max_active = std::max(t.lr_max_active, tr.max_active);
+                t.lr_qr_fallbacks += tr.qr_fallback;
+                t.frac_inner += ftr.inner;
+            }
+            for (Index t = 0; t < ci; ++t) {
+                iso(v, t) = fr[1 + t];
+            }
+            bool converged = true;
+            for (Index f = 0; f < nlf; ++f) {
+                if (!(std::fabs(fr[f] - previous(v, f)) < fraction_tol)) {
+                    converged = false;
+                }
+            }
+            if (!converged) {
+                for (Index f = 0; f < nlf; ++f) {
+                    previous(v, f) = fr[f];
+                }
+                still.push_back(v);
+            }
+        }
+        std::swap(looping, still);
+    }
+    const Index ca = hr.anisotropic.cols;
+    const Index nf = static_cast<Index>(hr.blocks.size()) + ci;
+    for (Index v = 0; v < nv; ++v) {
+        const Vec a = aniso_signal(v, hr.weighted_isotropic);
+        NnlsTrace tr;
+        const Vec fod =
+            solve_ridge_nnls(hr_solver.design, a, 0.0, hr_solver.tol, traces ? &tr : nullptr);
+        if (traces) {
+            traces[v].hr = tr;
+        }
+        for (Index c = 0; c < ca; ++c) {
+            fods_out[v * ca + c] = fod[c];
+        }
+    }
+    for (Index v = 0; v < nv; ++v) {
+        const Vec fod(fods_out + v * ca, fods_out + (v + 1) * ca);
+        const Vec fr = compute_fractions(fod, hr, row(signals, v), false, true, final_tol, nullptr);
+        double total = 0.0;
+        for (double f : fr) {
+            total += f;
+        }
+        total += FRACTION_SUM_OFFSET;
+        for (Index f = 0; f < nf; ++f) {
+            fractions_out[v * nf + f] = fr[f] / total;
+        }
+    }
+}
+
+// parallel_for_blocks (parallel.hpp): blocks of block_size items taken from a shared counter;
+// the first exception is rethrown after all threads finish.
+template <class Body>
+void parallel_for_blocks(Index n_items, Index block_size, int n_threads, Body body) {
+    const Index n_blocks = (n_items + block_size - 1) / block_size;
+    if (n_blocks == 0) {
+        return;
+    }
+    std::atomic<Index> next{0};
+    std::atomic<bool> failed{false};
+    std::exception_ptr first;
+    std::mutex mu;
+    auto run = [&] {
+        try {
+            while (!failed.load(std::memory_order_relaxed)) {
+                const Index b = next.fetch_add(1);
+                if (b >= n_blocks) {
+                    break;
+                }
+                const Index begin = b * block_size;
+                const Index end = std::min(begin + block_size, n_items);
+                body(begin, end);
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(mu);
+            if (!first) {
+                first = std::current_exception();
+            }
+            failed.store(true);
+        }
+    };
+    const Index workers = std::min<Index>(n_threads, n_blocks);
+    std::vector<std::thread> threads;
+    for (Index w = 1; w < workers; ++w) {
+        threads.emplace_back(run);
+    }
+    run();
+    for (auto& t : threads) {
+        t.join();
+    }
+    if (first) {
+        std::rethrow_exception(first);
+    }
+}
+
+TissueKernel make_kernel(const double* k_aniso, const double* k_iso, const double* w_k_iso,
+                         Index m, Index ca, Index ci, const std::int64_t* blocks,
+                         Index n_blocks) {
+    TissueKernel k;
+    k.anisotropic = from_ptr(k_aniso, m, ca);
+    k.isotropic = from_ptr(k_iso, m, ci);
+    k.weighted_isotropic = from_ptr(w_k_iso, m, ci);
+    for (Index b = 0; b < n_blocks; ++b) {
+        k.blocks.push_back({blocks[2 * b], blocks[2 * b + 1]});
+    }
+    return k;
+}
+
+template <class F>
+int guarded(F f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return 1;
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+const char* oracle_last_error() { return g_error.c_str(); }
+
+double oracle_default_gradient_tolerance(const double* design, std::int64_t rows,
+                                         std::int64_t cols) {
+    return default_gradient_tolerance(from_ptr(design, rows, cols));
+}
+
+// Generic NNLS; returns 1 when converged, 0 when the inner-step limit stopped it, -1 on error.
+int oracle_nnls(const double* design, std::int64_t rows, std::int64_t cols, const double* data,
+                double tol, std::int64_t max_inner, double* solution) {
+    NnlsResult r;
+    const int status = guarded([&] {
+        const Mat c = from_ptr(design, rows, cols);
+        const Vec d(data, data + rows);
+        r = solve_nnls_with_limit(c, d, tol, max_inner, nullptr);
+    });
+    if (status) {
+        return -1;
+    }
+    std::copy(r.solution.begin(), r.solution.end(), solution);
+    return r.converged ? 1 : 0;
+}
+
+int oracle_ridge_nnls(const double* kernel, std::int64_t rows, std::int64_t cols,
+                      const double* signal, double ridge, double l1, double tol,
+                      double* solution, std::int64_t* trace) {
+    return guarded([&] {
+        const RidgeDesign d = make_ridge_design(from_ptr(kernel, rows, cols), ridge);
+        NnlsTrace tr;
+        const Vec x = solve_ridge_nnls(d, Vec(signal, signal + rows), l1, tol, &tr);
+        std::copy(x.begin(), x.end(), solution);
+        if (trace) {
+            trace[0] = tr.outer;
+            trace[1] = tr.inner;
+            trace[2] = tr.max_active;
+            trace[3] = tr.qr_fallback;
+            trace[4] = tr.final_active;
+        }
+    });
+}
+
+int oracle_ridge_normal_equations(const double* kernel, std::int64_t rows, std::int64_t cols,
+                                  double ridge) {
+    return make_ridge_design(from_ptr(kernel, rows, cols), ridge).normal_equations ? 1 : 0;
+}
+
+// The ridge design's Gram matrix K^T K + r^2 I, (cols, cols).
+void oracle_ridge_gram(const double* kernel, std::int64_t rows, std::int64_t cols, double ridge,
+                       double* gram) {
+    const RidgeDesign d = make_ridge_design(from_ptr(kernel, rows, cols), ridge);
+    std::copy(d.gram.v.begin(), d.gram.v.end(), gram);
+}
+
+// remove_cholesky_column on a (n, n) row-major factor, in place.
+void oracle_remove_cholesky_column(double* factor, std::int64_t n, std::int64_t position) {
+    Mat f = from_ptr(factor, n, n);
+    remove_cholesky_column(f, n, position);
+    std::copy(f.v.begin(), f.v.end(), factor);
+}
+
+double oracle_median(const double* values, std::int64_t n) {
+    return median(Vec(values, values + n));
+}
+
+int oracle_least_squares(const double* design, std::int64_t rows, std::int64_t cols,
+                         const double* data, double* coefficients) {
+    return guarded([&] {
+        const Vec x = least_squares(from_ptr(design, rows, cols), Vec(data, data + rows));
+        std::copy(x.begin(), x.end(), coefficients);
+    });
+}
+
+int oracle_compute_fractions(const double* fod, const double* k_aniso, const double* k_iso,
+                             std::int64_t m, std::int64_t ca, std::int64_t ci,
+                             const std::int64_t* blocks, std::int64_t n_blocks,
+                             const double* signal, int lumped, int has_tol, double tol,
+                             double* fractions) {
+    return guarded([&] {
+        const TissueKernel k = make_kernel(k_aniso, k_iso, k_iso, m, ca, ci, blocks, n_blocks);
+        const Vec f = compute_fractions(Vec(fod, fod + ca), k, Vec(signal, signal + m),
+                                        lumped != 0, has_tol != 0, tol, nullptr);
+        std::copy(f.begin(), f.end(), fractions);
+    });
+}
+
+// fit_single_tissue: one ridge NNLS per voxel with a prepared solver.
+int oracle_fit_single_tissue(const double* signals, std::int64_t n_voxels, std::int64_t m,
+                             const double* kernel, std::int64_t n, double ridge, double tol,
+                             int n_threads, double* fods) {
+    return guarded([&] {
+        const RidgeDesign d = make_ridge_design(from_ptr(kernel, m, n), ridge);
+        parallel_for_blocks(n_voxels, 128, n_threads, [&](Index begin, Index end) {
+            for (Index v = begin; v < end; ++v) {
+                const Vec s(signals + v * m, signals + (v + 1) * m);
+                const Vec x = solve_ridge_nnls(d, s, 0.0, tol, nullptr);
+                std::copy(x.begin(), x.end(), fods + v * n);
+            }
+        });
+    });
+}
+
+// fit_multi_tissue. trace (optional): per voxel 10 int64s
+// [rounds, lr_outer_sum, lr_inner_sum, lr_final_active_sum, lr_max_active, lr_qr_fallbacks,
+//  hr_outer, hr_inner, hr_max_active, hr_qr_fallback].
+int oracle_fit_multi_tissue(const double* signals, std::int64_t n_voxels, std::int64_t m,
+                            const double* weights, const double* lr_k_aniso,
+                            const double* lr_k_iso, const double* lr_w_k_iso,
+                            const double* lr_w_k_aniso, std::int64_t lr_ca,
+                            const std::int64_t* lr_blocks, std::int64_t lr_n_blocks,
+                            double lr_ridge, double lr_tol, const double* hr_k_aniso,
+                            const double* hr_k_iso, const double* hr_w_k_iso,
+                            const double* hr_w_k_aniso, std::int64_t hr_ca,
+                            const std::int64_t* hr_blocks, std::int64_t hr_n_blocks,
+                            double hr_ridge, double hr_tol, std::int64_t ci,
+                            std::int64_t max_rounds, double fraction_tol, double final_tol,
+                            int n_threads, double* fods, double* fractions,
+                            std::int64_t* trace) {
+    return guarded([&] {
+        const TissueKernel lr =
+            make_kernel(lr_k_aniso, lr_k_iso, lr_w_k_iso, m, lr_ca, ci, lr_blocks, lr_n_blocks);
+        const TissueKernel hr =
+            make_kernel(hr_k_aniso, hr_k_iso, hr_w_k_iso, m, hr_ca, ci, hr_blocks, hr_n_blocks);
+        const Solver lr_solver{make_ridge_design(from_ptr(lr_w_k_aniso, m, lr_ca), lr_ridge),
+                               lr_tol};
+        const Solver hr_solver{make_ridge_design(from_ptr(hr_w_k_aniso, m, hr_ca), hr_ridge),
+                               hr_tol};
+        const Vec w(weights, weights + m);
+        const Index nf = hr_n_blocks + ci;
+        parallel_for_blocks(n_voxels, 128, n_threads, [&](Index begin, Index end) {
+            const Index nb = end - begin;
+            const Mat block = from_ptr(signals + begin * m, nb, m);
+            std::vector<MtTrace> tr(trace ? static_cast<size_t>(nb) : 0);
+            fit_multi_tissue_block(block, w, lr, lr_solver, hr, hr_solver, max_rounds,
+                                   fraction_tol, final_tol, fods + begin * hr_ca,
+                                   fractions + begin * nf, trace ? tr.data() : nullptr);
+            if (trace) {
+                for (Index v = 0; v < nb; ++v) {
+                    std::int64_t* t = trace + (begin + v) * 10;
+                    t[0] = tr[v].rounds;
+                    t[1] = tr[v].lr_sum.outer;
+                    t[2] = tr[v].lr_sum.inner;
+                    t[3] = tr[v].lr_sum.max_active;
+                    t[4] = tr[v].lr_max_active;
+                    t[5] = tr[v].lr_qr_fallbacks;
+                    t[6] = tr[v].hr.outer;
+                    t[7] = tr[v].hr.inner;
+                    t[8] = tr[v].hr.max_active;
+                    t[9] = tr[v].hr.qr_fallback;
+                }
+            }
+        });
+    });
+}
+
+} // extern "C"
