@@ -40,6 +40,16 @@ const char* lsd_last_error(void);
 int lsd_abi_version(void);
 
 /*
+ * Opt-in kernel timing (no counterpart in the reference, which has no profiling hooks): with
+ * profiling on, every kernel launch is bracketed by HIP events on its own stream;
+ * lsd_profile_collect waits for them, and returns per kernel name (newline-separated in names)
+ * the summed milliseconds and launch counts, then forgets them.
+ */
+int lsd_profile_enable(int on);
+int lsd_profile_collect(char* names, int64_t names_bytes, double* ms, int64_t* counts,
+                        int64_t max_entries, int64_t* n_entries);
+
+/*
  * An L2-regularised NNLS FOD solver for kernel K (host, m x n): its device copy, the Gram matrix
  * K^T K + ridge^2 I, and whether fits use the normal equations (ridge 0, or
  * (||K||_F^2 + ridge^2) / ridge^2 <= 1e8).

@@ -20,6 +20,11 @@ _vp = ctypes.c_void_p
 SIGNATURES = {
     "lsd_last_error": (ctypes.c_char_p, []),
     "lsd_abi_version": (ctypes.c_int, []),
+    "lsd_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "lsd_profile_collect": (
+        ctypes.c_int,
+        [ctypes.c_char_p, ctypes.c_int64, _d, _i64, ctypes.c_int64, _i64],
+    ),
     "lsd_solver_create_l2nnls": (
         ctypes.c_int,
         [_d, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
@@ -59,6 +64,11 @@ def load() -> ctypes.CDLL:
         raise LibraryMissing(
             f"{LIB_PATH} is not built; build it with `python -m larmor_amd.build`."
         )
+    # torch ships its own HIP runtime (torch/lib/libamdhip64.so, SONAME libamdhip64.so.7).
+    # Importing torch first makes the loader bind this library to that same runtime, so torch's
+    # allocations and streams and these kernels share one HIP context.
+    import torch  # noqa: F401
+
     lib = ctypes.CDLL(str(LIB_PATH))
     for name, (restype, argtypes) in SIGNATURES.items():
         fn = getattr(lib, name)
@@ -76,6 +86,23 @@ def check(status: int) -> None:
     if status == 1:
         raise ValueError(message)
     raise RuntimeError(message)
+
+
+def profile_enable(on: bool = True) -> None:
+    """Turns per-launch event timing on or off."""
+    check(load().lsd_profile_enable(1 if on else 0))
+
+
+def profile_collect() -> dict[str, tuple[float, int]]:
+    """{kernel name: (summed ms, launches)} since the last collect; waits for the events."""
+    lib = load()
+    names = ctypes.create_string_buffer(4096)
+    ms = (ctypes.c_double * 64)()
+    counts = (ctypes.c_int64 * 64)()
+    n = ctypes.c_int64()
+    check(lib.lsd_profile_collect(names, 4096, ms, counts, 64, ctypes.byref(n)))
+    labels = names.value.decode().split("\n")
+    return {labels[i]: (ms[i], counts[i]) for i in range(n.value)}
 
 
 def declared_symbols() -> list[str]:

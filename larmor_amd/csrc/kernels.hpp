@@ -1,0 +1,510 @@
+// Kernels of liblarmor_sd.so and the launch helpers shared by its translation units.
+//
+// The C-ABI replaces the compiled core that larmor's Python fit API calls for this path
+// (cpp/src/python/sd.cpp: FodSolver.l2nnls, fit_single_tissue, DeconvKernel,
+// fit_multi_tissue) with device-resident solver and kernel objects and stream-ordered entry
+// points on device buffers.  Declarations and per-function reference citations are in
+// include/larmor_sd.h.
+//
+// Execution model: one 64-lane wavefront per voxel, voxels strided over a grid sized to the
+// chip.  A multi-tissue fit runs two kernels: the low-resolution loop (FOD NNLS <-> fraction
+// NNLS rounds, giving each voxel's isotropic fractions) and the final high-resolution FOD and
+// fractions.  A voxel whose active set outgrows the LDS Cholesky factor is appended to a device
+// list and solved again by a rerun kernel that keeps its factor in global memory; the rerun
+// kernels read the list length on the device, so a fit is a fixed sequence of launches with no
+// host synchronisation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/larmor_sd.h"
+#include "fractions_wave.hpp"
+#include "ridge_wave.hpp"
+
+namespace lsd {
+
+constexpr int NQ = 4;             // most fraction columns (groups + isotropic tissues)
+constexpr int KCAP_LDS = 32;      // LDS factor capacity (active columns)
+constexpr int WAVES_PER_BLOCK = 4;
+constexpr int RERUN_WAVES = 32;   // waves of a rerun launch
+constexpr double FRACTION_SUM_OFFSET = 1e-16;
+
+constexpr int lds_factor_doubles(int kcap) { return (kcap + 1) * kcap + WAVE; }
+
+// ---------------------------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------------------------
+
+// corr_c = sum over rows, in order, of K(r, c) a_r, minus l1 (nnls.cpp:302).
+template <int NCPL, int RPL>
+__device__ __forceinline__ void correlations(const RidgeDesignDev& D, const RowVec<RPL>& a, double l1,
+                             double (&corr)[NCPL]) {
+    const int l = lane_id();
+#pragma unroll
+    for (int t = 0; t < NCPL; ++t) {
+        corr[t] = 0.0;
+    }
+#pragma unroll
+    for (int ts = 0; ts < RPL; ++ts) {
+        const int rows_here = D.m - WAVE * ts;
+        const int count = rows_here < WAVE ? rows_here : WAVE;
+        for (int ll = 0; ll < count; ++ll) {
+            const int r = WAVE * ts + ll;
+            const double ar = readlane(a.v[ts], ll);
+            const double* krow = D.kernel + static_cast<long>(r) * D.n;
+#pragma unroll
+            for (int t = 0; t < NCPL; ++t) {
+                const int c = l + WAVE * t;
+                if (c < D.n) {
+                    corr[t] = corr[t] + krow[c] * ar;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NCPL; ++t) {
+        corr[t] = corr[t] - l1;
+    }
+}
+
+template <int RPL>
+__device__ __forceinline__ void load_signal(const double* signals, long v, int m, RowVec<RPL>& s) {
+    const int l = lane_id();
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+        const int r = l + WAVE * t;
+        s.v[t] = r < m ? signals[v * m + r] : 0.0;
+    }
+}
+
+// max(w s - W_iso p, 0) per row (multi_tissue.cpp:20-24), p the isotropic fractions.
+template <int RPL>
+__device__ __forceinline__ void anisotropic_signal(const RowVec<RPL>& s, const double* weights,
+                                   const double* w_iso, int m, int ci, const double (&p)[NQ],
+                                   RowVec<RPL>& a) {
+    const int l = lane_id();
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+        const int r = l + WAVE * t;
+        double out = 0.0;
+        if (r < m) {
+            const double ws = s.v[t] * weights[r];
+            double acc = 0.0;
+#pragma unroll
+            for (int tt = 0; tt < NQ; ++tt) {
+                if (tt < ci) {
+                    acc = acc + w_iso[static_cast<long>(r) * ci + tt] * p[tt];
+                }
+            }
+            out = fmax(ws - acc, 0.0);
+        }
+        a.v[t] = out;
+    }
+}
+
+struct SolveScratch {
+    double* base;     // this launch's slab array
+    long per_wave;    // doubles per wave
+};
+
+// ---------------------------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------------------------
+
+struct VoxelSet {
+    long n_vox;               // all voxels, when list is null
+    const int* list;          // voxel indices to process, or null
+    const int* list_count;    // device count of list
+    int* overflow_list;       // where voxels that overflow are recorded (null: none expected)
+    int* overflow_count;
+};
+
+__device__ __forceinline__ long voxel_count(const VoxelSet& vs) {
+    return vs.list ? static_cast<long>(*vs.list_count) : vs.n_vox;
+}
+
+__device__ __forceinline__ void record_overflow(const VoxelSet& vs, long v) {
+    if (lane_id() == 0 && vs.overflow_count != nullptr) {
+        const int at = atomicAdd(vs.overflow_count, 1);
+        vs.overflow_list[at] = static_cast<int>(v);
+    }
+}
+
+struct SingleArgs {
+    const double* signals;
+    int m;
+    RidgeDesignDev design;
+    double tol;
+    double* fods;
+    int* status;
+    VoxelSet vs;
+    SolveScratch scratch;
+};
+
+// fit_single_tissue: one ridge NNLS per voxel (solver.cpp:14-21 through sd.cpp:fit_single_tissue).
+template <int NCPL, int RPL, int KCAP, bool GLOBAL_L>
+__global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) single_kernel(SingleArgs A) {
+    constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
+    extern __shared__ double smem[];
+    const int w = threadIdx.x / WAVE;
+    const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
+    const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
+    double* slab = A.scratch.base + gw * A.scratch.per_wave;
+    double* qrs = slab;
+    double* L = GLOBAL_L ? slab + static_cast<long>(RPLQ * WAVE) * (KCAP + 1)
+                         : smem + w * lds_factor_doubles(KCAP);
+    const long count = voxel_count(A.vs);
+    const int l = lane_id();
+    for (long it = gw; it < count; it += n_waves) {
+        const long v = A.vs.list ? A.vs.list[it] : it;
+        RowVec<RPL> s;
+        load_signal<RPL>(A.signals, v, A.m, s);
+        double corr[NCPL];
+        correlations<NCPL, RPL>(A.design, s, 0.0, corr);
+        RidgeNnlsWave<NCPL, KCAP, RPLQ> solver(A.design, A.tol, 3 * A.design.n, 0.0, L, qrs);
+        const int st = solver.template run<RPL>(corr, s);
+        if (st & ST_OVERFLOW) {
+            record_overflow(A.vs, v);
+            continue;
+        }
+#pragma unroll
+        for (int t = 0; t < NCPL; ++t) {
+            const int c = l + WAVE * t;
+            if (c < A.design.n) {
+                A.fods[v * A.design.n + c] = solver.x[t];
+            }
+        }
+        if (l == 0) {
+            A.status[v] = st;
+        }
+    }
+}
+
+struct LoopArgs {
+    const double* signals;
+    int m;
+    const double* weights;
+    RidgeDesignDev design;    // low-resolution solver
+    FractionKernelDev kernel; // low-resolution kernel (unweighted)
+    const double* w_iso;      // (m, ci) weighted isotropic columns
+    double tol;
+    int max_rounds;
+    double fraction_tol;
+    double* iso_out;          // (n_vox, ci)
+    int* rounds_out;
+    int* status;
+    VoxelSet vs;
+    SolveScratch scratch;
+};
+
+// The low-resolution rounds of fit_multi_tissue_block (multi_tissue.cpp:47-80): per round the
+// anisotropic signal, its low-resolution FOD and the lumped fractions, until no fraction moves by
+// fraction_tol or max_rounds rounds ran; the voxel keeps its last isotropic fractions.
+template <int NCPL, int RPL, int KCAP, bool GLOBAL_L>
+__global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) loop_kernel(LoopArgs A) {
+    constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
+    extern __shared__ double smem[];
+    const int w = threadIdx.x / WAVE;
+    const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
+    const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
+    double* slab = A.scratch.base + gw * A.scratch.per_wave;
+    double* qrs = slab;
+    double* L = GLOBAL_L ? slab + static_cast<long>(RPLQ * WAVE) * (KCAP + 1)
+                         : smem + w * lds_factor_doubles(KCAP);
+    const long count = voxel_count(A.vs);
+    const int l = lane_id();
+    const int ci = A.kernel.ci;
+    const int nlf = 1 + ci;
+    for (long it = gw; it < count; it += n_waves) {
+        const long v = A.vs.list ? A.vs.list[it] : it;
+        RowVec<RPL> s;
+        load_signal<RPL>(A.signals, v, A.m, s);
+        double p[NQ];
+        double prev[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            p[q] = 0.0;
+            prev[q] = __builtin_inf();
+        }
+        int status = 0;
+        int rounds = 0;
+        for (int round = 0; round < A.max_rounds; ++round) {
+            RowVec<RPL> a;
+            anisotropic_signal<RPL>(s, A.weights, A.w_iso, A.m, ci, p, a);
+            double corr[NCPL];
+            correlations<NCPL, RPL>(A.design, a, 0.0, corr);
+            RidgeNnlsWave<NCPL, KCAP, RPLQ> solver(A.design, A.tol, 3 * A.design.n, 0.0, L, qrs);
+            const int st = solver.template run<RPL>(corr, a);
+            status |= st;
+            if (st & ST_OVERFLOW) {
+                break;
+            }
+            double fr[NQ];
+            if (!fractions_wave<NCPL, RPL, NQ>(A.kernel, solver.x, s, true, false, 0.0, fr)) {
+                status |= ST_NOT_CONVERGED;
+            }
+            rounds += 1;
+            bool converged = true;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                if (q >= 1 && q < nlf) {
+                    p[q - 1] = fr[q];
+                }
+                if (q < nlf && !(fabs(fr[q] - prev[q]) < A.fraction_tol)) {
+                    converged = false;
+                }
+            }
+            if (converged) {
+                break;
+            }
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                prev[q] = q < nlf ? fr[q] : prev[q];
+            }
+        }
+        if (status & ST_OVERFLOW) {
+            record_overflow(A.vs, v);
+            continue;
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            if (l == q && q < ci) {
+                A.iso_out[v * ci + q] = p[q];
+            }
+        }
+        if (l == 0) {
+            A.rounds_out[v] = rounds;
+            A.status[v] = status;
+        }
+    }
+}
+
+struct FinalArgs {
+    const double* signals;
+    int m;
+    const double* weights;
+    RidgeDesignDev design;    // high-resolution solver
+    FractionKernelDev kernel; // high-resolution kernel (unweighted)
+    const double* w_iso;
+    double tol;
+    double final_tol;
+    const double* iso_in;     // (n_vox, ci)
+    double* fods;             // (n_vox, ca)
+    double* fractions;        // (n_vox, n_blocks + ci)
+    int* status;
+    VoxelSet vs;
+    SolveScratch scratch;
+};
+
+// The high-resolution FOD and the final per-block fractions, normalised to sum to 1
+// (multi_tissue.cpp:82-99).
+template <int NCPL, int RPL, int KCAP, bool GLOBAL_L>
+__global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) final_kernel(FinalArgs A) {
+    constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
+    extern __shared__ double smem[];
+    const int w = threadIdx.x / WAVE;
+    const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
+    const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
+    double* slab = A.scratch.base + gw * A.scratch.per_wave;
+    double* qrs = slab;
+    double* L = GLOBAL_L ? slab + static_cast<long>(RPLQ * WAVE) * (KCAP + 1)
+                         : smem + w * lds_factor_doubles(KCAP);
+    const long count = voxel_count(A.vs);
+    const int l = lane_id();
+    const int ci = A.kernel.ci;
+    const int nf = A.kernel.n_blocks + ci;
+    for (long it = gw; it < count; it += n_waves) {
+        const long v = A.vs.list ? A.vs.list[it] : it;
+        RowVec<RPL> s;
+        load_signal<RPL>(A.signals, v, A.m, s);
+        double p[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            p[q] = q < ci ? A.iso_in[v * ci + q] : 0.0;
+        }
+        RowVec<RPL> a;
+        anisotropic_signal<RPL>(s, A.weights, A.w_iso, A.m, ci, p, a);
+        double corr[NCPL];
+        correlations<NCPL, RPL>(A.design, a, 0.0, corr);
+        RidgeNnlsWave<NCPL, KCAP, RPLQ> solver(A.design, A.tol, 3 * A.design.n, 0.0, L, qrs);
+        int st = solver.template run<RPL>(corr, a);
+        if (st & ST_OVERFLOW) {
+            record_overflow(A.vs, v);
+            continue;
+        }
+        if (A.fods) {
+#pragma unroll
+            for (int t = 0; t < NCPL; ++t) {
+                const int c = l + WAVE * t;
+                if (c < A.design.n) {
+                    A.fods[v * A.design.n + c] = solver.x[t];
+                }
+            }
+        }
+        double fr[NQ];
+        if (!fractions_wave<NCPL, RPL, NQ>(A.kernel, solver.x, s, false, true, A.final_tol, fr)) {
+            st |= ST_NOT_CONVERGED;
+        }
+        double total = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            if (q < nf) {
+                total = total + fr[q];
+            }
+        }
+        total = total + FRACTION_SUM_OFFSET;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            if (l == q && q < nf) {
+                A.fractions[v * nf + q] = fr[q] / total;
+            }
+        }
+        if (l == 0) {
+            A.status[v] |= st;
+        }
+    }
+}
+
+// Kernel families, for the launch helpers.
+template <int NCPL, int RPL, int KCAP, bool G>
+struct single_family {
+    static constexpr auto kernel = single_kernel<NCPL, RPL, KCAP, G>;
+};
+template <int NCPL, int RPL, int KCAP, bool G>
+struct loop_family {
+    static constexpr auto kernel = loop_kernel<NCPL, RPL, KCAP, G>;
+};
+template <int NCPL, int RPL, int KCAP, bool G>
+struct final_family {
+    static constexpr auto kernel = final_kernel<NCPL, RPL, KCAP, G>;
+};
+
+
+// ---------------------------------------------------------------------------------------------
+// Host-side launch helpers (shared by the instantiation units)
+// ---------------------------------------------------------------------------------------------
+
+extern thread_local std::string g_error;
+
+inline int fail(const std::string& message) {
+    g_error = message;
+    return LSD_ERR_INVALID;
+}
+
+inline int hip_fail(hipError_t e, const char* what) {
+    g_error = std::string(what) + ": " + hipGetErrorString(e);
+    return LSD_ERR_HIP;
+}
+
+#define LSD_HIP(call)                                                                          \
+    do {                                                                                       \
+        const hipError_t lsd_e = (call);                                                       \
+        if (lsd_e != hipSuccess) {                                                             \
+            return hip_fail(lsd_e, #call);                                                     \
+        }                                                                                      \
+    } while (0)
+
+inline int device_alloc(void** p, size_t bytes, hipStream_t stream) {
+    if (bytes == 0) {
+        bytes = 8;
+    }
+    LSD_HIP(hipMallocAsync(p, bytes, stream));
+    return 0;
+}
+
+template <int RPL, int KCAP>
+constexpr long slab_doubles(bool global_l) {
+    constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
+    return static_cast<long>(RPLQ * WAVE) * (KCAP + 1) +
+           (global_l ? static_cast<long>(lds_factor_doubles(KCAP)) : 0);
+}
+
+// Opt-in per-launch timing: events recorded on the launch stream around each kernel.
+
+// Opt-in per-launch timing: events recorded on the launch stream around each kernel.
+struct ProfiledLaunch {
+    const char* name;
+    hipEvent_t start;
+    hipEvent_t stop;
+};
+extern bool g_profiling;
+extern std::vector<ProfiledLaunch> g_launches;
+
+// Launches kern over `waves` waves; lds_bytes of dynamic LDS per block.
+template <class Kern, class Args>
+inline int launch(const char* name, Kern kern, const Args& args, int waves, size_t lds_bytes,
+           hipStream_t stream) {
+    const int blocks = (waves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+    ProfiledLaunch rec{name, nullptr, nullptr};
+    if (g_profiling) {
+        LSD_HIP(hipEventCreate(&rec.start));
+        LSD_HIP(hipEventCreate(&rec.stop));
+        LSD_HIP(hipEventRecord(rec.start, stream));
+    }
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(WAVE * WAVES_PER_BLOCK), lds_bytes, stream, args);
+    LSD_HIP(hipGetLastError());
+    if (g_profiling) {
+        LSD_HIP(hipEventRecord(rec.stop, stream));
+        g_launches.push_back(rec);
+    }
+    return 0;
+}
+
+
+// The fast launch (LDS factor) of a kernel family, then its rerun of the overflowed voxels with
+// the factor in global memory, sized for every column (KG = NCPL * 64 >= n).
+template <template <int, int, int, bool> class Family, int NCPL, int RPL, class Args>
+int run_pair(const char* name, const char* rerun_name, const Args& fast, Args rerun, int waves,
+             int* d_overflow_list, int* d_overflow_count, hipStream_t stream,
+             void** rerun_scratch) {
+    constexpr int KG = NCPL * WAVE;
+    const size_t lds = sizeof(double) * lds_factor_doubles(KCAP_LDS) * WAVES_PER_BLOCK;
+    int rc = launch(name, Family<NCPL, RPL, KCAP_LDS, false>::kernel, fast, waves, lds, stream);
+    if (rc) return rc;
+    const long per = slab_doubles<RPL, KG>(true);
+    rc = device_alloc(rerun_scratch, sizeof(double) * per * RERUN_WAVES, stream);
+    if (rc) return rc;
+    rerun.scratch = SolveScratch{static_cast<double*>(*rerun_scratch), per};
+    rerun.vs.list = d_overflow_list;
+    rerun.vs.list_count = d_overflow_count;
+    rerun.vs.overflow_list = nullptr;
+    rerun.vs.overflow_count = nullptr;
+    return launch(rerun_name, Family<NCPL, RPL, KG, true>::kernel, rerun, RERUN_WAVES, 0, stream);
+}
+
+// Launcher of one (family, NCPL, RPL) pair, defined in its own translation unit.
+#define LSD_LAUNCHER_DECL(FAMILY, ARGS, NC, RP)                                                \
+    int launch_##FAMILY##_##NC##_##RP(const ARGS& fast, const ARGS& rerun, int waves,          \
+                                        int* ol, int* oc, hipStream_t stream, void** scratch);
+#define LSD_LAUNCHER_DEF(FAMILY, ARGS, NC, RP)                                                 \
+    int launch_##FAMILY##_##NC##_##RP(const ARGS& fast, const ARGS& rerun, int waves,          \
+                                        int* ol, int* oc, hipStream_t stream, void** scratch) { \
+        return run_pair<FAMILY##_family, NC, RP>(#FAMILY, #FAMILY "_rerun", fast, rerun, waves, \
+                                                 ol, oc, stream, scratch);                     \
+    }
+
+// Every instantiated (family, NCPL, RPL): X(FAMILY, ARGS, NC, RP).
+#ifdef LSD_FAST_BUILD
+#define LSD_FOR_EACH_LAUNCHER(X)                                                               \
+    X(single, SingleArgs, 1, 4) X(single, SingleArgs, 5, 4) X(loop, LoopArgs, 1, 4)            \
+    X(final, FinalArgs, 5, 4)
+#else
+#define LSD_FOR_EACH_RPL(X, FAMILY, ARGS, NC) X(FAMILY, ARGS, NC, 4) X(FAMILY, ARGS, NC, 6) X(FAMILY, ARGS, NC, 8)
+#define LSD_FOR_EACH_LAUNCHER(X)                                                               \
+    LSD_FOR_EACH_RPL(X, single, SingleArgs, 1) LSD_FOR_EACH_RPL(X, single, SingleArgs, 3)        \
+    LSD_FOR_EACH_RPL(X, single, SingleArgs, 5) LSD_FOR_EACH_RPL(X, single, SingleArgs, 10)       \
+    LSD_FOR_EACH_RPL(X, loop, LoopArgs, 1) LSD_FOR_EACH_RPL(X, loop, LoopArgs, 3)                \
+    LSD_FOR_EACH_RPL(X, final, FinalArgs, 1) LSD_FOR_EACH_RPL(X, final, FinalArgs, 3)            \
+    LSD_FOR_EACH_RPL(X, final, FinalArgs, 5) LSD_FOR_EACH_RPL(X, final, FinalArgs, 10)
+#endif
+
+LSD_FOR_EACH_LAUNCHER(LSD_LAUNCHER_DECL)
+
+} // namespace lsd

@@ -42,12 +42,14 @@ def acquisition(n_b0: int, shells: list[tuple[float, int]], rng: np.random.Gener
 
 
 def ellipsoid_mask(shape: tuple[int, int, int], share: float) -> np.ndarray:
-    """An ellipsoid centred in the grid holding about `share` of the voxels."""
-    scale = (share / (np.pi / 6.0)) ** (1.0 / 3.0)
-    axes = [np.linspace(-1.0, 1.0, s) for s in shape]
+    """An ellipsoid centred in the grid, its semi-axes proportional to the grid's, scaled so
+    that it holds `share` of the voxels (to the voxel)."""
+    axes = [(np.arange(s) + 0.5) / s * 2.0 - 1.0 for s in shape]
     grid = np.meshgrid(*axes, indexing="ij")
     radius = sum(g**2 for g in grid)
-    return radius <= scale**2
+    n_in = int(round(share * radius.size))
+    threshold = np.partition(radius.ravel(), n_in - 1)[n_in - 1]
+    return radius <= threshold
 
 
 def random_unit(rng: np.random.Generator, n: int) -> np.ndarray:

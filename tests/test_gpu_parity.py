@@ -58,3 +58,69 @@ def test_single_tissue_bit_identical_to_oracle():
     solver = _core.FodSolver.l2nnls(k, 0.1, 1e-6)
     got = _core.fit_single_tissue(s, solver, 1)
     np.testing.assert_array_equal(got, ref)
+
+
+def test_batch_invariance_alone_equals_in_the_mix():
+    """A voxel's result does not depend on its batch mates (cpp/tests/sd/test_multi_tissue.cpp:56
+    asks 1e-12; the GPU gives identity)."""
+    signals, w, lr, hr = _mfod_setup(seed=7)
+    s = np.ascontiguousarray(signals[:150])
+    lr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(lr.w_k_aniso), 0.0, 1e-6)
+    hr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(hr.w_k_aniso), 0.1, 1e-6)
+    args = (w, _deconv(lr), lr_solver, _deconv(hr), hr_solver, 50, 1e-7, 1e-2, 1)
+    fods, fr = _core.fit_multi_tissue(s, *args)
+    for v in (0, 17, 149):
+        f1, r1 = _core.fit_multi_tissue(np.ascontiguousarray(s[v : v + 1]), *args)
+        np.testing.assert_array_equal(f1[0], fods[v])
+        np.testing.assert_array_equal(r1[0], fr[v])
+    rev = np.ascontiguousarray(s[::-1])
+    f2, r2 = _core.fit_multi_tissue(rev, *args)
+    np.testing.assert_array_equal(f2[::-1], fods)
+    np.testing.assert_array_equal(r2[::-1], fr)
+
+
+def test_edge_cases_empty_zero_and_saturated_signals():
+    signals, w, lr, hr = _mfod_setup(seed=9)
+    lr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(lr.w_k_aniso), 0.0, 1e-6)
+    hr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(hr.w_k_aniso), 0.1, 1e-6)
+    args = (w, _deconv(lr), lr_solver, _deconv(hr), hr_solver, 50, 1e-7, 1e-2, 1)
+    empty = np.zeros((0, signals.shape[1]))
+    f0, r0 = _core.fit_multi_tissue(empty, *args)
+    assert f0.shape == (0, 300) and r0.shape == (0, 3)
+    odd = np.ascontiguousarray(np.vstack([np.zeros(signals.shape[1]), np.ones(signals.shape[1]),
+                                          signals[:3]]))
+    fods, fr = _core.fit_multi_tissue(odd, *args)
+    ref_fods, ref_fr = oracle.fit_multi_tissue(odd, w, lr, hr, 0.0, 0.1, 1e-6)
+    np.testing.assert_array_equal(fods, ref_fods)
+    np.testing.assert_array_equal(fr, ref_fr)
+    assert (fods[0] == 0).all()
+
+
+def test_mritoolkit_fixture_on_gpu():
+    """The reference's 512-voxel MRIToolkit mFOD case (two anisotropic blocks, NODDI grey
+    matter, 600 high-resolution columns with active sets up to ~220, which exercise the
+    global-memory factor rerun): identical to the oracle, and within the reference's own
+    tolerances of MRIToolkit (tests/sd/test_matlab_match.py:143-165)."""
+    from pathlib import Path
+
+    from larmor_amd.sd.rf import NoddiRF
+
+    f = np.load(Path(__file__).resolve().parent / "golden" / "mfod_fixture.npz")
+    bvals, bvecs = f["bvals"], f["bvecs"]
+    rfs = [AnisotropicRF([1.7e-3, 0.2e-3, 0.2e-3], 0.0), NoddiRF(), IsotropicRF(3e-3)]
+    w = np.ones(len(bvals))
+    hr = Kernel(rfs, bvals, bvecs, dirs=DIRS_300, weights=w)
+    lr = Kernel(rfs, bvals, bvecs, dirs=[get_dirs(min(len(bvals), rf.low_resolution_dirs))[:-1]
+                                         for rf in rfs[:2]], weights=w)
+    sig = f["signal"].astype(np.float64)
+    s = np.ascontiguousarray(sig / sig[:, bvals <= bvals.min()].mean(axis=1, keepdims=True))
+    lr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(lr.w_k_aniso), 0.0, 1e-6)
+    hr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(hr.w_k_aniso), 0.1, 1e-6)
+    fods, fr = _core.fit_multi_tissue(s, w, _deconv(lr), lr_solver, _deconv(hr), hr_solver, 50,
+                                      1e-7, 1e-2, 1)
+    ref_fods, ref_fr = oracle.fit_multi_tissue(s, w, lr, hr, 0.0, 0.1, 1e-6)
+    np.testing.assert_array_equal(fods, ref_fods)
+    np.testing.assert_array_equal(fr, ref_fr)
+    mrt = f["fod"].astype(np.float64)
+    assert (np.abs(fods - mrt).max(axis=1) / mrt.max(axis=1)).max() <= 1e-3
+    np.testing.assert_allclose(fr, f["fractions"].astype(np.float64), rtol=0, atol=1e-4)
