@@ -98,13 +98,39 @@ struct FractionKernelDev {
     int n_blocks;
 };
 
+// The columns of a fraction fit: up to NG group signals in registers, then the isotropic
+// kernel columns read from memory.
+template <int NG, int RPL>
+struct FractionDesign {
+    dvec<RPL> grp[NG];
+    int n_groups;
+    const double* iso_t; // (ci, m)
+    int m;
+
+    __device__ __forceinline__ double get(int c, int t) const {
+        if (c < n_groups) {
+            double x = 0.0;
+#pragma unroll
+            for (int q = 0; q < NG; ++q) {
+                x = c == q ? static_cast<double>(grp[q][t]) : x;
+            }
+            return x;
+        }
+        const int r = lane_id() + WAVE * t;
+        return r < m ? iso_t[static_cast<long>(c - n_groups) * m + r] : 0.0;
+    }
+};
+
+constexpr int MAX_GROUPS = 2; // anisotropic blocks of a per-block fraction fit
+
 // compute_fractions of one voxel: fod in column slots, s the unweighted signal.  lumped takes
-// all anisotropic columns as one group.  fr receives the fractions, groups first.  Returns false
-// when the NNLS hit its inner-step limit.
+// all anisotropic columns as one group.  fr receives the fractions, groups first.  work is the
+// wave's LDS slab for the pivoted QR (RPL * 64 * NQ doubles).  Returns false when the NNLS hit
+// its inner-step limit.
 template <int NCPL, int RPL, int NQ>
 __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const double (&fod)[NCPL],
-                               const RowVec<RPL>& s, bool lumped, bool has_tol, double tol_in,
-                               double (&fr)[NQ]) {
+                                               const RowVec<RPL>& s, bool lumped, bool has_tol,
+                                               double tol_in, double (&fr)[NQ], double* work) {
     const int l = lane_id();
     const double threshold = median_cols<NCPL>(fod, K.ca) * (1.0 - MEDIAN_TIE_TOLERANCE);
     double thr[NCPL];
@@ -112,77 +138,63 @@ __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const
     for (int t = 0; t < NCPL; ++t) {
         thr[t] = fod[t] < threshold ? 0.0 : fod[t];
     }
-    const int n_groups = lumped ? 1 : K.n_blocks;
-    RegCols<NQ, RPL> C;
+    FractionDesign<MAX_GROUPS, RPL> C;
+    C.n_groups = lumped ? 1 : K.n_blocks;
+    C.iso_t = K.iso_t;
+    C.m = K.m;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        C.col[q] = 0.0;
-    }
-    for (int grp = 0; grp < n_groups; ++grp) {
-        const int begin = lumped ? 0 : K.blocks[2 * grp];
-        const int end = lumped ? K.ca : K.blocks[2 * grp + 1];
-        double gs[RPL];
+    for (int grp = 0; grp < MAX_GROUPS; ++grp) {
+        dvec<RPL> gs = 0.0;
+        if (grp < C.n_groups) {
+            const int begin = lumped ? 0 : K.blocks[2 * grp];
+            const int end = lumped ? K.ca : K.blocks[2 * grp + 1];
+            // Zero terms add exactly nothing, so only the non-zero thresholded values are
+            // visited, in column order.
 #pragma unroll
-        for (int t = 0; t < RPL; ++t) {
-            gs[t] = 0.0;
-        }
-        // Zero terms add exactly nothing, so only the non-zero thresholded values are visited,
-        // in column order.
+            for (int t2 = 0; t2 < NCPL; ++t2) {
+                const int c0 = l + WAVE * t2;
+                unsigned long long mask = ballot(c0 >= begin && c0 < end && thr[t2] != 0.0);
+                while (mask) {
+                    const int b = __ffsll(static_cast<long long>(mask)) - 1;
+                    mask &= mask - 1;
+                    const int c = WAVE * t2 + b;
+                    const double f = readlane(thr[t2], b);
+                    const double* krow = K.aniso_t + static_cast<long>(c) * K.m;
 #pragma unroll
-        for (int t2 = 0; t2 < NCPL; ++t2) {
-            const int c0 = l + WAVE * t2;
-            unsigned long long mask = ballot(c0 >= begin && c0 < end && thr[t2] != 0.0);
-            while (mask) {
-                const int b = __ffsll(static_cast<long long>(mask)) - 1;
-                mask &= mask - 1;
-                const int c = WAVE * t2 + b;
-                const double f = readlane(thr[t2], b);
-                const double* krow = K.aniso_t + static_cast<long>(c) * K.m;
+                    for (int t = 0; t < RPL; ++t) {
+                        const int r = l + WAVE * t;
+                        if (r < K.m) {
+                            gs[t] = gs[t] + krow[r] * f;
+                        }
+                    }
+                }
+            }
+            if (K.m > 0) {
+                double largest = __builtin_nan("");
 #pragma unroll
                 for (int t = 0; t < RPL; ++t) {
                     const int r = l + WAVE * t;
                     if (r < K.m) {
-                        gs[t] = gs[t] + krow[r] * f;
+                        largest = fmax(largest, static_cast<double>(gs[t]));
+                    }
+                }
+#pragma unroll
+                for (int o = WAVE / 2; o >= 1; o >>= 1) {
+                    largest = fmax(largest, __shfl_xor(largest, o));
+                }
+                if (largest > 0.0) {
+#pragma unroll
+                    for (int t = 0; t < RPL; ++t) {
+                        gs[t] = gs[t] / largest;
                     }
                 }
             }
         }
-        if (K.m > 0) {
-            double largest = __builtin_nan("");
-#pragma unroll
-            for (int t = 0; t < RPL; ++t) {
-                const int r = l + WAVE * t;
-                if (r < K.m) {
-                    largest = fmax(largest, gs[t]);
-                }
-            }
-#pragma unroll
-            for (int o = WAVE / 2; o >= 1; o >>= 1) {
-                largest = fmax(largest, __shfl_xor(largest, o));
-            }
-            if (largest > 0.0) {
-#pragma unroll
-                for (int t = 0; t < RPL; ++t) {
-                    gs[t] = gs[t] / largest;
-                }
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < RPL; ++t) {
-            C.set(grp, t, gs[t]);
-        }
+        C.grp[grp] = gs;
     }
-    for (int tt = 0; tt < K.ci; ++tt) {
-        const double* irow = K.iso_t + static_cast<long>(tt) * K.m;
-#pragma unroll
-        for (int t = 0; t < RPL; ++t) {
-            const int r = l + WAVE * t;
-            C.set(n_groups + tt, t, r < K.m ? irow[r] : 0.0);
-        }
-    }
-    const int ncols = n_groups + K.ci;
+    const int ncols = C.n_groups + K.ci;
     const double tol = has_tol ? tol_in : default_tolerance_wave<NQ, RPL>(C, K.m, ncols);
-    return nnls_small<NQ, RPL>(C, K.m, ncols, s, tol, 3 * ncols, fr);
+    return nnls_small<NQ, RPL>(C, K.m, ncols, s, tol, 3 * ncols, fr, work);
 }
 
 } // namespace lsd

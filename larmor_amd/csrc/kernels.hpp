@@ -34,9 +34,16 @@ constexpr int NQ = 4;             // most fraction columns (groups + isotropic t
 constexpr int KCAP_LDS = 32;      // LDS factor capacity (active columns)
 constexpr int WAVES_PER_BLOCK = 4;
 constexpr int RERUN_WAVES = 32;   // waves of a rerun launch
+constexpr int RERUN_WAVES_ALL = 512; // waves when every voxel needs the global-memory kernel
 constexpr double FRACTION_SUM_OFFSET = 1e-16;
 
-constexpr int lds_factor_doubles(int kcap) { return (kcap + 1) * kcap + WAVE; }
+// Per-wave slab: the Cholesky factor during an FOD solve (stride kcap + 1, plus one wave of
+// slack for the uniform reads), then the pivoted-QR work columns of the fraction fit (NQ columns
+// of rpl * 64 rows).
+constexpr int lds_factor_doubles(int kcap, int rpl) {
+    return ((kcap + 1) * kcap + WAVE) > NQ * rpl * WAVE ? ((kcap + 1) * kcap + WAVE)
+                                                          : NQ * rpl * WAVE;
+}
 
 // ---------------------------------------------------------------------------------------------
 // Device helpers
@@ -159,7 +166,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) single_kernel(SingleAr
     double* slab = A.scratch.base + gw * A.scratch.per_wave;
     double* qrs = slab;
     double* L = GLOBAL_L ? slab + static_cast<long>(RPLQ * WAVE) * (KCAP + 1)
-                         : smem + w * lds_factor_doubles(KCAP);
+                         : smem + w * lds_factor_doubles(KCAP, RPL);
     const long count = voxel_count(A.vs);
     const int l = lane_id();
     for (long it = gw; it < count; it += n_waves) {
@@ -168,7 +175,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) single_kernel(SingleAr
         load_signal<RPL>(A.signals, v, A.m, s);
         double corr[NCPL];
         correlations<NCPL, RPL>(A.design, s, 0.0, corr);
-        RidgeNnlsWave<NCPL, KCAP, RPLQ> solver(A.design, A.tol, 3 * A.design.n, 0.0, L, qrs);
+        RidgeNnlsWave<NCPL, KCAP, RPLQ, GLOBAL_L> solver(A.design, A.tol, 3 * A.design.n, 0.0, L, qrs);
         const int st = solver.template run<RPL>(corr, s);
         if (st & ST_OVERFLOW) {
             record_overflow(A.vs, v);
@@ -217,7 +224,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) loop_kernel(LoopArgs A
     double* slab = A.scratch.base + gw * A.scratch.per_wave;
     double* qrs = slab;
     double* L = GLOBAL_L ? slab + static_cast<long>(RPLQ * WAVE) * (KCAP + 1)
-                         : smem + w * lds_factor_doubles(KCAP);
+                         : smem + w * lds_factor_doubles(KCAP, RPL);
     const long count = voxel_count(A.vs);
     const int l = lane_id();
     const int ci = A.kernel.ci;
@@ -240,14 +247,14 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) loop_kernel(LoopArgs A
             anisotropic_signal<RPL>(s, A.weights, A.w_iso, A.m, ci, p, a);
             double corr[NCPL];
             correlations<NCPL, RPL>(A.design, a, 0.0, corr);
-            RidgeNnlsWave<NCPL, KCAP, RPLQ> solver(A.design, A.tol, 3 * A.design.n, 0.0, L, qrs);
+            RidgeNnlsWave<NCPL, KCAP, RPLQ, GLOBAL_L> solver(A.design, A.tol, 3 * A.design.n, 0.0, L, qrs);
             const int st = solver.template run<RPL>(corr, a);
             status |= st;
             if (st & ST_OVERFLOW) {
                 break;
             }
             double fr[NQ];
-            if (!fractions_wave<NCPL, RPL, NQ>(A.kernel, solver.x, s, true, false, 0.0, fr)) {
+            if (!fractions_wave<NCPL, RPL, NQ>(A.kernel, solver.x, s, true, false, 0.0, fr, L)) {
                 status |= ST_NOT_CONVERGED;
             }
             rounds += 1;
@@ -315,7 +322,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) final_kernel(FinalArgs
     double* slab = A.scratch.base + gw * A.scratch.per_wave;
     double* qrs = slab;
     double* L = GLOBAL_L ? slab + static_cast<long>(RPLQ * WAVE) * (KCAP + 1)
-                         : smem + w * lds_factor_doubles(KCAP);
+                         : smem + w * lds_factor_doubles(KCAP, RPL);
     const long count = voxel_count(A.vs);
     const int l = lane_id();
     const int ci = A.kernel.ci;
@@ -333,7 +340,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) final_kernel(FinalArgs
         anisotropic_signal<RPL>(s, A.weights, A.w_iso, A.m, ci, p, a);
         double corr[NCPL];
         correlations<NCPL, RPL>(A.design, a, 0.0, corr);
-        RidgeNnlsWave<NCPL, KCAP, RPLQ> solver(A.design, A.tol, 3 * A.design.n, 0.0, L, qrs);
+        RidgeNnlsWave<NCPL, KCAP, RPLQ, GLOBAL_L> solver(A.design, A.tol, 3 * A.design.n, 0.0, L, qrs);
         int st = solver.template run<RPL>(corr, a);
         if (st & ST_OVERFLOW) {
             record_overflow(A.vs, v);
@@ -349,7 +356,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) final_kernel(FinalArgs
             }
         }
         double fr[NQ];
-        if (!fractions_wave<NCPL, RPL, NQ>(A.kernel, solver.x, s, false, true, A.final_tol, fr)) {
+        if (!fractions_wave<NCPL, RPL, NQ>(A.kernel, solver.x, s, false, true, A.final_tol, fr, L)) {
             st |= ST_NOT_CONVERGED;
         }
         double total = 0.0;
@@ -423,7 +430,7 @@ template <int RPL, int KCAP>
 constexpr long slab_doubles(bool global_l) {
     constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
     return static_cast<long>(RPLQ * WAVE) * (KCAP + 1) +
-           (global_l ? static_cast<long>(lds_factor_doubles(KCAP)) : 0);
+           (global_l ? static_cast<long>(lds_factor_doubles(KCAP, RPL)) : 0);
 }
 
 // Opt-in per-launch timing: events recorded on the launch stream around each kernel.
@@ -465,10 +472,21 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
              int* d_overflow_list, int* d_overflow_count, hipStream_t stream,
              void** rerun_scratch) {
     constexpr int KG = NCPL * WAVE;
-    const size_t lds = sizeof(double) * lds_factor_doubles(KCAP_LDS) * WAVES_PER_BLOCK;
+    const long per = slab_doubles<RPL, KG>(true);
+    if (fast.design.normal_equations == 0) {
+        // QR fits throughout: every voxel takes the global-memory kernel.
+        const int all_waves = std::min(waves, RERUN_WAVES_ALL);
+        int rc = device_alloc(rerun_scratch, sizeof(double) * per * all_waves, stream);
+        if (rc) return rc;
+        rerun.scratch = SolveScratch{static_cast<double*>(*rerun_scratch), per};
+        rerun.vs.overflow_list = nullptr;
+        rerun.vs.overflow_count = nullptr;
+        return launch(rerun_name, Family<NCPL, RPL, KG, true>::kernel, rerun, all_waves, 0,
+                      stream);
+    }
+    const size_t lds = sizeof(double) * lds_factor_doubles(KCAP_LDS, RPL) * WAVES_PER_BLOCK;
     int rc = launch(name, Family<NCPL, RPL, KCAP_LDS, false>::kernel, fast, waves, lds, stream);
     if (rc) return rc;
-    const long per = slab_doubles<RPL, KG>(true);
     rc = device_alloc(rerun_scratch, sizeof(double) * per * RERUN_WAVES, stream);
     if (rc) return rc;
     rerun.scratch = SolveScratch{static_cast<double*>(*rerun_scratch), per};

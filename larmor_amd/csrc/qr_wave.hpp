@@ -66,7 +66,8 @@ struct RegCols {
     }
 };
 
-// Columns in a global scratch slab, column-major with leading dimension RPL * 64.
+// Columns in a per-wave slab (LDS or global memory), column-major with leading dimension
+// RPL * 64.
 template <int RPL>
 struct MemCols {
     static constexpr int kRpl = RPL;
@@ -79,8 +80,8 @@ struct MemCols {
     __device__ __forceinline__ void set(int c, int t, double x) {
         base[static_cast<long>(c) * kLd + t * WAVE + lane_id()] = x;
     }
-    // A uniform element, read as a coalesced vector load of its 64-row segment plus readlane
-    // (never through the scalar cache, as the slab is written in the same kernel).
+    // A uniform element, read as a vector load of its 64-row segment plus readlane (never
+    // through the scalar cache, as the slab is written in the same kernel).
     __device__ __forceinline__ double elem(int c, int r) const {
         const double x = base[static_cast<long>(c) * kLd + (r / WAVE) * WAVE + lane_id()];
         return readlane(x, r % WAVE);
@@ -330,9 +331,10 @@ __device__ __forceinline__ void least_squares_wave(Cols& A, int rows, int cols, 
 }
 
 // Default NNLS gradient tolerance of a design (nnls.cpp:default_gradient_tolerance):
-// 10 eps max_c sum_r |C(r, c)| max(rows, cols), the column sums being wave sums.
-template <int NQ, int RPL>
-__device__ __forceinline__ double default_tolerance_wave(const RegCols<NQ, RPL>& C, int rows, int cols) {
+// 10 eps max_c sum_r |C(r, c)| max(rows, cols), the column sums being wave sums.  P provides
+// the columns: P.get(c, t) is column c at the lane's row slot t (c uniform).
+template <int NQ, int RPL, class P>
+__device__ __forceinline__ double default_tolerance_wave(const P& C, int rows, int cols) {
     double norm1 = 0.0;
     if (rows * cols > 0) {
 #pragma unroll
@@ -348,11 +350,14 @@ __device__ __forceinline__ double default_tolerance_wave(const RegCols<NQ, RPL>&
     return 10.0 * DBL_EPS * norm1 * static_cast<double>(rows > cols ? rows : cols);
 }
 
-// The generic Lawson-Hanson NNLS on up to NQ register columns (solve_nnls_with_limit).
-// x receives the solution (uniform).  Returns false when the inner-step limit stopped it.
-template <int NQ, int RPL>
-__device__ __forceinline__ bool nnls_small(const RegCols<NQ, RPL>& C, int rows, int ncols, const RowVec<RPL>& d,
-                           double tol, int max_inner, double (&x)[NQ]) {
+// The generic Lawson-Hanson NNLS on up to NQ columns (solve_nnls_with_limit).  C provides the
+// columns (C.get(c, t), c uniform); each least-squares fit copies the active columns into the
+// wave's LDS slab `work` (RPL * 64 * NQ doubles) and runs the pivoted QR there.  x receives the
+// solution (uniform).  Returns false when the inner-step limit stopped it.
+template <int NQ, int RPL, class P>
+__device__ __forceinline__ bool nnls_small(const P& C, int rows, int ncols, const RowVec<RPL>& d,
+                                           double tol, int max_inner, double (&x)[NQ],
+                                           double* work) {
     bool active[NQ];
     double g[NQ];
     double cand[NQ];
@@ -383,7 +388,7 @@ __device__ __forceinline__ bool nnls_small(const RegCols<NQ, RPL>& C, int rows, 
         }
     };
     auto fit_on = [&](double (&out)[NQ]) {
-        RegCols<NQ, RPL> A;
+        MemCols<RPL> A{work};
         int ka = 0;
 #pragma unroll
         for (int c = 0; c < NQ; ++c) {

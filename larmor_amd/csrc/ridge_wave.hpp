@@ -57,7 +57,11 @@ constexpr int qr_rows_per_lane(int max_m) {
     return (max_m + KCAP + WAVE - 1) / WAVE;
 }
 
-template <int NCPL, int KCAP, int RPLQ>
+// QR: whether this instantiation carries the pivoted-QR fits.  The fast (LDS) kernels do not:
+// a voxel whose factor refuses a pivot, or whose design needs QR fits throughout, is handed to
+// the rerun kernel (reported as ST_OVERFLOW), which carries them, so the fast kernels keep their
+// registers for the common path.
+template <int NCPL, int KCAP, int RPLQ, bool QR = true>
 struct RidgeNnlsWave {
     static constexpr int NSK = (KCAP + WAVE - 1) / WAVE; // factor-position slots
     static constexpr int S = KCAP + 1;                   // factor column stride
@@ -381,11 +385,16 @@ struct RidgeNnlsWave {
             }
         }
         if (factor_failed) {
-            if (!this->template solve_qr<RPL>(signal)) {
+            if constexpr (QR) {
+                if (!this->template solve_qr<RPL>(signal)) {
+                    status |= ST_OVERFLOW;
+                    return false;
+                }
+                return true;
+            } else {
                 status |= ST_OVERFLOW;
                 return false;
             }
-            return true;
         }
         solve_factor(corr);
         return true;

@@ -246,8 +246,9 @@ int lsd_tissue_kernel_create(const double* k_aniso, const double* k_iso, const d
         return fail("aniso_blocks must be non-empty ranges within 0.." + std::to_string(ca) +
                     ", got " + text + "]");
     }
-    if (n_blocks + ci > NQ || 1 + ci > NQ) {
-        return fail("at most " + std::to_string(NQ) +
+    if (n_blocks + ci > NQ || 1 + ci > NQ || n_blocks > MAX_GROUPS) {
+        return fail("at most " + std::to_string(MAX_GROUPS) + " anisotropic blocks and " +
+                    std::to_string(NQ) +
                     " fraction columns (anisotropic blocks plus isotropic tissues) are supported");
     }
     auto* k = new lsd_tissue_kernel();
