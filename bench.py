@@ -9,11 +9,14 @@ One step = the whole masked volume through the hot path with the normalised sign
 HBM: the low-resolution FOD <-> fraction rounds, the high-resolution FOD and final fractions
 (liblarmor_sd.so kernels) and the projection of the amplitudes onto SH up to lmax 8.
 
-Prints one JSON line.  `roofline` prices the dominant kernel (the low-resolution loop) at its
-algorithmic FP64 flops, 4 m n_lr per voxel and round, against MI355X's FP64 peak, from the
-kernel's own duration measured with HIP events on its launch stream.  `cpu_baseline` is the
-CPU FP64 oracle (a C++ restatement of the reference algorithm, `kind: port`) on a bounded
-random sample of the same voxels, on the host's cores.
+Prints one JSON line.  `roofline` prices the dominant kernel (the one with the most time per
+step) at its algorithmic FP64 flops against MI355X's FP64 peak, from the kernel's own launch
+durations measured with HIP events on its launch stream: 2 m n per voxel solve for the FOD
+kernels (the correlation K^T W s with the n-column design) and 2 m n per voxel fit for the
+fraction kernels (the anisotropic signal K x).  The low-resolution kernels run once per voxel
+and round, so their unit count is the sum of the voxels' rounds.  `cpu_baseline` is the CPU
+FP64 oracle (a C++ restatement of the reference algorithm, `kind: port`) on a bounded random
+sample of the same voxels, on the host's cores.
 """
 
 import argparse
@@ -133,14 +136,24 @@ def main() -> int:
     total_rounds = int(rounds.sum().item())
 
     value = n_vox * args.steps / elapsed
-    loop_ms, loop_launches = kernel_times.get("loop", (float("nan"), 1))
-    loop_avg_s = loop_ms / loop_launches / 1e3
     n_lr = lr.w_k_aniso.shape[1]
-    flops_per_launch = total_rounds * 4.0 * m * n_lr
-    achieved = flops_per_launch / loop_avg_s / 1e12
+    n_hr = hr.w_k_aniso.shape[1]
+    # Algorithmic FP64 flops of each kernel per step.
+    step_flops = {
+        "fod_lr": total_rounds * 2.0 * m * n_lr,
+        "fraction_loop": total_rounds * 2.0 * m * n_lr,
+        "fod_hr": n_vox * 2.0 * m * n_hr,
+        "fraction_final": n_vox * 2.0 * m * n_hr,
+    }
+    dominant = max(step_flops, key=lambda k: kernel_times.get(k, (0.0, 0))[0])
+    dom_ms, dom_launches = kernel_times[dominant]
+    launches_per_step = dom_launches / args.steps
+    avg_launch_s = dom_ms / dom_launches / 1e3
+    flops_per_launch = step_flops[dominant] / launches_per_step
+    achieved = flops_per_launch / avg_launch_s / 1e12
     print("kernel ms per step: " + ", ".join(
-        f"{k} {v[0] / max(v[1], 1):.2f} x{v[1] // args.steps}" for k, v in kernel_times.items()),
-        file=sys.stderr, flush=True)
+        f"{k} {v[0] / args.steps:.2f} ({v[1] // args.steps} launches)"
+        for k, v in kernel_times.items()), file=sys.stderr, flush=True)
     print(f"rounds: mean {total_rounds / n_vox:.2f}; status bits: "
           f"{torch.bincount(status.to(torch.int64)).tolist()}", file=sys.stderr, flush=True)
 
@@ -173,8 +186,9 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": None,
-            "kernel": "loop_kernel (low-resolution FOD/fraction rounds)",
-            "kernel_ms": loop_avg_s * 1e3,
+            "kernel": dominant,
+            "kernel_avg_launch_ms": avg_launch_s * 1e3,
+            "launches_per_step": launches_per_step,
             "flops_per_launch": flops_per_launch,
         },
     }

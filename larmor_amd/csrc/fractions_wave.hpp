@@ -99,13 +99,14 @@ struct FractionKernelDev {
 };
 
 // The columns of a fraction fit: up to NG group signals in registers, then the isotropic
-// kernel columns read from memory.
+// kernel columns, read from the block's LDS copy (column c at c * RPL * 64, rows past m zero).
+// Reading them from global memory instead lets the compiler hoist every load out of the NNLS
+// loop and hold them all in registers.
 template <int NG, int RPL>
 struct FractionDesign {
     dvec<RPL> grp[NG];
     int n_groups;
-    const double* iso_t; // (ci, m)
-    int m;
+    const double* iso_lds;
 
     __device__ __forceinline__ double get(int c, int t) const {
         if (c < n_groups) {
@@ -116,8 +117,7 @@ struct FractionDesign {
             }
             return x;
         }
-        const int r = lane_id() + WAVE * t;
-        return r < m ? iso_t[static_cast<long>(c - n_groups) * m + r] : 0.0;
+        return iso_lds[(c - n_groups) * (RPL * WAVE) + lane_id() + WAVE * t];
     }
 };
 
@@ -125,12 +125,13 @@ constexpr int MAX_GROUPS = 2; // anisotropic blocks of a per-block fraction fit
 
 // compute_fractions of one voxel: fod in column slots, s the unweighted signal.  lumped takes
 // all anisotropic columns as one group.  fr receives the fractions, groups first.  work is the
-// wave's LDS slab for the pivoted QR (RPL * 64 * NQ doubles).  Returns false when the NNLS hit
-// its inner-step limit.
+// wave's LDS slab for the pivoted QR (RPL * 64 * NQ doubles), iso_lds the block's LDS copy of the
+// isotropic columns.  Returns false when the NNLS hit its inner-step limit.
 template <int NCPL, int RPL, int NQ>
 __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const double (&fod)[NCPL],
                                                const RowVec<RPL>& s, bool lumped, bool has_tol,
-                                               double tol_in, double (&fr)[NQ], double* work) {
+                                               double tol_in, double (&fr)[NQ], double* work,
+                                               const double* iso_lds) {
     const int l = lane_id();
     const double threshold = median_cols<NCPL>(fod, K.ca) * (1.0 - MEDIAN_TIE_TOLERANCE);
     double thr[NCPL];
@@ -140,8 +141,7 @@ __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const
     }
     FractionDesign<MAX_GROUPS, RPL> C;
     C.n_groups = lumped ? 1 : K.n_blocks;
-    C.iso_t = K.iso_t;
-    C.m = K.m;
+    C.iso_lds = iso_lds;
 #pragma unroll
     for (int grp = 0; grp < MAX_GROUPS; ++grp) {
         dvec<RPL> gs = 0.0;

@@ -37,13 +37,9 @@ constexpr int RERUN_WAVES = 32;   // waves of a rerun launch
 constexpr int RERUN_WAVES_ALL = 512; // waves when every voxel needs the global-memory kernel
 constexpr double FRACTION_SUM_OFFSET = 1e-16;
 
-// Per-wave slab: the Cholesky factor during an FOD solve (stride kcap + 1, plus one wave of
-// slack for the uniform reads), then the pivoted-QR work columns of the fraction fit (NQ columns
-// of rpl * 64 rows).
-constexpr int lds_factor_doubles(int kcap, int rpl) {
-    return ((kcap + 1) * kcap + WAVE) > NQ * rpl * WAVE ? ((kcap + 1) * kcap + WAVE)
-                                                          : NQ * rpl * WAVE;
-}
+// Per-wave slab of the FOD kernels: the Cholesky factor (stride kcap + 1, plus one wave of slack
+// for the uniform reads).
+constexpr int lds_factor_doubles(int kcap, int /*rpl*/) { return (kcap + 1) * kcap + WAVE; }
 
 // ---------------------------------------------------------------------------------------------
 // Device helpers
@@ -144,20 +140,28 @@ __device__ __forceinline__ void record_overflow(const VoxelSet& vs, long v) {
     }
 }
 
-struct SingleArgs {
-    const double* signals;
+// The FOD solve of a list of voxels: the signal (single-tissue) or the anisotropic signal
+// max(w s - W_iso p, 0) of each voxel's isotropic fractions p (multi-tissue,
+// multi_tissue.cpp:20-24), its correlations and the ridge NNLS (solver.cpp:14-21,
+// nnls.cpp:294-350); the amplitudes go to x_out (n_vox, n).
+struct FodArgs {
+    const double* signals;    // (n_vox, m)
     int m;
+    int multi;                // 1: anisotropic signal of iso; 0: the signal itself
+    const double* weights;    // (m), multi only
+    const double* w_iso;      // (m, ci), multi only
+    const double* iso;        // (n_vox, ci), multi only
+    int ci;
     RidgeDesignDev design;
     double tol;
-    double* fods;
-    int* status;
+    double* x_out;            // (n_vox, n)
+    int* status;              // per voxel, OR-ed
     VoxelSet vs;
     SolveScratch scratch;
 };
 
-// fit_single_tissue: one ridge NNLS per voxel (solver.cpp:14-21 through sd.cpp:fit_single_tissue).
 template <int NCPL, int RPL, int KCAP, bool GLOBAL_L>
-__global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) single_kernel(SingleArgs A) {
+__global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) fod_kernel(FodArgs A) {
     constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
     extern __shared__ double smem[];
     const int w = threadIdx.x / WAVE;
@@ -173,9 +177,20 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) single_kernel(SingleAr
         const long v = A.vs.list ? A.vs.list[it] : it;
         RowVec<RPL> s;
         load_signal<RPL>(A.signals, v, A.m, s);
+        if (A.multi) {
+            double p[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                p[q] = q < A.ci ? A.iso[v * A.ci + q] : 0.0;
+            }
+            RowVec<RPL> a;
+            anisotropic_signal<RPL>(s, A.weights, A.w_iso, A.m, A.ci, p, a);
+            s = a;
+        }
         double corr[NCPL];
         correlations<NCPL, RPL>(A.design, s, 0.0, corr);
-        RidgeNnlsWave<NCPL, KCAP, RPLQ, GLOBAL_L> solver(A.design, A.tol, 3 * A.design.n, 0.0, L, qrs);
+        RidgeNnlsWave<NCPL, KCAP, RPLQ, GLOBAL_L> solver(A.design, A.tol, 3 * A.design.n, 0.0, L,
+                                                         qrs);
         const int st = solver.template run<RPL>(corr, s);
         if (st & ST_OVERFLOW) {
             record_overflow(A.vs, v);
@@ -185,214 +200,128 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) single_kernel(SingleAr
         for (int t = 0; t < NCPL; ++t) {
             const int c = l + WAVE * t;
             if (c < A.design.n) {
-                A.fods[v * A.design.n + c] = solver.x[t];
+                A.x_out[v * A.design.n + c] = solver.x[t];
             }
         }
-        if (l == 0) {
-            A.status[v] = st;
-        }
-    }
-}
-
-struct LoopArgs {
-    const double* signals;
-    int m;
-    const double* weights;
-    RidgeDesignDev design;    // low-resolution solver
-    FractionKernelDev kernel; // low-resolution kernel (unweighted)
-    const double* w_iso;      // (m, ci) weighted isotropic columns
-    double tol;
-    int max_rounds;
-    double fraction_tol;
-    double* iso_out;          // (n_vox, ci)
-    int* rounds_out;
-    int* status;
-    VoxelSet vs;
-    SolveScratch scratch;
-};
-
-// The low-resolution rounds of fit_multi_tissue_block (multi_tissue.cpp:47-80): per round the
-// anisotropic signal, its low-resolution FOD and the lumped fractions, until no fraction moves by
-// fraction_tol or max_rounds rounds ran; the voxel keeps its last isotropic fractions.
-template <int NCPL, int RPL, int KCAP, bool GLOBAL_L>
-__global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) loop_kernel(LoopArgs A) {
-    constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
-    extern __shared__ double smem[];
-    const int w = threadIdx.x / WAVE;
-    const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
-    const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
-    double* slab = A.scratch.base + gw * A.scratch.per_wave;
-    double* qrs = slab;
-    double* L = GLOBAL_L ? slab + static_cast<long>(RPLQ * WAVE) * (KCAP + 1)
-                         : smem + w * lds_factor_doubles(KCAP, RPL);
-    const long count = voxel_count(A.vs);
-    const int l = lane_id();
-    const int ci = A.kernel.ci;
-    const int nlf = 1 + ci;
-    for (long it = gw; it < count; it += n_waves) {
-        const long v = A.vs.list ? A.vs.list[it] : it;
-        RowVec<RPL> s;
-        load_signal<RPL>(A.signals, v, A.m, s);
-        double p[NQ];
-        double prev[NQ];
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            p[q] = 0.0;
-            prev[q] = __builtin_inf();
-        }
-        int status = 0;
-        int rounds = 0;
-        for (int round = 0; round < A.max_rounds; ++round) {
-            RowVec<RPL> a;
-            anisotropic_signal<RPL>(s, A.weights, A.w_iso, A.m, ci, p, a);
-            double corr[NCPL];
-            correlations<NCPL, RPL>(A.design, a, 0.0, corr);
-            RidgeNnlsWave<NCPL, KCAP, RPLQ, GLOBAL_L> solver(A.design, A.tol, 3 * A.design.n, 0.0, L, qrs);
-            const int st = solver.template run<RPL>(corr, a);
-            status |= st;
-            if (st & ST_OVERFLOW) {
-                break;
-            }
-            double fr[NQ];
-            if (!fractions_wave<NCPL, RPL, NQ>(A.kernel, solver.x, s, true, false, 0.0, fr, L)) {
-                status |= ST_NOT_CONVERGED;
-            }
-            rounds += 1;
-            bool converged = true;
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                if (q >= 1 && q < nlf) {
-                    p[q - 1] = fr[q];
-                }
-                if (q < nlf && !(fabs(fr[q] - prev[q]) < A.fraction_tol)) {
-                    converged = false;
-                }
-            }
-            if (converged) {
-                break;
-            }
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                prev[q] = q < nlf ? fr[q] : prev[q];
-            }
-        }
-        if (status & ST_OVERFLOW) {
-            record_overflow(A.vs, v);
-            continue;
-        }
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            if (l == q && q < ci) {
-                A.iso_out[v * ci + q] = p[q];
-            }
-        }
-        if (l == 0) {
-            A.rounds_out[v] = rounds;
-            A.status[v] = status;
+        if (l == 0 && st != 0) {
+            A.status[v] |= st;
         }
     }
 }
 
-struct FinalArgs {
-    const double* signals;
+// The tissue fractions of a list of voxels from their FOD amplitudes (fractions.cpp:45-76).
+// A round of the low-resolution loop (multi_tissue.cpp:59-78) fits the lumped fractions with the
+// default tolerance, keeps the isotropic ones, counts the round and appends the voxel to the
+// next round's list unless no fraction moved by fraction_tol; the final fit (multi_tissue.cpp:
+// 92-99) fits the per-block fractions with final_tol and normalises them to sum to 1.
+struct FractionArgs {
+    const double* signals;    // (n_vox, m), unweighted
     int m;
-    const double* weights;
-    RidgeDesignDev design;    // high-resolution solver
-    FractionKernelDev kernel; // high-resolution kernel (unweighted)
-    const double* w_iso;
-    double tol;
+    const double* x;          // (n_vox, ca) FOD amplitudes
+    FractionKernelDev kernel;
+    int final_fit;
     double final_tol;
-    const double* iso_in;     // (n_vox, ci)
-    double* fods;             // (n_vox, ca)
-    double* fractions;        // (n_vox, n_blocks + ci)
+    double fraction_tol;
+    double* iso;              // (n_vox, ci)     loop rounds
+    double* prev;             // (n_vox, 1 + ci) loop rounds
+    int* rounds;              // (n_vox)         loop rounds
+    int* next_list;           // loop rounds: the voxels of the next round
+    int* next_count;
+    double* fractions;        // (n_vox, n_blocks + ci), final fit
     int* status;
-    VoxelSet vs;
-    SolveScratch scratch;
+    const int* list;          // voxels to fit, or null for all
+    const int* list_count;
+    long n_vox;
 };
 
-// The high-resolution FOD and the final per-block fractions, normalised to sum to 1
-// (multi_tissue.cpp:82-99).
-template <int NCPL, int RPL, int KCAP, bool GLOBAL_L>
-__global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) final_kernel(FinalArgs A) {
-    constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
+template <int NCPL, int RPL>
+__global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) fraction_kernel(FractionArgs A) {
     extern __shared__ double smem[];
     const int w = threadIdx.x / WAVE;
     const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
     const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
-    double* slab = A.scratch.base + gw * A.scratch.per_wave;
-    double* qrs = slab;
-    double* L = GLOBAL_L ? slab + static_cast<long>(RPLQ * WAVE) * (KCAP + 1)
-                         : smem + w * lds_factor_doubles(KCAP, RPL);
-    const long count = voxel_count(A.vs);
+    double* work = smem + w * (NQ * RPL * WAVE);
+    // The block's copy of the isotropic columns, after the waves' QR slabs.
+    double* iso_lds = smem + WAVES_PER_BLOCK * (NQ * RPL * WAVE);
+    for (int i = threadIdx.x; i < A.kernel.ci * RPL * WAVE; i += blockDim.x) {
+        const int c = i / (RPL * WAVE);
+        const int r = i % (RPL * WAVE);
+        iso_lds[i] = r < A.m ? A.kernel.iso_t[static_cast<long>(c) * A.m + r] : 0.0;
+    }
+    __syncthreads();
+    const long count = A.list ? static_cast<long>(*A.list_count) : A.n_vox;
     const int l = lane_id();
+    const int ca = A.kernel.ca;
     const int ci = A.kernel.ci;
-    const int nf = A.kernel.n_blocks + ci;
     for (long it = gw; it < count; it += n_waves) {
-        const long v = A.vs.list ? A.vs.list[it] : it;
+        const long v = A.list ? A.list[it] : it;
         RowVec<RPL> s;
         load_signal<RPL>(A.signals, v, A.m, s);
-        double p[NQ];
+        double fod[NCPL];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            p[q] = q < ci ? A.iso_in[v * ci + q] : 0.0;
-        }
-        RowVec<RPL> a;
-        anisotropic_signal<RPL>(s, A.weights, A.w_iso, A.m, ci, p, a);
-        double corr[NCPL];
-        correlations<NCPL, RPL>(A.design, a, 0.0, corr);
-        RidgeNnlsWave<NCPL, KCAP, RPLQ, GLOBAL_L> solver(A.design, A.tol, 3 * A.design.n, 0.0, L, qrs);
-        int st = solver.template run<RPL>(corr, a);
-        if (st & ST_OVERFLOW) {
-            record_overflow(A.vs, v);
-            continue;
-        }
-        if (A.fods) {
-#pragma unroll
-            for (int t = 0; t < NCPL; ++t) {
-                const int c = l + WAVE * t;
-                if (c < A.design.n) {
-                    A.fods[v * A.design.n + c] = solver.x[t];
-                }
-            }
+        for (int t = 0; t < NCPL; ++t) {
+            const int c = l + WAVE * t;
+            fod[t] = c < ca ? A.x[v * ca + c] : 0.0;
         }
         double fr[NQ];
-        if (!fractions_wave<NCPL, RPL, NQ>(A.kernel, solver.x, s, false, true, A.final_tol, fr, L)) {
-            st |= ST_NOT_CONVERGED;
+        const bool ok = A.final_fit
+                            ? fractions_wave<NCPL, RPL, NQ>(A.kernel, fod, s, false, true,
+                                                            A.final_tol, fr, work, iso_lds)
+                            : fractions_wave<NCPL, RPL, NQ>(A.kernel, fod, s, true, false, 0.0,
+                                                            fr, work, iso_lds);
+        if (!ok && l == 0) {
+            A.status[v] |= ST_NOT_CONVERGED;
         }
-        double total = 0.0;
+        if (A.final_fit) {
+            const int nf = A.kernel.n_blocks + ci;
+            double total = 0.0;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                if (q < nf) {
+                    total = total + fr[q];
+                }
+            }
+            total = total + FRACTION_SUM_OFFSET;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                if (l == q && q < nf) {
+                    A.fractions[v * nf + q] = fr[q] / total;
+                }
+            }
+            continue;
+        }
+        const int nlf = 1 + ci;
+        bool converged = true;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            if (q < nf) {
-                total = total + fr[q];
+            if (q < nlf && !(fabs(fr[q] - A.prev[v * nlf + q]) < A.fraction_tol)) {
+                converged = false;
             }
         }
-        total = total + FRACTION_SUM_OFFSET;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            if (l == q && q < nf) {
-                A.fractions[v * nf + q] = fr[q] / total;
+            if (l == q && q >= 1 && q < nlf) {
+                A.iso[v * ci + (q - 1)] = fr[q];
+            }
+            if (!converged && l == q && q < nlf) {
+                A.prev[v * nlf + q] = fr[q];
             }
         }
         if (l == 0) {
-            A.status[v] |= st;
+            A.rounds[v] += 1;
+            if (!converged) {
+                const int at = atomicAdd(A.next_count, 1);
+                A.next_list[at] = static_cast<int>(v);
+            }
         }
     }
 }
 
 // Kernel families, for the launch helpers.
 template <int NCPL, int RPL, int KCAP, bool G>
-struct single_family {
-    static constexpr auto kernel = single_kernel<NCPL, RPL, KCAP, G>;
+struct fod_family {
+    static constexpr auto kernel = fod_kernel<NCPL, RPL, KCAP, G>;
 };
-template <int NCPL, int RPL, int KCAP, bool G>
-struct loop_family {
-    static constexpr auto kernel = loop_kernel<NCPL, RPL, KCAP, G>;
-};
-template <int NCPL, int RPL, int KCAP, bool G>
-struct final_family {
-    static constexpr auto kernel = final_kernel<NCPL, RPL, KCAP, G>;
-};
-
 
 // ---------------------------------------------------------------------------------------------
 // Host-side launch helpers (shared by the instantiation units)
@@ -497,30 +426,34 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
     return launch(rerun_name, Family<NCPL, RPL, KG, true>::kernel, rerun, RERUN_WAVES, 0, stream);
 }
 
-// Launcher of one (family, NCPL, RPL) pair, defined in its own translation unit.
-#define LSD_LAUNCHER_DECL(FAMILY, ARGS, NC, RP)                                                \
-    int launch_##FAMILY##_##NC##_##RP(const ARGS& fast, const ARGS& rerun, int waves,          \
-                                        int* ol, int* oc, hipStream_t stream, void** scratch);
-#define LSD_LAUNCHER_DEF(FAMILY, ARGS, NC, RP)                                                 \
-    int launch_##FAMILY##_##NC##_##RP(const ARGS& fast, const ARGS& rerun, int waves,          \
-                                        int* ol, int* oc, hipStream_t stream, void** scratch) { \
-        return run_pair<FAMILY##_family, NC, RP>(#FAMILY, #FAMILY "_rerun", fast, rerun, waves, \
-                                                 ol, oc, stream, scratch);                     \
+// Launchers of one (NCPL, RPL) instantiation, defined in its own translation unit.
+// Names label the launches in the timing records (string literals: they are kept).
+#define LSD_LAUNCHER_DECL(NC, RP)                                                              \
+    int launch_fod_##NC##_##RP(const char* name, const char* rerun_name, const FodArgs& fast,   \
+                               const FodArgs& rerun, int waves, int* ol, int* oc,               \
+                               hipStream_t stream, void** scratch);                            \
+    int launch_fraction_##NC##_##RP(const char* name, const FractionArgs& args, int waves,      \
+                                    hipStream_t stream);
+#define LSD_LAUNCHER_DEF(NC, RP)                                                               \
+    int launch_fod_##NC##_##RP(const char* name, const char* rerun_name, const FodArgs& fast,   \
+                               const FodArgs& rerun, int waves, int* ol, int* oc,               \
+                               hipStream_t stream, void** scratch) {                           \
+        return run_pair<fod_family, NC, RP>(name, rerun_name, fast, rerun, waves, ol, oc,       \
+                                            stream, scratch);                                  \
+    }                                                                                          \
+    int launch_fraction_##NC##_##RP(const char* name, const FractionArgs& args, int waves,      \
+                                    hipStream_t stream) {                                      \
+        const size_t lds = sizeof(double) * (NQ * WAVES_PER_BLOCK + args.kernel.ci) * RP * WAVE; \
+        return launch(name, fraction_kernel<NC, RP>, args, waves, lds, stream);                 \
     }
 
-// Every instantiated (family, NCPL, RPL): X(FAMILY, ARGS, NC, RP).
+// Every instantiated (NCPL, RPL): X(NC, RP).
 #ifdef LSD_FAST_BUILD
-#define LSD_FOR_EACH_LAUNCHER(X)                                                               \
-    X(single, SingleArgs, 1, 4) X(single, SingleArgs, 5, 4) X(loop, LoopArgs, 1, 4)            \
-    X(final, FinalArgs, 5, 4)
+#define LSD_FOR_EACH_LAUNCHER(X) X(1, 4) X(5, 4)
 #else
-#define LSD_FOR_EACH_RPL(X, FAMILY, ARGS, NC) X(FAMILY, ARGS, NC, 4) X(FAMILY, ARGS, NC, 6) X(FAMILY, ARGS, NC, 8)
+#define LSD_FOR_EACH_RPL(X, NC) X(NC, 4) X(NC, 6) X(NC, 8)
 #define LSD_FOR_EACH_LAUNCHER(X)                                                               \
-    LSD_FOR_EACH_RPL(X, single, SingleArgs, 1) LSD_FOR_EACH_RPL(X, single, SingleArgs, 3)        \
-    LSD_FOR_EACH_RPL(X, single, SingleArgs, 5) LSD_FOR_EACH_RPL(X, single, SingleArgs, 10)       \
-    LSD_FOR_EACH_RPL(X, loop, LoopArgs, 1) LSD_FOR_EACH_RPL(X, loop, LoopArgs, 3)                \
-    LSD_FOR_EACH_RPL(X, final, FinalArgs, 1) LSD_FOR_EACH_RPL(X, final, FinalArgs, 3)            \
-    LSD_FOR_EACH_RPL(X, final, FinalArgs, 5) LSD_FOR_EACH_RPL(X, final, FinalArgs, 10)
+    LSD_FOR_EACH_RPL(X, 1) LSD_FOR_EACH_RPL(X, 3) LSD_FOR_EACH_RPL(X, 5) LSD_FOR_EACH_RPL(X, 10)
 #endif
 
 LSD_FOR_EACH_LAUNCHER(LSD_LAUNCHER_DECL)

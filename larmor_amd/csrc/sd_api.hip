@@ -2,7 +2,6 @@
 // entry points declared in include/larmor_sd.h, which replace the spherical-deconvolution
 // bindings of larmor's compiled core (cpp/src/python/sd.cpp).
 
-#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -16,6 +15,19 @@ __global__ void zero_ints(int* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         p[i] = 0;
+    }
+}
+
+// Starts the loop state: isotropic fractions 0, previous fractions +inf, no rounds, no status.
+// Clears the per-voxel state of a fit; rounds may be null (single-tissue fits have none).
+__global__ void init_loop_state(double* iso, long n_iso, double* prev, long n_prev, int* rounds,
+                                int* status, long n_vox) {
+    const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n_iso) iso[i] = 0.0;
+    if (i < n_prev) prev[i] = __builtin_inf();
+    if (i < n_vox) {
+        if (rounds != nullptr) rounds[i] = 0;
+        status[i] = 0;
     }
 }
 
@@ -38,21 +50,59 @@ int rpl_for(int64_t m) {
     return -1;
 }
 
-// The launcher of a family's (NCPL, RPL) instantiation.
-template <class Args>
-int dispatch(int ncpl, int rpl, const Args& fast, const Args& rerun, int waves, int* ol, int* oc,
-             hipStream_t stream, void** rerun_scratch) {
-#define LSD_TRY(FAMILY, ARGS, NC, RP)                                                          \
-    if constexpr (std::is_same_v<Args, ARGS>) {                                                \
-        if (ncpl == NC && rpl == RP)                                                           \
-            return launch_##FAMILY##_##NC##_##RP(fast, rerun, waves, ol, oc, stream,            \
-                                                 rerun_scratch);                               \
-    }
+// The launchers of an (NCPL, RPL) instantiation.
+int dispatch_fod(const char* name, const char* rerun_name, int ncpl, int rpl, const FodArgs& fast,
+                 const FodArgs& rerun, int waves, int* ol, int* oc, hipStream_t stream,
+                 void** rerun_scratch) {
+#define LSD_TRY(NC, RP)                                                                        \
+    if (ncpl == NC && rpl == RP)                                                               \
+        return launch_fod_##NC##_##RP(name, rerun_name, fast, rerun, waves, ol, oc, stream,     \
+                                      rerun_scratch);
     LSD_FOR_EACH_LAUNCHER(LSD_TRY)
 #undef LSD_TRY
     return fail("no kernel instantiation for " + std::to_string(ncpl) + " column slots and " +
                 std::to_string(rpl) + " row slots");
 }
+
+int dispatch_fraction(const char* name, int ncpl, int rpl, const FractionArgs& args, int waves,
+                      hipStream_t stream) {
+#define LSD_TRY(NC, RP)                                                                        \
+    if (ncpl == NC && rpl == RP) return launch_fraction_##NC##_##RP(name, args, waves, stream);
+    LSD_FOR_EACH_LAUNCHER(LSD_TRY)
+#undef LSD_TRY
+    return fail("no kernel instantiation for " + std::to_string(ncpl) + " column slots and " +
+                std::to_string(rpl) + " row slots");
+}
+
+// Stream-ordered temporaries of one call, freed (stream-ordered) when the call returns.
+class StreamAllocs {
+public:
+    explicit StreamAllocs(hipStream_t stream) : stream_(stream) {}
+    StreamAllocs(const StreamAllocs&) = delete;
+    StreamAllocs& operator=(const StreamAllocs&) = delete;
+    ~StreamAllocs() {
+        for (void* p : held_) {
+            (void)hipFreeAsync(p, stream_);
+        }
+    }
+    template <class T>
+    int get(T** out, long count) {
+        void* p = nullptr;
+        const int rc = device_alloc(&p, sizeof(T) * static_cast<size_t>(std::max(1L, count)),
+                                    stream_);
+        if (rc) return rc;
+        held_.push_back(p);
+        *out = static_cast<T*>(p);
+        return 0;
+    }
+    void adopt(void* p) {
+        if (p) held_.push_back(p);
+    }
+
+private:
+    hipStream_t stream_;
+    std::vector<void*> held_;
+};
 
 } // namespace lsd
 
@@ -310,41 +360,38 @@ int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
     const int ncpl = ncpl_for(solver->n);
     const int rpl = rpl_for(m);
     if (ncpl < 0 || rpl < 0) {
-        return fail("problem size " + shape_tuple(m, solver->n) + " exceeds 512 measurements or 640 directions");
+        return fail("problem size " + shape_tuple(m, solver->n) +
+                    " exceeds 512 measurements or 640 directions");
     }
     if (n_voxels == 0) {
         return 0;
     }
-    const int waves = main_waves(n_voxels);
-    void* scratch = nullptr;
-    void* rerun_scratch = nullptr;
-    void* counters = nullptr;
-    void* olist = nullptr;
-    long per = 0;
-#define SLAB_CASE(RP) if (rpl == RP) per = slab_doubles<RP, KCAP_LDS>(false);
-    SLAB_CASE(4) SLAB_CASE(6) SLAB_CASE(8)
-#undef SLAB_CASE
-    int rc = device_alloc(&scratch, sizeof(double) * per * waves, stream);
-    if (!rc) rc = device_alloc(&counters, sizeof(int) * 4, stream);
-    if (!rc) rc = device_alloc(&olist, sizeof(int) * n_voxels, stream);
+    StreamAllocs mem(stream);
+    int* counters = nullptr;
+    int* olist = nullptr;
+    double* scratch = nullptr;
+    int rc = mem.get(&counters, 2);
+    if (!rc) rc = mem.get(&olist, n_voxels);
+    if (!rc) rc = mem.get(&scratch, 1);
+    if (!rc && status == nullptr) rc = mem.get(&status, n_voxels);
     if (rc) return rc;
-    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, stream, static_cast<int*>(counters), 4);
-    SingleArgs fast{};
-    fast.signals = signals;
-    fast.m = static_cast<int>(m);
-    fast.design = design_of(solver);
-    fast.tol = solver->tol;
-    fast.fods = fods;
-    fast.status = status;
-    fast.vs = VoxelSet{n_voxels, nullptr, nullptr, static_cast<int*>(olist), static_cast<int*>(counters)};
-    fast.scratch = SolveScratch{static_cast<double*>(scratch), per};
-    SingleArgs rerun = fast;
-    rc = dispatch(ncpl, rpl, fast, rerun, waves, static_cast<int*>(olist),
-                  static_cast<int*>(counters), stream, &rerun_scratch);
-    (void)hipFreeAsync(scratch, stream);
-    (void)hipFreeAsync(counters, stream);
-    (void)hipFreeAsync(olist, stream);
-    if (rerun_scratch) (void)hipFreeAsync(rerun_scratch, stream);
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, stream, counters, 2);
+    hipLaunchKernelGGL(init_loop_state, dim3((n_voxels + 255) / 256), dim3(256), 0, stream,
+                       nullptr, 0L, nullptr, 0L, nullptr, status, static_cast<long>(n_voxels));
+    FodArgs fa{};
+    fa.signals = signals;
+    fa.m = static_cast<int>(m);
+    fa.multi = 0;
+    fa.design = design_of(solver);
+    fa.tol = solver->tol;
+    fa.x_out = fods;
+    fa.status = status;
+    fa.vs = VoxelSet{n_voxels, nullptr, nullptr, olist, counters};
+    fa.scratch = SolveScratch{scratch, 0};
+    void* rerun_scratch = nullptr;
+    rc = dispatch_fod("fod_single", "fod_single_rerun", ncpl, rpl, fa, fa, main_waves(n_voxels),
+                      olist, counters, stream, &rerun_scratch);
+    mem.adopt(rerun_scratch);
     return rc;
 }
 
@@ -385,69 +432,107 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
     if (n_voxels == 0) {
         return 0;
     }
+    const long nv = static_cast<long>(n_voxels);
     const int ci = static_cast<int>(lr->ci);
+    const int nlf = 1 + ci;
     const int waves = main_waves(n_voxels);
-    long per = 0;
-#define SLAB_CASE(RP) if (rpl == RP) per = slab_doubles<RP, KCAP_LDS>(false);
-    SLAB_CASE(4) SLAB_CASE(6) SLAB_CASE(8)
-#undef SLAB_CASE
-    void* scratch = nullptr;
-    void* counters = nullptr;
-    void* olist = nullptr;
-    void* iso = nullptr;
-    void* rerun1 = nullptr;
-    void* rerun2 = nullptr;
-    rc = device_alloc(&scratch, sizeof(double) * per * waves, stream);
-    if (!rc) rc = device_alloc(&counters, sizeof(int) * 4, stream);
-    if (!rc) rc = device_alloc(&olist, sizeof(int) * 2 * n_voxels, stream);
-    if (!rc) rc = device_alloc(&iso, sizeof(double) * std::max(1, ci) * n_voxels, stream);
+    StreamAllocs mem(stream);
+    // counters: per round its overflow and next-list counts, then the final solve's overflow.
+    const long n_counters = 2 * (max_rounds + 1);
+    int* counters = nullptr;
+    int* olist = nullptr;
+    int* lists = nullptr;
+    double* scratch = nullptr;
+    double* iso = nullptr;
+    double* prev = nullptr;
+    double* x_lr = nullptr;
+    rc = mem.get(&counters, n_counters);
+    if (!rc) rc = mem.get(&olist, nv);
+    if (!rc) rc = mem.get(&lists, 2 * nv);
+    if (!rc) rc = mem.get(&scratch, 1);
+    if (!rc) rc = mem.get(&iso, std::max(1L, nv * ci));
+    if (!rc) rc = mem.get(&prev, nv * nlf);
+    if (!rc) rc = mem.get(&x_lr, nv * lr->ca);
+    if (!rc && fods == nullptr) rc = mem.get(&fods, nv * hr->ca);
+    if (!rc && rounds == nullptr) rc = mem.get(&rounds, nv);
+    if (!rc && status == nullptr) rc = mem.get(&status, nv);
     if (rc) return rc;
-    int* cnt = static_cast<int*>(counters);
-    int* list1 = static_cast<int*>(olist);
-    int* list2 = list1 + n_voxels;
-    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, stream, cnt, 4);
+    hipLaunchKernelGGL(zero_ints, dim3((n_counters + 255) / 256), dim3(256), 0, stream, counters,
+                       static_cast<int>(n_counters));
+    hipLaunchKernelGGL(init_loop_state, dim3((nv * nlf + 255) / 256), dim3(256), 0, stream, iso,
+                       nv * ci, prev, nv * nlf, rounds, status, nv);
 
-    LoopArgs la{};
-    la.signals = signals;
-    la.m = static_cast<int>(m);
-    la.weights = weights;
-    la.design = design_of(lr_solver);
-    la.kernel = fractions_of(lr);
-    la.w_iso = lr->d_w_iso;
-    la.tol = lr_solver->tol;
-    la.max_rounds = static_cast<int>(max_rounds);
-    la.fraction_tol = fraction_tol;
-    la.iso_out = static_cast<double*>(iso);
-    la.rounds_out = rounds;
-    la.status = status;
-    la.vs = VoxelSet{n_voxels, nullptr, nullptr, list1, cnt};
-    la.scratch = SolveScratch{static_cast<double*>(scratch), per};
-    rc = dispatch(ncpl_lr, rpl, la, la, waves, list1, cnt, stream, &rerun1);
+    FodArgs fod{};
+    fod.signals = signals;
+    fod.m = static_cast<int>(m);
+    fod.multi = 1;
+    fod.weights = weights;
+    fod.iso = iso;
+    fod.ci = ci;
+    fod.status = status;
+    fod.scratch = SolveScratch{scratch, 0};
+    FractionArgs fr{};
+    fr.signals = signals;
+    fr.m = static_cast<int>(m);
+    fr.fraction_tol = fraction_tol;
+    fr.final_tol = final_tol;
+    fr.iso = iso;
+    fr.prev = prev;
+    fr.rounds = rounds;
+    fr.status = status;
+    fr.n_vox = nv;
 
-    if (!rc) {
-        FinalArgs fa{};
-        fa.signals = signals;
-        fa.m = static_cast<int>(m);
-        fa.weights = weights;
-        fa.design = design_of(hr_solver);
-        fa.kernel = fractions_of(hr);
-        fa.w_iso = hr->d_w_iso;
-        fa.tol = hr_solver->tol;
-        fa.final_tol = final_tol;
-        fa.iso_in = static_cast<double*>(iso);
-        fa.fods = fods;
-        fa.fractions = fractions;
-        fa.status = status;
-        fa.vs = VoxelSet{n_voxels, nullptr, nullptr, list2, cnt + 1};
-        fa.scratch = SolveScratch{static_cast<double*>(scratch), per};
-        rc = dispatch(ncpl_hr, rpl, fa, fa, waves, list2, cnt + 1, stream, &rerun2);
+    // The low-resolution rounds: round r fits the voxels of its list (all of them in round 0)
+    // and lists those that go on to round r + 1; lists and counts stay on the device.
+    const int* list = nullptr;
+    const int* count = nullptr;
+    for (int64_t r = 0; r < max_rounds && !rc; ++r) {
+        int* ovf_count = counters + 2 * r;
+        int* next_count = counters + 2 * r + 1;
+        int* next_list = lists + (r % 2) * nv;
+        fod.design = design_of(lr_solver);
+        fod.tol = lr_solver->tol;
+        fod.w_iso = lr->d_w_iso;
+        fod.x_out = x_lr;
+        fod.vs = VoxelSet{nv, list, count, olist, ovf_count};
+        void* rerun_scratch = nullptr;
+        rc = dispatch_fod("fod_lr", "fod_lr_rerun", ncpl_lr, rpl, fod, fod, waves, olist,
+                          ovf_count, stream, &rerun_scratch);
+        mem.adopt(rerun_scratch);
+        if (rc) break;
+        fr.x = x_lr;
+        fr.kernel = fractions_of(lr);
+        fr.final_fit = 0;
+        fr.list = list;
+        fr.list_count = count;
+        fr.next_list = next_list;
+        fr.next_count = next_count;
+        rc = dispatch_fraction("fraction_loop", ncpl_lr, rpl, fr, waves, stream);
+        list = next_list;
+        count = next_count;
     }
-    (void)hipFreeAsync(scratch, stream);
-    (void)hipFreeAsync(counters, stream);
-    (void)hipFreeAsync(olist, stream);
-    (void)hipFreeAsync(iso, stream);
-    if (rerun1) (void)hipFreeAsync(rerun1, stream);
-    if (rerun2) (void)hipFreeAsync(rerun2, stream);
+    if (!rc) {
+        // The high-resolution FOD of every voxel, then its final fractions.
+        int* ovf_count = counters + 2 * max_rounds;
+        fod.design = design_of(hr_solver);
+        fod.tol = hr_solver->tol;
+        fod.w_iso = hr->d_w_iso;
+        fod.x_out = fods;
+        fod.vs = VoxelSet{nv, nullptr, nullptr, olist, ovf_count};
+        void* rerun_scratch = nullptr;
+        rc = dispatch_fod("fod_hr", "fod_hr_rerun", ncpl_hr, rpl, fod, fod, waves, olist,
+                          ovf_count, stream, &rerun_scratch);
+        mem.adopt(rerun_scratch);
+    }
+    if (!rc) {
+        fr.x = fods;
+        fr.kernel = fractions_of(hr);
+        fr.final_fit = 1;
+        fr.fractions = fractions;
+        fr.list = nullptr;
+        fr.list_count = nullptr;
+        rc = dispatch_fraction("fraction_final", ncpl_hr, rpl, fr, waves, stream);
+    }
     return rc;
 }
 
