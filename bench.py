@@ -14,7 +14,9 @@ step) at its algorithmic FP64 flops against MI355X's FP64 peak, from the kernel'
 durations measured with HIP events on its launch stream: 2 m n per voxel solve for the FOD
 kernels (the correlation K^T W s with the n-column design) and 2 m n per voxel fit for the
 fraction kernels (the anisotropic signal K x).  The low-resolution kernels run once per voxel
-and round, so their unit count is the sum of the voxels' rounds.  `cpu_baseline` is the CPU
+and round, so their unit count is the sum of the voxels' rounds.  `traffic` is the kernel's HBM
+bytes per launch from the committed rocprofv3 counter summary (null when it has no entry).
+`cpu_baseline` is the CPU
 FP64 oracle (a C++ restatement of the reference algorithm, `kind: port`) on a bounded random
 sample of the same voxels, on the host's cores.
 """
@@ -75,6 +77,33 @@ def cpu_baseline(signals, w, lr, hr, target_seconds=15.0):
     }
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (profiles/mfod_3shell_traffic.json, counters scaled by a 1 GiB copy), or None."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "mfod_3shell_traffic.json")
+    try:
+        with open(path) as fh:
+            entry = json.load(fh)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if entry is None else entry["hbm_bytes_per_dispatch"]
+
+
+def copy_probe(mib: int) -> int:
+    """Copies mib MiB device to device three times; each copy reads and writes mib MiB."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    src = torch.rand(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"copied {mib} MiB x 3", file=sys.stderr, flush=True)
+    return 0
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -82,7 +111,12 @@ def main() -> int:
     parser.add_argument("--warmup", type=int, default=1)
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--cpu-seconds", type=float, default=15.0)
+    parser.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                        help="only run three device-to-device copies of MIB MiB (the known-size "
+                             "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = parser.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print("bench.py measures one GPU; the path shards voxels across GPUs with no exchange, "
               "and runs on more than one GPU are not part of this build.", file=sys.stderr)
@@ -185,7 +219,7 @@ def main() -> int:
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(dominant),
             "kernel": dominant,
             "kernel_avg_launch_ms": avg_launch_s * 1e3,
             "launches_per_step": launches_per_step,
