@@ -111,6 +111,32 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m,
                          double final_tol, double* fods, double* fractions, int32_t* rounds,
                          int32_t* status, void* stream);
 
+/*
+ * A constrained spherical deconvolution problem (host arrays): the linear start initial (c0 x m),
+ * the forward model F (m x c), the weighted SH basis on the constraint directions C (k x c),
+ * with the Gram matrix F^T F + norm_weight I made here.  c must be 6, 15, 28 or 45 (lmax 2-8);
+ * k <= 512.
+ * Replaces: fit_csd's problem set-up  cpp/src/python/sd.cpp:171-192,
+ *           make_csd_problem cpp/src/sd/csd.cpp:11-21.
+ * Errors: "shapes do not match: ...", "max_iter must be at least 0".
+ */
+typedef struct lsd_csd lsd_csd;
+int lsd_csd_create(const double* initial, int64_t c0, int64_t m, const double* forward,
+                   int64_t c, const double* constraint, int64_t k, double norm_weight,
+                   double threshold, int64_t max_iter, lsd_csd** out);
+void lsd_csd_destroy(lsd_csd* problem);
+
+/*
+ * CSD of each voxel: signals (device, n_voxels x m) -> fods (device, n_voxels x c), converged
+ * (device uint8, n_voxels: 1 when the constrained set repeated, or max_iter is 0) and, when not
+ * NULL, solves (device int32, n_voxels: Cholesky solves run).
+ * Replaces: fit_csd(signals, initial, forward, constraint, norm_weight, threshold, max_iter,
+ *           n_threads)  cpp/src/python/sd.cpp:171-206, solve_csd cpp/src/sd/csd.cpp:23-65.
+ * Errors: "shapes do not match: signals (r, c), initial ..., forward ..., constraint ...".
+ */
+int lsd_fit_csd(const double* signals, int64_t n_voxels, int64_t m, const lsd_csd* problem,
+                double* fods, uint8_t* converged, int32_t* solves, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

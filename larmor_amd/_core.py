@@ -246,3 +246,66 @@ def fit_multi_tissue(
     if return_diagnostics:
         out += (_out(rounds, host), _out(status, host))
     return out
+
+
+class CsdProblem:
+    """A CSD problem resident on the device: the linear start, forward model, weighted
+    constraint basis and the Gram matrix F^T F + norm_weight I (csd.cpp:11-21)."""
+
+    def __init__(self, initial, forward, constraint, norm_weight: float, threshold: float,
+                 max_iter: int):
+        ini = _host_matrix(initial, "initial")
+        fw = _host_matrix(forward, "forward")
+        con = _host_matrix(constraint, "constraint")
+        lib = _lib.load()
+        handle = ctypes.c_void_p()
+        _lib.check(lib.lsd_csd_create(
+            _ptr(ini), ini.shape[0], ini.shape[1], _ptr(fw), fw.shape[1], _ptr(con),
+            con.shape[0], float(norm_weight), float(threshold), int(max_iter),
+            ctypes.byref(handle)))
+        self._handle = handle
+        self.n_coefficients = fw.shape[1]
+        self.n_meas = fw.shape[0]
+
+    def __del__(self):
+        _release(self, "lsd_csd_destroy")
+
+
+def fit_csd(signals, initial, forward, constraint, norm_weight: float, threshold: float,
+            max_iter: int, n_threads: int, *, return_solves: bool = False):
+    """Fits every row of signals (n, m) by constrained spherical deconvolution (sd.cpp:171-206).
+
+    Returns the FODs (n, c) and whether each voxel's constrained set repeated (n,) bool; with
+    return_solves also the Cholesky solves run per voxel.
+    """
+    torch = _torch()
+    for a, name in ((initial, "initial"), (forward, "forward"), (constraint, "constraint")):
+        _check_f64(a, name)
+    dev, host = _device_signals(signals, torch)
+    n, m = dev.shape
+    ini_shape, fw_shape, con_shape = tuple(initial.shape), tuple(forward.shape), tuple(constraint.shape)
+    if (len(ini_shape) != 2 or len(fw_shape) != 2 or len(con_shape) != 2 or m != fw_shape[0]
+            or ini_shape[1] != fw_shape[0] or ini_shape[0] > fw_shape[1]
+            or con_shape[1] != fw_shape[1]):
+        def tup(t):
+            return "(" + ", ".join(str(x) for x in t) + ")"
+        raise ValueError(
+            f"shapes do not match: signals ({n}, {m}), initial {tup(ini_shape)}, forward "
+            f"{tup(fw_shape)}, constraint {tup(con_shape)}"
+        )
+    if max_iter < 0:
+        raise ValueError("max_iter must be at least 0")
+    _require_threads(n_threads)
+    problem = CsdProblem(initial, forward, constraint, norm_weight, threshold, max_iter)
+    c = problem.n_coefficients
+    fods = torch.empty((n, c), dtype=torch.float64, device="cuda")
+    converged = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    solves = torch.zeros(n, dtype=torch.int32, device="cuda")
+    lib = _lib.load()
+    _lib.check(lib.lsd_fit_csd(dev.data_ptr(), n, m, problem._handle, fods.data_ptr(),
+                               converged.data_ptr(), solves.data_ptr(), _stream_ptr(torch)))
+    conv = converged.to(torch.bool)
+    out = (_out(fods, host), _out(conv, host))
+    if return_solves:
+        out += (_out(solves, host),)
+    return out

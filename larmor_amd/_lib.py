@@ -48,6 +48,15 @@ SIGNATURES = {
         [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64,
          ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp, _vp],
     ),
+    "lsd_csd_create": (
+        ctypes.c_int,
+        [_d, ctypes.c_int64, ctypes.c_int64, _d, ctypes.c_int64, _d, ctypes.c_int64,
+         ctypes.c_double, ctypes.c_double, ctypes.c_int64, ctypes.POINTER(_vp)],
+    ),
+    "lsd_csd_destroy": (None, [_vp]),
+    "lsd_fit_csd": (
+        ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]
+    ),
 }
 
 

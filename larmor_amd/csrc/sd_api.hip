@@ -3,6 +3,7 @@
 // bindings of larmor's compiled core (cpp/src/python/sd.cpp).
 
 
+#include "csd_wave.hpp"
 #include "kernels.hpp"
 
 namespace lsd {
@@ -167,7 +168,26 @@ int upload(double** dst, const std::vector<double>& src) {
     return 0;
 }
 
+// CSD launcher (inst_csd.hip).
+int csd_direction_slots(int64_t k);
+bool csd_supported_coefficients(int64_t c);
+int launch_csd(int n, int nk, const CsdArgs& args, int waves, hipStream_t stream);
+
 } // namespace lsd
+
+struct lsd_csd {
+    int64_t m = 0;
+    int64_t c = 0;
+    int64_t c0 = 0;
+    int64_t k = 0;
+    double threshold = 0.0;
+    int64_t max_iter = 0;
+    double* d_initial_t = nullptr;    // (m, c0)
+    double* d_forward = nullptr;      // (m, c)
+    double* d_constraint = nullptr;   // (k, c)
+    double* d_constraint_t = nullptr; // (c, k)
+    double* d_gram = nullptr;         // (c, c)
+};
 
 extern "C" {
 
@@ -534,6 +554,110 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
         rc = dispatch_fraction("fraction_final", ncpl_hr, rpl, fr, waves, stream);
     }
     return rc;
+}
+
+int lsd_csd_create(const double* initial, int64_t c0, int64_t m, const double* forward,
+                   int64_t c, const double* constraint, int64_t k, double norm_weight,
+                   double threshold, int64_t max_iter, lsd_csd** out) {
+    if (c0 < 0 || m < 0 || c < 0 || k < 0 || c0 > c) {
+        return fail("shapes do not match: initial " + shape_tuple(c0, m) + ", forward " +
+                    shape_tuple(m, c) + ", constraint " + shape_tuple(k, c));
+    }
+    if (max_iter < 0) {
+        return fail("max_iter must be at least 0");
+    }
+    if (!csd_supported_coefficients(c)) {
+        return fail("CSD on the GPU takes 6, 15, 28 or 45 coefficients (lmax 2 to 8), got " +
+                    std::to_string(c));
+    }
+    if (csd_direction_slots(k) < 0) {
+        return fail("CSD on the GPU takes at most 512 constraint directions, got " +
+                    std::to_string(k));
+    }
+    auto* p = new lsd_csd();
+    p->m = m;
+    p->c = c;
+    p->c0 = c0;
+    p->k = k;
+    p->threshold = threshold;
+    p->max_iter = max_iter;
+    // make_csd_problem (csd.cpp:11-21): F^T F summed over the measurements in order, then the
+    // norm weight on the diagonal.
+    std::vector<double> gram(static_cast<size_t>(c * c));
+    for (int64_t a = 0; a < c; ++a) {
+        for (int64_t b = 0; b < c; ++b) {
+            double acc = 0.0;
+            for (int64_t r = 0; r < m; ++r) {
+                acc = acc + forward[r * c + a] * forward[r * c + b];
+            }
+            gram[a * c + b] = acc;
+        }
+    }
+    for (int64_t a = 0; a < c; ++a) {
+        gram[a * c + a] = gram[a * c + a] + norm_weight;
+    }
+    std::vector<double> init_t(static_cast<size_t>(m * c0));
+    for (int64_t i = 0; i < c0; ++i) {
+        for (int64_t r = 0; r < m; ++r) {
+            init_t[r * c0 + i] = initial[i * m + r];
+        }
+    }
+    std::vector<double> con_t(static_cast<size_t>(c * k));
+    for (int64_t d = 0; d < k; ++d) {
+        for (int64_t j = 0; j < c; ++j) {
+            con_t[j * k + d] = constraint[d * c + j];
+        }
+    }
+    int rc = upload(&p->d_initial_t, init_t);
+    if (!rc) rc = upload(&p->d_forward, std::vector<double>(forward, forward + m * c));
+    if (!rc) rc = upload(&p->d_constraint, std::vector<double>(constraint, constraint + k * c));
+    if (!rc) rc = upload(&p->d_constraint_t, con_t);
+    if (!rc) rc = upload(&p->d_gram, gram);
+    if (rc) {
+        lsd_csd_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return 0;
+}
+
+void lsd_csd_destroy(lsd_csd* p) {
+    if (!p) return;
+    for (double* d : {p->d_initial_t, p->d_forward, p->d_constraint, p->d_constraint_t, p->d_gram}) {
+        if (d) (void)hipFree(d);
+    }
+    delete p;
+}
+
+int lsd_fit_csd(const double* signals, int64_t n_voxels, int64_t m, const lsd_csd* p,
+                double* fods, uint8_t* converged, int32_t* solves, void* stream_) {
+    auto stream = static_cast<hipStream_t>(stream_);
+    if (m != p->m) {
+        return fail("shapes do not match: signals " + shape_tuple(n_voxels, m) + ", initial " +
+                    shape_tuple(p->c0, p->m) + ", forward " + shape_tuple(p->m, p->c) +
+                    ", constraint " + shape_tuple(p->k, p->c));
+    }
+    if (n_voxels == 0) {
+        return 0;
+    }
+    CsdArgs a{};
+    a.signals = signals;
+    a.n_vox = n_voxels;
+    a.m = static_cast<int>(m);
+    a.c0 = static_cast<int>(p->c0);
+    a.initial_t = p->d_initial_t;
+    a.forward = p->d_forward;
+    a.constraint = p->d_constraint;
+    a.constraint_t = p->d_constraint_t;
+    a.gram = p->d_gram;
+    a.k = static_cast<int>(p->k);
+    a.threshold = p->threshold;
+    a.max_iter = static_cast<int>(p->max_iter);
+    a.fods = fods;
+    a.converged = converged;
+    a.solves = solves;
+    return launch_csd(static_cast<int>(p->c), csd_direction_slots(p->k), a,
+                      main_waves(n_voxels), stream);
 }
 
 } // extern "C"

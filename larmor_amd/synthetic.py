@@ -128,6 +128,21 @@ def phantom(
     return Phantom(data=data, bvals=bvals, bvecs=bvecs, mask=mask, fractions=fractions)
 
 
+def tensor_zonal(bval: float, l_par: float = 1.7e-3, l_perp: float = 0.3e-3,
+                 lmax: int = 8) -> np.ndarray:
+    """Zonal SH coefficients c_l (l = 0, 2, ..., lmax) of the axially symmetric tensor response
+    exp(-b (l_perp + (l_par - l_perp) cos^2 t)) on one shell, S0 = 1: c_l = 2 pi times the
+    integral over cos t of the response times sqrt((2l + 1) / 4 pi) P_l (64-point
+    Gauss-Legendre, exact to rounding for these smooth profiles)."""
+    from scipy.special import eval_legendre
+
+    x, w = np.polynomial.legendre.leggauss(64)
+    profile = np.exp(-bval * (l_perp + (l_par - l_perp) * x * x))
+    degrees = np.arange(0, lmax + 1, 2)
+    return np.array([2.0 * np.pi * np.sum(w * profile * np.sqrt((2 * l + 1) / (4 * np.pi))
+                                          * eval_legendre(l, x)) for l in degrees])
+
+
 # The named workloads of the benchmark and the parity tests.
 CSD_CONFIG = dict(shape=(32, 32, 20), n_b0=5, shells=[(1000.0, 60)], seed=1, snr=20.0,
                   mask_share=None, tissues=False)

@@ -59,6 +59,11 @@ def _load():
         + [ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int, _d, _d,
            _i64]
     )
+    lib.oracle_fit_csd.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, _d, ctypes.c_int64, _d,
+                                   ctypes.c_int64, _d, ctypes.c_int64, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_int64, ctypes.c_int, _d,
+                                   ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_int32)]
+    lib.oracle_csd_gram.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, _d]
     _lib = lib
     return lib
 
@@ -192,3 +197,30 @@ def fit_multi_tissue(signals, weights, lr, hr, lr_ridge, hr_ridge, tol_x, max_ro
              tr.ctypes.data_as(_i64) if trace else None]
     _check(_load().oracle_fit_multi_tissue(*args))
     return (fods, fractions, tr) if trace else (fods, fractions)
+
+
+def fit_csd(signals, initial, forward, constraint, norm_weight, threshold, max_iter,
+            n_threads=None, return_solves=False):
+    """CSD of every row of signals; returns (fods (n, c), converged (n,) bool[, solves (n,)])."""
+    s, ini, fw, con = _c(signals), _c(initial), _c(forward), _c(constraint)
+    n, m = s.shape
+    c = fw.shape[1]
+    fods = np.zeros((n, c))
+    conv = np.zeros(n, dtype=np.uint8)
+    solves = np.zeros(n, dtype=np.int32)
+    _check(_load().oracle_fit_csd(
+        _p(s), n, m, _p(ini), ini.shape[0], _p(fw), c, _p(con), con.shape[0], float(norm_weight),
+        float(threshold), int(max_iter), n_threads or os.cpu_count() or 1, _p(fods),
+        conv.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+        solves.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    if return_solves:
+        return fods, conv.astype(bool), solves
+    return fods, conv.astype(bool)
+
+
+def csd_gram(forward, norm_weight):
+    fw = _c(forward)
+    m, c = fw.shape
+    out = np.zeros((c, c))
+    _check(_load().oracle_csd_gram(_p(fw), m, c, float(norm_weight), _p(out)))
+    return out

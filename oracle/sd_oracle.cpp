@@ -10,6 +10,7 @@
 //   cpp/src/sd/fractions.cpp   median, compute_fractions
 //   cpp/src/sd/multi_tissue.cpp fit_multi_tissue_block
 //   cpp/src/sd/solver.cpp      L2NnlsSolver::solve_block
+//   cpp/src/sd/csd.cpp         make_csd_problem, solve_csd (LLT restated as below)
 //   cpp/src/common/parallel.hpp parallel_for_blocks (blocks of 128 voxels, dynamic)
 // Eigen's ColPivHouseholderQR (Eigen 5.0.1, the reference's pinned dependency) is restated from
 // its published algorithm (LAPACK xGEQP3 column pivoting with the LAWN 176 norm downdate).
@@ -932,6 +933,159 @@ void parallel_for_blocks(Index n_items, Index block_size, int n_threads, Body bo
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Constrained spherical deconvolution (cpp/src/sd/csd.cpp).
+
+struct CsdProblem {
+    Mat initial;    // (c0, m)
+    Mat forward;    // (m, c)
+    Mat constraint; // (k, c), weighted
+    Mat gram;       // (c, c)
+    double threshold = 0.0;
+    Index max_iterations = 0;
+};
+
+// make_csd_problem (csd.cpp:11-21): gram = F^T F, each entry summed over the measurements in
+// order, then norm_weight added to the diagonal.
+CsdProblem make_csd_problem(Mat initial, Mat forward, Mat constraint, double norm_weight,
+                            double threshold, Index max_iterations) {
+    const Index c = forward.cols;
+    Mat gram(c, c);
+    for (Index i = 0; i < c; ++i) {
+        for (Index j = 0; j < c; ++j) {
+            double sum = 0.0;
+            for (Index r = 0; r < forward.rows; ++r) {
+                sum = sum + forward(r, i) * forward(r, j);
+            }
+            gram(i, j) = sum;
+        }
+    }
+    for (Index i = 0; i < c; ++i) {
+        gram(i, i) = gram(i, i) + norm_weight;
+    }
+    CsdProblem p;
+    p.initial = std::move(initial);
+    p.forward = std::move(forward);
+    p.constraint = std::move(constraint);
+    p.gram = std::move(gram);
+    p.threshold = threshold;
+    p.max_iterations = max_iterations;
+    return p;
+}
+
+// The Cholesky factor of a symmetric matrix (lower triangle used), left-looking:
+// L_ij = (A_ij - L_i0 L_j0 - L_i1 L_j1 - ... - L_i(j-1) L_j(j-1)) / L_jj, subtracted in order,
+// and L_jj = sqrt(A_jj - L_j0^2 - ...).  Eigen's LLT (csd.cpp:34,60) factors the same matrix.
+Mat cholesky_lower(const Mat& a) {
+    const Index n = a.rows;
+    Mat l(n, n);
+    for (Index j = 0; j < n; ++j) {
+        double d = a(j, j);
+        for (Index p = 0; p < j; ++p) {
+            d = d - l(j, p) * l(j, p);
+        }
+        const double ljj = std::sqrt(d);
+        l(j, j) = ljj;
+        for (Index i = j + 1; i < n; ++i) {
+            double v = a(i, j);
+            for (Index p = 0; p < j; ++p) {
+                v = v - l(i, p) * l(j, p);
+            }
+            l(i, j) = v / ljj;
+        }
+    }
+    return l;
+}
+
+// L L^T x = b: forward then backward substitution; a zero right-hand entry stays zero (as
+// Eigen's triangular solves skip it), the terms subtracted in the order the column-oriented
+// solves meet them (forward: earlier positions first; backward: later positions first).
+Vec cholesky_solve(const Mat& l, const Vec& b) {
+    const Index n = l.rows;
+    Vec y(b);
+    for (Index i = 0; i < n; ++i) {
+        double v = y[i];
+        for (Index p = 0; p < i; ++p) {
+            v = v - l(i, p) * y[p];
+        }
+        y[i] = v != 0.0 ? v / l(i, i) : v;
+    }
+    Vec x(y);
+    for (Index i = n - 1; i >= 0; --i) {
+        double v = x[i];
+        for (Index p = n - 1; p > i; --p) {
+            v = v - l(p, i) * x[p];
+        }
+        x[i] = v != 0.0 ? v / l(i, i) : v;
+    }
+    return x;
+}
+
+struct CsdOut {
+    Vec fod;
+    bool converged = true;
+    Index solves = 0;
+};
+
+// solve_csd (csd.cpp:23-65): the linear start initial * s (first c0 coefficients), then solves of
+// (gram + sum over constrained directions d, in increasing d, of r_d r_d^T) f = F^T s until the
+// set {d : (C f)_d < threshold} repeats or max_iterations solves have run.  Sums over the
+// measurements and over the coefficients run in index order.
+CsdOut solve_csd(const CsdProblem& p, const Vec& s) {
+    const Index m = p.forward.rows;
+    const Index c = p.forward.cols;
+    const Index k = p.constraint.rows;
+    CsdOut out;
+    out.fod.assign(static_cast<size_t>(c), 0.0);
+    for (Index i = 0; i < p.initial.rows; ++i) {
+        double sum = 0.0;
+        for (Index r = 0; r < m; ++r) {
+            sum = sum + p.initial(i, r) * s[r];
+        }
+        out.fod[i] = sum;
+    }
+    Vec projected(static_cast<size_t>(c));
+    for (Index i = 0; i < c; ++i) {
+        double sum = 0.0;
+        for (Index r = 0; r < m; ++r) {
+            sum = sum + p.forward(r, i) * s[r];
+        }
+        projected[i] = sum;
+    }
+    std::vector<Index> constrained, previous;
+    bool first = true;
+    out.converged = p.max_iterations == 0;
+    for (Index it = 0; it < p.max_iterations; ++it) {
+        constrained.clear();
+        for (Index d = 0; d < k; ++d) {
+            double a = 0.0;
+            for (Index j = 0; j < c; ++j) {
+                a = a + p.constraint(d, j) * out.fod[j];
+            }
+            if (a < p.threshold) {
+                constrained.push_back(d);
+            }
+        }
+        if (!first && constrained == previous) {
+            out.converged = true;
+            break;
+        }
+        first = false;
+        Mat system = p.gram;
+        for (Index d : constrained) {
+            for (Index i = 0; i < c; ++i) {
+                for (Index j = 0; j < c; ++j) {
+                    system(i, j) = system(i, j) + p.constraint(d, i) * p.constraint(d, j);
+                }
+            }
+        }
+        out.fod = cholesky_solve(cholesky_lower(system), projected);
+        out.solves += 1;
+        std::swap(previous, constrained);
+    }
+    return out;
+}
+
 TissueKernel make_kernel(const double* k_aniso, const double* k_iso, const double* w_k_iso,
                          Index m, Index ca, Index ci, const std::int64_t* blocks,
                          Index n_blocks) {
@@ -1111,6 +1265,39 @@ int oracle_fit_multi_tissue(const double* signals, std::int64_t n_voxels, std::i
                 }
             }
         });
+    });
+}
+
+// fit_csd (cpp/src/python/sd.cpp:171-206): converged (uint8) and solves (int32, optional) per voxel.
+int oracle_fit_csd(const double* signals, std::int64_t n_voxels, std::int64_t m,
+                   const double* initial, std::int64_t c0, const double* forward, std::int64_t c,
+                   const double* constraint, std::int64_t k, double norm_weight, double threshold,
+                   std::int64_t max_iterations, int n_threads, double* fods,
+                   std::uint8_t* converged, std::int32_t* solves) {
+    return guarded([&] {
+        const CsdProblem p =
+            make_csd_problem(from_ptr(initial, c0, m), from_ptr(forward, m, c),
+                             from_ptr(constraint, k, c), norm_weight, threshold, max_iterations);
+        parallel_for_blocks(n_voxels, 128, n_threads, [&](Index begin, Index end) {
+            for (Index v = begin; v < end; ++v) {
+                const CsdOut r = solve_csd(p, Vec(signals + v * m, signals + (v + 1) * m));
+                std::copy(r.fod.begin(), r.fod.end(), fods + v * c);
+                converged[v] = r.converged ? 1 : 0;
+                if (solves) {
+                    solves[v] = static_cast<std::int32_t>(r.solves);
+                }
+            }
+        });
+    });
+}
+
+// The CSD Gram matrix F^T F + norm_weight I, for checking the library's.
+int oracle_csd_gram(const double* forward, std::int64_t m, std::int64_t c, double norm_weight,
+                    double* gram) {
+    return guarded([&] {
+        const CsdProblem p =
+            make_csd_problem(Mat(0, m), from_ptr(forward, m, c), Mat(0, c), norm_weight, 0.0, 0);
+        std::copy(p.gram.v.begin(), p.gram.v.end(), gram);
     });
 }
 

@@ -1,0 +1,146 @@
+"""The CPU oracle's constrained spherical deconvolution against what pins it.
+
+- the reference's known answers (cpp/tests/sd/test_csd.cpp:33-52): no iteration gives the
+  linear start; the penalised coefficient of a toy problem has the closed form
+  -0.5 / (1 + w^2); running out of iterations is reported;
+- the fixed point that defines a converged CSD solve: the final FOD solves
+  (G + sum over {d : (C f)_d < threshold} of c_d c_d^T) f = F^T s, checked with numpy's dense
+  solve on realistic single-shell problems (a dropped term, a wrong sign or a transposed operand
+  in the rank update, the Cholesky or the triangular solves breaks it);
+- the Gram matrix F^T F + norm_weight I against numpy;
+- the constraint suppresses negative lobes (tests/sd/test_csd.py:69-78);
+- the host problem set-up's checks (tests/sd/test_csd.py:81-93, csd_solver.py:59-103).
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from larmor_amd import synthetic
+from larmor_amd.sd.csd import CSDModel
+from larmor_amd.sd.csd_solver import CSDSolver, csd_problem
+from larmor_amd.sd.rf import IsotropicRF, ZonalSHRF
+from larmor_amd.sd.sh import build_basis_matrix
+from larmor_amd.sd.sphere import get_dirs, get_theta_phi
+
+WEIGHT = 10.0
+# The reference test's white-matter response at b = 1000 (tests/sd/test_csd.py:36).
+RESPONSE = np.array([1800.0, -700.0, 180.0, -35.0, 5.0])
+
+
+def toy(max_iter):
+    constraint = np.zeros((2, 3))
+    constraint[0, 1] = WEIGHT
+    constraint[1, 2] = WEIGHT
+    return np.eye(3), np.eye(3), constraint, 0.0, 0.0, max_iter
+
+
+SIGNAL = np.array([[1.0, -0.5, 0.2]])
+
+
+def test_no_iteration_gives_the_linear_start():  # test_csd.cpp:33-37
+    fods, conv = oracle.fit_csd(SIGNAL, *toy(0))
+    assert conv[0]
+    np.testing.assert_array_equal(fods[0], SIGNAL[0])
+
+
+def test_negative_coefficient_penalised_and_set_settles():  # test_csd.cpp:39-46
+    fods, conv = oracle.fit_csd(SIGNAL, *toy(50))
+    assert conv[0]
+    np.testing.assert_allclose(fods[0], [1.0, -0.5 / (1.0 + WEIGHT**2), 0.2], rtol=1e-15)
+
+
+def test_running_out_of_iterations_is_reported():  # test_csd.cpp:48-52
+    assert not oracle.fit_csd(SIGNAL, *toy(1))[1][0]
+    assert oracle.fit_csd(SIGNAL, *toy(2))[1][0]
+
+
+def single_shell(n_dirs=60, seed=1, n_vox=200, snr=20.0):
+    """Signals of 1-3 fibres in the response's units on one shell, with Rician noise."""
+    rng = np.random.default_rng(seed)
+    _, bvecs = synthetic.acquisition(0, [(1000.0, n_dirs)], rng)
+    rf = ZonalSHRF(bvals=np.array([1000.0]), coefficients=RESPONSE[None])
+    signals = np.empty((n_vox, n_dirs))
+    for v in range(n_vox):
+        k = rng.integers(1, 4)
+        fibres = synthetic.random_unit(rng, k)
+        w = rng.dirichlet(np.ones(k))
+        signals[v] = rf.signal(np.full(n_dirs, 1000.0), bvecs, fibres) @ w
+    sigma = RESPONSE[0] / snr
+    re = signals + rng.normal(scale=sigma, size=signals.shape)
+    im = rng.normal(scale=sigma, size=signals.shape)
+    return bvecs, np.sqrt(re * re + im * im)
+
+
+@pytest.mark.parametrize("lmax", [4, 6, 8])
+def test_converged_fod_is_the_fixed_point(lmax):
+    bvecs, signals = single_shell()
+    p = csd_problem(bvecs, RESPONSE, lmax, CSDSolver())
+    fods, conv, solves = oracle.fit_csd(signals, p.initial, p.forward, p.constraint,
+                                        p.norm_weight, p.threshold, p.max_iter,
+                                        return_solves=True)
+    assert conv.all() and (solves >= 1).all()
+    gram = p.forward.T @ p.forward + p.norm_weight * np.eye(p.forward.shape[1])
+    for s, f in zip(signals, fods):
+        active = p.constraint[(p.constraint @ f) < p.threshold]
+        expected = np.linalg.solve(gram + active.T @ active, p.forward.T @ s)
+        np.testing.assert_allclose(f, expected, rtol=1e-9, atol=1e-9 * np.abs(expected).max())
+
+
+def test_linear_start_is_the_initial_matrix():
+    bvecs, signals = single_shell(n_vox=50)
+    p = csd_problem(bvecs, RESPONSE, 8, CSDSolver(max_iter=0))
+    fods, conv = oracle.fit_csd(signals, p.initial, p.forward, p.constraint, p.norm_weight,
+                                p.threshold, 0)
+    assert conv.all()
+    start = signals @ p.initial.T
+    np.testing.assert_allclose(fods[:, : p.initial.shape[0]], start, rtol=1e-12, atol=1e-9)
+    assert (fods[:, p.initial.shape[0]:] == 0).all()
+
+
+def test_gram_matches_numpy():
+    bvecs, _ = single_shell(n_vox=1)
+    p = csd_problem(bvecs, RESPONSE, 8, CSDSolver())
+    gram = oracle.csd_gram(p.forward, p.norm_weight)
+    expected = p.forward.T @ p.forward + p.norm_weight * np.eye(45)
+    np.testing.assert_allclose(gram, expected, rtol=1e-13, atol=1e-14 * np.abs(expected).max())
+    assert (gram == gram.T).all()
+
+
+def test_constraint_suppresses_negative_lobes():  # tests/sd/test_csd.py:69-78
+    rng = np.random.default_rng(2)
+    _, bvecs = synthetic.acquisition(0, [(1000.0, 60)], rng)
+    rf = ZonalSHRF(bvals=np.array([1000.0]), coefficients=RESPONSE[None])
+    fibres = np.array([[1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    signal = rf.signal(np.full(60, 1000.0), bvecs, fibres).mean(axis=1)
+    signal = signal + np.random.default_rng(2).normal(0, 40.0, signal.shape)
+    basis = build_basis_matrix(*get_theta_phi(get_dirs(2000)), 8)
+    out = {}
+    for it in (0, 50):
+        p = csd_problem(bvecs, RESPONSE, 8, CSDSolver(max_iter=it))
+        f, _ = oracle.fit_csd(np.maximum(signal, 0)[None], p.initial, p.forward, p.constraint,
+                              p.norm_weight, p.threshold, it)
+        out[it] = basis @ f[0]
+    neg_linear = -out[0].min() / out[0].max()
+    neg_constrained = -out[50].min() / out[50].max()
+    assert neg_constrained < neg_linear / 3
+
+
+def test_settings_that_cannot_be_solved_are_refused():  # tests/sd/test_csd.py:81-93
+    with pytest.raises(ValueError, match="non-zero"):
+        CSDSolver(directions=np.array([[0.0, 0.0, 1.0], [0.0, 0.0, 0.0]]))
+    with pytest.raises(ValueError, match="max_iter"):
+        CSDSolver(max_iter=-1)
+    with pytest.raises(ValueError, match="init_lmax"):
+        CSDSolver(init_lmax=3)
+    with pytest.raises(ValueError, match="neg_lambda"):
+        CSDSolver(neg_lambda=-1.0)
+    bvecs, _ = single_shell(n_vox=1)
+    with pytest.raises(ValueError, match="singular"):
+        csd_problem(bvecs, RESPONSE, 10, CSDSolver(norm_lambda=0.0))
+    with pytest.raises(ValueError, match="at least 6"):
+        csd_problem(bvecs[:5], RESPONSE, 8, CSDSolver())
+    with pytest.raises(ValueError, match="ZonalSHRF"):
+        CSDModel(IsotropicRF(3e-3))
+    with pytest.raises(ValueError, match="lmax"):
+        CSDModel(ZonalSHRF(bvals=np.array([1000.0]), coefficients=RESPONSE[None]), lmax=7)
