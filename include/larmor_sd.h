@@ -137,6 +137,57 @@ void lsd_csd_destroy(lsd_csd* problem);
 int lsd_fit_csd(const double* signals, int64_t n_voxels, int64_t m, const lsd_csd* problem,
                 double* fods, uint8_t* converged, int32_t* solves, void* stream);
 
+/*
+ * The sphere a peak search climbs from (host arrays): unit vertices (nv x 3, nv <= 704), the 6
+ * neighbours of each vertex (nv x 6; a vertex with 5 lists itself as the sixth) and, optionally,
+ * the tournier SH basis at the vertices (nv x n_coeffs) for searches that compute the amplitudes
+ * on the device.
+ * Replaces: the sphere argument checks and copy of peaks_chunk  cpp/src/python/sh.cpp:118-146.
+ * Errors: "a neighbour index is outside the sphere".
+ */
+typedef struct lsd_sphere lsd_sphere;
+int lsd_sphere_create(const double* vertices, int64_t nv, const int64_t* neighbours,
+                      const double* basis, int64_t n_coeffs, lsd_sphere** out);
+void lsd_sphere_destroy(lsd_sphere* sphere);
+
+/*
+ * The peaks of each SH series: coeffs (device, n_voxels x n_coeffs, tournier basis; all degrees
+ * when full) with its amplitudes at the sphere's vertices on_sphere (device, n_voxels x nv), or
+ * NULL to compute them from the sphere's basis.  Outputs (device): directions (n_voxels x
+ * n_peaks x 3) and amplitudes (n_voxels x n_peaks), largest first, NaN-padded; overflow (device
+ * int32, may be NULL) counts voxels with more than 128 distinct peaks (the rest are dropped).
+ * n_peaks <= 64; lmax <= 12 (symmetric) or 8 (full).
+ * Replaces: peaks_chunk(coeffs, lmax, on_sphere, vertices, neighbours, n_peaks,
+ *           relative_threshold, threshold, n_threads, full)  cpp/src/python/sh.cpp:112-175,
+ *           voxel_peaks cpp/src/sh/peaks.cpp:79-133, climb cpp/src/sh/peaks.cpp:33-77.
+ * Errors: "lmax must be at most 20, and even unless full, got L", "lmax L has N coefficients,
+ * got C", "n_peaks must be at least 1".
+ */
+int lsd_find_peaks(const double* coeffs, int64_t n_voxels, int64_t n_coeffs, int64_t lmax,
+                   int full, const double* on_sphere, const lsd_sphere* sphere, int64_t n_peaks,
+                   double relative_threshold, double threshold, double* directions,
+                   double* amplitudes, int32_t* overflow, void* stream);
+
+/*
+ * The value and derivatives of series i at (theta[i], phi[i]) (all device): out (n x 6) =
+ * (value, d_theta, d_phi / sin, d_theta_theta, d_theta_phi / sin, d_phi_phi / sin^2).
+ * Replaces: sh_derivatives(sh, lmax, theta, phi, full)  cpp/src/python/sh.cpp:67-83,
+ *           derivatives cpp/src/sh/harmonics.cpp:139-185 (batched over series).
+ */
+int lsd_sh_derivatives(const double* coeffs, int64_t n, int64_t n_coeffs, int64_t lmax, int full,
+                       const double* theta, const double* phi, double* out, void* stream);
+
+/*
+ * One climb per series from starts[i] (all device): directions (n x 3) and amplitudes (n), NaN
+ * where the climb did not stop within max_iters.
+ * Replaces: climb(sh, lmax, start, max_iters, tol, max_step, full)  cpp/src/python/sh.cpp:85-108
+ *           (batched over series).
+ * Errors: "max_iters must be at least 0".
+ */
+int lsd_sh_climb(const double* coeffs, int64_t n, int64_t n_coeffs, int64_t lmax, int full,
+                 const double* starts, int64_t max_iters, double tol, double max_step,
+                 double* directions, double* amplitudes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

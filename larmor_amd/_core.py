@@ -309,3 +309,144 @@ def fit_csd(signals, initial, forward, constraint, norm_weight: float, threshold
     if return_solves:
         out += (_out(solves, host),)
     return out
+
+
+# The largest lmax of the reference's compiled SH code (cpp/src/sh/harmonics.hpp MAX_LMAX).  The
+# GPU kernels hold lmax <= 12 for a symmetric series and <= 8 for a full one, and say so when
+# asked for more.
+MAX_LMAX = 20
+
+
+def _n_sh(lmax: int, full: bool) -> int:
+    return (lmax + 1) ** 2 if full else (lmax + 1) * (lmax + 2) // 2
+
+
+def _check_lmax(lmax: int, full: bool, n_coeffs: int) -> None:
+    """check_lmax (cpp/src/python/sh.cpp:41-54)."""
+    if lmax < 0 or lmax > MAX_LMAX or (not full and lmax % 2 != 0):
+        raise ValueError(f"lmax must be at most {MAX_LMAX}, and even unless full, got {lmax}")
+    expected = _n_sh(lmax, full)
+    if expected != n_coeffs:
+        raise ValueError(f"lmax {lmax} has {expected} coefficients, got {n_coeffs}")
+
+
+class Sphere:
+    """A search sphere resident on the device: vertices, neighbours and (optionally) the tournier
+    basis at the vertices for the device-side amplitudes."""
+
+    def __init__(self, vertices, neighbours, basis=None):
+        vert = np.ascontiguousarray(vertices, dtype=np.float64)
+        nb = np.ascontiguousarray(neighbours, dtype=np.int64)
+        b = None if basis is None else np.ascontiguousarray(basis, dtype=np.float64)
+        lib = _lib.load()
+        handle = ctypes.c_void_p()
+        _lib.check(lib.lsd_sphere_create(
+            _ptr(vert), vert.shape[0], nb.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            None if b is None else _ptr(b), 0 if b is None else b.shape[1], ctypes.byref(handle)))
+        self._handle = handle
+        self.n_vertices = vert.shape[0]
+        self.n_coeffs = 0 if b is None else b.shape[1]
+
+    def __del__(self):
+        _release(self, "lsd_sphere_destroy")
+
+
+def _as_device(a, torch):
+    if isinstance(a, np.ndarray):
+        return torch.from_numpy(np.ascontiguousarray(a)).to("cuda"), True
+    if a.device.type != "cuda":
+        return a.contiguous().to("cuda"), True
+    return a.contiguous(), False
+
+
+def find_peaks_device(coeffs, lmax: int, sphere: Sphere, n_peaks: int, relative_threshold: float,
+                      threshold: float, *, full: bool = False, on_sphere=None):
+    """The peaks of every row of coeffs (n, C) on the device; amplitudes on the sphere from
+    on_sphere (n, V) or, when None, from the sphere's basis.  Returns (directions (n, P, 3),
+    amplitudes (n, P), voxels whose distinct peaks overflowed)."""
+    torch = _torch()
+    dev, host = _as_device(coeffs, torch)
+    n, c = dev.shape
+    _check_lmax(lmax, full, c)
+    if n_peaks < 1:
+        raise ValueError("n_peaks must be at least 1")
+    osph = None
+    if on_sphere is not None:
+        osph, _ = _as_device(on_sphere, torch)
+    dirs = torch.empty((n, n_peaks, 3), dtype=torch.float64, device="cuda")
+    amps = torch.empty((n, n_peaks), dtype=torch.float64, device="cuda")
+    overflow = torch.zeros(1, dtype=torch.int32, device="cuda")
+    lib = _lib.load()
+    _lib.check(lib.lsd_find_peaks(
+        dev.data_ptr(), n, c, int(lmax), int(bool(full)),
+        None if osph is None else osph.data_ptr(), sphere._handle, int(n_peaks),
+        float(relative_threshold), float(threshold), dirs.data_ptr(), amps.data_ptr(),
+        overflow.data_ptr(), _stream_ptr(torch)))
+    return _out(dirs, host), _out(amps, host), int(overflow.item())
+
+
+def peaks_chunk(coeffs, lmax: int, on_sphere, vertices, neighbours, n_peaks: int,
+                relative_threshold: float, threshold: float, n_threads: int, full: bool = False):
+    """Finds the peaks of every voxel of a chunk (cpp/src/python/sh.cpp:112-198): coeffs (n, C)
+    tournier, on_sphere (n, V), vertices (V, 3), neighbours (V, 6).  Returns directions
+    (n, n_peaks, 3) and amplitudes (n, n_peaks), largest first, NaN-padded."""
+    _check_f64(coeffs, "coeffs")
+    _check_f64(on_sphere, "on_sphere")
+    if not _is_c_contiguous(coeffs) or not _is_c_contiguous(on_sphere):
+        raise ValueError("the array must be C-contiguous")
+    _check_lmax(lmax, full, coeffs.shape[1])
+    nv = vertices.shape[0]
+    if (vertices.shape[1] != 3 or neighbours.shape != (nv, 6)
+            or on_sphere.shape[0] != coeffs.shape[0] or on_sphere.shape[1] != nv):
+        def tup(t):
+            return "(" + ", ".join(str(x) for x in t) + ")"
+        raise ValueError(
+            f"shapes do not match: coeffs {tup(coeffs.shape)}, on_sphere {tup(on_sphere.shape)}, "
+            f"vertices {tup(vertices.shape)}, neighbours {tup(neighbours.shape)}"
+        )
+    nb = np.asarray(neighbours)
+    if nb.size and (nb.min() < 0 or nb.max() >= nv):
+        raise ValueError("a neighbour index is outside the sphere")
+    if n_peaks < 1:
+        raise ValueError("n_peaks must be at least 1")
+    _require_threads(n_threads)
+    sphere = Sphere(np.asarray(vertices, dtype=np.float64), nb)
+    dirs, amps, _ = find_peaks_device(coeffs, lmax, sphere, n_peaks, relative_threshold,
+                                      threshold, full=full, on_sphere=on_sphere)
+    return dirs, amps
+
+
+def sh_derivatives(sh, lmax: int, theta: float, phi: float, full: bool = False):
+    """The series' value and derivatives at (theta, phi) (cpp/src/python/sh.cpp:67-83):
+    (f, df/dtheta, df/dphi / sin, d2f/dtheta2, d2f/dtheta dphi / sin, d2f/dphi2 / sin^2)."""
+    torch = _torch()
+    s = np.ascontiguousarray(np.asarray(sh, dtype=np.float64))
+    _check_lmax(lmax, full, s.shape[0])
+    dev = torch.from_numpy(s[None]).to("cuda")
+    th = torch.tensor([float(theta)], dtype=torch.float64, device="cuda")
+    ph = torch.tensor([float(phi)], dtype=torch.float64, device="cuda")
+    out = torch.empty((1, 6), dtype=torch.float64, device="cuda")
+    _lib.check(_lib.load().lsd_sh_derivatives(dev.data_ptr(), 1, s.shape[0], int(lmax),
+                                              int(bool(full)), th.data_ptr(), ph.data_ptr(),
+                                              out.data_ptr(), _stream_ptr(torch)))
+    return tuple(float(x) for x in out.cpu().numpy()[0])
+
+
+def climb(sh, lmax: int, start, max_iters: int = 50, tol: float = 1e-4, max_step: float = 0.2,
+          full: bool = False):
+    """Climbs from start to the nearest peak (cpp/src/python/sh.cpp:85-108): the peak's
+    direction (3,) and amplitude, both NaN if the climb did not stop."""
+    torch = _torch()
+    s = np.ascontiguousarray(np.asarray(sh, dtype=np.float64))
+    _check_lmax(lmax, full, s.shape[0])
+    if max_iters < 0:
+        raise ValueError("max_iters must be at least 0")
+    dev = torch.from_numpy(s[None]).to("cuda")
+    st = torch.tensor(np.asarray(start, dtype=np.float64).reshape(1, 3), device="cuda")
+    d = torch.empty((1, 3), dtype=torch.float64, device="cuda")
+    a = torch.empty(1, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.load().lsd_sh_climb(dev.data_ptr(), 1, s.shape[0], int(lmax), int(bool(full)),
+                                        st.data_ptr(), int(max_iters), float(tol),
+                                        float(max_step), d.data_ptr(), a.data_ptr(),
+                                        _stream_ptr(torch)))
+    return d.cpu().numpy()[0], float(a.item())

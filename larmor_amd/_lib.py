@@ -57,6 +57,25 @@ SIGNATURES = {
     "lsd_fit_csd": (
         ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]
     ),
+    "lsd_sphere_create": (
+        ctypes.c_int,
+        [_d, ctypes.c_int64, _i64, _d, ctypes.c_int64, ctypes.POINTER(_vp)],
+    ),
+    "lsd_sphere_destroy": (None, [_vp]),
+    "lsd_find_peaks": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, _vp, _vp,
+         ctypes.c_int64, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp],
+    ),
+    "lsd_sh_derivatives": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, _vp, _vp, _vp, _vp],
+    ),
+    "lsd_sh_climb": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, _vp, ctypes.c_int64,
+         ctypes.c_double, ctypes.c_double, _vp, _vp, _vp],
+    ),
 }
 
 

@@ -50,12 +50,6 @@ constexpr int csd_lds_doubles(int n) { return n * (n + 1) / 2; }
 // Offset of column i in the packed lower triangle (column-major: column i holds rows i..n-1).
 __device__ __forceinline__ int packed_column(int n, int i) { return i * n - (i * (i - 1)) / 2; }
 
-// Orders this wave's LDS accesses across lanes: writes before it are seen by reads after it.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
 template <int N, int NK>
 __global__ void __launch_bounds__(WAVE * 4) csd_kernel(CsdArgs A) {
     static_assert(N <= WAVE, "one system row per lane");

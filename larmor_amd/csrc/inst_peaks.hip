@@ -1,0 +1,85 @@
+// Instantiations and launchers of the peak search, derivative and climb kernels: symmetric
+// series of even lmax 0-12 and full series of lmax 1-8.
+#include "kernels.hpp"
+#include "peaks_wave.hpp"
+
+namespace lsd {
+
+namespace {
+
+template <int LMAX, bool FULL>
+int launch_peaks_t(const PeaksArgs& a, int waves, hipStream_t stream) {
+    const size_t lds = static_cast<size_t>(peaks_lds_bytes_per_wave()) * WAVES_PER_BLOCK;
+    return launch("peaks", peaks_kernel<LMAX, FULL>, a, waves, lds, stream);
+}
+
+template <int LMAX, bool FULL>
+int launch_derivs_t(const double* coeffs, long n, int nc, const double* theta, const double* phi,
+                    double* out, ShConstantsDev k, hipStream_t stream) {
+    const int threads = WAVE * WAVES_PER_BLOCK;
+    const long blocks = (n + threads - 1) / threads;
+    hipLaunchKernelGGL((sh_derivatives_kernel<LMAX, FULL>), dim3(blocks), dim3(threads), 0, stream,
+                       coeffs, n, nc, theta, phi, out, k);
+    LSD_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int LMAX, bool FULL>
+int launch_climb_t(const double* coeffs, long n, int nc, const double* starts, int max_iters,
+                   double tol, double max_step, double* dirs, double* amps, ShConstantsDev k,
+                   hipStream_t stream) {
+    const int threads = WAVE * WAVES_PER_BLOCK;
+    const long blocks = (n + threads - 1) / threads;
+    hipLaunchKernelGGL((sh_climb_kernel<LMAX, FULL>), dim3(blocks), dim3(threads), 0, stream,
+                       coeffs, n, nc, starts, max_iters, tol, max_step, dirs, amps, k);
+    LSD_HIP(hipGetLastError());
+    return 0;
+}
+
+// X(LMAX, FULL) over every instantiated layout.
+#define LSD_FOR_EACH_SH(X)                                                                     \
+    X(0, false) X(2, false) X(4, false) X(6, false) X(8, false) X(10, false) X(12, false)        \
+    X(1, true) X(2, true) X(3, true) X(4, true) X(5, true) X(6, true) X(7, true) X(8, true)
+
+} // namespace
+
+bool sh_supported(int lmax, bool full) {
+#define LSD_TRY(L, F)                                                                          \
+    if (lmax == L && full == F) return true;
+    LSD_FOR_EACH_SH(LSD_TRY)
+#undef LSD_TRY
+    return false;
+}
+
+int launch_peaks(int lmax, bool full, const PeaksArgs& a, int waves, hipStream_t stream) {
+#define LSD_TRY(L, F)                                                                          \
+    if (lmax == L && full == F) return launch_peaks_t<L, F>(a, waves, stream);
+    LSD_FOR_EACH_SH(LSD_TRY)
+#undef LSD_TRY
+    return fail("no peak kernel for lmax " + std::to_string(lmax));
+}
+
+int launch_sh_derivatives(int lmax, bool full, const double* coeffs, long n, int nc,
+                          const double* theta, const double* phi, double* out, ShConstantsDev k,
+                          hipStream_t stream) {
+#define LSD_TRY(L, F)                                                                          \
+    if (lmax == L && full == F)                                                                \
+        return launch_derivs_t<L, F>(coeffs, n, nc, theta, phi, out, k, stream);
+    LSD_FOR_EACH_SH(LSD_TRY)
+#undef LSD_TRY
+    return fail("no derivative kernel for lmax " + std::to_string(lmax));
+}
+
+int launch_sh_climb(int lmax, bool full, const double* coeffs, long n, int nc,
+                    const double* starts, int max_iters, double tol, double max_step,
+                    double* dirs, double* amps, ShConstantsDev k, hipStream_t stream) {
+#define LSD_TRY(L, F)                                                                          \
+    if (lmax == L && full == F)                                                                \
+        return launch_climb_t<L, F>(coeffs, n, nc, starts, max_iters, tol, max_step, dirs,     \
+                                    amps, k, stream);
+    LSD_FOR_EACH_SH(LSD_TRY)
+#undef LSD_TRY
+    return fail("no climb kernel for lmax " + std::to_string(lmax));
+}
+
+} // namespace lsd

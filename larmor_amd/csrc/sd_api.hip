@@ -5,6 +5,11 @@
 
 #include "csd_wave.hpp"
 #include "kernels.hpp"
+#include "peaks_wave.hpp"
+
+#include <cmath>
+#include <map>
+#include <mutex>
 
 namespace lsd {
 
@@ -173,7 +178,98 @@ int csd_direction_slots(int64_t k);
 bool csd_supported_coefficients(int64_t c);
 int launch_csd(int n, int nk, const CsdArgs& args, int waves, hipStream_t stream);
 
+// Peak, derivative and climb launchers (inst_peaks.hip).
+bool sh_supported(int lmax, bool full);
+int launch_peaks(int lmax, bool full, const PeaksArgs& a, int waves, hipStream_t stream);
+int launch_sh_derivatives(int lmax, bool full, const double* coeffs, long n, int nc,
+                          const double* theta, const double* phi, double* out, ShConstantsDev k,
+                          hipStream_t stream);
+int launch_sh_climb(int lmax, bool full, const double* coeffs, long n, int nc,
+                    const double* starts, int max_iters, double tol, double max_step,
+                    double* dirs, double* amps, ShConstantsDev k, hipStream_t stream);
+
+constexpr int SH_MAX_LMAX = 20;
+
+// The Legendre recurrence and ladder constants up to lmax 20 on the current device, made once per
+// device with the expressions of harmonics.cpp (RecurrenceConstants, up, down) and kept for the
+// life of the process.
+int sh_constants(ShConstantsDev* out) {
+    static std::mutex mu;
+    static std::map<int, ShConstantsDev> per_device;
+    int dev = 0;
+    LSD_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = per_device.find(dev);
+    if (it != per_device.end()) {
+        *out = it->second;
+        return 0;
+    }
+    constexpr int T = SH_TABLE;
+    std::vector<double> table(static_cast<size_t>(T + T * T + T * T + T * T + T), 0.0);
+    double* diagonal = table.data();
+    double* factor = diagonal + T;
+    double* up = factor + T * T;
+    double* down = up + T * T;
+    double* offdiag = down + T * T;
+    double ratio = 1.0;
+    for (int m = 0; m <= SH_MAX_LMAX; ++m) {
+        const double mf = static_cast<double>(m);
+        if (m > 0) {
+            ratio = ratio * ((2.0 * mf - 1.0) / (2.0 * mf));
+        }
+        const double sign = m % 2 == 1 ? -1.0 : 1.0;
+        diagonal[m] = sign * std::sqrt((2.0 * mf + 1.0) / (4.0 * M_PI) * ratio);
+        offdiag[m] = std::sqrt(2.0 * mf + 3.0);
+        for (int l = 0; l <= SH_MAX_LMAX; ++l) {
+            const double lf = static_cast<double>(l);
+            factor[l * T + m] = l > m ? std::sqrt((4.0 * lf * lf - 1.0) / (lf * lf - mf * mf)) : 0.0;
+            if (m <= l) {
+                up[l * T + m] = std::sqrt(static_cast<double>((l - m) * (l + m + 1)));
+                down[l * T + m] = std::sqrt(static_cast<double>((l + m) * (l - m + 1)));
+            }
+        }
+    }
+    double* d = nullptr;
+    const int rc = upload(&d, table);
+    if (rc) return rc;
+    ShConstantsDev k{d, d + T, d + T + T * T, d + T + 2 * T * T, d + T + 3 * T * T};
+    per_device[dev] = k;
+    *out = k;
+    return 0;
+}
+
+int n_sh_coeffs(int64_t lmax, bool full) {
+    return static_cast<int>(full ? (lmax + 1) * (lmax + 1) : (lmax + 1) * (lmax + 2) / 2);
+}
+
+// check_lmax (cpp/src/python/sh.cpp:41-54), then the GPU's own limits.
+int check_sh_layout(int64_t lmax, bool full, int64_t n_coeffs) {
+    if (lmax < 0 || lmax > SH_MAX_LMAX || (!full && lmax % 2 != 0)) {
+        return fail("lmax must be at most " + std::to_string(SH_MAX_LMAX) +
+                    ", and even unless full, got " + std::to_string(lmax));
+    }
+    const int expected = n_sh_coeffs(lmax, full);
+    if (expected != n_coeffs) {
+        return fail("lmax " + std::to_string(lmax) + " has " + std::to_string(expected) +
+                    " coefficients, got " + std::to_string(n_coeffs));
+    }
+    if (!sh_supported(static_cast<int>(lmax), full)) {
+        return fail("lmax " + std::to_string(lmax) +
+                    " is above what the GPU peak search holds (12 for a symmetric series, 8 "
+                    "for a full one)");
+    }
+    return 0;
+}
+
 } // namespace lsd
+
+struct lsd_sphere {
+    int64_t nv = 0;
+    int64_t n_coeffs = 0;          // columns of the basis, 0 without one
+    double* d_vertices = nullptr;  // (nv, 3)
+    int* d_neighbours = nullptr;   // (nv, 6)
+    double* d_basis_t = nullptr;   // (n_coeffs, nv)
+};
 
 struct lsd_csd {
     int64_t m = 0;
@@ -658,6 +754,141 @@ int lsd_fit_csd(const double* signals, int64_t n_voxels, int64_t m, const lsd_cs
     a.solves = solves;
     return launch_csd(static_cast<int>(p->c), csd_direction_slots(p->k), a,
                       main_waves(n_voxels), stream);
+}
+
+int lsd_sphere_create(const double* vertices, int64_t nv, const int64_t* neighbours,
+                      const double* basis, int64_t n_coeffs, lsd_sphere** out) {
+    if (nv < 0 || nv > PEAK_VERTEX_SLOTS * WAVE) {
+        return fail("the GPU peak search takes spheres of at most " +
+                    std::to_string(PEAK_VERTEX_SLOTS * WAVE) + " vertices, got " +
+                    std::to_string(nv));
+    }
+    std::vector<int> nb(static_cast<size_t>(nv * 6));
+    for (int64_t i = 0; i < nv * 6; ++i) {
+        if (neighbours[i] < 0 || neighbours[i] >= nv) {
+            return fail("a neighbour index is outside the sphere");
+        }
+        nb[i] = static_cast<int>(neighbours[i]);
+    }
+    auto* sp = new lsd_sphere();
+    sp->nv = nv;
+    sp->n_coeffs = basis ? n_coeffs : 0;
+    int rc = upload(&sp->d_vertices, std::vector<double>(vertices, vertices + nv * 3));
+    if (!rc) {
+        const hipError_t e = hipMalloc(reinterpret_cast<void**>(&sp->d_neighbours),
+                                       std::max<size_t>(4, nb.size() * sizeof(int)));
+        rc = e == hipSuccess
+                 ? 0
+                 : hip_fail(e, "hipMalloc(neighbours)");
+        if (!rc && !nb.empty()) {
+            const hipError_t e2 = hipMemcpy(sp->d_neighbours, nb.data(), nb.size() * sizeof(int),
+                                            hipMemcpyHostToDevice);
+            rc = e2 == hipSuccess ? 0 : hip_fail(e2, "hipMemcpy(neighbours)");
+        }
+    }
+    if (!rc && basis) {
+        std::vector<double> bt(static_cast<size_t>(nv * n_coeffs));
+        for (int64_t d = 0; d < nv; ++d) {
+            for (int64_t q = 0; q < n_coeffs; ++q) {
+                bt[q * nv + d] = basis[d * n_coeffs + q];
+            }
+        }
+        rc = upload(&sp->d_basis_t, bt);
+    }
+    if (rc) {
+        lsd_sphere_destroy(sp);
+        return rc;
+    }
+    *out = sp;
+    return 0;
+}
+
+void lsd_sphere_destroy(lsd_sphere* sp) {
+    if (!sp) return;
+    if (sp->d_vertices) (void)hipFree(sp->d_vertices);
+    if (sp->d_neighbours) (void)hipFree(sp->d_neighbours);
+    if (sp->d_basis_t) (void)hipFree(sp->d_basis_t);
+    delete sp;
+}
+
+int lsd_find_peaks(const double* coeffs, int64_t n_voxels, int64_t n_coeffs, int64_t lmax,
+                   int full, const double* on_sphere, const lsd_sphere* sphere, int64_t n_peaks,
+                   double relative_threshold, double threshold, double* directions,
+                   double* amplitudes, int32_t* overflow, void* stream_) {
+    auto stream = static_cast<hipStream_t>(stream_);
+    int rc = check_sh_layout(lmax, full != 0, n_coeffs);
+    if (rc) return rc;
+    if (n_peaks < 1) {
+        return fail("n_peaks must be at least 1");
+    }
+    if (n_peaks > WAVE) {
+        return fail("the GPU peak search keeps at most 64 peaks per voxel, got " +
+                    std::to_string(n_peaks));
+    }
+    if (on_sphere == nullptr && sphere->n_coeffs != n_coeffs) {
+        return fail("the sphere has no basis for " + std::to_string(n_coeffs) +
+                    " coefficients; pass on_sphere or create it with one");
+    }
+    if (n_voxels == 0) {
+        return 0;
+    }
+    ShConstantsDev k{};
+    rc = sh_constants(&k);
+    if (rc) return rc;
+    StreamAllocs mem(stream);
+    int* counter = overflow;
+    if (!counter) {
+        rc = mem.get(&counter, 1);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, stream, counter, 1);
+    PeaksArgs a{};
+    a.coeffs = coeffs;
+    a.n_vox = n_voxels;
+    a.n_coeffs = static_cast<int>(n_coeffs);
+    a.on_sphere = on_sphere;
+    a.basis_t = sphere->d_basis_t;
+    a.vertices = sphere->d_vertices;
+    a.neighbours = sphere->d_neighbours;
+    a.nv = static_cast<int>(sphere->nv);
+    a.n_peaks = static_cast<int>(n_peaks);
+    a.relative_threshold = relative_threshold;
+    a.threshold = threshold;
+    a.k = k;
+    a.directions = directions;
+    a.amplitudes = amplitudes;
+    a.overflow = counter;
+    return launch_peaks(static_cast<int>(lmax), full != 0, a, main_waves(n_voxels), stream);
+}
+
+int lsd_sh_derivatives(const double* coeffs, int64_t n, int64_t n_coeffs, int64_t lmax, int full,
+                       const double* theta, const double* phi, double* out, void* stream_) {
+    int rc = check_sh_layout(lmax, full != 0, n_coeffs);
+    if (rc) return rc;
+    if (n == 0) return 0;
+    ShConstantsDev k{};
+    rc = sh_constants(&k);
+    if (rc) return rc;
+    return launch_sh_derivatives(static_cast<int>(lmax), full != 0, coeffs, n,
+                                 static_cast<int>(n_coeffs), theta, phi, out, k,
+                                 static_cast<hipStream_t>(stream_));
+}
+
+int lsd_sh_climb(const double* coeffs, int64_t n, int64_t n_coeffs, int64_t lmax, int full,
+                 const double* starts, int64_t max_iters, double tol, double max_step,
+                 double* directions, double* amplitudes, void* stream_) {
+    int rc = check_sh_layout(lmax, full != 0, n_coeffs);
+    if (rc) return rc;
+    if (max_iters < 0) {
+        return fail("max_iters must be at least 0");
+    }
+    if (n == 0) return 0;
+    ShConstantsDev k{};
+    rc = sh_constants(&k);
+    if (rc) return rc;
+    return launch_sh_climb(static_cast<int>(lmax), full != 0, coeffs, n,
+                           static_cast<int>(n_coeffs), starts, static_cast<int>(max_iters), tol,
+                           max_step, directions, amplitudes, k, static_cast<hipStream_t>(stream_));
 }
 
 } // extern "C"

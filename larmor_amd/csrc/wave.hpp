@@ -44,6 +44,12 @@ __device__ __forceinline__ int gather(int v, int src) { return __shfl(v, src); }
 
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
 
+// Orders this wave's LDS accesses across lanes: writes before it are seen by reads after it.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // The canonical butterfly of 64 lane partials; every lane gets the same sum.
 __device__ __forceinline__ double butterfly_sum(double p) {
 #pragma unroll

@@ -64,6 +64,14 @@ def _load():
                                    ctypes.c_double, ctypes.c_int64, ctypes.c_int, _d,
                                    ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_int32)]
     lib.oracle_csd_gram.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, _d]
+    lib.oracle_sh_derivatives.argtypes = [_d, ctypes.c_int64, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double, _d]
+    lib.oracle_sh_derivatives.restype = None
+    lib.oracle_climb.argtypes = [_d, ctypes.c_int64, ctypes.c_int, _d, ctypes.c_int64,
+                                 ctypes.c_double, ctypes.c_double, _d, _d]
+    lib.oracle_peaks_chunk.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                       ctypes.c_int, _d, ctypes.c_int64, _d, _i64, ctypes.c_int64,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, _d, _d]
     _lib = lib
     return lib
 
@@ -223,4 +231,47 @@ def csd_gram(forward, norm_weight):
     m, c = fw.shape
     out = np.zeros((c, c))
     _check(_load().oracle_csd_gram(_p(fw), m, c, float(norm_weight), _p(out)))
+    return out
+
+
+def sh_derivatives(sh, lmax, theta, phi, full=False):
+    """(value, d_theta, d_phi / sin, d_theta_theta, d_theta_phi / sin, d_phi_phi / sin^2)."""
+    c = _c(sh)
+    out = np.zeros(6)
+    _load().oracle_sh_derivatives(_p(c), int(lmax), int(full), float(theta), float(phi), _p(out))
+    return tuple(out)
+
+
+def climb(sh, lmax, start, max_iters=50, tol=1e-4, max_step=0.2, full=False):
+    c = _c(sh)
+    st = _c(start)
+    d = np.zeros(3)
+    a = np.zeros(1)
+    _load().oracle_climb(_p(c), int(lmax), int(full), _p(st), int(max_iters), float(tol),
+                         float(max_step), _p(d), _p(a))
+    return d, float(a[0])
+
+
+def peaks_chunk(coeffs, lmax, on_sphere, vertices, neighbours, n_peaks, relative_threshold,
+                threshold, n_threads=None, full=False):
+    c, o, v = _c(coeffs), _c(on_sphere), _c(vertices)
+    nb = _c(neighbours, np.int64)
+    n = c.shape[0]
+    dirs = np.zeros((n, n_peaks, 3))
+    amps = np.zeros((n, n_peaks))
+    _check(_load().oracle_peaks_chunk(
+        _p(c), n, c.shape[1], int(lmax), int(full), _p(o), v.shape[0], _p(v),
+        nb.ctypes.data_as(_i64), int(n_peaks), float(relative_threshold), float(threshold),
+        n_threads or os.cpu_count() or 1, _p(dirs), _p(amps)))
+    return dirs, amps
+
+
+def sphere_amplitudes(coeffs, basis):
+    """Amplitudes (n, V) = coeffs (n, C) at the basis rows (V, C), each a sum over the
+    coefficients in order (the order of the device's own amplitude evaluation)."""
+    c = _c(coeffs)
+    b = _c(basis)
+    out = np.zeros((c.shape[0], b.shape[0]))
+    for q in range(c.shape[1]):
+        out = out + b[None, :, q] * c[:, q:q + 1]
     return out
