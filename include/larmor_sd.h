@@ -188,6 +188,44 @@ int lsd_sh_climb(const double* coeffs, int64_t n, int64_t n_coeffs, int64_t lmax
                  const double* starts, int64_t max_iters, double tol, double max_step,
                  double* directions, double* amplitudes, void* stream);
 
+/*
+ * A least-squares problem with inequality constraints, min ||a x - s||^2 + ridge^2 ||x||^2
+ * subject to g x >= 0, prepared for many data vectors (host arrays a (m x n), g (k x g_cols)):
+ * the column-pivoted QR of [a; ridge I] and the least-distance form E = g P R^-1 (unit rows),
+ * E E^T, made on the host and kept on the device.  n <= 63, k <= 512.
+ * Replaces: LsiDesign(a, g, ridge)  cpp/src/python/sd.cpp:360-366,
+ *           make_lsi_design cpp/src/sd/lsi.cpp:19-69.
+ * Errors: "the constraints have C columns but the design has N", "the ridge must be at least 0",
+ * "the design does not have full column rank: the measurements cannot determine all N
+ * parameters".
+ */
+typedef struct lsd_lsi lsd_lsi;
+int lsd_lsi_create(const double* a, int64_t m, int64_t n, const double* g, int64_t k,
+                   int64_t g_cols, double ridge, lsd_lsi** out);
+void lsd_lsi_destroy(lsd_lsi* design);
+int64_t lsd_lsi_n_meas(const lsd_lsi* design);
+int64_t lsd_lsi_n_params(const lsd_lsi* design);
+int64_t lsd_lsi_n_constraints(const lsd_lsi* design);
+
+/*
+ * The constrained solution of each voxel: signals (device, n_voxels x m) -> x (device,
+ * n_voxels x n), converged (device uint8: 0 when the NNLS stopped at its inner-step limit) and,
+ * when not NULL, status (device int32: LSD_STATUS_QR_FALLBACK when the QR-based NNLS finished
+ * the voxel).
+ * Replaces: fit_lsi(signals, design, n_threads)  cpp/src/python/sd.cpp:368-394,
+ *           solve_lsi cpp/src/sd/lsi.cpp:245-296.
+ * Errors: "signals (r, c) do not have the design's M measurements".
+ */
+int lsd_fit_lsi(const double* signals, int64_t n_voxels, int64_t m, const lsd_lsi* design,
+                double* x, uint8_t* converged, int32_t* status, void* stream);
+
+/*
+ * lsd_fit_lsi with the QR-based NNLS for every voxel.
+ * Replaces: solve_lsi_by_qr cpp/src/sd/lsi.cpp:298-300 (the fallback, on its own).
+ */
+int lsd_fit_lsi_by_qr(const double* signals, int64_t n_voxels, int64_t m, const lsd_lsi* design,
+                      double* x, uint8_t* converged, int32_t* status, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -450,3 +450,50 @@ def climb(sh, lmax: int, start, max_iters: int = 50, tol: float = 1e-4, max_step
                                         float(max_step), d.data_ptr(), a.data_ptr(),
                                         _stream_ptr(torch)))
     return d.cpu().numpy()[0], float(a.item())
+
+
+class LsiDesign:
+    """min ||a x - s||^2 + ridge^2 ||x||^2 subject to g x >= 0, prepared once on the host and kept
+    on the device (cpp/src/python/sd.cpp:360-366, 521-532)."""
+
+    def __init__(self, a, g, ridge: float):
+        am = _host_matrix(a, "a")
+        gm = _host_matrix(g, "g")
+        lib = _lib.load()
+        handle = ctypes.c_void_p()
+        _lib.check(lib.lsd_lsi_create(_ptr(am), am.shape[0], am.shape[1], _ptr(gm), gm.shape[0],
+                                      gm.shape[1], float(ridge), ctypes.byref(handle)))
+        self._handle = handle
+        self.n_meas = am.shape[0]
+        self.n_params = am.shape[1]
+        self.n_constraints = gm.shape[0]
+
+    def __del__(self):
+        _release(self, "lsd_lsi_destroy")
+
+
+def fit_lsi(signals, design: LsiDesign, n_threads: int, *, return_status: bool = False,
+            only_qr: bool = False):
+    """The constrained solution (v, n) of every row of signals (v, m) and whether each voxel's
+    NNLS converged (v,) (sd.cpp:368-394, 534-542).  With return_status also the per-voxel
+    status (LSD_STATUS_QR_FALLBACK where the QR-based NNLS finished the voxel); only_qr runs the
+    QR-based NNLS for every voxel (the reference's solve_lsi_by_qr)."""
+    torch = _torch()
+    dev, host = _device_signals(signals, torch)
+    v, m = dev.shape
+    if m != design.n_meas:
+        raise ValueError(
+            f"signals ({v}, {m}) do not have the design's {design.n_meas} measurements"
+        )
+    _require_threads(n_threads)
+    x = torch.empty((v, design.n_params), dtype=torch.float64, device="cuda")
+    converged = torch.zeros(v, dtype=torch.uint8, device="cuda")
+    status = torch.zeros(v, dtype=torch.int32, device="cuda")
+    lib = _lib.load()
+    fn = lib.lsd_fit_lsi_by_qr if only_qr else lib.lsd_fit_lsi
+    _lib.check(fn(dev.data_ptr(), v, m, design._handle, x.data_ptr(), converged.data_ptr(),
+                  status.data_ptr(), _stream_ptr(torch)))
+    out = (_out(x, host), _out(converged.to(torch.bool), host))
+    if return_status:
+        out += (_out(status, host),)
+    return out

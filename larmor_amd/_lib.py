@@ -76,6 +76,21 @@ SIGNATURES = {
         [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int, _vp, ctypes.c_int64,
          ctypes.c_double, ctypes.c_double, _vp, _vp, _vp],
     ),
+    "lsd_lsi_create": (
+        ctypes.c_int,
+        [_d, ctypes.c_int64, ctypes.c_int64, _d, ctypes.c_int64, ctypes.c_int64, ctypes.c_double,
+         ctypes.POINTER(_vp)],
+    ),
+    "lsd_lsi_destroy": (None, [_vp]),
+    "lsd_lsi_n_meas": (ctypes.c_int64, [_vp]),
+    "lsd_lsi_n_params": (ctypes.c_int64, [_vp]),
+    "lsd_lsi_n_constraints": (ctypes.c_int64, [_vp]),
+    "lsd_fit_lsi": (
+        ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]
+    ),
+    "lsd_fit_lsi_by_qr": (
+        ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]
+    ),
 }
 
 

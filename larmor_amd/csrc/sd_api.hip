@@ -5,6 +5,8 @@
 
 #include "csd_wave.hpp"
 #include "kernels.hpp"
+#include "lsi_host.hpp"
+#include "lsi_wave.hpp"
 #include "peaks_wave.hpp"
 
 #include <cmath>
@@ -178,6 +180,15 @@ int csd_direction_slots(int64_t k);
 bool csd_supported_coefficients(int64_t c);
 int launch_csd(int n, int nk, const CsdArgs& args, int waves, hipStream_t stream);
 
+// LSI launchers (inst_lsi.hip).
+int lsi_constraint_slots(int64_t k);
+int launch_lsi(int nks, const LsiArgs& fast, const LsiArgs& qr, int waves, int qr_waves,
+               hipStream_t stream);
+// Waves of the LSI QR path, and the doubles of its per-wave column slab.
+constexpr int LSI_QR_WAVES = 128;
+constexpr long LSI_QR_SLAB = static_cast<long>(LSI_QR_ROWS) * WAVE * LSI_QR_COLS;
+constexpr long LSI_FACTOR_SLAB = static_cast<long>(LSI_KCAP + 1) * LSI_KCAP;
+
 // Peak, derivative and climb launchers (inst_peaks.hip).
 bool sh_supported(int lmax, bool full);
 int launch_peaks(int lmax, bool full, const PeaksArgs& a, int waves, hipStream_t stream);
@@ -269,6 +280,16 @@ struct lsd_sphere {
     double* d_vertices = nullptr;  // (nv, 3)
     int* d_neighbours = nullptr;   // (nv, 6)
     double* d_basis_t = nullptr;   // (n_coeffs, nv)
+};
+
+struct lsd_lsi {
+    int64_t m = 0, n = 0, k = 0;
+    double* d_q_t = nullptr;
+    double* d_back = nullptr;
+    double* d_e = nullptr;
+    double* d_e_t = nullptr;
+    double* d_gram = nullptr;
+    double* d_abs_sums = nullptr;
 };
 
 struct lsd_csd {
@@ -889,6 +910,112 @@ int lsd_sh_climb(const double* coeffs, int64_t n, int64_t n_coeffs, int64_t lmax
     return launch_sh_climb(static_cast<int>(lmax), full != 0, coeffs, n,
                            static_cast<int>(n_coeffs), starts, static_cast<int>(max_iters), tol,
                            max_step, directions, amplitudes, k, static_cast<hipStream_t>(stream_));
+}
+
+int lsd_lsi_create(const double* a, int64_t m, int64_t n, const double* g, int64_t k,
+                   int64_t g_cols, double ridge, lsd_lsi** out) {
+    host::LsiHostDesign d;
+    const std::string err = host::make_lsi_design(a, m, n, g, k, g_cols, ridge, d);
+    if (!err.empty()) {
+        return fail(err);
+    }
+    if (n > WAVE - 1) {
+        return fail("the GPU LSI solve takes at most 63 parameters, got " + std::to_string(n));
+    }
+    if (lsi_constraint_slots(k) < 0) {
+        return fail("the GPU LSI solve takes at most 512 constraints, got " + std::to_string(k));
+    }
+    auto* p = new lsd_lsi();
+    p->m = m;
+    p->n = n;
+    p->k = k;
+    int rc = upload(&p->d_q_t, d.q_t);
+    if (!rc) rc = upload(&p->d_back, d.back);
+    if (!rc) rc = upload(&p->d_e, d.e);
+    if (!rc) rc = upload(&p->d_e_t, d.e_t);
+    if (!rc) rc = upload(&p->d_gram, d.gram);
+    if (!rc) rc = upload(&p->d_abs_sums, d.abs_sums);
+    if (rc) {
+        lsd_lsi_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return 0;
+}
+
+void lsd_lsi_destroy(lsd_lsi* p) {
+    if (!p) return;
+    for (double* d : {p->d_q_t, p->d_back, p->d_e, p->d_e_t, p->d_gram, p->d_abs_sums}) {
+        if (d) (void)hipFree(d);
+    }
+    delete p;
+}
+
+int64_t lsd_lsi_n_meas(const lsd_lsi* p) { return p->m; }
+int64_t lsd_lsi_n_params(const lsd_lsi* p) { return p->n; }
+int64_t lsd_lsi_n_constraints(const lsd_lsi* p) { return p->k; }
+
+static int fit_lsi_impl(const double* signals, int64_t n_voxels, int64_t m, const lsd_lsi* p,
+                        double* x, uint8_t* converged, int32_t* status, bool only_qr,
+                        void* stream_);
+
+int lsd_fit_lsi(const double* signals, int64_t n_voxels, int64_t m, const lsd_lsi* p, double* x,
+                uint8_t* converged, int32_t* status, void* stream_) {
+    return fit_lsi_impl(signals, n_voxels, m, p, x, converged, status, false, stream_);
+}
+
+int lsd_fit_lsi_by_qr(const double* signals, int64_t n_voxels, int64_t m, const lsd_lsi* p,
+                      double* x, uint8_t* converged, int32_t* status, void* stream_) {
+    return fit_lsi_impl(signals, n_voxels, m, p, x, converged, status, true, stream_);
+}
+
+static int fit_lsi_impl(const double* signals, int64_t n_voxels, int64_t m, const lsd_lsi* p,
+                        double* x, uint8_t* converged, int32_t* status, bool only_qr,
+                        void* stream_) {
+    auto stream = static_cast<hipStream_t>(stream_);
+    if (m != p->m) {
+        return fail("signals " + shape_tuple(n_voxels, m) + " do not have the design's " +
+                    std::to_string(p->m) + " measurements");
+    }
+    if (n_voxels == 0) {
+        return 0;
+    }
+    const int waves = main_waves(n_voxels);
+    const int qr_waves = std::min<int>(LSI_QR_WAVES, waves);
+    StreamAllocs mem(stream);
+    int* counter = nullptr;
+    int* list = nullptr;
+    double* factor = nullptr;
+    double* qr = nullptr;
+    int rc = mem.get(&counter, 1);
+    if (!rc) rc = mem.get(&list, n_voxels);
+    if (!rc) rc = mem.get(&factor, LSI_FACTOR_SLAB * waves);
+    if (!rc) rc = mem.get(&qr, LSI_QR_SLAB * qr_waves);
+    if (!rc && status == nullptr) rc = mem.get(&status, n_voxels);
+    if (rc) return rc;
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, stream, counter, 1);
+    LsiArgs a{};
+    a.signals = signals;
+    a.n_vox = n_voxels;
+    a.d = LsiDev{static_cast<int>(p->m), static_cast<int>(p->n), static_cast<int>(p->k),
+                 p->d_q_t, p->d_back, p->d_e, p->d_e_t, p->d_gram, p->d_abs_sums};
+    a.x_out = x;
+    a.converged = converged;
+    a.status = status;
+    a.fallback_list = list;
+    a.fallback_count = counter;
+    a.scratch = factor;
+    a.per_wave = LSI_FACTOR_SLAB;
+    LsiArgs q = a;
+    q.list = only_qr ? nullptr : list;
+    q.list_count = only_qr ? nullptr : counter;
+    q.fallback_list = nullptr;
+    q.fallback_count = nullptr;
+    q.scratch = qr;
+    q.per_wave = LSI_QR_SLAB;
+    // QR only: the fast kernel gets no voxels, the QR kernel all of them.
+    if (only_qr) a.n_vox = 0;
+    return launch_lsi(lsi_constraint_slots(p->k), a, q, waves, qr_waves, stream);
 }
 
 } // extern "C"

@@ -64,6 +64,13 @@ def _load():
                                    ctypes.c_double, ctypes.c_int64, ctypes.c_int, _d,
                                    ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_int32)]
     lib.oracle_csd_gram.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, _d]
+    _u8 = ctypes.POINTER(ctypes.c_uint8)
+    lib.oracle_fit_lsi.argtypes = [_d, ctypes.c_int64, _d, ctypes.c_int64, ctypes.c_int64, _d,
+                                   ctypes.c_int64, ctypes.c_double, ctypes.c_int, _d, _u8, _i64]
+    lib.oracle_lsi_design.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, _d, ctypes.c_int64,
+                                      ctypes.c_double, _d, _d, _d, _d, _d]
+    lib.oracle_solve_lsi_by_qr.argtypes = [_d, _d, ctypes.c_int64, ctypes.c_int64, _d,
+                                           ctypes.c_int64, ctypes.c_double, _d, _u8]
     lib.oracle_sh_derivatives.argtypes = [_d, ctypes.c_int64, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_double, _d]
     lib.oracle_sh_derivatives.restype = None
@@ -275,3 +282,42 @@ def sphere_amplitudes(coeffs, basis):
     for q in range(c.shape[1]):
         out = out + b[None, :, q] * c[:, q:q + 1]
     return out
+
+
+def fit_lsi(signals, a, g, ridge, n_threads=None, trace=False):
+    """LsiDesign(a, g, ridge) and fit_lsi: (x (v, n), converged (v,) bool[, trace (v, 4)])."""
+    s, am, gm = _c(signals), _c(a), _c(g)
+    v, m = s.shape
+    n = am.shape[1]
+    x = np.zeros((v, n))
+    conv = np.zeros(v, dtype=np.uint8)
+    tr = np.zeros((v, 4), dtype=np.int64)
+    _check(_load().oracle_fit_lsi(_p(s), v, _p(am), m, n, _p(gm), gm.shape[0], float(ridge),
+                                  n_threads or os.cpu_count() or 1, _p(x),
+                                  conv.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                  tr.ctypes.data_as(_i64)))
+    if trace:
+        return x, conv.astype(bool), tr
+    return x, conv.astype(bool)
+
+
+def lsi_design(a, g, ridge):
+    """(q_transposed, back_substitution, constraints, constraint_gram, constraint_abs_sums)."""
+    am, gm = _c(a), _c(g)
+    m, n = am.shape
+    k = gm.shape[0]
+    q_t, back, e = np.zeros((n, m)), np.zeros((n, n)), np.zeros((k, n))
+    gram, sums = np.zeros((k, k)), np.zeros(k)
+    _check(_load().oracle_lsi_design(_p(am), m, n, _p(gm), k, float(ridge), _p(q_t), _p(back),
+                                     _p(e), _p(gram), _p(sums)))
+    return q_t, back, e, gram, sums
+
+
+def solve_lsi_by_qr(signal, a, g, ridge):
+    s, am, gm = _c(signal), _c(a), _c(g)
+    m, n = am.shape
+    x = np.zeros(n)
+    conv = np.zeros(1, dtype=np.uint8)
+    _check(_load().oracle_solve_lsi_by_qr(_p(s), _p(am), m, n, _p(gm), gm.shape[0], float(ridge),
+                                          _p(x), conv.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+    return x, bool(conv[0])

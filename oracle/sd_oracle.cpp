@@ -13,6 +13,7 @@
 //   cpp/src/sd/csd.cpp         make_csd_problem, solve_csd (LLT restated as below)
 //   cpp/src/sh/harmonics.cpp   fill_legendre, Legendre derivatives, derivatives
 //   cpp/src/sh/peaks.cpp       climb, voxel_peaks
+//   cpp/src/sd/lsi.cpp         make_lsi_design, ldp_multipliers, solve_lsi
 //   cpp/src/common/parallel.hpp parallel_for_blocks (blocks of 128 voxels, dynamic)
 // Eigen's ColPivHouseholderQR (Eigen 5.0.1, the reference's pinned dependency) is restated from
 // its published algorithm (LAPACK xGEQP3 column pivoting with the LAWN 176 norm downdate).
@@ -1365,6 +1366,428 @@ void sh_voxel_peaks(const double* c, Index lmax, bool full, const double* on_sph
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Least squares with inequality constraints (cpp/src/sd/lsi.cpp).
+
+struct LsiDesign {
+    Index m = 0, n = 0, k = 0;
+    Mat q_transposed;      // (n, m)
+    Mat back_substitution; // (n, n): P R^-1
+    Mat constraints;       // (k, n): E = G P R^-1, unit rows
+    Mat constraint_gram;   // (k, k): E E^T
+    Vec constraint_abs_sums;
+};
+
+// make_lsi_design (lsi.cpp:19-69): the pivoted QR of [A; ridge I] (the QR above, wave-sum order),
+// the thin Q's measurement rows transposed, P R^-1 (R^-1 by column-oriented back substitution of
+// the identity), E = G P R^-1 with unit rows, E E^T and the rows' absolute sums.  Every other sum
+// runs in index order.
+LsiDesign make_lsi_design(const Mat& a, const Mat& g, double ridge) {
+    const Index m = a.rows;
+    const Index n = a.cols;
+    if (g.cols != n) {
+        throw std::invalid_argument("the constraints have " + std::to_string(g.cols) +
+                                    " columns but the design has " + std::to_string(n));
+    }
+    if (!(ridge >= 0.0)) {
+        throw std::invalid_argument("the ridge must be at least 0");
+    }
+    const bool has_ridge = ridge > 0.0;
+    const Index rows = m + (has_ridge ? n : 0);
+    Mat stacked(rows, n);
+    for (Index r = 0; r < m; ++r) {
+        for (Index c = 0; c < n; ++c) {
+            stacked(r, c) = a(r, c);
+        }
+    }
+    if (has_ridge) {
+        for (Index c = 0; c < n; ++c) {
+            stacked(m + c, c) = ridge;
+        }
+    }
+    PivotedQR qr;
+    qr.compute(stacked);
+    const double threshold = static_cast<double>(std::max(rows, n)) * EPS;
+    if (rows < n || qr.rank(threshold) < n) {
+        throw std::invalid_argument(
+            "the design does not have full column rank: the measurements cannot determine all " +
+            std::to_string(n) + " parameters");
+    }
+    LsiDesign d;
+    d.m = m;
+    d.n = n;
+    d.k = g.rows;
+    // Thin Q: Q e_j for j < n, the reflectors applied last to first.
+    d.q_transposed = Mat(n, m);
+    for (Index j = 0; j < n; ++j) {
+        Vec col(static_cast<size_t>(rows), 0.0);
+        col[j] = 1.0;
+        for (Index h = n - 1; h >= 0; --h) {
+            const double tau = qr.hcoeffs[h];
+            if (rows - h == 1) {
+                col[h] *= (1.0 - tau);
+            } else if (tau != 0.0) {
+                double tmp = wave_sum(h + 1, rows, [&](Index r) { return qr.qr(r, h) * col[r]; });
+                tmp += col[h];
+                col[h] -= tau * tmp;
+                for (Index r = h + 1; r < rows; ++r) {
+                    col[r] -= (tau * qr.qr(r, h)) * tmp;
+                }
+            }
+        }
+        for (Index r = 0; r < m; ++r) {
+            d.q_transposed(j, r) = col[r];
+        }
+    }
+    // R^-1, one identity column at a time, then P R^-1.
+    Mat rinv(n, n);
+    for (Index j = 0; j < n; ++j) {
+        Vec x(static_cast<size_t>(n), 0.0);
+        x[j] = 1.0;
+        for (Index i = n - 1; i >= 0; --i) {
+            if (x[i] != 0.0) {
+                x[i] = x[i] / qr.qr(i, i);
+            }
+            for (Index r = 0; r < i; ++r) {
+                x[r] -= qr.qr(r, i) * x[i];
+            }
+        }
+        for (Index i = 0; i < n; ++i) {
+            rinv(i, j) = x[i];
+        }
+    }
+    d.back_substitution = Mat(n, n);
+    for (Index p = 0; p < n; ++p) {
+        for (Index j = 0; j < n; ++j) {
+            d.back_substitution(qr.perm[p], j) = rinv(p, j);
+        }
+    }
+    d.constraints = Mat(d.k, n);
+    for (Index c = 0; c < d.k; ++c) {
+        for (Index j = 0; j < n; ++j) {
+            double sum = 0.0;
+            for (Index i = 0; i < n; ++i) {
+                sum = sum + g(c, i) * d.back_substitution(i, j);
+            }
+            d.constraints(c, j) = sum;
+        }
+        double sq = 0.0;
+        for (Index j = 0; j < n; ++j) {
+            sq = sq + d.constraints(c, j) * d.constraints(c, j);
+        }
+        const double norm = std::sqrt(sq);
+        if (norm > 0.0) {
+            for (Index j = 0; j < n; ++j) {
+                d.constraints(c, j) = d.constraints(c, j) / norm;
+            }
+        }
+    }
+    d.constraint_gram = Mat(d.k, d.k);
+    d.constraint_abs_sums.assign(static_cast<size_t>(d.k), 0.0);
+    for (Index a2 = 0; a2 < d.k; ++a2) {
+        for (Index b = 0; b < d.k; ++b) {
+            double sum = 0.0;
+            for (Index j = 0; j < n; ++j) {
+                sum = sum + d.constraints(a2, j) * d.constraints(b, j);
+            }
+            d.constraint_gram(a2, b) = sum;
+        }
+        double abs_sum = 0.0;
+        for (Index j = 0; j < n; ++j) {
+            abs_sum = abs_sum + std::fabs(d.constraints(a2, j));
+        }
+        d.constraint_abs_sums[a2] = abs_sum;
+    }
+    return d;
+}
+
+// The growing Cholesky factor of H = E E^T + f f^T on the active columns in the order they
+// joined (lsi.cpp GrowingCholesky): append by a forward solve (a zero right-hand entry stays
+// zero) with the pivot test of nnls.cpp, removal by the Givens downdate, solves forward then
+// backward, column-oriented.
+class LdpCholesky {
+public:
+    LdpCholesky(const LsiDesign& d, const Vec& f) : design(d), bounds(f), factor(d.k, d.k) {}
+
+    double h(Index row, Index column) const {
+        return design.constraint_gram(row, column) + bounds[row] * bounds[column];
+    }
+
+    bool append(Index column) {
+        const Index pos = static_cast<Index>(columns.size());
+        Vec cross(static_cast<size_t>(pos));
+        for (Index r = 0; r < pos; ++r) {
+            cross[r] = h(columns[r], column);
+        }
+        forward(cross, pos);
+        double sq = 0.0;
+        for (Index r = 0; r < pos; ++r) {
+            sq += cross[r] * cross[r];
+        }
+        const double hcc = h(column, column);
+        const double pivot = hcc - sq;
+        if (!(pivot > MIN_RELATIVE_PIVOT * hcc)) {
+            return false;
+        }
+        for (Index r = 0; r < pos; ++r) {
+            factor(pos, r) = cross[r];
+        }
+        factor(pos, pos) = std::sqrt(pivot);
+        columns.push_back(column);
+        return true;
+    }
+
+    void keep(const std::vector<char>& active) {
+        for (Index pos = static_cast<Index>(columns.size()) - 1; pos >= 0; --pos) {
+            if (!active[columns[pos]]) {
+                remove_cholesky_column(factor, static_cast<Index>(columns.size()), pos);
+                columns.erase(columns.begin() + pos);
+            }
+        }
+    }
+
+    Vec solve(const Vec& rhs) const {
+        const Index k = static_cast<Index>(columns.size());
+        Vec w(static_cast<size_t>(k));
+        for (Index p = 0; p < k; ++p) {
+            w[p] = rhs[columns[p]];
+        }
+        forward(w, k);
+        for (Index j = k - 1; j >= 0; --j) {
+            if (w[j] != 0.0) {
+                w[j] = w[j] / factor(j, j);
+            }
+            for (Index i = 0; i < j; ++i) {
+                w[i] -= factor(j, i) * w[j];
+            }
+        }
+        Vec full(rhs.size(), 0.0);
+        for (Index p = 0; p < k; ++p) {
+            full[columns[p]] = w[p];
+        }
+        return full;
+    }
+
+    Index size() const { return static_cast<Index>(columns.size()); }
+
+private:
+    void forward(Vec& w, Index k) const {
+        for (Index j = 0; j < k; ++j) {
+            if (w[j] != 0.0) {
+                w[j] = w[j] / factor(j, j);
+            }
+            for (Index i = j + 1; i < k; ++i) {
+                w[i] -= factor(i, j) * w[j];
+            }
+        }
+    }
+
+    const LsiDesign& design;
+    const Vec& bounds;
+    Mat factor;
+    std::vector<Index> columns;
+};
+
+// Entering steps after which an LDP solve counts as failed (handed to the QR path): each step
+// adds a column, and a finite solve takes far fewer.
+Index ldp_outer_guard(Index k) { return 8 * k + 64; }
+
+struct LsiTrace {
+    int fallback = 0; // 1 when the QR-based NNLS finished the voxel
+    Index outer = 0;
+    Index inner = 0;
+    Index max_factor = 0;
+};
+
+// ldp_multipliers (lsi.cpp:191-240): Lawson-Hanson on H, the gradient f (1 - f.u) minus
+// u_c (E E^T)[:, c] over the non-zero u_c in column order (f.u over the non-zero u_c in order).
+// Returns false when a pivot is refused, the inner steps run out or the outer guard is hit.
+bool ldp_multipliers(const LsiDesign& d, const Vec& f, Vec& u, LsiTrace* tr) {
+    const Index k = d.k;
+    double norm1 = 0.0;
+    for (Index c = 0; c < k; ++c) {
+        const double v = d.constraint_abs_sums[c] + std::fabs(f[c]);
+        if (c == 0 || v > norm1) {
+            norm1 = v;
+        }
+    }
+    const double tol = 10.0 * EPS * norm1 * static_cast<double>(std::max(d.n + 1, k));
+    const Index max_inner = 3 * k;
+    std::vector<char> active(static_cast<size_t>(k), 0);
+    LdpCholesky chol(d, f);
+    u.assign(static_cast<size_t>(k), 0.0);
+    Index n_inner = 0;
+    Index n_outer = 0;
+    while (true) {
+        double dot = 0.0;
+        for (Index c = 0; c < k; ++c) {
+            if (u[c] != 0.0) {
+                dot = dot + f[c] * u[c];
+            }
+        }
+        Vec g(static_cast<size_t>(k));
+        for (Index j = 0; j < k; ++j) {
+            g[j] = f[j] * (1.0 - dot);
+        }
+        for (Index c = 0; c < k; ++c) {
+            if (u[c] != 0.0) {
+                for (Index j = 0; j < k; ++j) {
+                    g[j] = g[j] - u[c] * d.constraint_gram(c, j);
+                }
+            }
+        }
+        Index entering = -1;
+        for (Index c = 0; c < k; ++c) {
+            if (!active[c] && (entering < 0 || g[c] > g[entering])) {
+                entering = c;
+            }
+        }
+        if (entering < 0 || g[entering] <= tol) {
+            return true;
+        }
+        if (++n_outer > ldp_outer_guard(k)) {
+            return false;
+        }
+        if (tr) tr->outer += 1;
+        active[entering] = 1;
+        if (!chol.append(entering)) {
+            return false;
+        }
+        if (tr) tr->max_factor = std::max(tr->max_factor, chol.size());
+        Vec cand = chol.solve(f);
+        while (true) {
+            double step = std::numeric_limits<double>::infinity();
+            Index blocking = -1;
+            for (Index c = 0; c < k; ++c) {
+                if (!active[c] || cand[c] > 0.0) {
+                    continue;
+                }
+                const double distance = u[c] - cand[c];
+                const double s = distance > 0.0 ? u[c] / distance : 0.0;
+                if (s < step) {
+                    step = s;
+                    blocking = c;
+                }
+            }
+            if (blocking < 0) {
+                break;
+            }
+            if (++n_inner > max_inner) {
+                return false;
+            }
+            if (tr) tr->inner += 1;
+            for (Index c = 0; c < k; ++c) {
+                u[c] = u[c] + step * (cand[c] - u[c]);
+            }
+            u[blocking] = 0.0;
+            for (Index c = 0; c < k; ++c) {
+                if (c == blocking || std::fabs(u[c]) < tol) {
+                    active[c] = 0;
+                    u[c] = 0.0;
+                }
+            }
+            chol.keep(active);
+            cand = chol.solve(f);
+        }
+        u = cand;
+    }
+}
+
+struct LsiOut {
+    Vec x;
+    bool converged = true;
+};
+
+// solve_lsi (lsi.cpp:245-292): r = Q^T s; f = -(E r) / max|E r| (all sums in index order); the
+// LDP multipliers u by the Cholesky NNLS or, when that fails, by solve_nnls_with_limit on
+// [E^T; f^T] with target e_n; x = P R^-1 (scale (E^T u) / (1 - f.u) + r), the sums over the
+// non-zero u_c in order.
+LsiOut solve_lsi(const LsiDesign& d, const Vec& s, LsiTrace* tr, bool only_qr = false) {
+    const Index n = d.n;
+    const Index k = d.k;
+    Vec rotated(static_cast<size_t>(n));
+    for (Index i = 0; i < n; ++i) {
+        double sum = 0.0;
+        for (Index r = 0; r < d.m; ++r) {
+            sum = sum + d.q_transposed(i, r) * s[r];
+        }
+        rotated[i] = sum;
+    }
+    // scale = max |raw_c|, NaN when any raw_c is NaN.
+    Vec raw(static_cast<size_t>(k));
+    double scale = 0.0;
+    bool any_nan = false;
+    for (Index c = 0; c < k; ++c) {
+        double sum = 0.0;
+        for (Index i = 0; i < n; ++i) {
+            sum = sum + d.constraints(c, i) * rotated[i];
+        }
+        raw[c] = -sum;
+        const double a = std::fabs(raw[c]);
+        any_nan = any_nan || std::isnan(a);
+        scale = a > scale ? a : scale;
+    }
+    if (any_nan) {
+        scale = std::numeric_limits<double>::quiet_NaN();
+    }
+    LsiOut out;
+    auto back = [&](const Vec& z) {
+        Vec x(static_cast<size_t>(n));
+        for (Index i = 0; i < n; ++i) {
+            double sum = 0.0;
+            for (Index j = 0; j < n; ++j) {
+                sum = sum + d.back_substitution(i, j) * z[j];
+            }
+            x[i] = sum;
+        }
+        return x;
+    };
+    if (!(scale > 0.0)) {
+        out.x = back(rotated);
+        return out;
+    }
+    Vec f(static_cast<size_t>(k));
+    for (Index c = 0; c < k; ++c) {
+        f[c] = raw[c] / scale;
+    }
+    Vec u;
+    if (only_qr || !ldp_multipliers(d, f, u, tr)) {
+        if (tr) tr->fallback = 1;
+        Mat c(n + 1, k);
+        for (Index j = 0; j < k; ++j) {
+            for (Index i = 0; i < n; ++i) {
+                c(i, j) = d.constraints(j, i);
+            }
+            c(n, j) = f[j];
+        }
+        Vec target(static_cast<size_t>(n + 1), 0.0);
+        target[n] = 1.0;
+        NnlsResult r = solve_nnls_with_limit(c, target, default_gradient_tolerance(c), 3 * k,
+                                             nullptr);
+        u = std::move(r.solution);
+        out.converged = r.converged;
+    }
+    double dot = 0.0;
+    for (Index c = 0; c < k; ++c) {
+        if (u[c] != 0.0) {
+            dot = dot + f[c] * u[c];
+        }
+    }
+    const double denom = 1.0 - dot;
+    Vec z(static_cast<size_t>(n));
+    for (Index j = 0; j < n; ++j) {
+        double sum = 0.0;
+        for (Index c = 0; c < k; ++c) {
+            if (u[c] != 0.0) {
+                sum = sum + d.constraints(c, j) * u[c];
+            }
+        }
+        z[j] = (scale * sum) / denom + rotated[j];
+    }
+    out.x = back(z);
+    return out;
+}
+
 TissueKernel make_kernel(const double* k_aniso, const double* k_iso, const double* w_k_iso,
                          Index m, Index ca, Index ci, const std::int64_t* blocks,
                          Index n_blocks) {
@@ -1622,6 +2045,56 @@ int oracle_peaks_chunk(const double* coeffs, std::int64_t n_voxels, std::int64_t
                                directions + v * n_peaks * 3, amplitudes + v * n_peaks);
             }
         });
+    });
+}
+
+// LsiDesign(a, g, ridge) then fit_lsi (cpp/src/python/sd.cpp:360-394): x (n_voxels, n),
+// converged (uint8), trace (optional, 4 int64 per voxel: fallback, outer, inner, max factor).
+int oracle_fit_lsi(const double* signals, std::int64_t n_voxels, const double* a, std::int64_t m,
+                   std::int64_t n, const double* g, std::int64_t k, double ridge, int n_threads,
+                   double* x, std::uint8_t* converged, std::int64_t* trace) {
+    return guarded([&] {
+        const LsiDesign d = make_lsi_design(from_ptr(a, m, n), from_ptr(g, k, n), ridge);
+        parallel_for_blocks(n_voxels, 128, n_threads, [&](Index begin, Index end) {
+            for (Index v = begin; v < end; ++v) {
+                LsiTrace tr;
+                const LsiOut o = solve_lsi(d, Vec(signals + v * m, signals + (v + 1) * m), &tr);
+                std::copy(o.x.begin(), o.x.end(), x + v * n);
+                converged[v] = o.converged ? 1 : 0;
+                if (trace) {
+                    trace[4 * v + 0] = tr.fallback;
+                    trace[4 * v + 1] = tr.outer;
+                    trace[4 * v + 2] = tr.inner;
+                    trace[4 * v + 3] = tr.max_factor;
+                }
+            }
+        });
+    });
+}
+
+// The design matrices: q_t (n, m), back (n, n), e (k, n), gram (k, k), abs_sums (k).
+int oracle_lsi_design(const double* a, std::int64_t m, std::int64_t n, const double* g,
+                      std::int64_t k, double ridge, double* q_t, double* back, double* e,
+                      double* gram, double* abs_sums) {
+    return guarded([&] {
+        const LsiDesign d = make_lsi_design(from_ptr(a, m, n), from_ptr(g, k, n), ridge);
+        std::copy(d.q_transposed.v.begin(), d.q_transposed.v.end(), q_t);
+        std::copy(d.back_substitution.v.begin(), d.back_substitution.v.end(), back);
+        std::copy(d.constraints.v.begin(), d.constraints.v.end(), e);
+        std::copy(d.constraint_gram.v.begin(), d.constraint_gram.v.end(), gram);
+        std::copy(d.constraint_abs_sums.begin(), d.constraint_abs_sums.end(), abs_sums);
+    });
+}
+
+// solve_lsi_by_qr (lsi.cpp:298-300) of one data vector: the QR-based NNLS only.
+int oracle_solve_lsi_by_qr(const double* signal, const double* a, std::int64_t m, std::int64_t n,
+                           const double* g, std::int64_t k, double ridge, double* x,
+                           std::uint8_t* converged) {
+    return guarded([&] {
+        const LsiDesign d = make_lsi_design(from_ptr(a, m, n), from_ptr(g, k, n), ridge);
+        const LsiOut o = solve_lsi(d, Vec(signal, signal + m), nullptr, true);
+        std::copy(o.x.begin(), o.x.end(), x);
+        *converged = o.converged ? 1 : 0;
     });
 }
 
