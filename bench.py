@@ -7,7 +7,7 @@ D = 3.0e-3 mixed by Dirichlet(2, 1, 0.5) fractions; Rician noise at SNR 25.
 
 One step = the whole masked volume through the hot path with the normalised signals already in
 HBM: the low-resolution FOD <-> fraction rounds, the high-resolution FOD and final fractions
-(liblarmor_sd.so kernels) and the projection of the amplitudes onto SH up to lmax 8.
+and the projection of the amplitudes onto SH up to lmax 8 (all liblarmor_sd.so kernels).
 
 Prints one JSON line.  `roofline` prices the dominant kernel (the one with the most time per
 step) at its algorithmic FP64 flops against MI355X's FP64 peak, from the kernel's own launch
@@ -149,8 +149,7 @@ def main() -> int:
         fods, fractions, rounds, status = _core.fit_multi_tissue(
             dev, weights, lr_k, lr_solver, hr_k, hr_solver, MAX_ROUNDS, 1e-7,
             FINAL_FRACTION_TOL_X, 1, return_diagnostics=True)
-        sh = fods @ projection
-        sh[:, 0] = fods.sum(dim=1) / np.sqrt(4.0 * np.pi)
+        sh = _core.sh_project(fods, projection, exact_integral=True)
         return sh, fractions, rounds, status
 
     for _ in range(args.warmup):

@@ -226,6 +226,16 @@ int lsd_fit_lsi(const double* signals, int64_t n_voxels, int64_t m, const lsd_ls
 int lsd_fit_lsi_by_qr(const double* signals, int64_t n_voxels, int64_t m, const lsd_lsi* design,
                       double* x, uint8_t* converged, int32_t* status, void* stream);
 
+/*
+ * FOD amplitudes to SH coefficients (all device): coeffs (n x c) = amplitudes (n x k) times
+ * projection (k x c, the least-squares projection of the basis on the amplitude directions
+ * times their density, c <= 96); with exact_integral, coeffs[:, 0] = the sum of each row of
+ * amplitudes / sqrt(4 pi).
+ * Replaces: _to_sh  src/larmor/sd/model.py:412-440 (numpy lstsq through fit_sh, sh.py:134-177).
+ */
+int lsd_sh_project(const double* amplitudes, int64_t n, int64_t k, const double* projection,
+                   int64_t c, int exact_integral, double* coeffs, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -497,3 +497,22 @@ def fit_lsi(signals, design: LsiDesign, n_threads: int, *, return_status: bool =
     if return_status:
         out += (_out(status, host),)
     return out
+
+
+def sh_project(amplitudes, projection, *, exact_integral: bool = False):
+    """SH coefficients (n, c) = amplitudes (n, k) @ projection (k, c) on the device; with
+    exact_integral, c00 = sum(amplitudes) / sqrt(4 pi) (model.py:412-440)."""
+    torch = _torch()
+    amps, host = _as_device(amplitudes, torch)
+    proj, _ = _as_device(projection, torch)
+    _check_f64(amps, "amplitudes")
+    _check_f64(proj, "projection")
+    n, k = amps.shape
+    if proj.shape[0] != k:
+        raise ValueError(f"amplitudes ({n}, {k}) and projection {tuple(proj.shape)} do not match")
+    c = proj.shape[1]
+    out = torch.empty((n, c), dtype=torch.float64, device="cuda")
+    _lib.check(_lib.load().lsd_sh_project(amps.data_ptr(), n, k, proj.data_ptr(), c,
+                                          int(bool(exact_integral)), out.data_ptr(),
+                                          _stream_ptr(torch)))
+    return _out(out, host)

@@ -91,6 +91,10 @@ SIGNATURES = {
     "lsd_fit_lsi_by_qr": (
         ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp]
     ),
+    "lsd_sh_project": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int64, ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int, _vp, _vp],
+    ),
 }
 
 

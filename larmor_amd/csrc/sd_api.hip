@@ -180,6 +180,10 @@ int csd_direction_slots(int64_t k);
 bool csd_supported_coefficients(int64_t c);
 int launch_csd(int n, int nk, const CsdArgs& args, int waves, hipStream_t stream);
 
+// SH projection (inst_sh.hip).
+int launch_sh_project(const double* amps, long n, int k, const double* p, int c,
+                      int exact_integral, double* out, hipStream_t stream);
+
 // LSI launchers (inst_lsi.hip).
 int lsi_constraint_slots(int64_t k);
 int launch_lsi(int nks, const LsiArgs& fast, const LsiArgs& qr, int waves, int qr_waves,
@@ -1016,6 +1020,18 @@ static int fit_lsi_impl(const double* signals, int64_t n_voxels, int64_t m, cons
     // QR only: the fast kernel gets no voxels, the QR kernel all of them.
     if (only_qr) a.n_vox = 0;
     return launch_lsi(lsi_constraint_slots(p->k), a, q, waves, qr_waves, stream);
+}
+
+int lsd_sh_project(const double* amplitudes, int64_t n, int64_t k, const double* projection,
+                   int64_t c, int exact_integral, double* coeffs, void* stream) {
+    if (n < 0 || k < 0 || c < 0) {
+        return fail("shapes must be non-negative");
+    }
+    if (n == 0) {
+        return 0;
+    }
+    return launch_sh_project(amplitudes, n, static_cast<int>(k), projection, static_cast<int>(c),
+                             exact_integral, coeffs, static_cast<hipStream_t>(stream));
 }
 
 } // extern "C"
