@@ -10,7 +10,7 @@ namespace {
 template <int N, int NK>
 int launch_csd_n(const CsdArgs& args, int waves, hipStream_t stream) {
     const size_t lds = sizeof(double) * csd_lds_doubles(N) * WAVES_PER_BLOCK;
-    return launch("csd", csd_kernel<N, NK>, args, waves, lds, stream);
+    return launch_fill("csd", csd_kernel<N, NK>, args, waves, lds, stream);
 }
 
 template <int N>

@@ -16,7 +16,7 @@ namespace {
 template <int NKS>
 int launch_lsi_t(const LsiArgs& fast, const LsiArgs& qr, int waves, int qr_waves,
                  hipStream_t stream) {
-    int rc = launch("lsi", lsi_kernel<NKS>, fast, waves, 0, stream);
+    int rc = launch_fill("lsi", lsi_kernel<NKS>, fast, waves, 0, stream);
     if (rc) return rc;
     return launch("lsi_qr", lsi_qr_kernel<NKS>, qr, qr_waves, 0, stream);
 }

@@ -10,7 +10,7 @@ namespace {
 template <int LMAX, bool FULL>
 int launch_peaks_t(const PeaksArgs& a, int waves, hipStream_t stream) {
     const size_t lds = static_cast<size_t>(peaks_lds_bytes_per_wave()) * WAVES_PER_BLOCK;
-    return launch("peaks", peaks_kernel<LMAX, FULL>, a, waves, lds, stream);
+    return launch_fill("peaks", peaks_kernel<LMAX, FULL>, a, waves, lds, stream);
 }
 
 template <int LMAX, bool FULL>

@@ -31,7 +31,10 @@
 namespace lsd {
 
 constexpr int NQ = 4;             // most fraction columns (groups + isotropic tissues)
-constexpr int KCAP_LDS = 32;      // LDS factor capacity (active columns)
+// LDS factor capacity (active columns) of the fast FOD kernels: 24 for the low-resolution solve
+// (its active sets stay below 24 in all but a handful of voxels, which take the rerun kernel),
+// so the LDS does not cap its occupancy; 32 for the others.
+constexpr int kcap_lds(int ncpl) { return ncpl == 1 ? 24 : 32; }
 constexpr int WAVES_PER_BLOCK = 4;
 constexpr int RERUN_WAVES = 32;   // waves of a rerun launch
 constexpr int RERUN_WAVES_ALL = 512; // waves when every voxel needs the global-memory kernel
@@ -160,8 +163,14 @@ struct FodArgs {
     SolveScratch scratch;
 };
 
+// The fast low-resolution instantiation is asked for 5 waves per SIMD (it sits a few registers
+// above the 96 that allow it).
+constexpr int fod_waves_per_eu(int ncpl, bool global_l) { return (!global_l && ncpl == 1) ? 5 : 1; }
+
 template <int NCPL, int RPL, int KCAP, bool GLOBAL_L>
-__global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) fod_kernel(FodArgs A) {
+__global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
+    __attribute__((amdgpu_waves_per_eu(fod_waves_per_eu(NCPL, GLOBAL_L))))
+    fod_kernel(FodArgs A) {
     constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
     extern __shared__ double smem[];
     const int w = threadIdx.x / WAVE;
@@ -394,6 +403,27 @@ inline int launch(const char* name, Kern kern, const Args& args, int waves, size
 }
 
 
+// Launches a persistent kernel (one that strides over its items) on at most `waves` waves and
+// no more than the device keeps resident at once: one full-occupancy wave set.
+template <class Kern, class Args>
+inline int launch_fill(const char* name, Kern kern, const Args& args, int waves, size_t lds_bytes,
+                       hipStream_t stream) {
+    int blocks_per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kern, WAVE * WAVES_PER_BLOCK,
+                                                     lds_bytes) == hipSuccess &&
+        blocks_per_cu > 0) {
+        int dev = 0;
+        int cus = 0;
+        LSD_HIP(hipGetDevice(&dev));
+        LSD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const long resident = static_cast<long>(blocks_per_cu) * cus * WAVES_PER_BLOCK;
+        if (resident > 0 && resident < waves) {
+            waves = static_cast<int>(resident);
+        }
+    }
+    return launch(name, kern, args, waves, lds_bytes, stream);
+}
+
 // The fast launch (LDS factor) of a kernel family, then its rerun of the overflowed voxels with
 // the factor in global memory, sized for every column (KG = NCPL * 64 >= n).
 template <template <int, int, int, bool> class Family, int NCPL, int RPL, class Args>
@@ -413,8 +443,9 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
         return launch(rerun_name, Family<NCPL, RPL, KG, true>::kernel, rerun, all_waves, 0,
                       stream);
     }
-    const size_t lds = sizeof(double) * lds_factor_doubles(KCAP_LDS, RPL) * WAVES_PER_BLOCK;
-    int rc = launch(name, Family<NCPL, RPL, KCAP_LDS, false>::kernel, fast, waves, lds, stream);
+    constexpr int KC = kcap_lds(NCPL);
+    const size_t lds = sizeof(double) * lds_factor_doubles(KC, RPL) * WAVES_PER_BLOCK;
+    int rc = launch_fill(name, Family<NCPL, RPL, KC, false>::kernel, fast, waves, lds, stream);
     if (rc) return rc;
     rc = device_alloc(rerun_scratch, sizeof(double) * per * RERUN_WAVES, stream);
     if (rc) return rc;
@@ -444,7 +475,7 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
     int launch_fraction_##NC##_##RP(const char* name, const FractionArgs& args, int waves,      \
                                     hipStream_t stream) {                                      \
         const size_t lds = sizeof(double) * (NQ * WAVES_PER_BLOCK + args.kernel.ci) * RP * WAVE; \
-        return launch(name, fraction_kernel<NC, RP>, args, waves, lds, stream);                 \
+        return launch_fill(name, fraction_kernel<NC, RP>, args, waves, lds, stream);            \
     }
 
 // Every instantiated (NCPL, RPL): X(NC, RP).

@@ -82,6 +82,11 @@ struct RidgeNnlsWave {
     LaneVec<NSK, int> fcol; // column at each factor position
     int k;             // factor size (uniform)
     bool factor_failed;
+    // y = L^-1 corr_A, valid on positions [0, y_valid): kept across appends (the new row of L
+    // gives y_k by one sequential dot, the order the full column-oriented solve uses) and cut
+    // back to the removed position by a downdate.
+    LaneVec<NSK> y;
+    int y_valid;
 
     __device__ RidgeNnlsWave(const RidgeDesignDev& design, double tol_, int max_inner_, double l1_,
                              double* L_, double* qrs_)
@@ -117,6 +122,33 @@ struct RidgeNnlsWave {
         return sq;
     }
 
+    // L w = b on positions [0, kk) when positions [0, p0) of b already hold their final values
+    // (a prefix of the same solve): the remaining positions receive the same updates, in the same
+    // order, as in forward().
+    __device__ __forceinline__ void forward_from(LaneVec<NSK>& b, int p0, int kk) {
+        if (p0 >= kk) {
+            return;
+        }
+        const int l = lane_id();
+        for (int j = 0; j < kk; ++j) {
+            const double bj = b.get(j);
+            double wj = bj;
+            if (j >= p0) {
+                const double ljj = Luni(j, j);
+                wj = bj != 0.0 ? bj / ljj : bj;
+            }
+#pragma unroll
+            for (int u = 0; u < NSK; ++u) {
+                const int i = l + WAVE * u;
+                if (i == j) {
+                    b.v[u] = wj;
+                } else if (i > j && i < kk && i >= p0) {
+                    b.v[u] = b.v[u] - Lat(i, j) * wj;
+                }
+            }
+        }
+    }
+
     // L^T z = w on positions [0, kk), column-oriented.
     __device__ __forceinline__ void backward(LaneVec<NSK>& b, int kk) {
         const int l = lane_id();
@@ -149,8 +181,19 @@ struct RidgeNnlsWave {
         return out;
     }
 
-    // Appends column c to the factor; false when its pivot is refused.
-    __device__ __forceinline__ bool append(int c) {
+    // The value of a column-spread vector at uniform column c.
+    __device__ __forceinline__ double uniform_col(const double (&v)[NCPL], int c) const {
+        double x = 0.0;
+#pragma unroll
+        for (int t = 0; t < NCPL; ++t) {
+            x = (c / WAVE) == t ? v[t] : x;
+        }
+        return readlane(x, c % WAVE);
+    }
+
+    // Appends column c to the factor; false when its pivot is refused.  With y valid on every
+    // position, also extends y to the new one.
+    __device__ __forceinline__ bool append(int c, const double (&corr)[NCPL]) {
         const int l = lane_id();
         LaneVec<NSK> b;
 #pragma unroll
@@ -171,8 +214,24 @@ struct RidgeNnlsWave {
                 Lat(k, p) = b.v[u];
             }
         }
+        const double lkk = sqrt(pivot);
         if (l == k % WAVE) {
-            Lat(k, k) = sqrt(pivot);
+            Lat(k, k) = lkk;
+        }
+        if (y_valid == k) {
+            // y_k = (corr_c - L_k0 y_0 - L_k1 y_1 - ...) / L_kk, subtracted in order.
+            const double cc = uniform_col(corr, c);
+            LaneVec<NSK> prod;
+#pragma unroll
+            for (int u = 0; u < NSK; ++u) {
+                prod.v[u] = b.v[u] * y.v[u];
+            }
+            double v = cc;
+            for (int j = 0; j < k; ++j) {
+                v = v - prod.get(j);
+            }
+            y.set(k, v != 0.0 ? v / lkk : v);
+            y_valid = k + 1;
         }
         fcol.set(k, c);
 #pragma unroll
@@ -238,6 +297,7 @@ struct RidgeNnlsWave {
             pos[t] = pos[t] == p ? -1 : (pos[t] > p ? pos[t] - 1 : pos[t]);
         }
         k -= 1;
+        y_valid = y_valid < p ? y_valid : p;
     }
 
     // G_AA z = corr_A from the factor, scattered to the columns (zero elsewhere).
@@ -248,9 +308,11 @@ struct RidgeNnlsWave {
         for (int u = 0; u < NSK; ++u) {
             const int p = l + WAVE * u;
             const double v = gather_cols(corr, p < k ? fcol.v[u] : 0);
-            b.v[u] = p < k ? v : 0.0;
+            b.v[u] = p < k ? (p < y_valid ? static_cast<double>(y.v[u]) : v) : 0.0;
         }
-        forward(b, k);
+        forward_from(b, y_valid, k);
+        y = b;
+        y_valid = k;
         backward(b, k);
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {
@@ -374,7 +436,7 @@ struct RidgeNnlsWave {
                         status |= ST_OVERFLOW;
                         return false;
                     }
-                    if (!append(WAVE * t + b)) {
+                    if (!append(WAVE * t + b, corr)) {
                         failed = true;
                     }
                 }
@@ -407,6 +469,8 @@ struct RidgeNnlsWave {
         const int l = lane_id();
         int status = 0;
         k = 0;
+        y_valid = 0;
+        y.fill(0.0);
         factor_failed = D.normal_equations == 0;
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {

@@ -160,7 +160,9 @@ int main_waves(int64_t n_vox) {
     if (cus <= 0) {
         cus = 256;
     }
-    const int64_t target = static_cast<int64_t>(cus) * 16;
+    // Up to 8 waves per SIMD; each persistent launch then clamps to what its kernel keeps
+    // resident (launch_fill).
+    const int64_t target = static_cast<int64_t>(cus) * 32;
     int64_t waves = std::min<int64_t>(target, std::max<int64_t>(n_vox, 1));
     waves = (waves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK * WAVES_PER_BLOCK;
     return static_cast<int>(waves);
