@@ -182,6 +182,7 @@ __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const
                 for (int o = WAVE / 2; o >= 1; o >>= 1) {
                     largest = fmax(largest, __shfl_xor(largest, o));
                 }
+                largest = readlane(largest, 0);
                 if (largest > 0.0) {
 #pragma unroll
                     for (int t = 0; t < RPL; ++t) {

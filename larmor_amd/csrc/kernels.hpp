@@ -41,8 +41,11 @@ constexpr int RERUN_WAVES_ALL = 512; // waves when every voxel needs the global-
 constexpr double FRACTION_SUM_OFFSET = 1e-16;
 
 // Per-wave slab of the FOD kernels: the Cholesky factor (stride kcap + 1, plus one wave of slack
-// for the uniform reads).
-constexpr int lds_factor_doubles(int kcap, int /*rpl*/) { return (kcap + 1) * kcap + WAVE; }
+// for the uniform reads), then its diagonal and the diagonal's reciprocals (2 kcap).  An even
+// count keeps every slab 16-byte aligned.
+constexpr int lds_factor_doubles(int kcap, int /*rpl*/) {
+    return (kcap + 1) * kcap + WAVE + 2 * kcap;
+}
 
 // ---------------------------------------------------------------------------------------------
 // Device helpers
@@ -165,7 +168,7 @@ struct FodArgs {
 
 // The fast low-resolution instantiation is asked for 5 waves per SIMD (it sits a few registers
 // above the 96 that allow it).
-constexpr int fod_waves_per_eu(int ncpl, bool global_l) { return (!global_l && ncpl == 1) ? 5 : 1; }
+constexpr int fod_waves_per_eu(int ncpl, bool global_l) { return (!global_l && ncpl == 1) ? 6 : 1; }
 
 template <int NCPL, int RPL, int KCAP, bool GLOBAL_L>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
@@ -173,7 +176,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     fod_kernel(FodArgs A) {
     constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
     extern __shared__ double smem[];
-    const int w = threadIdx.x / WAVE;
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
     const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
     double* slab = A.scratch.base + gw * A.scratch.per_wave;
@@ -246,7 +249,7 @@ struct FractionArgs {
 template <int NCPL, int RPL>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) fraction_kernel(FractionArgs A) {
     extern __shared__ double smem[];
-    const int w = threadIdx.x / WAVE;
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
     const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
     double* work = smem + w * (NQ * RPL * WAVE);

@@ -102,7 +102,9 @@ struct MemCols {
     }
 };
 
-// The wave sum over rows lo <= r < hi of f(t), f evaluated on the lane's slot t.
+// The wave sum over rows lo <= r < hi of f(t), f evaluated on the lane's slot t.  f is
+// evaluated on every slot (its reads stay inside the lane's slots) and the terms outside the
+// range are left out by a select, so the lanes do not diverge.
 template <int RPL, class F>
 __device__ __forceinline__ double wave_sum_f(int lo, int hi, F f) {
     const int l = lane_id();
@@ -110,9 +112,9 @@ __device__ __forceinline__ double wave_sum_f(int lo, int hi, F f) {
 #pragma unroll
     for (int t = 0; t < RPL; ++t) {
         const int r = l + WAVE * t;
-        if (r >= lo && r < hi) {
-            s = s + f(t);
-        }
+        const double term = f(t);
+        const double sum = s + term;
+        s = (r >= lo && r < hi) ? sum : s;
     }
     return butterfly_sum(s);
 }

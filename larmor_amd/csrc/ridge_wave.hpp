@@ -87,10 +87,24 @@ struct RidgeNnlsWave {
     // back to the removed position by a downdate.
     LaneVec<NSK> y;
     int y_valid;
+    // diag[2 p], diag[2 p + 1]: the factor's diagonal at position p and its division reciprocal
+    // (after the factor and its slack); diag_ok: every diagonal in the divisor range.
+    double* diag;
+    bool diag_ok;
 
     __device__ RidgeNnlsWave(const RidgeDesignDev& design, double tol_, int max_inner_, double l1_,
                              double* L_, double* qrs_)
-        : D(design), tol(tol_), max_inner(max_inner_), l1(l1_), L(L_), qrs(qrs_) {}
+        : D(design), tol(tol_), max_inner(max_inner_), l1(l1_), L(L_), qrs(qrs_),
+          diag(L_ + S * KCAP + WAVE) {}
+
+    // L_jj and its reciprocal (one 16-byte read of a uniform address).
+    __device__ __forceinline__ double2 diag_at(int j) const {
+        return *reinterpret_cast<const double2*>(diag + 2 * j);
+    }
+    __device__ __forceinline__ double solve_divide(double a, int j) const {
+        const double2 d = diag_at(j);
+        return divide_nonzero_by(a, d.x, d.y, diag_ok);
+    }
 
     __device__ __forceinline__ int col(int t) const { return lane_id() + WAVE * t; }
     __device__ __forceinline__ double& Lat(int i, int j) { return L[j * S + i]; }
@@ -101,13 +115,13 @@ struct RidgeNnlsWave {
     }
 
     // L w = b on positions [0, kk), column-oriented; returns sum of w_j^2 in order.
-    __device__ __forceinline__ double forward(LaneVec<NSK>& b, int kk) {
+    __device__ __forceinline__ double forward(LaneVec<NSK>& b, int kk_) {
         const int l = lane_id();
+        const int kk = __builtin_amdgcn_readfirstlane(kk_);
         double sq = 0.0;
         for (int j = 0; j < kk; ++j) {
             const double bj = b.get(j);
-            const double ljj = Luni(j, j);
-            const double wj = bj != 0.0 ? bj / ljj : bj;
+            const double wj = solve_divide(bj, j);
             sq = sq + wj * wj;
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
@@ -125,7 +139,9 @@ struct RidgeNnlsWave {
     // L w = b on positions [0, kk) when positions [0, p0) of b already hold their final values
     // (a prefix of the same solve): the remaining positions receive the same updates, in the same
     // order, as in forward().
-    __device__ __forceinline__ void forward_from(LaneVec<NSK>& b, int p0, int kk) {
+    __device__ __forceinline__ void forward_from(LaneVec<NSK>& b, int p0_, int kk_) {
+        const int p0 = __builtin_amdgcn_readfirstlane(p0_);
+        const int kk = __builtin_amdgcn_readfirstlane(kk_);
         if (p0 >= kk) {
             return;
         }
@@ -134,8 +150,7 @@ struct RidgeNnlsWave {
             const double bj = b.get(j);
             double wj = bj;
             if (j >= p0) {
-                const double ljj = Luni(j, j);
-                wj = bj != 0.0 ? bj / ljj : bj;
+                wj = solve_divide(bj, j);
             }
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
@@ -150,12 +165,12 @@ struct RidgeNnlsWave {
     }
 
     // L^T z = w on positions [0, kk), column-oriented.
-    __device__ __forceinline__ void backward(LaneVec<NSK>& b, int kk) {
+    __device__ __forceinline__ void backward(LaneVec<NSK>& b, int kk_) {
         const int l = lane_id();
+        const int kk = __builtin_amdgcn_readfirstlane(kk_);
         for (int j = kk - 1; j >= 0; --j) {
             const double wj = b.get(j);
-            const double ljj = Luni(j, j);
-            const double zj = wj != 0.0 ? wj / ljj : wj;
+            const double zj = solve_divide(wj, j);
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
                 const int i = l + WAVE * u;
@@ -215,9 +230,13 @@ struct RidgeNnlsWave {
             }
         }
         const double lkk = sqrt(pivot);
+        const double rkk = division_reciprocal(lkk);
         if (l == k % WAVE) {
             Lat(k, k) = lkk;
+            diag[2 * k] = lkk;
+            diag[2 * k + 1] = rkk;
         }
+        const bool lkk_ok = divisor_in_range(lkk);
         if (y_valid == k) {
             // y_k = (corr_c - L_k0 y_0 - L_k1 y_1 - ...) / L_kk, subtracted in order.
             const double cc = uniform_col(corr, c);
@@ -230,7 +249,7 @@ struct RidgeNnlsWave {
             for (int j = 0; j < k; ++j) {
                 v = v - prod.get(j);
             }
-            y.set(k, v != 0.0 ? v / lkk : v);
+            y.set(k, divide_nonzero_by(v, lkk, rkk, lkk_ok));
             y_valid = k + 1;
         }
         fcol.set(k, c);
@@ -238,6 +257,7 @@ struct RidgeNnlsWave {
         for (int t = 0; t < NCPL; ++t) {
             pos[t] = col(t) == c ? k : pos[t];
         }
+        diag_ok = diag_ok && lkk_ok;
         k += 1;
         return true;
     }
@@ -262,8 +282,9 @@ struct RidgeNnlsWave {
             const double diagonal = Luni(column, column);
             const double above = Luni(column, column + 1);
             const double length = hypot_fixed(diagonal, above);
-            const double cosine = diagonal / length;
-            const double sine = above / length;
+            const double rl = division_reciprocal(length);
+            const double cosine = divide_by(diagonal, length, rl);
+            const double sine = divide_by(above, length, rl);
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
                 const int i = l + WAVE * u;
@@ -275,6 +296,22 @@ struct RidgeNnlsWave {
                 }
             }
         }
+        // The rotated diagonal and its reciprocals (each lane its own positions).
+        bool out_of_range = false;
+#pragma unroll
+        for (int u = 0; u < NSK; ++u) {
+            const int i = l + WAVE * u;
+            if (i < k - 1) {
+                double d = diag[2 * i];
+                if (i >= p) {
+                    d = Lat(i, i);
+                    diag[2 * i] = d;
+                    diag[2 * i + 1] = division_reciprocal(d);
+                }
+                out_of_range = out_of_range || !divisor_in_range(d);
+            }
+        }
+        diag_ok = ballot(out_of_range) == 0ull;
         // Positions above p move down one.
         LaneVec<NSK, int> shifted;
 #pragma unroll
@@ -471,6 +508,7 @@ struct RidgeNnlsWave {
         k = 0;
         y_valid = 0;
         y.fill(0.0);
+        diag_ok = true;
         factor_failed = D.normal_equations == 0;
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {

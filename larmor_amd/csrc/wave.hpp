@@ -50,26 +50,29 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-// The canonical butterfly of 64 lane partials; every lane gets the same sum.
+// The canonical butterfly of 64 lane partials.  Every lane computes the same sum (each step adds
+// the same two values, in either order); lane 0's, read back, is provably uniform to the
+// compiler, which keeps the control flow that depends on it scalar.
 __device__ __forceinline__ double butterfly_sum(double p) {
 #pragma unroll
     for (int o = WAVE / 2; o >= 1; o >>= 1) {
         p = p + __shfl_xor(p, o);
     }
-    return p;
+    return readlane(p, 0);
 }
 
-// max over lanes (exact, order-free).
+// max over lanes (exact, order-free; with NaN values, lane 0's result).
 __device__ __forceinline__ double wave_max(double p) {
 #pragma unroll
     for (int o = WAVE / 2; o >= 1; o >>= 1) {
         const double q = __shfl_xor(p, o);
         p = q > p ? q : p;
     }
-    return p;
+    return readlane(p, 0);
 }
 
 // The largest value and the smallest index holding it, over lanes; idx < 0 means "none".
+// (With NaN values the lanes' results could differ; lane 0's is the one returned.)
 __device__ __forceinline__ void wave_argmax_first(double& value, int& idx) {
 #pragma unroll
     for (int o = WAVE / 2; o >= 1; o >>= 1) {
@@ -79,6 +82,9 @@ __device__ __forceinline__ void wave_argmax_first(double& value, int& idx) {
         value = take ? ov : value;
         idx = take ? oi : idx;
     }
+    // Every lane holds the same result; lane 0's, read back, is provably uniform.
+    value = readlane(value, 0);
+    idx = readlane(idx, 0);
 }
 
 // The smallest value and the smallest index holding it, over lanes; idx < 0 means "none".
@@ -91,6 +97,9 @@ __device__ __forceinline__ void wave_argmin_first(double& value, int& idx) {
         value = take ? ov : value;
         idx = take ? oi : idx;
     }
+    // Every lane holds the same result; lane 0's, read back, is provably uniform.
+    value = readlane(value, 0);
+    idx = readlane(idx, 0);
 }
 
 // Register vectors.  Element selection by a run-time (uniform) index stays in registers for
@@ -172,6 +181,57 @@ __device__ __forceinline__ double hypot_fixed(double a, double b) {
     }
     const double r = small / big;
     return big * sqrt(1.0 + r * r);
+}
+
+// Division by a divisor used many times, split in the two halves of gfx950's double-division
+// sequence (V_DIV_SCALE, V_RCP and two Newton steps on the reciprocal; then the quotient a y, the
+// exact residual a - d (a y), the correction by V_DIV_FMAS and V_DIV_FIXUP): the reciprocal is
+// refined once per divisor and each quotient then costs three operations.  Where the sequence's
+// scaling and fix-up steps are the identity -- a divisor in [2^-100, 2^100) and a dividend of
+// magnitude in [2^-800, 2^600), so that no operand, the reciprocal or the quotient is subnormal or
+// near overflow and the exponents differ by less than 768 -- the three operations are the
+// sequence's own and the quotient is a / d to the bit.  Outside that range (zero, subnormal, huge
+// or non-finite operands) the callers divide.
+__device__ __forceinline__ int exponent_field(double x) {
+    return static_cast<int>((__double_as_longlong(x) >> 52) & 0x7ff);
+}
+
+__device__ __forceinline__ bool divisor_in_range(double d) {
+    return static_cast<unsigned>(exponent_field(d) - (1023 - 100)) < 200u && d > 0.0;
+}
+
+__device__ __forceinline__ bool dividend_in_range(double a) {
+    return static_cast<unsigned>(exponent_field(a) - (1023 - 800)) < 1400u;
+}
+
+// The refined reciprocal of d.
+__device__ __forceinline__ double division_reciprocal(double d) {
+    double y = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-d, y, 1.0);
+    return fma(y, e, y);
+}
+
+// a / d from y = division_reciprocal(d), both operands in range.
+__device__ __forceinline__ double divide_in_range(double a, double d, double y) {
+    const double q = a * y;
+    const double r = fma(-d, q, a);
+    return fma(r, y, q);
+}
+
+// a / d given y = division_reciprocal(d).
+__device__ __forceinline__ double divide_by(double a, double d, double y) {
+    return (divisor_in_range(d) && dividend_in_range(a)) ? divide_in_range(a, d, y) : a / d;
+}
+
+// The triangular solves' a != 0 ? a / d : a, with d_ok = divisor_in_range(d) known (a zero
+// dividend is never in range).
+__device__ __forceinline__ double divide_nonzero_by(double a, double d, double y, bool d_ok) {
+    if (d_ok && dividend_in_range(a)) {
+        return divide_in_range(a, d, y);
+    }
+    return a != 0.0 ? a / d : a;
 }
 
 } // namespace lsd
