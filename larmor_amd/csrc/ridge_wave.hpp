@@ -101,9 +101,14 @@ struct RidgeNnlsWave {
     __device__ __forceinline__ double2 diag_at(int j) const {
         return *reinterpret_cast<const double2*>(diag + 2 * j);
     }
+    // a != 0 ? a / L_jj : a; FAST: every diagonal is in the divisor range.
+    template <bool FAST>
     __device__ __forceinline__ double solve_divide(double a, int j) const {
         const double2 d = diag_at(j);
-        return divide_nonzero_by(a, d.x, d.y, diag_ok);
+        if (FAST && dividend_in_range(a)) {
+            return divide_in_range(a, d.x, d.y);
+        }
+        return a != 0.0 ? a / d.x : a;
     }
 
     __device__ __forceinline__ int col(int t) const { return lane_id() + WAVE * t; }
@@ -114,72 +119,75 @@ struct RidgeNnlsWave {
         return readlane(L[base + lane_id()], i % WAVE);
     }
 
-    // L w = b on positions [0, kk), column-oriented; returns sum of w_j^2 in order.
-    __device__ __forceinline__ double forward(LaneVec<NSK>& b, int kk_) {
+    // The column-oriented triangular solves update every lane's slot with a select instead of a
+    // branch: the factor read L(i, j) of a lane outside the update stays inside the slab (rows
+    // up to 64 NSK - 1 of a column lie within the column stride and the slack after the last
+    // column) and its product is discarded.
+
+    // L w = b on positions [p0, kk) when positions [0, p0) of b already hold their final values
+    // (p0 = 0: the whole solve); returns the sum of the w_j^2 in order (for p0 = 0).  Positions
+    // from p0 on receive the same updates, in the same order, as in the whole solve.
+    template <bool FAST>
+    __device__ __forceinline__ double forward_impl(LaneVec<NSK>& b, int p0, int kk) {
         const int l = lane_id();
-        const int kk = __builtin_amdgcn_readfirstlane(kk_);
         double sq = 0.0;
         for (int j = 0; j < kk; ++j) {
             const double bj = b.get(j);
-            const double wj = solve_divide(bj, j);
+            const double wj = j >= p0 ? solve_divide<FAST>(bj, j) : bj;
             sq = sq + wj * wj;
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
                 const int i = l + WAVE * u;
-                if (i == j) {
-                    b.v[u] = wj;
-                } else if (i > j && i < kk) {
-                    b.v[u] = b.v[u] - Lat(i, j) * wj;
-                }
+                const double updated = b.v[u] - Lat(i, j) * wj;
+                const bool below = i > j && i < kk && i >= p0;
+                b.v[u] = i == j ? wj : (below ? updated : b.v[u]);
             }
         }
         return sq;
     }
 
-    // L w = b on positions [0, kk) when positions [0, p0) of b already hold their final values
-    // (a prefix of the same solve): the remaining positions receive the same updates, in the same
-    // order, as in forward().
+    __device__ __forceinline__ double forward(LaneVec<NSK>& b, int kk_) {
+        const int kk = __builtin_amdgcn_readfirstlane(kk_);
+        return diag_ok ? forward_impl<true>(b, 0, kk) : forward_impl<false>(b, 0, kk);
+    }
+
     __device__ __forceinline__ void forward_from(LaneVec<NSK>& b, int p0_, int kk_) {
         const int p0 = __builtin_amdgcn_readfirstlane(p0_);
         const int kk = __builtin_amdgcn_readfirstlane(kk_);
         if (p0 >= kk) {
             return;
         }
+        if (diag_ok) {
+            forward_impl<true>(b, p0, kk);
+        } else {
+            forward_impl<false>(b, p0, kk);
+        }
+    }
+
+    // L^T z = w on positions [0, kk), column-oriented.  A lane past the factor reads row 0 of
+    // column j instead of the row its index names (outside the slab).
+    template <bool FAST>
+    __device__ __forceinline__ void backward_impl(LaneVec<NSK>& b, int kk) {
         const int l = lane_id();
-        for (int j = 0; j < kk; ++j) {
-            const double bj = b.get(j);
-            double wj = bj;
-            if (j >= p0) {
-                wj = solve_divide(bj, j);
-            }
+        for (int j = kk - 1; j >= 0; --j) {
+            const double wj = b.get(j);
+            const double zj = solve_divide<FAST>(wj, j);
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
                 const int i = l + WAVE * u;
-                if (i == j) {
-                    b.v[u] = wj;
-                } else if (i > j && i < kk && i >= p0) {
-                    b.v[u] = b.v[u] - Lat(i, j) * wj;
-                }
+                const int ir = i < KCAP ? i : 0;
+                const double updated = b.v[u] - Lat(j, ir) * zj;
+                b.v[u] = i == j ? zj : (i < j ? updated : b.v[u]);
             }
         }
     }
 
-    // L^T z = w on positions [0, kk), column-oriented.
     __device__ __forceinline__ void backward(LaneVec<NSK>& b, int kk_) {
-        const int l = lane_id();
         const int kk = __builtin_amdgcn_readfirstlane(kk_);
-        for (int j = kk - 1; j >= 0; --j) {
-            const double wj = b.get(j);
-            const double zj = solve_divide(wj, j);
-#pragma unroll
-            for (int u = 0; u < NSK; ++u) {
-                const int i = l + WAVE * u;
-                if (i == j) {
-                    b.v[u] = zj;
-                } else if (i < j) {
-                    b.v[u] = b.v[u] - Lat(j, i) * zj;
-                }
-            }
+        if (diag_ok) {
+            backward_impl<true>(b, kk);
+        } else {
+            backward_impl<false>(b, kk);
         }
     }
 
