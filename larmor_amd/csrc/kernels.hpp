@@ -164,6 +164,7 @@ struct FodArgs {
     int* status;              // per voxel, OR-ed
     VoxelSet vs;
     SolveScratch scratch;
+    const double* corr;       // (list position, n) precomputed correlations, or null
 };
 
 // The fast low-resolution instantiation is asked for 5 waves per SIMD (it sits a few registers
@@ -188,19 +189,30 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     for (long it = gw; it < count; it += n_waves) {
         const long v = A.vs.list ? A.vs.list[it] : it;
         RowVec<RPL> s;
-        load_signal<RPL>(A.signals, v, A.m, s);
-        if (A.multi) {
-            double p[NQ];
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                p[q] = q < A.ci ? A.iso[v * A.ci + q] : 0.0;
-            }
-            RowVec<RPL> a;
-            anisotropic_signal<RPL>(s, A.weights, A.w_iso, A.m, A.ci, p, a);
-            s = a;
-        }
         double corr[NCPL];
-        correlations<NCPL, RPL>(A.design, s, 0.0, corr);
+        if (!GLOBAL_L && A.corr != nullptr) {
+            // The correlation pass computed them (the fast kernels carry no QR fits, the only
+            // other use of the signal).
+            s.fill(0.0);
+#pragma unroll
+            for (int t = 0; t < NCPL; ++t) {
+                const int c = l + WAVE * t;
+                corr[t] = c < A.design.n ? A.corr[it * A.design.n + c] : 0.0;
+            }
+        } else {
+            load_signal<RPL>(A.signals, v, A.m, s);
+            if (A.multi) {
+                double p[NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    p[q] = q < A.ci ? A.iso[v * A.ci + q] : 0.0;
+                }
+                RowVec<RPL> a;
+                anisotropic_signal<RPL>(s, A.weights, A.w_iso, A.m, A.ci, p, a);
+                s = a;
+            }
+            correlations<NCPL, RPL>(A.design, s, 0.0, corr);
+        }
         RidgeNnlsWave<NCPL, KCAP, RPLQ, GLOBAL_L> solver(A.design, A.tol, 3 * A.design.n, 0.0, L,
                                                          qrs);
         const int st = solver.template run<RPL>(corr, s);
@@ -222,6 +234,107 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
 }
 
 // The tissue fractions of a list of voxels from their FOD amplitudes (fractions.cpp:45-76).
+// The correlations K^T a of a list of voxels, a the signal (single-tissue) or the anisotropic
+// signal max(w s - W_iso p, 0) (multi_tissue.cpp:20-24), one voxel per lane: the design rows are
+// wave-uniform, so each multiply takes its K entry as a scalar operand and a wave's 64 voxels
+// share every instruction.  Each correlation is the same in-order sum of K(r, c) a_r, rounded
+// the same way, as the FOD kernels' own (correlations() above).  Columns go in passes of NCB.
+struct CorrArgs {
+    const double* signals;    // (n_vox, m)
+    int m;
+    int multi;
+    const double* weights;    // (m), multi only
+    const double* w_iso;      // (m, ci), multi only
+    const double* iso;        // (n_vox, ci), multi only
+    int ci;
+    const double* kernel;     // (m, n), then at least NCB readable doubles
+    int n;
+    const int* list;          // voxels by list position, or null for all
+    const int* list_count;
+    long n_vox;
+    double* corr;             // (list position, n)
+};
+
+constexpr int CORR_COLUMNS = 48; // columns per pass of the correlation kernel
+constexpr int CORR_ROWS = 16;    // signal rows staged per LDS tile
+
+// Each wave takes 64 consecutive list positions.  Their signals are staged through the wave's
+// LDS tile CORR_ROWS rows at a time with coalesced loads (each load instruction covers 16 rows
+// of 4 voxels) and read back one voxel per lane.
+template <int NCB>
+__global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(CorrArgs A) {
+    __shared__ double tile[WAVES_PER_BLOCK][CORR_ROWS][WAVE + 1];
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
+    const int l = lane_id();
+    const long count = A.list ? static_cast<long>(*A.list_count) : A.n_vox;
+    const long first = (static_cast<long>(blockIdx.x) * WAVES_PER_BLOCK + w) * WAVE;
+    const long stride = static_cast<long>(gridDim.x) * WAVES_PER_BLOCK * WAVE;
+    for (long base = first; base < count; base += stride) {
+        const long it = base + l;
+        const bool on = it < count;
+        const long v = on ? (A.list ? A.list[it] : it) : 0;
+        double p[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            p[q] = (on && A.multi && q < A.ci) ? A.iso[v * A.ci + q] : 0.0;
+        }
+        for (int c0 = 0; c0 < A.n; c0 += NCB) {
+            double acc[NCB];
+#pragma unroll
+            for (int j = 0; j < NCB; ++j) {
+                acc[j] = 0.0;
+            }
+            for (int r0 = 0; r0 < A.m; r0 += CORR_ROWS) {
+                const int rows = min(CORR_ROWS, A.m - r0);
+#pragma unroll 4
+                for (int k = 0; k < CORR_ROWS; ++k) {
+                    const int e = k * WAVE + l;
+                    const int j = e / CORR_ROWS;
+                    const int rr = e % CORR_ROWS;
+                    const long vj = __shfl(v, j);
+                    const bool load = base + j < count && rr < rows;
+                    tile[w][rr][j] = load ? A.signals[vj * A.m + r0 + rr] : 0.0;
+                }
+                wave_lds_sync();
+                for (int rr = 0; rr < rows; ++rr) {
+                    const int r = r0 + rr;
+                    const double sr = tile[w][rr][l];
+                    double ar = sr;
+                    if (A.multi) {
+                        const double ws = sr * A.weights[r];
+                        double wp = 0.0;
+#pragma unroll
+                        for (int q = 0; q < NQ; ++q) {
+                            if (q < A.ci) {
+                                wp = wp + A.w_iso[static_cast<long>(r) * A.ci + q] * p[q];
+                            }
+                        }
+                        ar = fmax(ws - wp, 0.0);
+                    }
+                    // The row's pass through the constant address space (wave-uniform loads).
+                    // Columns past n read the next row (or the padding after the last) and are
+                    // not stored.
+                    using ConstRow = const __attribute__((address_space(4))) double*;
+                    const ConstRow krow = (ConstRow)(A.kernel + static_cast<long>(r) * A.n + c0);
+#pragma unroll
+                    for (int j = 0; j < NCB; ++j) {
+                        acc[j] = acc[j] + krow[j] * ar;
+                    }
+                }
+                wave_lds_sync();
+            }
+            if (on) {
+#pragma unroll
+                for (int j = 0; j < NCB; ++j) {
+                    if (c0 + j < A.n) {
+                        A.corr[it * A.n + c0 + j] = acc[j];
+                    }
+                }
+            }
+        }
+    }
+}
+
 // A round of the low-resolution loop (multi_tissue.cpp:59-78) fits the lumped fractions with the
 // default tolerance, keeps the isotropic ones, counts the round and appends the voxel to the
 // next round's list unless no fraction moved by fraction_tol; the final fit (multi_tissue.cpp:
@@ -246,8 +359,12 @@ struct FractionArgs {
     long n_vox;
 };
 
+// The low-resolution fraction kernel is asked for 5 waves per SIMD.
+constexpr int fraction_waves_per_eu(int ncpl) { return ncpl == 1 ? 5 : 1; }
+
 template <int NCPL, int RPL>
-__global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) fraction_kernel(FractionArgs A) {
+__global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
+    __attribute__((amdgpu_waves_per_eu(fraction_waves_per_eu(NCPL)))) fraction_kernel(FractionArgs A) {
     extern __shared__ double smem[];
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
@@ -443,6 +560,7 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
         rerun.scratch = SolveScratch{static_cast<double*>(*rerun_scratch), per};
         rerun.vs.overflow_list = nullptr;
         rerun.vs.overflow_count = nullptr;
+        rerun.corr = nullptr;
         return launch(rerun_name, Family<NCPL, RPL, KG, true>::kernel, rerun, all_waves, 0,
                       stream);
     }
@@ -457,6 +575,7 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
     rerun.vs.list_count = d_overflow_count;
     rerun.vs.overflow_list = nullptr;
     rerun.vs.overflow_count = nullptr;
+    rerun.corr = nullptr; // indexed by the fast launch's list positions
     return launch(rerun_name, Family<NCPL, RPL, KG, true>::kernel, rerun, RERUN_WAVES, 0, stream);
 }
 

@@ -400,7 +400,10 @@ int lsd_solver_create_l2nnls(const double* kernel, int64_t m, int64_t n, double 
     }
     // nnls.cpp:280-292: the normal equations with no ridge, or a well-conditioned one.
     s->normal_equations = (r2 == 0.0 || fro + r2 <= 1e8 * r2) ? 1 : 0;
-    std::vector<double> k(kernel, kernel + m * n);
+    // The device copy carries CORR_COLUMNS zeros after the last row: the correlation pass reads
+    // whole column passes from each row.
+    std::vector<double> k(static_cast<size_t>(m * n + CORR_COLUMNS), 0.0);
+    std::copy(kernel, kernel + m * n, k.begin());
     int rc = upload(&s->d_kernel, k);
     if (!rc) rc = upload(&s->d_gram, gram);
     if (rc) {
@@ -589,6 +592,7 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
     double* iso = nullptr;
     double* prev = nullptr;
     double* x_lr = nullptr;
+    double* corr_lr = nullptr;
     rc = mem.get(&counters, n_counters);
     if (!rc) rc = mem.get(&olist, nv);
     if (!rc) rc = mem.get(&lists, 2 * nv);
@@ -596,6 +600,7 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
     if (!rc) rc = mem.get(&iso, std::max(1L, nv * ci));
     if (!rc) rc = mem.get(&prev, nv * nlf);
     if (!rc) rc = mem.get(&x_lr, nv * lr->ca);
+    if (!rc) rc = mem.get(&corr_lr, std::max(1L, nv * lr->ca));
     if (!rc && fods == nullptr) rc = mem.get(&fods, nv * hr->ca);
     if (!rc && rounds == nullptr) rc = mem.get(&rounds, nv);
     if (!rc && status == nullptr) rc = mem.get(&status, nv);
@@ -638,9 +643,28 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
         fod.w_iso = lr->d_w_iso;
         fod.x_out = x_lr;
         fod.vs = VoxelSet{nv, list, count, olist, ovf_count};
+        CorrArgs ca{};
+        ca.signals = signals;
+        ca.m = static_cast<int>(m);
+        ca.multi = 1;
+        ca.weights = weights;
+        ca.w_iso = lr->d_w_iso;
+        ca.iso = iso;
+        ca.ci = ci;
+        ca.kernel = fod.design.kernel;
+        ca.n = fod.design.n;
+        ca.list = list;
+        ca.list_count = count;
+        ca.n_vox = nv;
+        ca.corr = corr_lr;
+        rc = launch_fill("fod_lr_corr", correlation_kernel<CORR_COLUMNS>, ca,
+                         static_cast<int>((nv + WAVE - 1) / WAVE), 0, stream);
+        if (rc) break;
+        fod.corr = corr_lr;
         void* rerun_scratch = nullptr;
         rc = dispatch_fod("fod_lr", "fod_lr_rerun", ncpl_lr, rpl, fod, fod, waves, olist,
                           ovf_count, stream, &rerun_scratch);
+        fod.corr = nullptr;
         mem.adopt(rerun_scratch);
         if (rc) break;
         fr.x = x_lr;
