@@ -11,11 +11,19 @@ and the projection of the amplitudes onto SH up to lmax 8 (all liblarmor_sd.so k
 
 Prints one JSON line.  `roofline` prices the dominant kernel (the one with the most time per
 step) at its algorithmic FP64 flops against MI355X's FP64 peak, from the kernel's own launch
-durations measured with HIP events on its launch stream: 2 m n per voxel solve for the FOD
-kernels (the correlation K^T W s with the n-column design) and 2 m n per voxel fit for the
-fraction kernels (the anisotropic signal K x).  The low-resolution kernels run once per voxel
-and round, so their unit count is the sum of the voxels' rounds.  `traffic` is the kernel's HBM
-bytes per launch from the committed rocprofv3 counter summary (null when it has no entry).
+durations measured with HIP events on its launch stream.  The algorithmic flops of a kernel:
+- fod_lr_corr: 2 m n_lr per voxel and round (the correlations K^T a of the low-resolution
+  design);
+- fod_lr: the active-set work of its NNLS solves, which the kernel counts by formula as they
+  run (per append k^2 + 4k + 4, per solve k^2 plus the forward part, 2 n per non-zero
+  coefficient of each gradient, 3 n per inner step, 8 + 6 rows per Givens rotation; k the
+  factor size) and reports through the library's profiling hook;
+- fraction_loop: 2 m n_lr per voxel and round (the group column K x);
+- fod_hr: 2 m n_hr per voxel (its correlations) plus its counted active-set work;
+- fraction_final: 2 m n_hr per voxel.
+The low-resolution kernels run once per voxel and round, so their unit count is the sum of the
+voxels' rounds.  `traffic` is the kernel's HBM bytes per launch from the committed rocprofv3
+counter summary (null when it has no entry).
 `cpu_baseline` is the CPU
 FP64 oracle (a C++ restatement of the reference algorithm, `kind: port`) on a bounded random
 sample of the same voxels, on the host's cores.
@@ -172,14 +180,18 @@ def main() -> int:
     n_lr = lr.w_k_aniso.shape[1]
     n_hr = hr.w_k_aniso.shape[1]
     # Algorithmic FP64 flops of each kernel per step.
+    def counted(name):
+        return kernel_times.get(name, (0.0, 0, 0.0))[2] / args.steps
+
     step_flops = {
-        "fod_lr": total_rounds * 2.0 * m * n_lr,
+        "fod_lr_corr": total_rounds * 2.0 * m * n_lr,
+        "fod_lr": counted("fod_lr"),
         "fraction_loop": total_rounds * 2.0 * m * n_lr,
-        "fod_hr": n_vox * 2.0 * m * n_hr,
+        "fod_hr": n_vox * 2.0 * m * n_hr + counted("fod_hr"),
         "fraction_final": n_vox * 2.0 * m * n_hr,
     }
-    dominant = max(step_flops, key=lambda k: kernel_times.get(k, (0.0, 0))[0])
-    dom_ms, dom_launches = kernel_times[dominant]
+    dominant = max(step_flops, key=lambda k: kernel_times.get(k, (0.0, 0, 0.0))[0])
+    dom_ms, dom_launches, _ = kernel_times[dominant]
     launches_per_step = dom_launches / args.steps
     avg_launch_s = dom_ms / dom_launches / 1e3
     flops_per_launch = step_flops[dominant] / launches_per_step
@@ -187,6 +199,9 @@ def main() -> int:
     print("kernel ms per step: " + ", ".join(
         f"{k} {v[0] / args.steps:.2f} ({v[1] // args.steps} launches)"
         for k, v in kernel_times.items()), file=sys.stderr, flush=True)
+    print("counted FP64 work per step: " + ", ".join(
+        f"{k} {v[2] / args.steps / 1e9:.2f} GFLOP" for k, v in kernel_times.items() if v[2] > 0),
+        file=sys.stderr, flush=True)
     print(f"rounds: mean {total_rounds / n_vox:.2f}; status bits: "
           f"{torch.bincount(status.to(torch.int64)).tolist()}", file=sys.stderr, flush=True)
 
@@ -223,6 +238,7 @@ def main() -> int:
             "kernel_avg_launch_ms": avg_launch_s * 1e3,
             "launches_per_step": launches_per_step,
             "flops_per_launch": flops_per_launch,
+            "peak_kind": "FP64 (MI355X's vector and matrix FP64 peaks are both 78.6 TFLOP/s)",
         },
     }
     if not args.no_cpu_baseline:

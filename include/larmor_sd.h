@@ -41,13 +41,15 @@ int lsd_abi_version(void);
 
 /*
  * Opt-in kernel timing (no counterpart in the reference, which has no profiling hooks): with
- * profiling on, every kernel launch is bracketed by HIP events on its own stream;
- * lsd_profile_collect waits for them, and returns per kernel name (newline-separated in names)
- * the summed milliseconds and launch counts, then forgets them.
+ * profiling on, every kernel launch is bracketed by HIP events on its own stream, and the FOD
+ * kernels add up the FP64 operations of the solves they complete (counted by formula from each
+ * step's factor size: the algorithmic work).  lsd_profile_collect waits for them and returns per
+ * kernel name (newline-separated in names) the summed milliseconds, launch counts and counted
+ * work (work may be NULL; 0 for kernels that count none), then forgets them.
  */
 int lsd_profile_enable(int on);
 int lsd_profile_collect(char* names, int64_t names_bytes, double* ms, int64_t* counts,
-                        int64_t max_entries, int64_t* n_entries);
+                        double* work, int64_t max_entries, int64_t* n_entries);
 
 /*
  * An L2-regularised NNLS FOD solver for kernel K (host, m x n): its device copy, the Gram matrix

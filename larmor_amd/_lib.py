@@ -23,7 +23,7 @@ SIGNATURES = {
     "lsd_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "lsd_profile_collect": (
         ctypes.c_int,
-        [ctypes.c_char_p, ctypes.c_int64, _d, _i64, ctypes.c_int64, _i64],
+        [ctypes.c_char_p, ctypes.c_int64, _d, _i64, _d, ctypes.c_int64, _i64],
     ),
     "lsd_solver_create_l2nnls": (
         ctypes.c_int,
@@ -140,16 +140,18 @@ def profile_enable(on: bool = True) -> None:
     check(load().lsd_profile_enable(1 if on else 0))
 
 
-def profile_collect() -> dict[str, tuple[float, int]]:
-    """{kernel name: (summed ms, launches)} since the last collect; waits for the events."""
+def profile_collect() -> dict[str, tuple[float, int, float]]:
+    """{kernel name: (summed ms, launches, counted FP64 work)} since the last collect; waits for
+    the events.  The work is 0 for kernels that do not count theirs."""
     lib = load()
     names = ctypes.create_string_buffer(4096)
     ms = (ctypes.c_double * 64)()
     counts = (ctypes.c_int64 * 64)()
+    work = (ctypes.c_double * 64)()
     n = ctypes.c_int64()
-    check(lib.lsd_profile_collect(names, 4096, ms, counts, 64, ctypes.byref(n)))
+    check(lib.lsd_profile_collect(names, 4096, ms, counts, work, 64, ctypes.byref(n)))
     labels = names.value.decode().split("\n")
-    return {labels[i]: (ms[i], counts[i]) for i in range(n.value)}
+    return {labels[i]: (ms[i], counts[i], work[i]) for i in range(n.value)}
 
 
 def declared_symbols() -> list[str]:

@@ -165,6 +165,7 @@ struct FodArgs {
     VoxelSet vs;
     SolveScratch scratch;
     const double* corr;       // (list position, n) precomputed correlations, or null
+    unsigned long long* work; // profiled launches: the completed solves' FP64 operations
 };
 
 // The fast low-resolution instantiation is asked for 5 waves per SIMD (it sits a few registers
@@ -186,6 +187,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
                          : smem + w * lds_factor_doubles(KCAP, RPL);
     const long count = voxel_count(A.vs);
     const int l = lane_id();
+    unsigned long long wave_work = 0;
     for (long it = gw; it < count; it += n_waves) {
         const long v = A.vs.list ? A.vs.list[it] : it;
         RowVec<RPL> s;
@@ -220,6 +222,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
             record_overflow(A.vs, v);
             continue;
         }
+        wave_work += solver.work;
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {
             const int c = l + WAVE * t;
@@ -230,6 +233,9 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
         if (l == 0 && st != 0) {
             A.status[v] |= st;
         }
+    }
+    if (A.work != nullptr && l == 0 && wave_work != 0) {
+        atomicAdd(A.work, wave_work);
     }
 }
 
@@ -491,29 +497,48 @@ constexpr long slab_doubles(bool global_l) {
            (global_l ? static_cast<long>(lds_factor_doubles(KCAP, RPL)) : 0);
 }
 
-// Opt-in per-launch timing: events recorded on the launch stream around each kernel.
-
-// Opt-in per-launch timing: events recorded on the launch stream around each kernel.
+// Opt-in per-launch timing: events recorded on the launch stream around each kernel, and for
+// the kernels that count their algorithmic work, a device counter slot per launch.
 struct ProfiledLaunch {
     const char* name;
     hipEvent_t start;
     hipEvent_t stop;
+    unsigned long long* work;
 };
 extern bool g_profiling;
 extern std::vector<ProfiledLaunch> g_launches;
+extern unsigned long long* g_work_slots; // one zeroed counter per profiled launch
+constexpr long WORK_SLOTS = 16384;
+
+// Gives a launch's arguments its work counter (kernels that count none ignore it).
+template <class Args>
+inline bool attach_work(Args&, unsigned long long*) {
+    return false;
+}
+inline bool attach_work(FodArgs& a, unsigned long long* slot) {
+    a.work = slot;
+    return slot != nullptr;
+}
 
 // Launches kern over `waves` waves; lds_bytes of dynamic LDS per block.
 template <class Kern, class Args>
 inline int launch(const char* name, Kern kern, const Args& args, int waves, size_t lds_bytes,
            hipStream_t stream) {
     const int blocks = (waves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
-    ProfiledLaunch rec{name, nullptr, nullptr};
+    ProfiledLaunch rec{name, nullptr, nullptr, nullptr};
+    Args a = args;
     if (g_profiling) {
         LSD_HIP(hipEventCreate(&rec.start));
         LSD_HIP(hipEventCreate(&rec.stop));
         LSD_HIP(hipEventRecord(rec.start, stream));
+        const long at = static_cast<long>(g_launches.size());
+        unsigned long long* slot =
+            (g_work_slots != nullptr && at < WORK_SLOTS) ? g_work_slots + at : nullptr;
+        if (attach_work(a, slot)) {
+            rec.work = slot;
+        }
     }
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(WAVE * WAVES_PER_BLOCK), lds_bytes, stream, args);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(WAVE * WAVES_PER_BLOCK), lds_bytes, stream, a);
     LSD_HIP(hipGetLastError());
     if (g_profiling) {
         LSD_HIP(hipEventRecord(rec.stop, stream));

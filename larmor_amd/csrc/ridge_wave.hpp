@@ -91,6 +91,9 @@ struct RidgeNnlsWave {
     // (after the factor and its slack); diag_ok: every diagonal in the divisor range.
     double* diag;
     bool diag_ok;
+    // The solve's FP64 operations, counted by formula as its steps run (uniform): the
+    // algorithmic work that profiled launches report.
+    unsigned long long work;
 
     __device__ RidgeNnlsWave(const RidgeDesignDev& design, double tol_, int max_inner_, double l1_,
                              double* L_, double* qrs_)
@@ -225,6 +228,7 @@ struct RidgeNnlsWave {
             b.v[u] = p < k ? D.gram[static_cast<long>(c) * D.n + fcol.v[u]] : 0.0;
         }
         const double sq = forward(b, k);
+        work += static_cast<unsigned long long>(k) * k + 4ull * k + 4ull;
         const double gcc = D.gram[static_cast<long>(c) * D.n + c];
         const double pivot = gcc - sq;
         if (!(pivot > MIN_RELATIVE_PIVOT * gcc)) {
@@ -290,6 +294,7 @@ struct RidgeNnlsWave {
             const double diagonal = Luni(column, column);
             const double above = Luni(column, column + 1);
             const double length = hypot_fixed(diagonal, above);
+            work += 8ull + 6ull * static_cast<unsigned long long>(k - 1 - column);
             const double rl = division_reciprocal(length);
             const double cosine = divide_by(diagonal, length, rl);
             const double sine = divide_by(above, length, rl);
@@ -355,6 +360,11 @@ struct RidgeNnlsWave {
             const double v = gather_cols(corr, p < k ? fcol.v[u] : 0);
             b.v[u] = p < k ? (p < y_valid ? static_cast<double>(y.v[u]) : v) : 0.0;
         }
+        if (y_valid < k) {
+            const unsigned long long fresh = static_cast<unsigned long long>(k - y_valid);
+            work += fresh * fresh + 2ull * static_cast<unsigned long long>(y_valid) * fresh;
+        }
+        work += static_cast<unsigned long long>(k) * k;
         forward_from(b, y_valid, k);
         y = b;
         y_valid = k;
@@ -434,6 +444,9 @@ struct RidgeNnlsWave {
             d.v[t] = v;
         }
         LaneVec<NSK> coef;
+        // Householder QR of the rows x ka stack and the projection of the data.
+        work += 2ull * static_cast<unsigned long long>(rows) * ka * ka +
+                4ull * static_cast<unsigned long long>(rows) * ka;
         least_squares_wave<NSK>(A, rows, ka, d, coef);
         for (int p = 0; p < ka; ++p) {
             const int c = idx.get(p);
@@ -517,6 +530,7 @@ struct RidgeNnlsWave {
         y_valid = 0;
         y.fill(0.0);
         diag_ok = true;
+        work = 0;
         factor_failed = D.normal_equations == 0;
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {
@@ -584,6 +598,7 @@ struct RidgeNnlsWave {
                     }
                 }
                 wave_argmin_first(sv, si);
+                work += 3ull * static_cast<unsigned long long>(D.n);
 #pragma unroll
                 for (int t = 0; t < NCPL; ++t) {
                     x[t] = x[t] + sv * (cand[t] - x[t]);
@@ -610,6 +625,7 @@ struct RidgeNnlsWave {
                     const int b = __ffsll(static_cast<long long>(mask)) - 1;
                     mask &= mask - 1;
                     const int c = WAVE * t2 + b;
+                    work += 2ull * static_cast<unsigned long long>(D.n);
                     const double xc = readlane(x[t2], b);
                     const double* grow = D.gram + static_cast<long>(c) * D.n;
 #pragma unroll

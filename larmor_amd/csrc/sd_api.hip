@@ -18,6 +18,7 @@ namespace lsd {
 thread_local std::string g_error;
 bool g_profiling = false;
 std::vector<ProfiledLaunch> g_launches;
+unsigned long long* g_work_slots = nullptr;
 
 __global__ void zero_ints(int* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -319,16 +320,37 @@ const char* lsd_last_error(void) { return lsd::g_error.c_str(); }
 int lsd_abi_version(void) { return LSD_ABI_VERSION; }
 
 int lsd_profile_enable(int on) {
+    if (on != 0 && lsd::g_work_slots == nullptr) {
+        // The work counters of the profiled launches (allocated once; zero between collects).
+        void* p = nullptr;
+        LSD_HIP(hipMalloc(&p, sizeof(unsigned long long) * lsd::WORK_SLOTS));
+        LSD_HIP(hipMemset(p, 0, sizeof(unsigned long long) * lsd::WORK_SLOTS));
+        lsd::g_work_slots = static_cast<unsigned long long*>(p);
+    }
     lsd::g_profiling = on != 0;
     return 0;
 }
 
 int lsd_profile_collect(char* names, int64_t names_bytes, double* ms, int64_t* counts,
-                        int64_t max_entries, int64_t* n_entries) {
-    // Sum the elapsed time of each kernel name over the recorded launches, in first-seen order.
+                        double* work, int64_t max_entries, int64_t* n_entries) {
+    // Sum the elapsed time and counted work of each kernel name over the recorded launches, in
+    // first-seen order.
     std::vector<std::string> order;
     std::vector<double> total;
     std::vector<int64_t> count;
+    std::vector<double> flops;
+    std::vector<unsigned long long> slots;
+    if (lsd::g_work_slots != nullptr && !lsd::g_launches.empty()) {
+        for (auto& rec : lsd::g_launches) {
+            LSD_HIP(hipEventSynchronize(rec.stop));
+        }
+        const size_t n_slots =
+            std::min<size_t>(lsd::g_launches.size(), static_cast<size_t>(lsd::WORK_SLOTS));
+        slots.resize(n_slots);
+        LSD_HIP(hipMemcpy(slots.data(), lsd::g_work_slots, sizeof(unsigned long long) * n_slots,
+                          hipMemcpyDeviceToHost));
+        LSD_HIP(hipMemset(lsd::g_work_slots, 0, sizeof(unsigned long long) * n_slots));
+    }
     for (auto& rec : lsd::g_launches) {
         LSD_HIP(hipEventSynchronize(rec.stop));
         float elapsed = 0.0f;
@@ -341,9 +363,16 @@ int lsd_profile_collect(char* names, int64_t names_bytes, double* ms, int64_t* c
             order.push_back(rec.name);
             total.push_back(0.0);
             count.push_back(0);
+            flops.push_back(0.0);
         }
         total[at] += elapsed;
         count[at] += 1;
+        if (rec.work != nullptr) {
+            const size_t slot = static_cast<size_t>(rec.work - lsd::g_work_slots);
+            if (slot < slots.size()) {
+                flops[at] += static_cast<double>(slots[slot]);
+            }
+        }
         (void)hipEventDestroy(rec.start);
         (void)hipEventDestroy(rec.stop);
     }
@@ -353,6 +382,9 @@ int lsd_profile_collect(char* names, int64_t names_bytes, double* ms, int64_t* c
     for (int64_t i = 0; i < n; ++i) {
         ms[i] = total[i];
         counts[i] = count[i];
+        if (work != nullptr) {
+            work[i] = flops[i];
+        }
         joined += order[i] + "\n";
     }
     *n_entries = n;

@@ -124,3 +124,34 @@ def test_mritoolkit_fixture_on_gpu():
     mrt = f["fod"].astype(np.float64)
     assert (np.abs(fods - mrt).max(axis=1) / mrt.max(axis=1)).max() <= 1e-3
     np.testing.assert_allclose(fr, f["fractions"].astype(np.float64), rtol=0, atol=1e-4)
+
+
+def test_profiling_counts_work_and_leaves_results_unchanged():
+    from larmor_amd import _lib
+
+    signals, w, lr, hr = _mfod_setup(seed=7)
+    s = np.ascontiguousarray(signals[:256])
+    lr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(lr.w_k_aniso), 0.0, 1e-6)
+    hr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(hr.w_k_aniso), 0.1, 1e-6)
+    args = (w, _deconv(lr), lr_solver, _deconv(hr), hr_solver, 50, 1e-7, 1e-2, 1)
+    plain_fods, plain_fr = _core.fit_multi_tissue(s, *args)
+    runs = []
+    _lib.profile_enable(True)
+    try:
+        _lib.profile_collect()
+        for _ in range(2):
+            fods, fr = _core.fit_multi_tissue(s, *args)
+            np.testing.assert_array_equal(fods, plain_fods)
+            np.testing.assert_array_equal(fr, plain_fr)
+            runs.append(_lib.profile_collect())
+    finally:
+        _lib.profile_enable(False)
+    first, second = runs
+    print("profiled", {k: v[1:] for k, v in first.items()}, flush=True)
+    for name in ("fod_lr_corr", "fod_lr", "fraction_loop", "fod_hr", "fraction_final"):
+        assert name in first and first[name][0] > 0.0 and first[name][1] > 0
+    # The FOD kernels count their solves' work; the others count none; the count is a
+    # property of the solves, so it repeats exactly.
+    assert first["fod_lr"][2] > 0 and first["fod_hr"][2] > 0
+    assert first["fraction_loop"][2] == 0 and first["fod_lr_corr"][2] == 0
+    assert [v[1:] for v in first.values()] == [v[1:] for v in second.values()]
