@@ -5,7 +5,8 @@
 //  - the gradient is C^T (d - C x) = K^T s - l1 - (K^T K + r^2 I) x, summed over the non-zero
 //    coefficients in column order (nnls.cpp:305-313);
 //  - the entering column is the inactive one of largest gradient, the first on a tie; the loop
-//    ends when it does not exceed the tolerance;
+//    ends when it does not exceed the tolerance (a NaN gradient, which only non-finite signals
+//    produce, is never chosen);
 //  - least-squares fits on the active set come from the Cholesky factor L of the Gram matrix on
 //    the active columns, in the order they joined: a joining column appends the row
 //    L w = G_A,j, sqrt(G_jj - w.w), refused when that pivot is not above 1e-12 G_jj
@@ -548,16 +549,14 @@ struct RidgeNnlsWave {
                 status |= ST_OUTER_GUARD | ST_NOT_CONVERGED;
                 break;
             }
-            double bv = 0.0;
-            int bi = -1;
+            bool free_col[NCPL];
 #pragma unroll
             for (int t = 0; t < NCPL; ++t) {
-                if (col(t) < D.n && !act[t] && (bi < 0 || g[t] > bv)) {
-                    bv = g[t];
-                    bi = col(t);
-                }
+                free_col[t] = col(t) < D.n && !act[t];
             }
-            wave_argmax_first(bv, bi);
+            double bv = 0.0;
+            int bi = -1;
+            wave_argmax_cols<NCPL>(g, free_col, bv, bi);
             if (bi < 0 || bv <= tol) {
                 break;
             }
@@ -584,20 +583,17 @@ struct RidgeNnlsWave {
                     }
                     return status | ST_NOT_CONVERGED;
                 }
-                double sv = 0.0;
-                int si = -1;
+                double step[NCPL];
+                bool blocking[NCPL];
 #pragma unroll
                 for (int t = 0; t < NCPL; ++t) {
-                    if (col(t) < D.n && act[t] && cand[t] <= 0.0) {
-                        const double distance = x[t] - cand[t];
-                        const double s = distance > 0.0 ? x[t] / distance : 0.0;
-                        if (si < 0 || s < sv) {
-                            sv = s;
-                            si = col(t);
-                        }
-                    }
+                    blocking[t] = col(t) < D.n && act[t] && cand[t] <= 0.0;
+                    const double distance = x[t] - cand[t];
+                    step[t] = (blocking[t] && distance > 0.0) ? x[t] / distance : 0.0;
                 }
-                wave_argmin_first(sv, si);
+                double sv = 0.0;
+                int si = -1;
+                wave_argmin_cols<NCPL>(step, blocking, sv, si);
                 work += 3ull * static_cast<unsigned long long>(D.n);
 #pragma unroll
                 for (int t = 0; t < NCPL; ++t) {

@@ -102,6 +102,59 @@ __device__ __forceinline__ void wave_argmin_first(double& value, int& idx) {
     idx = readlane(idx, 0);
 }
 
+// The largest value among the flagged entries of a column-spread vector (column l + 64 t in
+// lane l, slot t) and the smallest column holding it: idx = -1 when none is flagged.  One max
+// butterfly, then a ballot per slot for the first column that holds the maximum.  For values
+// without NaN this is the in-order scan's "first strictly larger" choice; a NaN value is never
+// chosen.
+template <int NS>
+__device__ __forceinline__ void wave_argmax_cols(const double (&v)[NS], const bool (&flag)[NS],
+                                                 double& value, int& idx) {
+    double lane_max = -__builtin_inf();
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        any = any || flag[t];
+        lane_max = (flag[t] && v[t] > lane_max) ? v[t] : lane_max;
+    }
+    value = wave_max(lane_max);
+    idx = -1;
+    if (ballot(any) != 0ull) {
+#pragma unroll
+        for (int t = NS - 1; t >= 0; --t) {
+            const unsigned long long hit = ballot(flag[t] && v[t] == value);
+            idx = hit != 0ull ? WAVE * t + __ffsll(static_cast<long long>(hit)) - 1 : idx;
+        }
+    }
+}
+
+// The same for the smallest value.
+template <int NS>
+__device__ __forceinline__ void wave_argmin_cols(const double (&v)[NS], const bool (&flag)[NS],
+                                                 double& value, int& idx) {
+    double lane_min = __builtin_inf();
+    bool any = false;
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        any = any || flag[t];
+        lane_min = (flag[t] && v[t] < lane_min) ? v[t] : lane_min;
+    }
+#pragma unroll
+    for (int o = WAVE / 2; o >= 1; o >>= 1) {
+        const double q = __shfl_xor(lane_min, o);
+        lane_min = q < lane_min ? q : lane_min;
+    }
+    value = readlane(lane_min, 0);
+    idx = -1;
+    if (ballot(any) != 0ull) {
+#pragma unroll
+        for (int t = NS - 1; t >= 0; --t) {
+            const unsigned long long hit = ballot(flag[t] && v[t] == value);
+            idx = hit != 0ull ? WAVE * t + __ffsll(static_cast<long long>(hit)) - 1 : idx;
+        }
+    }
+}
+
 // Register vectors.  Element selection by a run-time (uniform) index stays in registers for
 // ext_vector_type values, where a plain array would be moved to scratch memory.
 template <int N>
