@@ -124,10 +124,10 @@ struct FractionDesign {
 constexpr int MAX_GROUPS = 2; // anisotropic blocks of a per-block fraction fit
 
 // compute_fractions of one voxel: fod in column slots, s the unweighted signal.  lumped takes
-// all anisotropic columns as one group.  fr receives the fractions, groups first.  work is the
+// all anisotropic columns as one group; NG group columns are held (1 suffices when lumped).  fr receives the fractions, groups first.  work is the
 // wave's LDS slab for the pivoted QR (RPL * 64 * NQ doubles), iso_lds the block's LDS copy of the
 // isotropic columns.  Returns false when the NNLS hit its inner-step limit.
-template <int NCPL, int RPL, int NQ>
+template <int NCPL, int RPL, int NQ, int NG = MAX_GROUPS>
 __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const double (&fod)[NCPL],
                                                const RowVec<RPL>& s, bool lumped, bool has_tol,
                                                double tol_in, double (&fr)[NQ], double* work,
@@ -139,11 +139,11 @@ __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const
     for (int t = 0; t < NCPL; ++t) {
         thr[t] = fod[t] < threshold ? 0.0 : fod[t];
     }
-    FractionDesign<MAX_GROUPS, RPL> C;
+    FractionDesign<NG, RPL> C;
     C.n_groups = lumped ? 1 : K.n_blocks;
     C.iso_lds = iso_lds;
 #pragma unroll
-    for (int grp = 0; grp < MAX_GROUPS; ++grp) {
+    for (int grp = 0; grp < NG; ++grp) {
         dvec<RPL> gs = 0.0;
         if (grp < C.n_groups) {
             const int begin = lumped ? 0 : K.blocks[2 * grp];

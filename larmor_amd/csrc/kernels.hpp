@@ -368,7 +368,7 @@ struct FractionArgs {
 // The low-resolution fraction kernel is asked for 5 waves per SIMD.
 constexpr int fraction_waves_per_eu(int ncpl) { return ncpl == 1 ? 5 : 1; }
 
-template <int NCPL, int RPL>
+template <int NCPL, int RPL, bool FINAL>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     __attribute__((amdgpu_waves_per_eu(fraction_waves_per_eu(NCPL)))) fraction_kernel(FractionArgs A) {
     extern __shared__ double smem[];
@@ -399,15 +399,16 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
             fod[t] = c < ca ? A.x[v * ca + c] : 0.0;
         }
         double fr[NQ];
-        const bool ok = A.final_fit
-                            ? fractions_wave<NCPL, RPL, NQ>(A.kernel, fod, s, false, true,
-                                                            A.final_tol, fr, work, iso_lds)
-                            : fractions_wave<NCPL, RPL, NQ>(A.kernel, fod, s, true, false, 0.0,
-                                                            fr, work, iso_lds);
+        // The final fit's per-block groups, or the loop's one lumped group.
+        const bool ok = FINAL
+                            ? fractions_wave<NCPL, RPL, NQ, MAX_GROUPS>(
+                                  A.kernel, fod, s, false, true, A.final_tol, fr, work, iso_lds)
+                            : fractions_wave<NCPL, RPL, NQ, 1>(A.kernel, fod, s, true, false, 0.0,
+                                                               fr, work, iso_lds);
         if (!ok && l == 0) {
             A.status[v] |= ST_NOT_CONVERGED;
         }
-        if (A.final_fit) {
+        if (FINAL) {
             const int nf = A.kernel.n_blocks + ci;
             double total = 0.0;
 #pragma unroll
@@ -622,7 +623,9 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
     int launch_fraction_##NC##_##RP(const char* name, const FractionArgs& args, int waves,      \
                                     hipStream_t stream) {                                      \
         const size_t lds = sizeof(double) * (NQ * WAVES_PER_BLOCK + args.kernel.ci) * RP * WAVE; \
-        return launch_fill(name, fraction_kernel<NC, RP>, args, waves, lds, stream);            \
+        return args.final_fit                                                                  \
+                   ? launch_fill(name, fraction_kernel<NC, RP, true>, args, waves, lds, stream)   \
+                   : launch_fill(name, fraction_kernel<NC, RP, false>, args, waves, lds, stream); \
     }
 
 // Every instantiated (NCPL, RPL): X(NC, RP).
