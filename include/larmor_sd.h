@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define LSD_ABI_VERSION 1
+#define LSD_ABI_VERSION 2
 #define LSD_ERR_INVALID 1 /* invalid argument (the reference raises ValueError) */
 #define LSD_ERR_HIP 2     /* a HIP runtime call failed */
 

@@ -8,6 +8,7 @@ import ctypes
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "liblarmor_sd.so"
+ABI_VERSION = 2  # LSD_ABI_VERSION of include/larmor_sd.h these bindings follow
 HEADER_PATH = Path(__file__).resolve().parents[1] / "include" / "larmor_sd.h"
 
 _lib = None
@@ -121,6 +122,12 @@ def load() -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.restype = restype
         fn.argtypes = argtypes
+    version = lib.lsd_abi_version()
+    if version != ABI_VERSION:
+        raise LibraryMissing(
+            f"{LIB_PATH} has ABI version {version}, these bindings need {ABI_VERSION}; rebuild it "
+            "with `python -m larmor_amd.build`."
+        )
     _lib = lib
     return lib
 
