@@ -155,3 +155,31 @@ def test_profiling_counts_work_and_leaves_results_unchanged():
     assert first["fod_lr"][2] > 0 and first["fod_hr"][2] > 0
     assert first["fraction_loop"][2] == 0 and first["fod_lr_corr"][2] == 0
     assert [v[1:] for v in first.values()] == [v[1:] for v in second.values()]
+
+
+@pytest.mark.parametrize("n_lr", [20, 60, 70])
+def test_multi_tissue_low_resolution_widths(n_lr):
+    """Low-resolution designs of fewer than, more than one pass of (48) and more than 64 columns:
+    the correlation pass's column passes and the multi-slot FOD kernel, bit-identical to the
+    oracle."""
+    cfg = dict(synthetic.MFOD_CONFIG)
+    cfg["shape"] = (10, 10, 8)
+    cfg["seed"] = 11
+    ph = synthetic.phantom(**cfg)
+    signals = synthetic.normalized_signals(ph.data, ph.bvals, ph.mask)
+    rfs = [AnisotropicRF([1.7e-3, 0.3e-3, 0.3e-3], 0.0), IsotropicRF(0.8e-3), IsotropicRF(3e-3)]
+    w = np.ones(len(ph.bvals))
+    hr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=DIRS_300, weights=w)
+    lr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=[get_dirs(n_lr + 1)[:-1]], weights=w)
+    assert lr.w_k_aniso.shape[1] == n_lr
+    s = np.ascontiguousarray(signals[:150])
+    print("lr width", n_lr, s.shape, flush=True)
+    ref_fods, ref_fr, tr = oracle.fit_multi_tissue(s, w, lr, hr, 0.0, 0.1, 1e-6, trace=True)
+    lr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(lr.w_k_aniso), 0.0, 1e-6)
+    hr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(hr.w_k_aniso), 0.1, 1e-6)
+    fods, fr, rounds, _ = _core.fit_multi_tissue(
+        s, w, _deconv(lr), lr_solver, _deconv(hr), hr_solver, 50, 1e-7, 1e-2, 1,
+        return_diagnostics=True)
+    np.testing.assert_array_equal(rounds, tr[:, 0])
+    np.testing.assert_array_equal(fods, ref_fods)
+    np.testing.assert_array_equal(fr, ref_fr)
