@@ -52,6 +52,18 @@ def _load():
     lib.oracle_fit_single_tissue.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, _d,
                                              ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int, _d]
+    lib.oracle_fit_single_tissue_drl.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, _d,
+                                                 ctypes.c_int64, _d, ctypes.c_double,
+                                                 ctypes.c_int64, ctypes.c_double, ctypes.c_int,
+                                                 _d, _i64]
+    lib.oracle_fit_multi_tissue_drl.argtypes = (
+        [_d, ctypes.c_int64, ctypes.c_int64, _d]
+        + [_d, _d, _d, ctypes.c_int64, _i64, ctypes.c_int64, _d, _d, ctypes.c_double,
+           ctypes.c_int64, ctypes.c_double]
+        + [_d, _d, _d, ctypes.c_int64, _i64, ctypes.c_int64, _d, _d, ctypes.c_double,
+           ctypes.c_int64, ctypes.c_double]
+        + [ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int, _d, _d,
+           _i64])
     lib.oracle_fit_multi_tissue.argtypes = (
         [_d, ctypes.c_int64, ctypes.c_int64, _d]
         + [_d, _d, _d, _d, ctypes.c_int64, _i64, ctypes.c_int64, ctypes.c_double, ctypes.c_double]
@@ -321,3 +333,50 @@ def solve_lsi_by_qr(signal, a, g, ridge):
     _check(_load().oracle_solve_lsi_by_qr(_p(s), _p(am), m, n, _p(gm), gm.shape[0], float(ridge),
                                           _p(x), conv.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
     return x, bool(conv[0])
+
+
+def fit_single_tissue_drl(signals, kernel_t, gram, nu, n_iter, init, n_threads=None,
+                          iterations=False):
+    """Damped Richardson-Lucy of each voxel (FodSolver.drl): kernel_t (c, m), gram (c, c).
+    Returns fods (v, c)[, iterations (v,): negative when the limit stopped the voxel]."""
+    s = _c(signals)
+    kt = _c(kernel_t)
+    g = _c(gram)
+    nv, m = s.shape
+    c = kt.shape[0]
+    out = np.zeros((nv, c))
+    its = np.zeros(nv, dtype=np.int64)
+    n_threads = n_threads or os.cpu_count() or 1
+    _check(_load().oracle_fit_single_tissue_drl(_p(s), nv, m, _p(kt), c, _p(g), nu, n_iter, init,
+                                                n_threads, _p(out), its.ctypes.data_as(_i64)))
+    return (out, its) if iterations else out
+
+
+def fit_multi_tissue_drl(signals, weights, lr, hr, lr_rl, hr_rl, max_rounds=50,
+                         fraction_tol=1e-3, final_tol=1e-2, n_threads=None, rounds=False):
+    """Multi-tissue fit with damped Richardson-Lucy at both resolutions (the GRL method).  lr, hr
+    are kernels as in fit_multi_tissue; lr_rl, hr_rl are (kernel_t, gram, nu, n_iter, init) of
+    each stage's FodSolver.drl.  Returns (fods, fractions[, rounds])."""
+    s = _c(signals)
+    nv, m = s.shape
+    w = _c(weights)
+    args = [_p(s), nv, m, _p(w)]
+    keep = []
+    ci = None
+    for k, (kt, g, nu, n_iter, init) in ((lr, lr_rl), (hr, hr_rl)):
+        ka, ki, wki, b = _c(k.k_aniso), _c(k.k_iso), _c(k.w_k_iso), _blocks(k.aniso_blocks)
+        kt, g = _c(kt), _c(g)
+        keep += [ka, ki, wki, b, kt, g]
+        ci = ki.shape[1]
+        args += [_p(ka), _p(ki), _p(wki), ka.shape[1], b.ctypes.data_as(_i64), len(b), _p(kt),
+                 _p(g), nu, n_iter, init]
+    ca_hr = hr.k_aniso.shape[1]
+    nf = len(_blocks(hr.aniso_blocks)) + ci
+    fods = np.zeros((nv, ca_hr))
+    fractions = np.zeros((nv, nf))
+    rd = np.zeros(nv, dtype=np.int64)
+    n_threads = n_threads or os.cpu_count() or 1
+    args += [ci, max_rounds, fraction_tol, final_tol, n_threads, _p(fods), _p(fractions),
+             rd.ctypes.data_as(_i64)]
+    _check(_load().oracle_fit_multi_tissue_drl(*args))
+    return (fods, fractions, rd) if rounds else (fods, fractions)

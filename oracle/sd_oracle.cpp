@@ -9,7 +9,8 @@
 //                              growing / downdated Cholesky factor and the QR fallback
 //   cpp/src/sd/fractions.cpp   median, compute_fractions
 //   cpp/src/sd/multi_tissue.cpp fit_multi_tissue_block
-//   cpp/src/sd/solver.cpp      L2NnlsSolver::solve_block
+//   cpp/src/sd/solver.cpp      L2NnlsSolver::solve_block, DrlSolver::solve_block
+//   cpp/src/sd/drl.cpp         damped_richardson_lucy (per voxel)
 //   cpp/src/sd/csd.cpp         make_csd_problem, solve_csd (LLT restated as below)
 //   cpp/src/sh/harmonics.cpp   fill_legendre, Legendre derivatives, derivatives
 //   cpp/src/sh/peaks.cpp       climb, voxel_peaks
@@ -776,9 +777,95 @@ Vec compute_fractions(const Vec& fod, const TissueKernel& kernel, const Vec& sig
     return solve_nnls_with_limit(columns, signal, tol, 3 * columns.cols, trace).solution;
 }
 
+// Damped Richardson-Lucy of one voxel (drl.cpp:39-99; the block's row moves do not change a
+// voxel's arithmetic).  kernel_t is K^T (c, m) and gram K^T K (c, c), as FodSolver.drl takes
+// them.  Sums: the mean and the sum of squares of the sample standard deviation, K^T s, each
+// f G and the FOD change are sequential in index order.
+constexpr double DRL_FOD_FLOOR = 1e-6;           // drl.cpp:13-14
+constexpr double DRL_DENOMINATOR_OFFSET = 1e-16; // drl.cpp:16-17
+constexpr double DRL_CONVERGENCE_SHARE = 1e-4;   // drl.cpp:19-21
+
+struct DrlSolver {
+    Mat kernel_t; // (c, m)
+    Mat gram;     // (c, c)
+    double nu = 0.0;
+    Index n_iter = 0;
+    double init = 0.0;
+};
+
+double power_8(double x) { // common/maths.hpp:26-30
+    const double x2 = x * x;
+    const double x4 = x2 * x2;
+    return x4 * x4;
+}
+
+// Returns the FOD; *iterations (optional) receives the iterations run, and *converged whether the
+// change test stopped them.
+Vec drl_voxel(const DrlSolver& d, const Vec& signal, Index* iterations, bool* converged) {
+    const Index m = static_cast<Index>(signal.size());
+    const Index c = d.gram.rows;
+    // drl.cpp:23-28, 49-56: damping strength from the sample standard deviation, threshold from
+    // the largest (non-NaN) signal value.
+    double total = 0.0;
+    for (Index i = 0; i < m; ++i) total += signal[i];
+    const double mean = total / static_cast<double>(m);
+    double squares = 0.0;
+    for (Index i = 0; i < m; ++i) squares += (signal[i] - mean) * (signal[i] - mean);
+    const double sd = std::sqrt(squares / static_cast<double>(m - 1));
+    const double strength = std::fmax(1.0 - 4.0 * sd, 0.0);
+    double largest = std::numeric_limits<double>::quiet_NaN();
+    for (Index i = 0; i < m; ++i) {
+        if (!std::isnan(signal[i]) && (std::isnan(largest) || signal[i] > largest)) {
+            largest = signal[i];
+        }
+    }
+    const double threshold = DRL_CONVERGENCE_SHARE * largest;
+    const double nu8 = power_8(d.nu);
+    Vec ks(static_cast<size_t>(c), 0.0);
+    for (Index j = 0; j < c; ++j) {
+        double acc = 0.0;
+        for (Index i = 0; i < m; ++i) acc += signal[i] * d.kernel_t(j, i);
+        ks[j] = acc;
+    }
+    Vec f(static_cast<size_t>(c), d.init);
+    Vec previous = f;
+    Vec predicted(static_cast<size_t>(c));
+    Index it = 0;
+    bool done = false;
+    for (; it < d.n_iter && !done; ++it) {
+        for (Index j = 0; j < c; ++j) {
+            double acc = 0.0;
+            for (Index k = 0; k < c; ++k) acc += f[k] * d.gram(k, j);
+            predicted[j] = acc;
+        }
+        double change = 0.0;
+        for (Index j = 0; j < c; ++j) {
+            const double a = f[j];
+            const double a8 = power_8(a);
+            const double damping = 1.0 - strength * (1.0 - a8 / (a8 + nu8));
+            const double ratio =
+                (ks[j] - predicted[j]) / (predicted[j] + DRL_DENOMINATOR_OFFSET);
+            const double updated = std::fmax(a * (1.0 + damping * ratio), DRL_FOD_FLOOR);
+            f[j] = updated;
+            change += std::fabs(updated - previous[j]);
+        }
+        if (change <= threshold) {
+            done = true;
+        } else {
+            previous = f;
+        }
+    }
+    if (iterations) *iterations = it;
+    if (converged) *converged = done;
+    return f;
+}
+
+// One stage's FOD solver: ridge NNLS (L2NnlsSolver) or damped Richardson-Lucy (DrlSolver).
 struct Solver {
     RidgeDesign design;
     double tol = 0.0;
+    bool drl = false;
+    DrlSolver rl;
 };
 
 struct MtTrace {
@@ -831,8 +918,9 @@ void fit_multi_tissue_block(const Mat& signals, const Vec& weights, const Tissue
         for (Index v : looping) {
             const Vec a = aniso_signal(v, lr.weighted_isotropic);
             NnlsTrace tr;
-            const Vec fod = solve_ridge_nnls(lr_solver.design, a, 0.0, lr_solver.tol,
-                                             traces ? &tr : nullptr);
+            const Vec fod = lr_solver.drl ? drl_voxel(lr_solver.rl, a, nullptr, nullptr)
+                                          : solve_ridge_nnls(lr_solver.design, a, 0.0,
+                                                             lr_solver.tol, traces ? &tr : nullptr);
             NnlsTrace ftr;
             const Vec fr = compute_fractions(fod, lr, row(signals, v), true, false, 0.0,
                                              traces ? &ftr : nullptr);
@@ -869,8 +957,9 @@ void fit_multi_tissue_block(const Mat& signals, const Vec& weights, const Tissue
     for (Index v = 0; v < nv; ++v) {
         const Vec a = aniso_signal(v, hr.weighted_isotropic);
         NnlsTrace tr;
-        const Vec fod =
-            solve_ridge_nnls(hr_solver.design, a, 0.0, hr_solver.tol, traces ? &tr : nullptr);
+        const Vec fod = hr_solver.drl ? drl_voxel(hr_solver.rl, a, nullptr, nullptr)
+                                      : solve_ridge_nnls(hr_solver.design, a, 0.0, hr_solver.tol,
+                                                         traces ? &tr : nullptr);
         if (traces) {
             traces[v].hr = tr;
         }
@@ -1938,10 +2027,12 @@ int oracle_fit_multi_tissue(const double* signals, std::int64_t n_voxels, std::i
             make_kernel(lr_k_aniso, lr_k_iso, lr_w_k_iso, m, lr_ca, ci, lr_blocks, lr_n_blocks);
         const TissueKernel hr =
             make_kernel(hr_k_aniso, hr_k_iso, hr_w_k_iso, m, hr_ca, ci, hr_blocks, hr_n_blocks);
-        const Solver lr_solver{make_ridge_design(from_ptr(lr_w_k_aniso, m, lr_ca), lr_ridge),
-                               lr_tol};
-        const Solver hr_solver{make_ridge_design(from_ptr(hr_w_k_aniso, m, hr_ca), hr_ridge),
-                               hr_tol};
+        Solver lr_solver;
+        lr_solver.design = make_ridge_design(from_ptr(lr_w_k_aniso, m, lr_ca), lr_ridge);
+        lr_solver.tol = lr_tol;
+        Solver hr_solver;
+        hr_solver.design = make_ridge_design(from_ptr(hr_w_k_aniso, m, hr_ca), hr_ridge);
+        hr_solver.tol = hr_tol;
         const Vec w(weights, weights + m);
         const Index nf = hr_n_blocks + ci;
         parallel_for_blocks(n_voxels, 128, n_threads, [&](Index begin, Index end) {
@@ -1965,6 +2056,70 @@ int oracle_fit_multi_tissue(const double* signals, std::int64_t n_voxels, std::i
                     t[8] = tr[v].hr.max_active;
                     t[9] = tr[v].hr.qr_fallback;
                 }
+            }
+        });
+    });
+}
+
+// fit_single_tissue with FodSolver.drl (solver.cpp:9-12, drl.cpp:39-99): kernel_t (c, m),
+// gram (c, c).  iterations (optional, int64 per voxel): the iterations each voxel ran, negated
+// when the iteration limit stopped it.
+int oracle_fit_single_tissue_drl(const double* signals, std::int64_t n_voxels, std::int64_t m,
+                                 const double* kernel_t, std::int64_t c, const double* gram,
+                                 double nu, std::int64_t n_iter, double init, int n_threads,
+                                 double* fods, std::int64_t* iterations) {
+    return guarded([&] {
+        const DrlSolver d{from_ptr(kernel_t, c, m), from_ptr(gram, c, c), nu, n_iter, init};
+        parallel_for_blocks(n_voxels, 128, n_threads, [&](Index begin, Index end) {
+            for (Index v = begin; v < end; ++v) {
+                const Vec sig(signals + v * m, signals + (v + 1) * m);
+                Index it = 0;
+                bool converged = false;
+                const Vec f = drl_voxel(d, sig, &it, &converged);
+                std::copy(f.begin(), f.end(), fods + v * c);
+                if (iterations) iterations[v] = converged ? it : -it;
+            }
+        });
+    });
+}
+
+// fit_multi_tissue with FodSolver.drl at both resolutions (the GRL method): each stage's
+// kernel_t (c, m), gram (c, c), nu, iteration limit and starting amplitude.  rounds (optional,
+// int64 per voxel).
+int oracle_fit_multi_tissue_drl(
+    const double* signals, std::int64_t n_voxels, std::int64_t m, const double* weights,
+    const double* lr_k_aniso, const double* lr_k_iso, const double* lr_w_k_iso, std::int64_t lr_ca,
+    const std::int64_t* lr_blocks, std::int64_t lr_n_blocks, const double* lr_kernel_t,
+    const double* lr_gram, double lr_nu, std::int64_t lr_n_iter, double lr_init,
+    const double* hr_k_aniso, const double* hr_k_iso, const double* hr_w_k_iso, std::int64_t hr_ca,
+    const std::int64_t* hr_blocks, std::int64_t hr_n_blocks, const double* hr_kernel_t,
+    const double* hr_gram, double hr_nu, std::int64_t hr_n_iter, double hr_init, std::int64_t ci,
+    std::int64_t max_rounds, double fraction_tol, double final_tol, int n_threads, double* fods,
+    double* fractions, std::int64_t* rounds) {
+    return guarded([&] {
+        const TissueKernel lr =
+            make_kernel(lr_k_aniso, lr_k_iso, lr_w_k_iso, m, lr_ca, ci, lr_blocks, lr_n_blocks);
+        const TissueKernel hr =
+            make_kernel(hr_k_aniso, hr_k_iso, hr_w_k_iso, m, hr_ca, ci, hr_blocks, hr_n_blocks);
+        Solver lr_solver;
+        lr_solver.drl = true;
+        lr_solver.rl = DrlSolver{from_ptr(lr_kernel_t, lr_ca, m), from_ptr(lr_gram, lr_ca, lr_ca),
+                                 lr_nu, lr_n_iter, lr_init};
+        Solver hr_solver;
+        hr_solver.drl = true;
+        hr_solver.rl = DrlSolver{from_ptr(hr_kernel_t, hr_ca, m), from_ptr(hr_gram, hr_ca, hr_ca),
+                                 hr_nu, hr_n_iter, hr_init};
+        const Vec w(weights, weights + m);
+        const Index nf = hr_n_blocks + ci;
+        parallel_for_blocks(n_voxels, 128, n_threads, [&](Index begin, Index end) {
+            const Index nb = end - begin;
+            const Mat block = from_ptr(signals + begin * m, nb, m);
+            std::vector<MtTrace> tr(rounds ? static_cast<size_t>(nb) : 0);
+            fit_multi_tissue_block(block, w, lr, lr_solver, hr, hr_solver, max_rounds,
+                                   fraction_tol, final_tol, fods + begin * hr_ca,
+                                   fractions + begin * nf, rounds ? tr.data() : nullptr);
+            if (rounds) {
+                for (Index v = 0; v < nb; ++v) rounds[begin + v] = tr[v].rounds;
             }
         });
     });
