@@ -61,6 +61,23 @@ int lsd_profile_collect(char* names, int64_t names_bytes, double* ms, int64_t* c
  */
 int lsd_solver_create_l2nnls(const double* kernel, int64_t m, int64_t n, double ridge,
                              double tol_x, lsd_solver** out);
+/*
+ * A damped Richardson-Lucy FOD solver from k_t = K^T (c x m) and k_t_k = K^T K (c x c), host
+ * arrays, with damping threshold nu, at most n_iter iterations, starting from init in every
+ * direction.  Fits with it run drl_kernel: per voxel the damping strength and stopping threshold
+ * from the signal, K^T s, then the multiplicative updates until the FOD changes in total by no
+ * more than 1e-4 of the signal's largest value (status bit 1 when n_iter stopped the voxel).
+ * Replaces: FodSolver.drl(k_t, k_t_k, nu, n_iter, init)  cpp/src/python/sd.cpp:127-145
+ *           (drl_solver), DrlSolver cpp/src/sd/solver.hpp:16-33, damped_richardson_lucy
+ *           cpp/src/sd/drl.cpp:39-99.
+ * Errors: "shapes do not match: k_t (c, m), k_t_k (r, s)", "n_iter must be at least 0".
+ */
+int lsd_solver_create_drl(const double* kernel_t, int64_t c, int64_t m, const double* gram,
+                          int64_t gram_rows, int64_t gram_cols, double nu, int64_t n_iter,
+                          double init, lsd_solver** out);
+/* 1 for a damped Richardson-Lucy solver, 0 for ridge NNLS (the variant of FodSolver's method,
+ * cpp/src/sd/solver.hpp:103-105). */
+int lsd_solver_is_drl(const lsd_solver* solver);
 void lsd_solver_destroy(lsd_solver* solver);
 /* FodSolver.n_dirs / n_meas  cpp/src/python/sd.cpp:399-443 */
 int64_t lsd_solver_n_dirs(const lsd_solver* solver);

@@ -80,11 +80,9 @@ def _release(obj, destroy: str) -> None:
 
 
 class FodSolver:
-    """An FOD solver with its kernel and Gram matrix resident on the device.
-
-    Only the L2NNLS solver is part of this path (FodSolver.l2nnls, sd.cpp:89-99); the reference's
-    `drl` and `lasso` solvers belong to other methods.
-    """
+    """An FOD solver with its matrices resident on the device: ridge NNLS (FodSolver.l2nnls,
+    sd.cpp:89-99) or damped Richardson-Lucy (FodSolver.drl, sd.cpp:127-145).  The reference's
+    `lasso` solver (elastic net) is not part of this package."""
 
     def __init__(self, handle, m: int, n: int):
         self._handle = handle
@@ -104,6 +102,26 @@ class FodSolver:
         )
         return FodSolver(handle, kernel.shape[0], kernel.shape[1])
 
+    @staticmethod
+    def drl(k_t, k_t_k, nu: float, n_iter: int, init: float) -> "FodSolver":
+        """Damped Richardson-Lucy from K^T (c, m) and K^T K (c, c), starting from init in every
+        direction, at most n_iter iterations per voxel (sd.cpp:127-145)."""
+        kt = _host_matrix(k_t, "k_t")
+        g = _host_matrix(k_t_k, "k_t_k")
+        lib = _lib.load()
+        handle = ctypes.c_void_p()
+        _lib.check(
+            lib.lsd_solver_create_drl(
+                _ptr(kt), kt.shape[0], kt.shape[1], _ptr(g), g.shape[0], g.shape[1], float(nu),
+                int(n_iter), float(init), ctypes.byref(handle),
+            )
+        )
+        return FodSolver(handle, kt.shape[1], kt.shape[0])
+
+    @property
+    def is_drl(self) -> bool:
+        return bool(_lib.load().lsd_solver_is_drl(self._handle))
+
     @property
     def n_dirs(self) -> int:
         return self._n
@@ -114,7 +132,7 @@ class FodSolver:
 
     @property
     def n_unconverged(self) -> int:
-        """Always 0 for L2NNLS, as in the reference (only the elastic net counts)."""
+        """Always 0, as in the reference (only the elastic net counts)."""
         return 0
 
     @property

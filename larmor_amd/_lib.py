@@ -31,6 +31,12 @@ SIGNATURES = {
         [_d, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double,
          ctypes.POINTER(_vp)],
     ),
+    "lsd_solver_create_drl": (
+        ctypes.c_int,
+        [_d, ctypes.c_int64, ctypes.c_int64, _d, ctypes.c_int64, ctypes.c_int64,
+         ctypes.c_double, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(_vp)],
+    ),
+    "lsd_solver_is_drl": (ctypes.c_int, [_vp]),
     "lsd_solver_destroy": (None, [_vp]),
     "lsd_solver_n_dirs": (ctypes.c_int64, [_vp]),
     "lsd_solver_n_meas": (ctypes.c_int64, [_vp]),

@@ -5,6 +5,7 @@
 
 #include "csd_wave.hpp"
 #include "kernels.hpp"
+#include "drl_wave.hpp"
 #include "lsi_host.hpp"
 #include "lsi_wave.hpp"
 #include "peaks_wave.hpp"
@@ -123,8 +124,13 @@ struct lsd_solver {
     double ridge = 0.0;
     double tol = 0.0;
     int normal_equations = 0;
-    double* d_kernel = nullptr;
-    double* d_gram = nullptr;
+    double* d_kernel = nullptr; // K (m, n), then CORR_COLUMNS (>= DRL_COLUMNS) zeros
+    double* d_gram = nullptr;   // (n, n); a dRL solver's then DRL_COLUMNS zeros
+    // Damped Richardson-Lucy (FodSolver.drl) instead of ridge NNLS.
+    bool drl = false;
+    double nu = 0.0;
+    int64_t n_iter = 0;
+    double init = 0.0;
 };
 
 struct lsd_tissue_kernel {
@@ -277,6 +283,50 @@ int check_sh_layout(int64_t lmax, bool full, int64_t n_coeffs) {
                     "for a full one)");
     }
     return 0;
+}
+
+// The position buffers of damped Richardson-Lucy solves over lists of at most ld voxels.
+struct DrlBuffers {
+    long ld = 0;
+    double* a = nullptr;
+    double* ks = nullptr;
+    double* f = nullptr;
+    double* prev = nullptr;
+    double* pred = nullptr;
+};
+
+int drl_buffers(const lsd_solver* solver, long ld, StreamAllocs& mem, DrlBuffers& b) {
+    b.ld = ld;
+    const long cl = static_cast<long>(solver->n) * ld;
+    int rc = mem.get(&b.a, static_cast<long>(solver->m) * ld);
+    if (!rc) rc = mem.get(&b.ks, cl);
+    if (!rc) rc = mem.get(&b.f, cl);
+    if (!rc) rc = mem.get(&b.prev, cl);
+    if (!rc) rc = mem.get(&b.pred, cl);
+    return rc;
+}
+
+// A damped Richardson-Lucy solve: base carries the signal inputs, the list and the outputs; the
+// solver supplies the matrices and settings.
+int run_drl(const char* name, const lsd_solver* solver, DrlArgs base, const DrlBuffers& b,
+            hipStream_t stream) {
+    DrlArgs a = base;
+    a.kernel = solver->d_kernel;
+    a.gram = solver->d_gram;
+    a.c = static_cast<int>(solver->n);
+    const double nu2 = solver->nu * solver->nu;
+    const double nu4 = nu2 * nu2;
+    a.nu8 = nu4 * nu4;
+    a.n_iter = static_cast<long>(solver->n_iter);
+    a.init = solver->init;
+    a.ld = b.ld;
+    a.a = b.a;
+    a.ks = b.ks;
+    a.f = b.f;
+    a.prev = b.prev;
+    a.pred = b.pred;
+    return launch_fill(name, drl_kernel<DRL_COLUMNS>, a, static_cast<int>((b.ld + WAVE - 1) / WAVE),
+                       0, stream);
 }
 
 } // namespace lsd
@@ -446,6 +496,45 @@ int lsd_solver_create_l2nnls(const double* kernel, int64_t m, int64_t n, double 
     return 0;
 }
 
+int lsd_solver_create_drl(const double* kernel_t, int64_t c, int64_t m, const double* gram,
+                          int64_t gram_rows, int64_t gram_cols, double nu, int64_t n_iter,
+                          double init, lsd_solver** out) {
+    if (gram_rows != c || gram_cols != c) {
+        return fail("shapes do not match: k_t " + shape_tuple(c, m) + ", k_t_k " +
+                    shape_tuple(gram_rows, gram_cols));
+    }
+    if (n_iter < 0) {
+        return fail("n_iter must be at least 0");
+    }
+    if (c < 0 || m < 0 || (c * m > 0 && kernel_t == nullptr) || (c > 0 && gram == nullptr)) {
+        return fail("k_t must be (c, m) and k_t_k (c, c) with c, m >= 0");
+    }
+    auto* s = new lsd_solver();
+    s->m = m;
+    s->n = c;
+    s->drl = true;
+    s->nu = nu;
+    s->n_iter = n_iter;
+    s->init = init;
+    // K (m, c) = k_t^T for the lane-per-voxel products, and K^T K as given, each followed by one
+    // pass of zero padding.
+    std::vector<double> k(static_cast<size_t>(m * c + CORR_COLUMNS), 0.0);
+    for (int64_t r = 0; r < m; ++r)
+        for (int64_t j = 0; j < c; ++j) k[r * c + j] = kernel_t[j * m + r];
+    std::vector<double> g(static_cast<size_t>(c * c + DRL_COLUMNS), 0.0);
+    std::copy(gram, gram + c * c, g.begin());
+    int rc = upload(&s->d_kernel, k);
+    if (!rc) rc = upload(&s->d_gram, g);
+    if (rc) {
+        lsd_solver_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return 0;
+}
+
+int lsd_solver_is_drl(const lsd_solver* s) { return s->drl ? 1 : 0; }
+
 void lsd_solver_destroy(lsd_solver* s) {
     if (!s) return;
     if (s->d_kernel) (void)hipFree(s->d_kernel);
@@ -534,6 +623,28 @@ int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
     if (m != solver->m) {
         return fail("signals " + shape_tuple(n_voxels, m) + " do not have the solver's " +
                     std::to_string(solver->m) + " measurements");
+    }
+    if (solver->drl) {
+        if (n_voxels == 0) {
+            return 0;
+        }
+        StreamAllocs mem(stream);
+        int rc = 0;
+        if (status == nullptr) rc = mem.get(&status, n_voxels);
+        if (rc) return rc;
+        hipLaunchKernelGGL(init_loop_state, dim3((n_voxels + 255) / 256), dim3(256), 0, stream,
+                           nullptr, 0L, nullptr, 0L, nullptr, status, static_cast<long>(n_voxels));
+        DrlBuffers b;
+        rc = drl_buffers(solver, n_voxels, mem, b);
+        if (rc) return rc;
+        DrlArgs da{};
+        da.signals = signals;
+        da.m = static_cast<int>(m);
+        da.multi = 0;
+        da.n_vox = n_voxels;
+        da.x_out = fods;
+        da.status = status;
+        return run_drl("drl_single", solver, da, b, stream);
     }
     const int ncpl = ncpl_for(solver->n);
     const int rpl = rpl_for(m);
@@ -632,7 +743,7 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
     if (!rc) rc = mem.get(&iso, std::max(1L, nv * ci));
     if (!rc) rc = mem.get(&prev, nv * nlf);
     if (!rc) rc = mem.get(&x_lr, nv * lr->ca);
-    if (!rc) rc = mem.get(&corr_lr, std::max(1L, nv * lr->ca));
+    if (!rc && !lr_solver->drl) rc = mem.get(&corr_lr, std::max(1L, nv * lr->ca));
     if (!rc && fods == nullptr) rc = mem.get(&fods, nv * hr->ca);
     if (!rc && rounds == nullptr) rc = mem.get(&rounds, nv);
     if (!rc && status == nullptr) rc = mem.get(&status, nv);
@@ -662,6 +773,12 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
     fr.status = status;
     fr.n_vox = nv;
 
+    DrlBuffers drl_lr;
+    if (lr_solver->drl) {
+        rc = drl_buffers(lr_solver, nv, mem, drl_lr);
+        if (rc) return rc;
+    }
+
     // The low-resolution rounds: round r fits the voxels of its list (all of them in round 0)
     // and lists those that go on to round r + 1; lists and counts stay on the device.
     const int* list = nullptr;
@@ -675,29 +792,47 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
         fod.w_iso = lr->d_w_iso;
         fod.x_out = x_lr;
         fod.vs = VoxelSet{nv, list, count, olist, ovf_count};
-        CorrArgs ca{};
-        ca.signals = signals;
-        ca.m = static_cast<int>(m);
-        ca.multi = 1;
-        ca.weights = weights;
-        ca.w_iso = lr->d_w_iso;
-        ca.iso = iso;
-        ca.ci = ci;
-        ca.kernel = fod.design.kernel;
-        ca.n = fod.design.n;
-        ca.list = list;
-        ca.list_count = count;
-        ca.n_vox = nv;
-        ca.corr = corr_lr;
-        rc = launch_fill("fod_lr_corr", correlation_kernel<CORR_COLUMNS>, ca,
-                         static_cast<int>((nv + WAVE - 1) / WAVE), 0, stream);
-        if (rc) break;
-        fod.corr = corr_lr;
-        void* rerun_scratch = nullptr;
-        rc = dispatch_fod("fod_lr", "fod_lr_rerun", ncpl_lr, rpl, fod, fod, waves, olist,
-                          ovf_count, stream, &rerun_scratch);
-        fod.corr = nullptr;
-        mem.adopt(rerun_scratch);
+        if (lr_solver->drl) {
+            DrlArgs da{};
+            da.signals = signals;
+            da.m = static_cast<int>(m);
+            da.multi = 1;
+            da.weights = weights;
+            da.w_iso = lr->d_w_iso;
+            da.iso = iso;
+            da.ci = ci;
+            da.list = list;
+            da.list_count = count;
+            da.n_vox = nv;
+            da.x_out = x_lr;
+            da.status = status;
+            rc = run_drl("drl_lr", lr_solver, da, drl_lr, stream);
+        }
+        if (!lr_solver->drl) {
+            CorrArgs ca{};
+            ca.signals = signals;
+            ca.m = static_cast<int>(m);
+            ca.multi = 1;
+            ca.weights = weights;
+            ca.w_iso = lr->d_w_iso;
+            ca.iso = iso;
+            ca.ci = ci;
+            ca.kernel = fod.design.kernel;
+            ca.n = fod.design.n;
+            ca.list = list;
+            ca.list_count = count;
+            ca.n_vox = nv;
+            ca.corr = corr_lr;
+            rc = launch_fill("fod_lr_corr", correlation_kernel<CORR_COLUMNS>, ca,
+                             static_cast<int>((nv + WAVE - 1) / WAVE), 0, stream);
+            if (rc) break;
+            fod.corr = corr_lr;
+            void* rerun_scratch = nullptr;
+            rc = dispatch_fod("fod_lr", "fod_lr_rerun", ncpl_lr, rpl, fod, fod, waves, olist,
+                              ovf_count, stream, &rerun_scratch);
+            fod.corr = nullptr;
+            mem.adopt(rerun_scratch);
+        }
         if (rc) break;
         fr.x = x_lr;
         fr.kernel = fractions_of(lr);
@@ -712,16 +847,33 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
     }
     if (!rc) {
         // The high-resolution FOD of every voxel, then its final fractions.
-        int* ovf_count = counters + 2 * max_rounds;
-        fod.design = design_of(hr_solver);
-        fod.tol = hr_solver->tol;
-        fod.w_iso = hr->d_w_iso;
-        fod.x_out = fods;
-        fod.vs = VoxelSet{nv, nullptr, nullptr, olist, ovf_count};
-        void* rerun_scratch = nullptr;
-        rc = dispatch_fod("fod_hr", "fod_hr_rerun", ncpl_hr, rpl, fod, fod, waves, olist,
-                          ovf_count, stream, &rerun_scratch);
-        mem.adopt(rerun_scratch);
+        if (hr_solver->drl) {
+            DrlBuffers drl_hr;
+            rc = drl_buffers(hr_solver, nv, mem, drl_hr);
+            DrlArgs da{};
+            da.signals = signals;
+            da.m = static_cast<int>(m);
+            da.multi = 1;
+            da.weights = weights;
+            da.w_iso = hr->d_w_iso;
+            da.iso = iso;
+            da.ci = ci;
+            da.n_vox = nv;
+            da.x_out = fods;
+            da.status = status;
+            if (!rc) rc = run_drl("drl_hr", hr_solver, da, drl_hr, stream);
+        } else {
+            int* ovf_count = counters + 2 * max_rounds;
+            fod.design = design_of(hr_solver);
+            fod.tol = hr_solver->tol;
+            fod.w_iso = hr->d_w_iso;
+            fod.x_out = fods;
+            fod.vs = VoxelSet{nv, nullptr, nullptr, olist, ovf_count};
+            void* rerun_scratch = nullptr;
+            rc = dispatch_fod("fod_hr", "fod_hr_rerun", ncpl_hr, rpl, fod, fod, waves, olist,
+                              ovf_count, stream, &rerun_scratch);
+            mem.adopt(rerun_scratch);
+        }
     }
     if (!rc) {
         fr.x = fods;

@@ -3,14 +3,11 @@
 from enum import Enum
 
 from larmor_amd.sd.rf import IsotropicRF, ResponseFunction
-from larmor_amd.sd.solvers import FRACTION_TOL, L2NNLS
+from larmor_amd.sd.solvers import DRL, FRACTION_TOL, L2NNLS
 
 
 class SDMethod(str, Enum):
-    """GRL (dRL solver, inner shells weighted 0.2) and mFOD (L2NNLS, equal shell weights).
-
-    Only mFOD's solver is part of this package; a GRL model must be given an L2NNLS solver.
-    """
+    """GRL (dRL solver, inner shells weighted 0.2) and mFOD (L2NNLS, equal shell weights)."""
 
     GRL = "GRL"
     MFOD = "mFOD"
@@ -28,12 +25,7 @@ class SDMethod(str, Enum):
         return self.value
 
     def default_solver(self):
-        if self is SDMethod.MFOD:
-            return L2NNLS()
-        raise NotImplementedError(
-            "GRL's default solver, damped Richardson-Lucy, is not part of larmor_amd; "
-            "pass solver=L2NNLS()."
-        )
+        return DRL() if self is SDMethod.GRL else L2NNLS()
 
     @property
     def inner_shell_weight(self) -> float:
