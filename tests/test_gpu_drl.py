@@ -112,3 +112,73 @@ def test_batch_invariance_and_edges(grl):  # noqa: F811
         _core.FodSolver.drl(k_t, k_t_k, nu=0.1, n_iter=-1, init=0.1)
     with pytest.raises(ValueError, match="measurements"):
         _core.fit_single_tissue(np.ascontiguousarray(s[:, :-1]), solver, 1)
+
+
+def _volume_case(n_rfs):
+    from larmor_amd import synthetic
+    from larmor_amd.dwi.inputs import estimate_s0, signal_rows
+    from larmor_amd.sd.rf import AnisotropicRF, IsotropicRF
+    from larmor_amd.types import GradientTable, Volume
+
+    cfg = dict(synthetic.MFOD_CONFIG)
+    cfg["shape"] = (8, 8, 5)
+    ph = synthetic.phantom(**cfg)
+    gtab = GradientTable(ph.bvals, ph.bvecs)
+    volume = Volume(ph.data, np.eye(4))
+    rfs = [AnisotropicRF([1.7e-3, 0.2e-3, 0.2e-3], 0.0), IsotropicRF(0.7e-3), IsotropicRF(3e-3)]
+    rfs = rfs[:n_rfs]
+    mask = ph.mask
+    s0 = estimate_s0(volume, gtab, mask)[mask]
+    signals = signal_rows(volume, np.nonzero(mask), slice(0, len(s0))) / s0[:, None]
+    np.minimum(signals, 1.0, out=signals)
+    return ph, gtab, volume, rfs, mask, np.ascontiguousarray(signals)
+
+
+def test_sd_model_grl_default_matches_oracle_pipeline():
+    """SDModel's default method (GRL: dRL, inner shells weighted 0.2) on a synthetic volume:
+    the fractions equal the oracle's GRL fit bit for bit, the SH FODs its amplitudes projected
+    (float32)."""
+    from larmor_amd.sd.kernel import Kernel
+    from larmor_amd.sd.model import SDModel, compute_shell_weights, low_resolution_dirs
+    from larmor_amd.sd.sh import projection_matrix
+    from larmor_amd.sd.solvers import FINAL_FRACTION_TOL_X
+    from larmor_amd.sd.sphere import DIRS_300
+
+    ph, gtab, volume, rfs, mask, signals = _volume_case(3)
+    print("grl model", signals.shape, flush=True)
+    fit = SDModel(rfs).fit(volume, gtab, mask=mask, n_jobs=1)
+    w = compute_shell_weights(ph.bvals, 0.2)
+    hr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=DIRS_300, weights=w)
+    lr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=low_resolution_dirs(rfs, len(ph.bvals)), weights=w)
+    nus = []
+    for kernel in (lr, hr):
+        k = kernel.k
+        iso = np.full((1, len(ph.bvals)), np.exp(-ph.bvals.max() * 0.7e-3))
+        nus.append(0.2 * oracle.fit_single_tissue_drl(iso, k.T.copy(), k.T @ k, 0.0, 200,
+                                                      1.0 / k.shape[1]).max())
+    fods, fractions = oracle.fit_multi_tissue_drl(
+        signals, w, lr, hr, _stage(lr, nus[0], 200), _stage(hr, nus[1], 600),
+        fraction_tol=1e-3, final_tol=FINAL_FRACTION_TOL_X)
+    np.testing.assert_array_equal(fit.tissue_fractions[mask], fractions.astype(np.float32))
+    sh = fods @ (projection_matrix(DIRS_300, 8).T * (300 / (4 * np.pi)))
+    np.testing.assert_allclose(fit.fods[0][mask], sh.astype(np.float32), rtol=2e-6, atol=1e-6)
+
+
+def test_sd_model_grl_single_tissue_matches_oracle():
+    from larmor_amd.sd.kernel import Kernel
+    from larmor_amd.sd.model import SDModel
+    from larmor_amd.sd.sh import projection_matrix
+    from larmor_amd.sd.sphere import DIRS_300
+
+    ph, gtab, volume, rfs, mask, signals = _volume_case(1)
+    fit = SDModel(rfs).fit(volume, gtab, mask=mask, n_jobs=1)
+    kernel = Kernel(rfs, ph.bvals, ph.bvecs, dirs=DIRS_300)
+    k = kernel.k
+    iso = np.full((1, len(ph.bvals)), np.exp(-ph.bvals.max() * 0.7e-3))
+    nu = 0.2 * oracle.fit_single_tissue_drl(iso, k.T.copy(), k.T @ k, 0.0, 200,
+                                            1.0 / k.shape[1]).max()
+    wk = kernel.w_k_aniso
+    ref = oracle.fit_single_tissue_drl(signals, wk.T.copy(), wk.T @ wk, nu, 200, 1.0 / wk.shape[1])
+    sh = ref @ (projection_matrix(DIRS_300, 8).T * (300 / (4 * np.pi)))
+    assert fit.tissue_fractions is None
+    np.testing.assert_allclose(fit.fods[0][mask], sh.astype(np.float32), rtol=2e-6, atol=1e-6)
