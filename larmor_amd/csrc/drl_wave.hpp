@@ -51,7 +51,13 @@ struct DrlArgs {
     double* pred;             // (c, ld) f K^T K
     double* x_out;            // (n_vox, c)
     int* status;              // (n_vox): ST_NOT_CONVERGED OR-ed in when the limit stopped it
+    unsigned long long* work; // profiled launches: the solves' FP64 operations
 };
+
+inline bool attach_work(DrlArgs& a, unsigned long long* slot) {
+    a.work = slot;
+    return slot != nullptr;
+}
 
 __device__ __forceinline__ double power_8(double x) { // common/maths.hpp:26-30
     const double x2 = x * x;
@@ -158,6 +164,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) drl_kernel(DrlArgs A) 
         }
         bool active = on;
         bool converged = false;
+        long ran = 0;
         for (long iter = 0; iter < A.n_iter; ++iter) {
             if (ballot(active) == 0ull) {
                 break;
@@ -199,6 +206,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) drl_kernel(DrlArgs A) 
                     change = change + fabs(updated - A.prev[j * ld + p]);
                     A.prev[j * ld + p] = updated;
                 }
+                ran = iter + 1;
                 if (change <= threshold) {
                     converged = true;
                     active = false;
@@ -211,6 +219,14 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) drl_kernel(DrlArgs A) 
             }
             if (!converged && A.n_iter > 0) {
                 A.status[v] |= ST_NOT_CONVERGED;
+            }
+            if (A.work != nullptr) {
+                // Per voxel: the signal statistics (4m), K^T s (2mc), and per iteration f K^T K
+                // (2c^2) and the update (16c: power, damping, ratio, step, change).
+                const unsigned long long c = static_cast<unsigned long long>(A.c);
+                const unsigned long long mm = static_cast<unsigned long long>(A.m);
+                atomicAdd(A.work, 4ull * mm + 2ull * mm * c +
+                                      static_cast<unsigned long long>(ran) * (2ull * c * c + 16ull * c));
             }
         }
     }

@@ -520,6 +520,8 @@ inline bool attach_work(FodArgs& a, unsigned long long* slot) {
     a.work = slot;
     return slot != nullptr;
 }
+struct DrlArgs;
+inline bool attach_work(DrlArgs& a, unsigned long long* slot); // drl_wave.hpp
 
 // Launches kern over `waves` waves; lds_bytes of dynamic LDS per block.
 template <class Kern, class Args>
