@@ -24,7 +24,7 @@ namespace lsd {
 constexpr double DRL_FOD_FLOOR = 1e-6;           // drl.cpp:13-14
 constexpr double DRL_DENOMINATOR_OFFSET = 1e-16; // drl.cpp:16-17
 constexpr double DRL_CONVERGENCE_SHARE = 1e-4;   // drl.cpp:19-21
-constexpr int DRL_COLUMNS = 48;                  // directions per pass of the products
+constexpr int DRL_COLUMNS = 32;                  // directions per pass of the products
 
 struct DrlArgs {
     const double* signals;    // (n_vox, m)
@@ -176,7 +176,24 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) drl_kernel(DrlArgs A) 
                 for (int jj = 0; jj < NCB; ++jj) {
                     acc[jj] = 0.0;
                 }
-                for (int k = 0; k < A.c; ++k) {
+                int k = 0;
+                for (; k + 4 <= A.c; k += 4) {
+                    double fq[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        fq[u] = active ? A.f[(k + u) * ld + p] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const ConstRow grow =
+                            (ConstRow)(A.gram + static_cast<long>(k + u) * A.c + j0);
+#pragma unroll
+                        for (int jj = 0; jj < NCB; ++jj) {
+                            acc[jj] = acc[jj] + fq[u] * grow[jj];
+                        }
+                    }
+                }
+                for (; k < A.c; ++k) {
                     const double fk = active ? A.f[k * ld + p] : 0.0;
                     const ConstRow grow = (ConstRow)(A.gram + static_cast<long>(k) * A.c + j0);
 #pragma unroll
