@@ -112,6 +112,165 @@ def copy_probe(mib: int) -> int:
     return 0
 
 
+def _cpu_rate(fn, n, threads):
+    t0 = time.perf_counter()
+    fn()
+    return n / (time.perf_counter() - t0), threads
+
+
+def rows_report() -> int:
+    """Times each row of the path beside its CPU oracle on a sample (--rows).  Prints one line per
+    row: GPU voxels/s on the stated workload, the oracle's voxels/s on its sample with the host's
+    threads, and the counted or dense FP64 rate of the row's kernels."""
+    import torch
+
+    import oracle
+    from larmor_amd import _core, _lib, synthetic
+    from larmor_amd.sd.csd_solver import CSDSolver, csd_problem
+    from larmor_amd.sd.kernel import Kernel
+    from larmor_amd.sd.model import compute_shell_weights, low_resolution_dirs
+    from larmor_amd.sd.msmt import constraint_matrix, forward_matrix, msmt_ridge
+    from larmor_amd.sd.peaks import _build_icosphere, _device_sphere, _get_amplitudes
+    from larmor_amd.sd.rf import AnisotropicRF, IsotropicRF, ZonalSHRF
+    from larmor_amd.sd.sh import projection_matrix
+    from larmor_amd.sd.solvers import (DRL, FINAL_FRACTION_TOL_X, MAX_ROUNDS, Stage,
+                                       _deconv_kernel)
+    from larmor_amd.sd.sphere import DIRS_300
+
+    threads = os.cpu_count() or 1
+    rng = np.random.default_rng(0)
+    lines = []
+
+    def report(row, workload, n, gpu_s, cpu, kernels):
+        profile = ", ".join(
+            f"{k} {v[0]:.1f} ms" + (f" ({v[2] / (v[0] / 1e3) / 1e12:.2f} TFLOP/s counted)" if v[2] else "")
+            for k, v in kernels.items())
+        line = (f"| {row} | {workload} | {n / gpu_s:,.0f} | {cpu[0]:,.0f} ({cpu[1]} threads) | "
+                f"{(n / gpu_s) / cpu[0]:.0f}x | {profile} |")
+        lines.append(line)
+        print(line, flush=True)
+
+    def profiled(fn):
+        """Wall time and per-kernel times of one run of fn after an unprofiled warmup."""
+        fn()
+        torch.cuda.synchronize()
+        _lib.profile_enable(True)
+        _lib.profile_collect()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        kt = _lib.profile_collect()
+        _lib.profile_enable(False)
+        return t, out, kt
+
+    cfg = dict(synthetic.MFOD_CONFIG)
+    ph = synthetic.phantom(**cfg)
+    signals = synthetic.normalized_signals(ph.data, ph.bvals, ph.mask)
+    n_vox, m = signals.shape
+    dev = torch.from_numpy(signals).to("cuda")
+    sample = np.ascontiguousarray(signals[rng.choice(n_vox, 2048, replace=False)])
+    rfs = [AnisotropicRF([1.7e-3, 0.3e-3, 0.3e-3], 0.0), IsotropicRF(0.8e-3), IsotropicRF(3.0e-3)]
+    vol = f"{n_vox} voxels x {m} measurements (the bench volume)"
+
+    # a2/a4: single-tissue ridge NNLS on the 300 HR directions.
+    k1 = Kernel(rfs[:1], ph.bvals, ph.bvecs, dirs=DIRS_300)
+    solver = _core.FodSolver.l2nnls(np.ascontiguousarray(k1.w_k_aniso), 0.1, 1e-6)
+    t, _, kt = profiled(lambda: _core.fit_single_tissue(dev, solver, 1))
+    cpu = _cpu_rate(lambda: oracle.fit_single_tissue(sample, k1.w_k_aniso, 0.1, 1e-6,
+                                                     n_threads=threads), len(sample), threads)
+    report("single-tissue L2NNLS (a2, a4)", vol + ", 300 directions", n_vox, t, cpu, kt)
+
+    # GRL: damped Richardson-Lucy multi-tissue (SDModel's default method).
+    w = compute_shell_weights(ph.bvals, 0.2)
+    hr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=DIRS_300, weights=w)
+    lr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=low_resolution_dirs(rfs, len(ph.bvals)), weights=w)
+    lr_s = DRL()._prepare(lr, ph.bvals, Stage.LOW_RES)
+    hr_s = DRL()._prepare(hr, ph.bvals, Stage.HIGH_RES)
+    wt = torch.from_numpy(np.ascontiguousarray(w)).to("cuda")
+    lk, hk = _deconv_kernel(lr), _deconv_kernel(hr)
+    t, _, kt = profiled(lambda: _core.fit_multi_tissue(dev, wt, lk, lr_s, hk, hr_s, MAX_ROUNDS,
+                                                       1e-3, FINAL_FRACTION_TOL_X, 1))
+
+    def stage(kernel, n_iter):
+        k = kernel.w_k_aniso
+        iso = np.full((1, m), np.exp(-ph.bvals.max() * 0.7e-3))
+        kk = kernel.k
+        nu = 0.2 * oracle.fit_single_tissue_drl(iso, kk.T.copy(), kk.T @ kk, 0.0, 200,
+                                                1.0 / kk.shape[1]).max()
+        return (k.T.copy(), k.T @ k, nu, n_iter - 1, 1e-6)
+
+    small = sample[:96]
+    cpu = _cpu_rate(lambda: oracle.fit_multi_tissue_drl(small, w, lr, hr, stage(lr, 200),
+                                                        stage(hr, 600), n_threads=threads),
+                    len(small), threads)
+    report("GRL (damped Richardson-Lucy)", vol + ", 44 LR / 300 HR directions", n_vox, t, cpu, kt)
+
+    # a8: SH projection of the 300 amplitudes to lmax 8.
+    amps = torch.rand((n_vox, 300), dtype=torch.float64, device="cuda")
+    proj = torch.as_tensor(projection_matrix(DIRS_300, 8).T * (300 / (4 * np.pi)),
+                           dtype=torch.float64, device="cuda")
+    t, _, kt = profiled(lambda: _core.sh_project(amps, proj, exact_integral=True))
+    host_amps = amps[:50000].cpu().numpy()
+    host_proj = proj.cpu().numpy()
+    cpu = _cpu_rate(lambda: host_amps @ host_proj, 50000, threads)
+    report("SH projection (a8)", f"{n_vox} x 300 amplitudes to 45 coefficients (numpy as the "
+           "CPU side)", n_vox, t, cpu, kt)
+
+    # a9: CSD, lmax 8, the single-shell CSD phantom at 4 x its volume.
+    ccfg = dict(synthetic.CSD_CONFIG)
+    ccfg["shape"] = (64, 64, 40)
+    cph = synthetic.phantom(**ccfg)
+    on_shell = cph.bvals > 50
+    cs = np.ascontiguousarray(
+        np.maximum(cph.data.reshape(-1, len(cph.bvals))[:, on_shell], 0.0), dtype=np.float64)
+    p = csd_problem(cph.bvecs[on_shell], synthetic.tensor_zonal(1000.0), 8, CSDSolver())
+    cdev = torch.from_numpy(cs).to("cuda")
+    t, _, kt = profiled(lambda: _core.fit_csd(cdev, p.initial, p.forward, p.constraint,
+                                              p.norm_weight, p.threshold, p.max_iter, 1))
+    csample = np.ascontiguousarray(cs[rng.choice(len(cs), 4096, replace=False)])
+    cpu = _cpu_rate(lambda: oracle.fit_csd(csample, p.initial, p.forward, p.constraint,
+                                           p.norm_weight, p.threshold, p.max_iter,
+                                           n_threads=threads), len(csample), threads)
+    report("CSD (a9)", f"{len(cs)} voxels x {cs.shape[1]} measurements, lmax 8, 300 constraints",
+           len(cs), t, cpu, kt)
+
+    # a10: MSMT-CSD (LSI): WM lmax 8 + GM + CSF on the bench volume's shells.
+    shells = np.array([0.0, 1000.0, 2000.0, 3000.0])
+
+    def zonal_tensor(par, perp):
+        return ZonalSHRF(bvals=shells, coefficients=np.array(
+            [synthetic.tensor_zonal(b, par, perp) for b in shells]))
+
+    def zonal_iso(d):
+        return ZonalSHRF(bvals=shells, coefficients=np.sqrt(4 * np.pi) * np.exp(-shells * d)[:, None])
+
+    mrfs = [zonal_tensor(1.7e-3, 0.3e-3), zonal_iso(0.8e-3), zonal_iso(3.0e-3)]
+    forward = forward_matrix(mrfs, [8, 0, 0], ph.bvals, ph.bvecs)
+    g = constraint_matrix([8, 0, 0], DIRS_300)
+    design = _core.LsiDesign(np.ascontiguousarray(forward), np.ascontiguousarray(g),
+                             msmt_ridge(forward))
+    t, _, kt = profiled(lambda: _core.fit_lsi(dev, design, 1))
+    lsample = sample[:1024]
+    cpu = _cpu_rate(lambda: oracle.fit_lsi(lsample, forward, g, msmt_ridge(forward),
+                                           n_threads=threads), len(lsample), threads)
+    report("MSMT-CSD / LSI (a10)", vol + ", 47 coefficients, 302 constraints", n_vox, t, cpu, kt)
+
+    # a11: peaks of lmax-8 FODs (3 per voxel) on the 642-vertex icosphere.
+    coeffs = torch.from_numpy(np.ascontiguousarray(
+        (rng.random((n_vox, 300)) ** 8) @ (projection_matrix(DIRS_300, 8).T * (300 / (4 * np.pi)))
+    )).to("cuda")
+    sphere = _device_sphere(8, False)
+    t, _, kt = profiled(lambda: _core.find_peaks_device(coeffs, 8, sphere, 3, 0.1, 0.0))
+    psample = coeffs[:4096].cpu().numpy()
+    vertices, neighbours = _build_icosphere()
+    cpu = _cpu_rate(lambda: oracle.peaks_chunk(psample, 8, _get_amplitudes(psample, 8, False),
+                                               vertices, neighbours, 3, 0.1, 0.0,
+                                               n_threads=threads), len(psample), threads)
+    report("peaks (a11)", f"{n_vox} lmax-8 FODs, 3 peaks, 642-vertex sphere", n_vox, t, cpu, kt)
+    return 0
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -119,12 +278,17 @@ def main() -> int:
     parser.add_argument("--warmup", type=int, default=1)
     parser.add_argument("--no-cpu-baseline", action="store_true")
     parser.add_argument("--cpu-seconds", type=float, default=15.0)
+    parser.add_argument("--rows", action="store_true",
+                        help="instead of the JSON line, time every row of the path (single-tissue "
+                             "NNLS, GRL, SH projection, CSD, MSMT-CSD, peaks) beside its CPU oracle")
     parser.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                         help="only run three device-to-device copies of MIB MiB (the known-size "
                              "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = parser.parse_args()
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.rows:
+        return rows_report()
     if args.gpus != 1:
         print("bench.py measures one GPU; the path shards voxels across GPUs with no exchange, "
               "and runs on more than one GPU are not part of this build.", file=sys.stderr)

@@ -36,7 +36,7 @@ constexpr int NQ = 4;             // most fraction columns (groups + isotropic t
 // so the LDS does not cap its occupancy; 32 for the others.
 constexpr int kcap_lds(int ncpl) { return ncpl == 1 ? 24 : 32; }
 constexpr int WAVES_PER_BLOCK = 4;
-constexpr int RERUN_WAVES = 256;  // waves of a rerun launch (one per CU: the overflow list is a few percent)
+constexpr int RERUN_WAVES = 1024; // waves of a rerun launch (4 per CU; the overflow list is a few percent)
 constexpr int RERUN_WAVES_ALL = 512; // waves when every voxel needs the global-memory kernel
 constexpr double FRACTION_SUM_OFFSET = 1e-16;
 
