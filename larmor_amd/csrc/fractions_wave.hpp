@@ -64,7 +64,7 @@ __device__ __forceinline__ double median_cols(const double (&v)[NCPL], int n) {
             } else {
 #pragma unroll
                 for (int t = 0; t < NP; ++t) {
-                    const double other = __shfl_xor(a[t], j);
+                    const double other = xor_lanes(a[t], j);
                     const int idx = t * WAVE + l;
                     const bool ascending = (idx & kk) == 0;
                     const bool lower = (l & j) == 0;
@@ -180,7 +180,7 @@ __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const
                 }
 #pragma unroll
                 for (int o = WAVE / 2; o >= 1; o >>= 1) {
-                    largest = fmax(largest, __shfl_xor(largest, o));
+                    largest = fmax(largest, xor_lanes(largest, o));
                 }
                 largest = readlane(largest, 0);
                 if (largest > 0.0) {

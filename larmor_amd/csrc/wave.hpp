@@ -44,6 +44,51 @@ __device__ __forceinline__ int gather(int v, int src) { return __shfl(v, src); }
 
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
 
+// The value of x in lane lane_id() ^ o, for o = 1, 2, 4, 8, 16 or 32 (a constant once the caller is
+// unrolled), with all 64 lanes active.  Within a 16-lane row it is a DPP move: quad permutations for
+// o = 1, 2, the rotation by 8 for o = 8, and for o = 4 the rotation by 4 or by 12 (a lane with bit 2
+// set reads lane - 4, one without it lane + 4).  Across rows it is gfx950's swap of odd rows of the
+// first operand with even rows of the second (o = 16) or of the upper half of the first with the
+// lower half of the second (o = 32): with x as both operands, a lane of an even row (lower half)
+// finds its partner in the second result, one of an odd row (upper half) in the first.  None of
+// these goes through the LDS crossbar, and the partner is exactly lane ^ o, as __shfl_xor's is.
+__device__ __forceinline__ unsigned xor_lanes(unsigned x, int o) {
+    const int l = lane_id();
+    const int xi = static_cast<int>(x);
+    if (o == 1) {
+        return static_cast<unsigned>(__builtin_amdgcn_mov_dpp(xi, 0xB1, 0xf, 0xf, false));  // quad_perm [1,0,3,2]
+    }
+    if (o == 2) {
+        return static_cast<unsigned>(__builtin_amdgcn_mov_dpp(xi, 0x4E, 0xf, 0xf, false));  // quad_perm [2,3,0,1]
+    }
+    if (o == 4) {
+        const int back = __builtin_amdgcn_mov_dpp(xi, 0x124, 0xf, 0xf, false);   // row_ror:4, lane - 4
+        const int ahead = __builtin_amdgcn_mov_dpp(xi, 0x12C, 0xf, 0xf, false);  // row_ror:12, lane + 4
+        return static_cast<unsigned>((l & 4) ? back : ahead);
+    }
+    if (o == 8) {
+        return static_cast<unsigned>(__builtin_amdgcn_mov_dpp(xi, 0x128, 0xf, 0xf, false));  // row_ror:8
+    }
+    if (o == 16) {
+        const auto s = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+        return (l & 16) ? s[0] : s[1];
+    }
+    const auto s = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return (l & 32) ? s[0] : s[1];
+}
+
+__device__ __forceinline__ int xor_lanes(int x, int o) {
+    return static_cast<int>(xor_lanes(static_cast<unsigned>(x), o));
+}
+
+__device__ __forceinline__ double xor_lanes(double x, int o) {
+    const auto bits = static_cast<unsigned long long>(__double_as_longlong(x));
+    const unsigned lo = xor_lanes(static_cast<unsigned>(bits & 0xffffffffull), o);
+    const unsigned hi = xor_lanes(static_cast<unsigned>(bits >> 32), o);
+    return __longlong_as_double(
+        static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
 // Orders this wave's LDS accesses across lanes: writes before it are seen by reads after it.
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -56,7 +101,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 __device__ __forceinline__ double butterfly_sum(double p) {
 #pragma unroll
     for (int o = WAVE / 2; o >= 1; o >>= 1) {
-        p = p + __shfl_xor(p, o);
+        p = p + xor_lanes(p, o);
     }
     return readlane(p, 0);
 }
@@ -65,7 +110,7 @@ __device__ __forceinline__ double butterfly_sum(double p) {
 __device__ __forceinline__ double wave_max(double p) {
 #pragma unroll
     for (int o = WAVE / 2; o >= 1; o >>= 1) {
-        const double q = __shfl_xor(p, o);
+        const double q = xor_lanes(p, o);
         p = q > p ? q : p;
     }
     return readlane(p, 0);
@@ -76,8 +121,8 @@ __device__ __forceinline__ double wave_max(double p) {
 __device__ __forceinline__ void wave_argmax_first(double& value, int& idx) {
 #pragma unroll
     for (int o = WAVE / 2; o >= 1; o >>= 1) {
-        const double ov = __shfl_xor(value, o);
-        const int oi = __shfl_xor(idx, o);
+        const double ov = xor_lanes(value, o);
+        const int oi = xor_lanes(idx, o);
         const bool take = oi >= 0 && (idx < 0 || ov > value || (ov == value && oi < idx));
         value = take ? ov : value;
         idx = take ? oi : idx;
@@ -91,8 +136,8 @@ __device__ __forceinline__ void wave_argmax_first(double& value, int& idx) {
 __device__ __forceinline__ void wave_argmin_first(double& value, int& idx) {
 #pragma unroll
     for (int o = WAVE / 2; o >= 1; o >>= 1) {
-        const double ov = __shfl_xor(value, o);
-        const int oi = __shfl_xor(idx, o);
+        const double ov = xor_lanes(value, o);
+        const int oi = xor_lanes(idx, o);
         const bool take = oi >= 0 && (idx < 0 || ov < value || (ov == value && oi < idx));
         value = take ? ov : value;
         idx = take ? oi : idx;
@@ -141,7 +186,7 @@ __device__ __forceinline__ void wave_argmin_cols(const double (&v)[NS], const bo
     }
 #pragma unroll
     for (int o = WAVE / 2; o >= 1; o >>= 1) {
-        const double q = __shfl_xor(lane_min, o);
+        const double q = xor_lanes(lane_min, o);
         lane_min = q < lane_min ? q : lane_min;
     }
     value = readlane(lane_min, 0);
