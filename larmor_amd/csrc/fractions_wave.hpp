@@ -122,6 +122,7 @@ struct FractionDesign {
 };
 
 constexpr int MAX_GROUPS = 2; // anisotropic blocks of a per-block fraction fit
+constexpr int GROUP_BATCH = 4; // kernel rows loaded together for a group signal
 
 // compute_fractions of one voxel: fod in column slots, s the unweighted signal.  lumped takes
 // all anisotropic columns as one group; NG group columns are held (1 suffices when lumped).  fr receives the fractions, groups first.  work is the
@@ -149,22 +150,39 @@ __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const
             const int begin = lumped ? 0 : K.blocks[2 * grp];
             const int end = lumped ? K.ca : K.blocks[2 * grp + 1];
             // Zero terms add exactly nothing, so only the non-zero thresholded values are
-            // visited, in column order.
+            // visited, in column order; their kernel rows are loaded GROUP_BATCH at a time, so
+            // the loads overlap, and then added in column order.
 #pragma unroll
             for (int t2 = 0; t2 < NCPL; ++t2) {
                 const int c0 = l + WAVE * t2;
                 unsigned long long mask = ballot(c0 >= begin && c0 < end && thr[t2] != 0.0);
                 while (mask) {
-                    const int b = __ffsll(static_cast<long long>(mask)) - 1;
-                    mask &= mask - 1;
-                    const int c = WAVE * t2 + b;
-                    const double f = readlane(thr[t2], b);
-                    const double* krow = K.aniso_t + static_cast<long>(c) * K.m;
+                    double fb[GROUP_BATCH];
+                    double kb[GROUP_BATCH][RPL];
+                    bool valid[GROUP_BATCH];
 #pragma unroll
-                    for (int t = 0; t < RPL; ++t) {
-                        const int r = l + WAVE * t;
-                        if (r < K.m) {
-                            gs[t] = gs[t] + krow[r] * f;
+                    for (int q = 0; q < GROUP_BATCH; ++q) {
+                        valid[q] = mask != 0ull;
+                        const int b = valid[q] ? __ffsll(static_cast<long long>(mask)) - 1 : 0;
+                        mask &= mask - 1;
+                        fb[q] = readlane(thr[t2], b);
+                        const double* krow = K.aniso_t + static_cast<long>(WAVE * t2 + b) * K.m;
+#pragma unroll
+                        for (int t = 0; t < RPL; ++t) {
+                            const int r = l + WAVE * t;
+                            kb[q][t] = (valid[q] && r < K.m) ? krow[r] : 0.0;
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < GROUP_BATCH; ++q) {
+                        if (valid[q]) {
+#pragma unroll
+                            for (int t = 0; t < RPL; ++t) {
+                                const int r = l + WAVE * t;
+                                if (r < K.m) {
+                                    gs[t] = gs[t] + kb[q][t] * fb[q];
+                                }
+                            }
                         }
                     }
                 }

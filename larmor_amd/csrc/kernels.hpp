@@ -392,6 +392,15 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
         const long v = A.list ? A.list[it] : it;
         RowVec<RPL> s;
         load_signal<RPL>(A.signals, v, A.m, s);
+        // The loop's previous fractions (lane q holds fraction q) and round count, loaded before
+        // the fit so that their latency overlaps it.
+        const int nlf = 1 + ci;
+        double prev = 0.0;
+        int rounds = 0;
+        if (!FINAL) {
+            prev = l < nlf ? A.prev[v * nlf + l] : 0.0;
+            rounds = l == 0 ? A.rounds[v] : 0;
+        }
         double fod[NCPL];
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {
@@ -426,25 +435,20 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
             }
             continue;
         }
-        const int nlf = 1 + ci;
-        bool converged = true;
+        double mine = 0.0;
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            if (q < nlf && !(fabs(fr[q] - A.prev[v * nlf + q]) < A.fraction_tol)) {
-                converged = false;
-            }
+            mine = l == q ? fr[q] : mine;
         }
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            if (l == q && q >= 1 && q < nlf) {
-                A.iso[v * ci + (q - 1)] = fr[q];
-            }
-            if (!converged && l == q && q < nlf) {
-                A.prev[v * nlf + q] = fr[q];
-            }
+        const bool converged = ballot(l < nlf && !(fabs(mine - prev) < A.fraction_tol)) == 0ull;
+        if (l >= 1 && l < nlf) {
+            A.iso[v * ci + (l - 1)] = mine;
+        }
+        if (!converged && l < nlf) {
+            A.prev[v * nlf + l] = mine;
         }
         if (l == 0) {
-            A.rounds[v] += 1;
+            A.rounds[v] = rounds + 1;
             if (!converged) {
                 const int at = atomicAdd(A.next_count, 1);
                 A.next_list[at] = static_cast<int>(v);
