@@ -66,6 +66,7 @@ template <int NCPL, int KCAP, int RPLQ, bool QR = true>
 struct RidgeNnlsWave {
     static constexpr int NSK = (KCAP + WAVE - 1) / WAVE; // factor-position slots
     static constexpr int S = KCAP + 1;                   // factor column stride
+    static constexpr bool PREFETCH = QR;                 // factor in global memory
 
     const RidgeDesignDev& D;
     double tol;
@@ -105,16 +106,6 @@ struct RidgeNnlsWave {
     __device__ __forceinline__ double2 diag_at(int j) const {
         return *reinterpret_cast<const double2*>(diag + 2 * j);
     }
-    // a != 0 ? a / L_jj : a; FAST: every diagonal is in the divisor range.
-    template <bool FAST>
-    __device__ __forceinline__ double solve_divide(double a, int j) const {
-        const double2 d = diag_at(j);
-        if (FAST && dividend_in_range(a)) {
-            return divide_in_range(a, d.x, d.y);
-        }
-        return a != 0.0 ? a / d.x : a;
-    }
-
     __device__ __forceinline__ int col(int t) const { return lane_id() + WAVE * t; }
     __device__ __forceinline__ double& Lat(int i, int j) { return L[j * S + i]; }
     __device__ __forceinline__ double Luni(int i, int j) const {
@@ -128,21 +119,63 @@ struct RidgeNnlsWave {
     // up to 64 NSK - 1 of a column lie within the column stride and the slack after the last
     // column) and its product is discarded.
 
+    // a != 0 ? a / L_jj : a from d = (L_jj, its reciprocal); FAST: every diagonal is in the
+    // divisor range.
+    template <bool FAST>
+    __device__ __forceinline__ double divide_by_diag(double a, double2 d) const {
+        if (FAST && dividend_in_range(a)) {
+            return divide_in_range(a, d.x, d.y);
+        }
+        return a != 0.0 ? a / d.x : a;
+    }
+
     // L w = b on positions [p0, kk) when positions [0, p0) of b already hold their final values
     // (p0 = 0: the whole solve); returns the sum of the w_j^2 in order (for p0 = 0).  Positions
-    // from p0 on receive the same updates, in the same order, as in the whole solve.
+    // from p0 on receive the same updates, in the same order, as in the whole solve.  With the
+    // factor in global memory (PREFETCH), step j's reads (the diagonal entry and its reciprocal,
+    // the factor column) are issued during step j - 1, so their latency is off the chain of
+    // dependent steps; from LDS the plain order issues fewer instructions and is faster.
     template <bool FAST>
     __device__ __forceinline__ double forward_impl(LaneVec<NSK>& b, int p0, int kk) {
         const int l = lane_id();
         double sq = 0.0;
+        double2 d_next = diag_at(0);
+        double l_next[NSK];
+        if constexpr (PREFETCH) {
+#pragma unroll
+            for (int u = 0; u < NSK; ++u) {
+                l_next[u] = Lat(l + WAVE * u, 0);
+            }
+        }
         for (int j = 0; j < kk; ++j) {
+            double2 d;
+            double lc[NSK];
+            if constexpr (PREFETCH) {
+                d = d_next;
+#pragma unroll
+                for (int u = 0; u < NSK; ++u) {
+                    lc[u] = l_next[u];
+                }
+                const int jn = j + 1 < kk ? j + 1 : j;
+                d_next = diag_at(jn);
+#pragma unroll
+                for (int u = 0; u < NSK; ++u) {
+                    l_next[u] = Lat(l + WAVE * u, jn);
+                }
+            } else {
+                d = diag_at(j);
+#pragma unroll
+                for (int u = 0; u < NSK; ++u) {
+                    lc[u] = Lat(l + WAVE * u, j);
+                }
+            }
             const double bj = b.get(j);
-            const double wj = j >= p0 ? solve_divide<FAST>(bj, j) : bj;
+            const double wj = j >= p0 ? divide_by_diag<FAST>(bj, d) : bj;
             sq = sq + wj * wj;
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
                 const int i = l + WAVE * u;
-                const double updated = b.v[u] - Lat(i, j) * wj;
+                const double updated = b.v[u] - lc[u] * wj;
                 const bool below = i > j && i < kk && i >= p0;
                 b.v[u] = i == j ? wj : (below ? updated : b.v[u]);
             }
@@ -169,18 +202,51 @@ struct RidgeNnlsWave {
     }
 
     // L^T z = w on positions [0, kk), column-oriented.  A lane past the factor reads row 0 of
-    // column j instead of the row its index names (outside the slab).
+    // column j instead of the row its index names (outside the slab).  As in the forward solve,
+    // with PREFETCH step j's reads are issued during step j + 1.
     template <bool FAST>
     __device__ __forceinline__ void backward_impl(LaneVec<NSK>& b, int kk) {
         const int l = lane_id();
-        for (int j = kk - 1; j >= 0; --j) {
-            const double wj = b.get(j);
-            const double zj = solve_divide<FAST>(wj, j);
+        const int top = kk > 0 ? kk - 1 : 0;
+        double2 d_next = diag_at(top);
+        double l_next[NSK];
+        if constexpr (PREFETCH) {
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
                 const int i = l + WAVE * u;
-                const int ir = i < KCAP ? i : 0;
-                const double updated = b.v[u] - Lat(j, ir) * zj;
+                l_next[u] = Lat(top, i < KCAP ? i : 0);
+            }
+        }
+        for (int j = kk - 1; j >= 0; --j) {
+            double2 d;
+            double lc[NSK];
+            if constexpr (PREFETCH) {
+                d = d_next;
+#pragma unroll
+                for (int u = 0; u < NSK; ++u) {
+                    lc[u] = l_next[u];
+                }
+                const int jn = j > 0 ? j - 1 : 0;
+                d_next = diag_at(jn);
+#pragma unroll
+                for (int u = 0; u < NSK; ++u) {
+                    const int i = l + WAVE * u;
+                    l_next[u] = Lat(jn, i < KCAP ? i : 0);
+                }
+            } else {
+                d = diag_at(j);
+#pragma unroll
+                for (int u = 0; u < NSK; ++u) {
+                    const int i = l + WAVE * u;
+                    lc[u] = Lat(j, i < KCAP ? i : 0);
+                }
+            }
+            const double wj = b.get(j);
+            const double zj = divide_by_diag<FAST>(wj, d);
+#pragma unroll
+            for (int u = 0; u < NSK; ++u) {
+                const int i = l + WAVE * u;
+                const double updated = b.v[u] - lc[u] * zj;
                 b.v[u] = i == j ? zj : (i < j ? updated : b.v[u]);
             }
         }
