@@ -18,7 +18,9 @@ int launch_lsi_t(const LsiArgs& fast, const LsiArgs& qr, int waves, int qr_waves
                  hipStream_t stream) {
     int rc = launch_fill("lsi", lsi_kernel<NKS>, fast, waves, 0, stream);
     if (rc) return rc;
-    return launch("lsi_qr", lsi_qr_kernel<NKS>, qr, qr_waves, 0, stream);
+    // One wave per block, its column slab in the block's LDS.
+    return launch("lsi_qr", lsi_qr_kernel<NKS>, qr, qr_waves, sizeof(double) * LSI_QR_SLAB, stream,
+                  1);
 }
 
 } // namespace

@@ -527,11 +527,11 @@ inline bool attach_work(FodArgs& a, unsigned long long* slot) {
 struct DrlArgs;
 inline bool attach_work(DrlArgs& a, unsigned long long* slot); // drl_wave.hpp
 
-// Launches kern over `waves` waves; lds_bytes of dynamic LDS per block.
+// Launches kern over `waves` waves, waves_per_block to a block with lds_bytes of dynamic LDS.
 template <class Kern, class Args>
 inline int launch(const char* name, Kern kern, const Args& args, int waves, size_t lds_bytes,
-           hipStream_t stream) {
-    const int blocks = (waves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+           hipStream_t stream, int waves_per_block = WAVES_PER_BLOCK) {
+    const int blocks = (waves + waves_per_block - 1) / waves_per_block;
     ProfiledLaunch rec{name, nullptr, nullptr, nullptr};
     Args a = args;
     if (g_profiling) {
@@ -545,7 +545,7 @@ inline int launch(const char* name, Kern kern, const Args& args, int waves, size
             rec.work = slot;
         }
     }
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(WAVE * WAVES_PER_BLOCK), lds_bytes, stream, a);
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(WAVE * waves_per_block), lds_bytes, stream, a);
     LSD_HIP(hipGetLastError());
     if (g_profiling) {
         LSD_HIP(hipEventRecord(rec.stop, stream));

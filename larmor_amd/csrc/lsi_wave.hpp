@@ -11,7 +11,8 @@
 //    exceed 1e-12 H_cc), blocking steps that zero the first blocking coefficient and any below
 //    the tolerance, Givens downdates, at most 3 k inner steps;
 //  - when that fails, the rerun kernel solves the same NNLS on the explicit matrix [E^T; f^T]
-//    with pivoted-QR least squares (solve_nnls_with_limit), its rows summed as wave sums;
+//    with pivoted-QR least squares (solve_nnls_with_limit): the QR's row sums are wave sums, the
+//    gradient and the tolerance's column sums run over the rows in order, one column per lane;
 //  - x = P R^-1 (scale (E^T u) / (1 - f.u) + r), the sums over the non-zero u_c in order.
 //
 // Layout: constraint (NNLS column) c in lane c % 64, slot c / 64 (NKS slots); parameter i in
@@ -37,8 +38,8 @@ struct LsiDev {
 };
 
 constexpr int LSI_KCAP = 64;
-constexpr int LSI_QR_ROWS = 2;    // row slots of [E^T; f^T]: n + 1 <= 128
-constexpr int LSI_QR_COLS = 128;  // active columns the QR fallback holds
+constexpr int LSI_QR_ROWS = 1;    // row slots of [E^T; f^T]: n + 1 <= 64
+constexpr int LSI_GRADIENT_BATCH = 2; // rows of E E^T loaded together by the gradient
 
 struct LsiArgs {
     const double* signals;      // (n_vox, m)
@@ -204,36 +205,48 @@ struct LdpWave {
         return out;
     }
 
-    // L w = b on positions [0, kk), column-oriented; returns the sum of the w_j^2 in order.
-    __device__ __forceinline__ double forward(double& b, int kk) {
+    // L w = b on positions [0, kk), column-oriented; returns the sum of the w_j^2 in order.  The
+    // factor lives in global memory: column j + 1 (which also holds L_{j+1,j+1}) is loaded
+    // during step j, so the load latency is off the chain of dependent steps.
+    __device__ __forceinline__ double forward(double& b, int kk_) {
         const int l = lane_id();
+        const int kk = __builtin_amdgcn_readfirstlane(kk_);
         double sq = 0.0;
+        double col_next = L[l];
         for (int j = 0; j < kk; ++j) {
+            const double colj = col_next;
+            const int jn = j + 1 < kk ? j + 1 : j;
+            col_next = L[jn * S + l];
             const double bj = readlane(b, j);
-            const double ljj = Luni(j, j);
+            const double ljj = readlane(colj, j);
             const double wj = bj != 0.0 ? bj / ljj : bj;
             sq = sq + wj * wj;
-            if (l == j) {
-                b = wj;
-            } else if (l > j && l < kk) {
-                b = b - Lat(l, j) * wj;
-            }
+            const double updated = b - colj * wj;
+            b = l == j ? wj : ((l > j && l < kk) ? updated : b);
         }
         return sq;
     }
 
-    // L^T z = w on positions [0, kk), column-oriented.
-    __device__ __forceinline__ void backward(double& b, int kk) {
+    // L^T z = w on positions [0, kk), column-oriented: step j reads row j (L_{j,l} for the
+    // lanes l < j, at stride S) and L_jj, both loaded during step j + 1.
+    __device__ __forceinline__ void backward(double& b, int kk_) {
         const int l = lane_id();
+        const int kk = __builtin_amdgcn_readfirstlane(kk_);
+        const int top = kk > 0 ? kk - 1 : 0;
+        const int lr = l < KCAP ? l : 0;
+        double row_next = L[lr * S + top];
+        double diag_next = L[top * S + l];
         for (int j = kk - 1; j >= 0; --j) {
+            const double rowj = row_next;
+            const double diagj = diag_next;
+            const int jn = j > 0 ? j - 1 : 0;
+            row_next = L[lr * S + jn];
+            diag_next = L[jn * S + l];
             const double wj = readlane(b, j);
-            const double ljj = Luni(j, j);
+            const double ljj = readlane(diagj, j);
             const double zj = wj != 0.0 ? wj / ljj : wj;
-            if (l == j) {
-                b = zj;
-            } else if (l < j) {
-                b = b - Lat(j, l) * zj;
-            }
+            const double updated = b - rowj * zj;
+            b = l == j ? zj : (l < j ? updated : b);
         }
     }
 
@@ -366,19 +379,36 @@ struct LdpWave {
             for (int t = 0; t < NKS; ++t) {
                 g[t] = f[t] * (1.0 - dot);
             }
+            // The rows of E E^T for the non-zero u_c are loaded LSI_GRADIENT_BATCH at a time,
+            // so their loads overlap, and subtracted in column order.
 #pragma unroll
             for (int t2 = 0; t2 < NKS; ++t2) {
                 unsigned long long bits = ballot(u[t2] != 0.0);
                 while (bits) {
-                    const int b = __ffsll(static_cast<long long>(bits)) - 1;
-                    bits &= bits - 1;
-                    const int c = WAVE * t2 + b;
-                    const double uc = readlane(u[t2], b);
-                    const double* row = d.gram + static_cast<long>(c) * d.k;
+                    double ub[LSI_GRADIENT_BATCH];
+                    double rb[LSI_GRADIENT_BATCH][NKS];
+                    bool valid[LSI_GRADIENT_BATCH];
 #pragma unroll
-                    for (int t = 0; t < NKS; ++t) {
-                        if (col(t) < d.k) {
-                            g[t] = g[t] - uc * row[col(t)];
+                    for (int q = 0; q < LSI_GRADIENT_BATCH; ++q) {
+                        valid[q] = bits != 0ull;
+                        const int b = valid[q] ? __ffsll(static_cast<long long>(bits)) - 1 : 0;
+                        bits &= bits - 1;
+                        ub[q] = readlane(u[t2], b);
+                        const double* row = d.gram + static_cast<long>(WAVE * t2 + b) * d.k;
+#pragma unroll
+                        for (int t = 0; t < NKS; ++t) {
+                            rb[q][t] = (valid[q] && col(t) < d.k) ? row[col(t)] : 0.0;
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < LSI_GRADIENT_BATCH; ++q) {
+                        if (valid[q]) {
+#pragma unroll
+                            for (int t = 0; t < NKS; ++t) {
+                                if (col(t) < d.k) {
+                                    g[t] = g[t] - ub[q] * rb[q][t];
+                                }
+                            }
                         }
                     }
                 }
@@ -462,13 +492,14 @@ struct LdpColumns {
     }
 };
 
-// solve_nnls_with_limit on [E^T; f^T] (rows n + 1, columns k) with target e_n: the QR fallback
-// of ldp_multipliers.  work: LSI_QR_ROWS * 64 * LSI_QR_COLS doubles (global).  Returns whether
-// it converged; u receives the multipliers.
-template <int NKS>
+// solve_nnls_with_limit on [E^T; f^T] (rows n + 1 <= 64 RP, columns k) with target e_n: the QR
+// fallback of ldp_multipliers.  The active columns (at most 64 RP: as many as the rows, which in
+// exact arithmetic bound an independent active set) are copied to `work`, RP * 64 * 64 RP
+// doubles in LDS or global memory.  Returns whether it converged; u receives the multipliers.
+template <int NKS, int RP>
 __device__ __forceinline__ bool ldp_by_qr(const LsiDev& d, const double (&f)[NKS],
                                           double (&u)[NKS], double* work) {
-    constexpr int RP = LSI_QR_ROWS;
+    constexpr int QCOLS = RP * WAVE;
     const int l = lane_id();
     const int rows = d.n + 1;
     const int cols = d.k;
@@ -478,14 +509,36 @@ __device__ __forceinline__ bool ldp_by_qr(const LsiDev& d, const double (&f)[NKS
     for (int t = 0; t < RP; ++t) {
         target.v[t] = (l + WAVE * t) == d.n ? 1.0 : 0.0;
     }
-    // Default tolerance: 10 eps max_j (wave sum of |C(:, j)|) max(rows, cols).
-    double norm1 = 0.0;
-    for (int j = 0; j < cols; ++j) {
-        const double fj = uniform_at<NKS>(f, j);
-        const double s = wave_sum_f<RP>(0, rows, [&](int t) { return fabs(C.get(j, t, fj)); });
-        if (j == 0 || s > norm1) {
-            norm1 = s;
+    // sum over rows r in order of term(C(r, j), res_r) for the lane's columns j: row r < n of
+    // column j is E[j, r] = e_t[r, j] (one coalesced load per row and slot), row n is f_j.
+    auto column_sums = [&](auto term, const RowVec<RP>& res, double (&out)[NKS]) {
+#pragma unroll
+        for (int t = 0; t < NKS; ++t) {
+            out[t] = 0.0;
         }
+        for (int r = 0; r < rows; ++r) {
+            const double rr = res.get(r);
+#pragma unroll
+            for (int t = 0; t < NKS; ++t) {
+                const int c = (l + WAVE * t) < cols ? l + WAVE * t : 0;
+                const double v = r < d.n ? d.e_t[static_cast<long>(r) * d.k + c] : f[t];
+                out[t] = out[t] + term(v, rr);
+            }
+        }
+    };
+    // Default tolerance: 10 eps max_j (sum over rows of |C(r, j)|) max(rows, cols).
+    double norm1 = 0.0;
+    {
+        double abs_sums[NKS];
+        RowVec<RP> none;
+        none.fill(0.0);
+        column_sums([](double v, double) { return fabs(v); }, none, abs_sums);
+        double lane_max = 0.0;
+#pragma unroll
+        for (int t = 0; t < NKS; ++t) {
+            lane_max = ((l + WAVE * t) < cols && abs_sums[t] > lane_max) ? abs_sums[t] : lane_max;
+        }
+        norm1 = wave_max(lane_max);
     }
     const double tol = 10.0 * DBL_EPS * norm1 * static_cast<double>(rows > cols ? rows : cols);
     const int max_inner = 3 * cols;
@@ -498,11 +551,12 @@ __device__ __forceinline__ bool ldp_by_qr(const LsiDev& d, const double (&f)[NKS
         act[t] = false;
     }
     auto gradient = [&]() {
-        // r = d - C u over the non-zero u_j in order, then g_j = wave sum of C(:, j) r.
-        double r[RP];
+        // r = d - C u over the non-zero u_j in order, then g_j = sum over rows in order of
+        // C(r, j) r_r.
+        RowVec<RP> res;
 #pragma unroll
         for (int t = 0; t < RP; ++t) {
-            r[t] = 0.0;
+            res.v[t] = 0.0;
         }
 #pragma unroll
         for (int t2 = 0; t2 < NKS; ++t2) {
@@ -515,27 +569,15 @@ __device__ __forceinline__ bool ldp_by_qr(const LsiDev& d, const double (&f)[NKS
                 const double fj = readlane(f[t2], b);
 #pragma unroll
                 for (int t = 0; t < RP; ++t) {
-                    r[t] = r[t] + C.get(j, t, fj) * uj;
+                    res.v[t] = res.v[t] + C.get(j, t, fj) * uj;
                 }
             }
         }
 #pragma unroll
         for (int t = 0; t < RP; ++t) {
-            r[t] = target.v[t] - r[t];
+            res.v[t] = target.v[t] - res.v[t];
         }
-#pragma unroll
-        for (int t = 0; t < NKS; ++t) {
-            g[t] = 0.0;
-        }
-        for (int j = 0; j < cols; ++j) {
-            const double fj = uniform_at<NKS>(f, j);
-            const double s =
-                wave_sum_f<RP>(0, rows, [&](int t) { return C.get(j, t, fj) * r[t]; });
-#pragma unroll
-            for (int t = 0; t < NKS; ++t) {
-                g[t] = (l + WAVE * t) == j ? s : g[t];
-            }
-        }
+        column_sums([](double v, double rr) { return v * rr; }, res, g);
     };
     // Least squares on the active columns in increasing order; false on overflow.
     auto fit_on = [&]() -> bool {
@@ -547,7 +589,7 @@ __device__ __forceinline__ bool ldp_by_qr(const LsiDev& d, const double (&f)[NKS
             while (bits) {
                 const int b = __ffsll(static_cast<long long>(bits)) - 1;
                 bits &= bits - 1;
-                if (ka >= LSI_QR_COLS) {
+                if (ka >= QCOLS) {
                     return false;
                 }
                 const int j = WAVE * t2 + b;
@@ -567,8 +609,8 @@ __device__ __forceinline__ bool ldp_by_qr(const LsiDev& d, const double (&f)[NKS
             return true;
         }
         RowVec<RP> dd = target;
-        LaneVec<LSI_QR_COLS / WAVE> coef;
-        least_squares_wave<LSI_QR_COLS / WAVE>(A, rows, ka, dd, coef);
+        LaneVec<RP> coef;
+        least_squares_wave<RP>(A, rows, ka, dd, coef);
         int p = 0;
 #pragma unroll
         for (int t2 = 0; t2 < NKS; ++t2) {
@@ -705,13 +747,17 @@ __global__ void __launch_bounds__(WAVE * 4) lsi_kernel(LsiArgs A) {
     }
 }
 
-// The QR path over the listed voxels (those the fast path handed over).
+// Doubles of the QR path's per-wave column slab: 64 rows (n + 1 <= 64) by 64 columns.
+constexpr long LSI_QR_SLAB = static_cast<long>(LSI_QR_ROWS) * WAVE * LSI_QR_ROWS * WAVE;
+
+// The QR path over the listed voxels (those the fast path handed over), one wave per block with
+// its column slab (32 KB) in the block's dynamic LDS.
 template <int NKS>
-__global__ void __launch_bounds__(WAVE * 4) lsi_qr_kernel(LsiArgs A) {
-    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
-    const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
-    const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
-    double* slab = A.scratch + gw * A.per_wave;
+__global__ void __launch_bounds__(WAVE) lsi_qr_kernel(LsiArgs A) {
+    extern __shared__ double smem[];
+    const long gw = blockIdx.x;
+    const long n_waves = gridDim.x;
+    double* slab = smem;
     const int l = lane_id();
     const LsiDev& d = A.d;
     const long count = A.list ? static_cast<long>(*A.list_count) : A.n_vox;
@@ -735,7 +781,7 @@ __global__ void __launch_bounds__(WAVE * 4) lsi_qr_kernel(LsiArgs A) {
             f[t] = raw[t] / scale;
         }
         double u[NKS];
-        const bool conv = ldp_by_qr<NKS>(d, f, u, slab);
+        const bool conv = ldp_by_qr<NKS, LSI_QR_ROWS>(d, f, u, slab);
         const double x = lsi_solution<NKS>(d, f, u, scale, rot);
         if (l < d.n) A.x_out[v * d.n + l] = x;
         if (l == 0) {

@@ -197,9 +197,8 @@ int launch_sh_project(const double* amps, long n, int k, const double* p, int c,
 int lsi_constraint_slots(int64_t k);
 int launch_lsi(int nks, const LsiArgs& fast, const LsiArgs& qr, int waves, int qr_waves,
                hipStream_t stream);
-// Waves of the LSI QR path, and the doubles of its per-wave column slab.
-constexpr int LSI_QR_WAVES = 128;
-constexpr long LSI_QR_SLAB = static_cast<long>(LSI_QR_ROWS) * WAVE * LSI_QR_COLS;
+// Waves (one per block) of the LSI QR path; each keeps its column slab in LDS.
+constexpr int LSI_QR_WAVES = 2048;
 constexpr long LSI_FACTOR_SLAB = static_cast<long>(LSI_KCAP + 1) * LSI_KCAP;
 
 // Peak, derivative and climb launchers (inst_peaks.hip).
@@ -1200,11 +1199,10 @@ static int fit_lsi_impl(const double* signals, int64_t n_voxels, int64_t m, cons
     int* counter = nullptr;
     int* list = nullptr;
     double* factor = nullptr;
-    double* qr = nullptr;
     int rc = mem.get(&counter, 1);
     if (!rc) rc = mem.get(&list, n_voxels);
     if (!rc) rc = mem.get(&factor, LSI_FACTOR_SLAB * waves);
-    if (!rc) rc = mem.get(&qr, LSI_QR_SLAB * qr_waves);
+
     if (!rc && status == nullptr) rc = mem.get(&status, n_voxels);
     if (rc) return rc;
     hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, stream, counter, 1);
@@ -1225,8 +1223,8 @@ static int fit_lsi_impl(const double* signals, int64_t n_voxels, int64_t m, cons
     q.list_count = only_qr ? nullptr : counter;
     q.fallback_list = nullptr;
     q.fallback_count = nullptr;
-    q.scratch = qr;
-    q.per_wave = LSI_QR_SLAB;
+    q.scratch = nullptr; // the QR path's slabs are in LDS
+    q.per_wave = 0;
     // QR only: the fast kernel gets no voxels, the QR kernel all of them.
     if (only_qr) a.n_vox = 0;
     return launch_lsi(lsi_constraint_slots(p->k), a, q, waves, qr_waves, stream);

@@ -316,12 +316,28 @@ Vec least_squares(const Mat& design, const Vec& data) {
     return coefficients;
 }
 
-double default_gradient_tolerance(const Mat& design) {
+// How the generic NNLS sums a column over the rows: as a wave sum, or in row order (the LDP QR
+// fallback, where the GPU gives each lane whole columns).
+enum class ColumnSum { Wave, InOrder };
+
+template <class Term>
+double column_sum(ColumnSum order, Index rows, Term term) {
+    if (order == ColumnSum::Wave) {
+        return wave_sum(0, rows, term);
+    }
+    double s = 0.0;
+    for (Index r = 0; r < rows; ++r) {
+        s = s + term(r);
+    }
+    return s;
+}
+
+double default_gradient_tolerance(const Mat& design, ColumnSum order = ColumnSum::Wave) {
     double norm1 = 0.0;
     if (design.rows * design.cols > 0) {
         for (Index c = 0; c < design.cols; ++c) {
-            const double s =
-                wave_sum(0, design.rows, [&](Index r) { return std::fabs(design(r, c)); });
+            const double s = column_sum(order, design.rows,
+                                        [&](Index r) { return std::fabs(design(r, c)); });
             if (c == 0 || s > norm1) {
                 norm1 = s;
             }
@@ -430,7 +446,8 @@ std::vector<Index> active_indices(const std::vector<char>& active) {
 
 // solve_nnls_with_limit (nnls.cpp): the generic NNLS, a pivoted QR per least-squares fit.
 NnlsResult solve_nnls_with_limit(const Mat& design, const Vec& data, double tol,
-                                 Index max_inner, NnlsTrace* trace) {
+                                 Index max_inner, NnlsTrace* trace,
+                                 ColumnSum order = ColumnSum::Wave) {
     const Index n = design.cols;
     const Index m = design.rows;
     auto gradient_at = [&](const Vec& x) {
@@ -444,7 +461,7 @@ NnlsResult solve_nnls_with_limit(const Mat& design, const Vec& data, double tol,
         }
         Vec g(static_cast<size_t>(n));
         for (Index c = 0; c < n; ++c) {
-            g[c] = wave_sum(0, m, [&](Index i) { return design(i, c) * r[i]; });
+            g[c] = column_sum(order, m, [&](Index i) { return design(i, c) * r[i]; });
         }
         return g;
     };
@@ -1788,7 +1805,8 @@ struct LsiOut {
 };
 
 // solve_lsi (lsi.cpp:245-292): r = Q^T s; f = -(E r) / max|E r| (all sums in index order); the
-// LDP multipliers u by the Cholesky NNLS or, when that fails, by solve_nnls_with_limit on
+// LDP multipliers u by the Cholesky NNLS or, when that fails, by solve_nnls_with_limit (its
+// gradient and tolerance summing each column over the rows in order) on
 // [E^T; f^T] with target e_n; x = P R^-1 (scale (E^T u) / (1 - f.u) + r), the sums over the
 // non-zero u_c in order.
 LsiOut solve_lsi(const LsiDesign& d, const Vec& s, LsiTrace* tr, bool only_qr = false) {
@@ -1851,8 +1869,9 @@ LsiOut solve_lsi(const LsiDesign& d, const Vec& s, LsiTrace* tr, bool only_qr = 
         }
         Vec target(static_cast<size_t>(n + 1), 0.0);
         target[n] = 1.0;
-        NnlsResult r = solve_nnls_with_limit(c, target, default_gradient_tolerance(c), 3 * k,
-                                             nullptr);
+        NnlsResult r = solve_nnls_with_limit(
+            c, target, default_gradient_tolerance(c, ColumnSum::InOrder), 3 * k, nullptr,
+            ColumnSum::InOrder);
         u = std::move(r.solution);
         out.converged = r.converged;
     }
