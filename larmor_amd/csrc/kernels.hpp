@@ -239,11 +239,10 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     }
 }
 
-// The tissue fractions of a list of voxels from their FOD amplitudes (fractions.cpp:45-76).
 // The correlations K^T a of a list of voxels, a the signal (single-tissue) or the anisotropic
 // signal max(w s - W_iso p, 0) (multi_tissue.cpp:20-24), one voxel per lane: the design rows are
-// wave-uniform, so each multiply takes its K entry as a scalar operand and a wave's 64 voxels
-// share every instruction.  Each correlation is the same in-order sum of K(r, c) a_r, rounded
+// wave-uniform, so a wave's 64 voxels share every instruction and each K entry is one LDS
+// broadcast read.  Each correlation is the same in-order sum of K(r, c) a_r, rounded
 // the same way, as the FOD kernels' own (correlations() above).  Columns go in passes of NCB.
 struct CorrArgs {
     const double* signals;    // (n_vox, m)
@@ -266,16 +265,22 @@ constexpr int CORR_ROWS = 16;    // signal rows staged per LDS tile
 
 // Each wave takes 64 consecutive list positions.  Their signals are staged through the wave's
 // LDS tile CORR_ROWS rows at a time with coalesced loads (each load instruction covers 16 rows
-// of 4 voxels) and read back one voxel per lane.
+// of 4 voxels) and read back one voxel per lane; the matching CORR_ROWS x NCB chunk of K is
+// staged once per block, shared by its waves.
 template <int NCB>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(CorrArgs A) {
     __shared__ double tile[WAVES_PER_BLOCK][CORR_ROWS][WAVE + 1];
+    __shared__ double ktile[CORR_ROWS][NCB];
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const int l = lane_id();
     const long count = A.list ? static_cast<long>(*A.list_count) : A.n_vox;
-    const long first = (static_cast<long>(blockIdx.x) * WAVES_PER_BLOCK + w) * WAVE;
-    const long stride = static_cast<long>(gridDim.x) * WAVES_PER_BLOCK * WAVE;
-    for (long base = first; base < count; base += stride) {
+    const long block_voxels = static_cast<long>(WAVES_PER_BLOCK) * WAVE;
+    const long stride = static_cast<long>(gridDim.x) * block_voxels;
+    // The block's waves move through the column passes and row chunks together (the kernel
+    // chunk in LDS is shared), so a wave past the list still takes part in every barrier.
+    for (long block_base = static_cast<long>(blockIdx.x) * block_voxels; block_base < count;
+         block_base += stride) {
+        const long base = block_base + static_cast<long>(w) * WAVE;
         const long it = base + l;
         const bool on = it < count;
         const long v = on ? (A.list ? A.list[it] : it) : 0;
@@ -292,6 +297,15 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
             }
             for (int r0 = 0; r0 < A.m; r0 += CORR_ROWS) {
                 const int rows = min(CORR_ROWS, A.m - r0);
+                // The kernel chunk K[r0 .. r0 + rows, c0 .. c0 + NCB), loaded once per block.
+                // Columns past n read the next row (or the padding after the last) and are
+                // not stored.
+                for (int e = threadIdx.x; e < CORR_ROWS * NCB; e += blockDim.x) {
+                    const int rr = e / NCB;
+                    const int j = e % NCB;
+                    ktile[rr][j] = rr < rows ? A.kernel[static_cast<long>(r0 + rr) * A.n + c0 + j]
+                                             : 0.0;
+                }
 #pragma unroll 4
                 for (int k = 0; k < CORR_ROWS; ++k) {
                     const int e = k * WAVE + l;
@@ -301,7 +315,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
                     const bool load = base + j < count && rr < rows;
                     tile[w][rr][j] = load ? A.signals[vj * A.m + r0 + rr] : 0.0;
                 }
-                wave_lds_sync();
+                __syncthreads();
                 for (int rr = 0; rr < rows; ++rr) {
                     const int r = r0 + rr;
                     const double sr = tile[w][rr][l];
@@ -317,17 +331,13 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
                         }
                         ar = fmax(ws - wp, 0.0);
                     }
-                    // The row's pass through the constant address space (wave-uniform loads).
-                    // Columns past n read the next row (or the padding after the last) and are
-                    // not stored.
-                    using ConstRow = const __attribute__((address_space(4))) double*;
-                    const ConstRow krow = (ConstRow)(A.kernel + static_cast<long>(r) * A.n + c0);
+                    // Every lane reads the same kernel entries: LDS broadcasts.
 #pragma unroll
                     for (int j = 0; j < NCB; ++j) {
-                        acc[j] = acc[j] + krow[j] * ar;
+                        acc[j] = acc[j] + ktile[rr][j] * ar;
                     }
                 }
-                wave_lds_sync();
+                __syncthreads();
             }
             if (on) {
 #pragma unroll
@@ -341,6 +351,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
     }
 }
 
+// The tissue fractions of a list of voxels from their FOD amplitudes (fractions.cpp:45-76).
 // A round of the low-resolution loop (multi_tissue.cpp:59-78) fits the lumped fractions with the
 // default tolerance, keeps the isotropic ones, counts the round and appends the voxel to the
 // next round's list unless no fraction moved by fraction_tol; the final fit (multi_tissue.cpp:
