@@ -125,9 +125,10 @@ constexpr int MAX_GROUPS = 2; // anisotropic blocks of a per-block fraction fit
 constexpr int GROUP_BATCH = 4; // kernel rows loaded together for a group signal
 
 // compute_fractions of one voxel: fod in column slots, s the unweighted signal.  lumped takes
-// all anisotropic columns as one group; NG group columns are held (1 suffices when lumped).  fr receives the fractions, groups first.  work is the
-// wave's LDS slab for the pivoted QR (RPL * 64 * NQ doubles), iso_lds the block's LDS copy of the
-// isotropic columns.  Returns false when the NNLS hit its inner-step limit.
+// all anisotropic columns as one group; NG group columns are held (1 suffices when lumped).  fr
+// receives the fractions, groups first.  work is the wave's LDS slab for the pivoted QR (RPL * 64
+// doubles per column of the fit), iso_lds the block's LDS copy of the isotropic columns.
+// Returns false when the NNLS hit its inner-step limit.
 template <int NCPL, int RPL, int NQ, int NG = MAX_GROUPS>
 __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const double (&fod)[NCPL],
                                                const RowVec<RPL>& s, bool lumped, bool has_tol,

@@ -386,9 +386,12 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
     const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
-    double* work = smem + w * (NQ * RPL * WAVE);
+    // The fit's columns: one lumped group (the loop) or one per block (the final fit), then the
+    // isotropic tissues.
+    const int qcols = (FINAL ? A.kernel.n_blocks : 1) + A.kernel.ci;
+    double* work = smem + w * (qcols * RPL * WAVE);
     // The block's copy of the isotropic columns, after the waves' QR slabs.
-    double* iso_lds = smem + WAVES_PER_BLOCK * (NQ * RPL * WAVE);
+    double* iso_lds = smem + WAVES_PER_BLOCK * (qcols * RPL * WAVE);
     for (int i = threadIdx.x; i < A.kernel.ci * RPL * WAVE; i += blockDim.x) {
         const int c = i / (RPL * WAVE);
         const int r = i % (RPL * WAVE);
@@ -639,7 +642,8 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
     }                                                                                          \
     int launch_fraction_##NC##_##RP(const char* name, const FractionArgs& args, int waves,      \
                                     hipStream_t stream) {                                      \
-        const size_t lds = sizeof(double) * (NQ * WAVES_PER_BLOCK + args.kernel.ci) * RP * WAVE; \
+        const int qcols = (args.final_fit ? args.kernel.n_blocks : 1) + args.kernel.ci;        \
+        const size_t lds = sizeof(double) * (qcols * WAVES_PER_BLOCK + args.kernel.ci) * RP * WAVE; \
         return args.final_fit                                                                  \
                    ? launch_fill(name, fraction_kernel<NC, RP, true>, args, waves, lds, stream)   \
                    : launch_fill(name, fraction_kernel<NC, RP, false>, args, waves, lds, stream); \

@@ -354,8 +354,8 @@ __device__ __forceinline__ double default_tolerance_wave(const P& C, int rows, i
 
 // The generic Lawson-Hanson NNLS on up to NQ columns (solve_nnls_with_limit).  C provides the
 // columns (C.get(c, t), c uniform); each least-squares fit copies the active columns into the
-// wave's LDS slab `work` (RPL * 64 * NQ doubles) and runs the pivoted QR there.  x receives the
-// solution (uniform).  Returns false when the inner-step limit stopped it.
+// wave's LDS slab `work` (RPL * 64 doubles per column, ncols columns) and runs the pivoted QR
+// there.  x receives the solution (uniform).  Returns false when the inner-step limit stopped it.
 template <int NQ, int RPL, class P>
 __device__ __forceinline__ bool nnls_small(const P& C, int rows, int ncols, const RowVec<RPL>& d,
                                            double tol, int max_inner, double (&x)[NQ],
