@@ -168,9 +168,10 @@ struct FodArgs {
     unsigned long long* work; // profiled launches: the completed solves' FP64 operations
 };
 
-// The fast low-resolution instantiation is asked for 5 waves per SIMD (it sits a few registers
-// above the 96 that allow it).
-constexpr int fod_waves_per_eu(int ncpl, bool global_l) { return (!global_l && ncpl == 1) ? 6 : 1; }
+// The fast low-resolution instantiation is asked for 7 waves per SIMD, what its LDS factor
+// (5.7 KB per wave) allows: the register budget that leaves costs a few spilled registers and
+// still beats 6 waves.
+constexpr int fod_waves_per_eu(int ncpl, bool global_l) { return (!global_l && ncpl == 1) ? 7 : 1; }
 
 template <int NCPL, int RPL, int KCAP, bool GLOBAL_L>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
