@@ -261,7 +261,7 @@ struct CorrArgs {
     double* corr;             // (list position, n)
 };
 
-constexpr int CORR_COLUMNS = 48; // columns per pass of the correlation kernel
+constexpr int CORR_COLUMNS = 44; // columns per pass of the correlation kernel
 constexpr int CORR_ROWS = 16;    // signal rows staged per LDS tile
 
 // Each wave takes 64 consecutive list positions.  Their signals are staged through the wave's
