@@ -592,7 +592,9 @@ inline int launch_fill(const char* name, Kern kern, const Args& args, int waves,
 }
 
 // The fast launch (LDS factor) of a kernel family, then its rerun of the overflowed voxels with
-// the factor in global memory, sized for every column (KG = NCPL * 64 >= n).
+// the factor in global memory, sized for every column (KG = NCPL * 64 >= n).  The rerun's
+// scratch is allocated into *rerun_scratch when that is null, and otherwise reused: a caller
+// that launches the same instantiation repeatedly (the low-resolution rounds) allocates once.
 template <template <int, int, int, bool> class Family, int NCPL, int RPL, class Args>
 int run_pair(const char* name, const char* rerun_name, const Args& fast, Args rerun, int waves,
              int* d_overflow_list, int* d_overflow_count, hipStream_t stream,
@@ -602,8 +604,10 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
     if (fast.design.normal_equations == 0) {
         // QR fits throughout: every voxel takes the global-memory kernel.
         const int all_waves = std::min(waves, RERUN_WAVES_ALL);
-        int rc = device_alloc(rerun_scratch, sizeof(double) * per * all_waves, stream);
-        if (rc) return rc;
+        if (*rerun_scratch == nullptr) {
+            int rc = device_alloc(rerun_scratch, sizeof(double) * per * all_waves, stream);
+            if (rc) return rc;
+        }
         rerun.scratch = SolveScratch{static_cast<double*>(*rerun_scratch), per};
         rerun.vs.overflow_list = nullptr;
         rerun.vs.overflow_count = nullptr;
@@ -615,8 +619,10 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
     const size_t lds = sizeof(double) * lds_factor_doubles(KC, RPL) * WAVES_PER_BLOCK;
     int rc = launch_fill(name, Family<NCPL, RPL, KC, false>::kernel, fast, waves, lds, stream);
     if (rc) return rc;
-    rc = device_alloc(rerun_scratch, sizeof(double) * per * RERUN_WAVES, stream);
-    if (rc) return rc;
+    if (*rerun_scratch == nullptr) {
+        rc = device_alloc(rerun_scratch, sizeof(double) * per * RERUN_WAVES, stream);
+        if (rc) return rc;
+    }
     rerun.scratch = SolveScratch{static_cast<double*>(*rerun_scratch), per};
     rerun.vs.list = d_overflow_list;
     rerun.vs.list_count = d_overflow_count;

@@ -782,6 +782,7 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
     // and lists those that go on to round r + 1; lists and counts stay on the device.
     const int* list = nullptr;
     const int* count = nullptr;
+    void* lr_rerun_scratch = nullptr; // allocated by the first round's rerun, reused after
     for (int64_t r = 0; r < max_rounds && !rc; ++r) {
         int* ovf_count = counters + 2 * r;
         int* next_count = counters + 2 * r + 1;
@@ -826,11 +827,13 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
                              static_cast<int>((nv + WAVE - 1) / WAVE), 0, stream);
             if (rc) break;
             fod.corr = corr_lr;
-            void* rerun_scratch = nullptr;
+            const bool first = lr_rerun_scratch == nullptr;
             rc = dispatch_fod("fod_lr", "fod_lr_rerun", ncpl_lr, rpl, fod, fod, waves, olist,
-                              ovf_count, stream, &rerun_scratch);
+                              ovf_count, stream, &lr_rerun_scratch);
             fod.corr = nullptr;
-            mem.adopt(rerun_scratch);
+            if (first) {
+                mem.adopt(lr_rerun_scratch);
+            }
         }
         if (rc) break;
         fr.x = x_lr;
