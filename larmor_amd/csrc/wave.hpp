@@ -45,13 +45,13 @@ __device__ __forceinline__ int gather(int v, int src) { return __shfl(v, src); }
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
 
 // The value of x in lane lane_id() ^ o, for o = 1, 2, 4, 8, 16 or 32 (a constant once the caller is
-// unrolled), with all 64 lanes active.  Within a 16-lane row it is a DPP move: quad permutations for
-// o = 1, 2, the rotation by 8 for o = 8, and for o = 4 the rotation by 4 or by 12 (a lane with bit 2
-// set reads lane - 4, one without it lane + 4).  Across rows it is gfx950's swap of odd rows of the
-// first operand with even rows of the second (o = 16) or of the upper half of the first with the
-// lower half of the second (o = 32): with x as both operands, a lane of an even row (lower half)
-// finds its partner in the second result, one of an odd row (upper half) in the first.  None of
-// these goes through the LDS crossbar, and the partner is exactly lane ^ o, as __shfl_xor's is.
+// unrolled), with all 64 lanes active.  o = 1, 2 and 8 are DPP moves within a 16-lane row (the
+// quad permutations and the rotation by 8); o = 4 and 16 are ds_swizzle in its bit mode (lane ^ o
+// within each 32-lane half), which runs on the LDS pipe without an address register and so leaves
+// the VALU, which bounds the solve kernels, free; o = 32 is gfx950's swap of the upper half of the
+// first operand with the lower half of the second: with x as both operands, a lane of the lower
+// half finds its partner in the second result, one of the upper half in the first.  The partner
+// is exactly lane ^ o, as __shfl_xor's is.
 __device__ __forceinline__ unsigned xor_lanes(unsigned x, int o) {
     const int l = lane_id();
     const int xi = static_cast<int>(x);
@@ -62,16 +62,15 @@ __device__ __forceinline__ unsigned xor_lanes(unsigned x, int o) {
         return static_cast<unsigned>(__builtin_amdgcn_mov_dpp(xi, 0x4E, 0xf, 0xf, false));  // quad_perm [2,3,0,1]
     }
     if (o == 4) {
-        const int back = __builtin_amdgcn_mov_dpp(xi, 0x124, 0xf, 0xf, false);   // row_ror:4, lane - 4
-        const int ahead = __builtin_amdgcn_mov_dpp(xi, 0x12C, 0xf, 0xf, false);  // row_ror:12, lane + 4
-        return static_cast<unsigned>((l & 4) ? back : ahead);
+        // and_mask 0x1f, or_mask 0, xor_mask 4
+        return static_cast<unsigned>(__builtin_amdgcn_ds_swizzle(xi, (4 << 10) | 0x1f));
     }
     if (o == 8) {
         return static_cast<unsigned>(__builtin_amdgcn_mov_dpp(xi, 0x128, 0xf, 0xf, false));  // row_ror:8
     }
     if (o == 16) {
-        const auto s = __builtin_amdgcn_permlane16_swap(x, x, false, false);
-        return (l & 16) ? s[0] : s[1];
+        // and_mask 0x1f, or_mask 0, xor_mask 16
+        return static_cast<unsigned>(__builtin_amdgcn_ds_swizzle(xi, (16 << 10) | 0x1f));
     }
     const auto s = __builtin_amdgcn_permlane32_swap(x, x, false, false);
     return (l & 32) ? s[0] : s[1];
