@@ -16,7 +16,8 @@ namespace {
 template <int NKS>
 int launch_lsi_t(const LsiArgs& fast, const LsiArgs& qr, int waves, int qr_waves,
                  hipStream_t stream) {
-    int rc = launch_fill("lsi", lsi_kernel<NKS>, fast, waves, 0, stream);
+    int rc = launch_fill("lsi", lsi_kernel<NKS>, fast, waves,
+                         sizeof(double) * LSI_DIAG_DOUBLES * WAVES_PER_BLOCK, stream);
     if (rc) return rc;
     // One wave per block, its column slab in the block's LDS.
     return launch("lsi_qr", lsi_qr_kernel<NKS>, qr, qr_waves, sizeof(double) * LSI_QR_SLAB, stream,

@@ -40,6 +40,7 @@ struct LsiDev {
 constexpr int LSI_KCAP = 64;
 constexpr int LSI_QR_ROWS = 1;    // row slots of [E^T; f^T]: n + 1 <= 64
 constexpr int LSI_GRADIENT_BATCH = 2; // rows of E E^T loaded together by the gradient
+constexpr int LSI_DIAG_DOUBLES = 2 * LSI_KCAP; // per-wave LDS table of the factor diagonal
 
 struct LsiArgs {
     const double* signals;      // (n_vox, m)
@@ -166,7 +167,11 @@ __device__ __forceinline__ double lsi_solution(const LsiDev& d, const double (&f
     return lsi_back(d, z);
 }
 
-// Lawson-Hanson on H = E E^T + f f^T with a growing Cholesky factor (ldp_multipliers).
+// Lawson-Hanson on H = E E^T + f f^T with a growing Cholesky factor (ldp_multipliers).  As in
+// the ridge solver: the factor's diagonal and its division reciprocals sit in a per-wave LDS table
+// (the split division of wave.hpp where every diagonal and the dividend are in range), and
+// y = L^-1 f_A is kept across appends (the new row gives y_k by one sequential dot, the order of
+// the column-oriented solve) and recomputed from a removed position on.
 template <int NKS>
 struct LdpWave {
     static constexpr int KCAP = LSI_KCAP;
@@ -174,22 +179,37 @@ struct LdpWave {
 
     const LsiDev& d;
     const double (&f)[NKS];
-    double* L;
+    double* L;    // factor, global slab, column-major with stride S
+    double* diag; // LDS: diag[2 p] = L_pp, diag[2 p + 1] = its division reciprocal
+    bool diag_ok; // every diagonal in the divisor range (uniform)
     double u[NKS];
     double g[NKS];
     double cand[NKS];
     bool act[NKS];
     int pos[NKS];
-    int fcol; // column at factor position lane_id() (lane < kf)
-    int kf;   // factor size (uniform)
+    int fcol;    // column at factor position lane_id() (lane < kf)
+    int kf;      // factor size (uniform)
+    double y;    // y = L^-1 f_A at the lane's position, valid on positions [0, y_valid)
+    int y_valid;
 
-    __device__ LdpWave(const LsiDev& d_, const double (&f_)[NKS], double* L_)
-        : d(d_), f(f_), L(L_) {}
+    __device__ LdpWave(const LsiDev& d_, const double (&f_)[NKS], double* L_, double* diag_)
+        : d(d_), f(f_), L(L_), diag(diag_) {}
 
     __device__ __forceinline__ int col(int t) const { return lane_id() + WAVE * t; }
     __device__ __forceinline__ double& Lat(int i, int j) { return L[j * S + i]; }
     __device__ __forceinline__ double Luni(int i, int j) const {
         return readlane(L[j * S + lane_id()], i);
+    }
+    __device__ __forceinline__ double2 diag_at(int j) const {
+        return *reinterpret_cast<const double2*>(diag + 2 * j);
+    }
+    // a != 0 ? a / L_jj : a from dd = (L_jj, its reciprocal).
+    template <bool FAST>
+    __device__ __forceinline__ double divide_by_diag(double a, double2 dd) const {
+        if (FAST && dividend_in_range(a)) {
+            return divide_in_range(a, dd.x, dd.y);
+        }
+        return a != 0.0 ? a / dd.x : a;
     }
 
     // f at column c (per lane).
@@ -205,52 +225,78 @@ struct LdpWave {
         return out;
     }
 
-    // L w = b on positions [0, kk), column-oriented; returns the sum of the w_j^2 in order.  The
-    // factor lives in global memory: column j + 1 (which also holds L_{j+1,j+1}) is loaded
-    // during step j, so the load latency is off the chain of dependent steps.
-    __device__ __forceinline__ double forward(double& b, int kk_) {
+    // L w = b on positions [p0, kk) when positions [0, p0) of b already hold their final values
+    // (p0 = 0: the whole solve), column-oriented; returns the sum of the w_j^2 in order (for
+    // p0 = 0).  The factor lives in global memory: column j + 1 is loaded during step j, so the
+    // load latency is off the chain of dependent steps.
+    template <bool FAST>
+    __device__ __forceinline__ double forward_impl(double& b, int p0, int kk) {
         const int l = lane_id();
-        const int kk = __builtin_amdgcn_readfirstlane(kk_);
         double sq = 0.0;
         double col_next = L[l];
         for (int j = 0; j < kk; ++j) {
             const double colj = col_next;
             const int jn = j + 1 < kk ? j + 1 : j;
             col_next = L[jn * S + l];
+            const double2 dd = diag_at(j);
             const double bj = readlane(b, j);
-            const double ljj = readlane(colj, j);
-            const double wj = bj != 0.0 ? bj / ljj : bj;
+            const double wj = j >= p0 ? divide_by_diag<FAST>(bj, dd) : bj;
             sq = sq + wj * wj;
             const double updated = b - colj * wj;
-            b = l == j ? wj : ((l > j && l < kk) ? updated : b);
+            b = l == j ? wj : ((l > j && l < kk && l >= p0) ? updated : b);
         }
         return sq;
     }
 
-    // L^T z = w on positions [0, kk), column-oriented: step j reads row j (L_{j,l} for the
-    // lanes l < j, at stride S) and L_jj, both loaded during step j + 1.
-    __device__ __forceinline__ void backward(double& b, int kk_) {
-        const int l = lane_id();
+    __device__ __forceinline__ double forward(double& b, int kk_) {
         const int kk = __builtin_amdgcn_readfirstlane(kk_);
+        return diag_ok ? forward_impl<true>(b, 0, kk) : forward_impl<false>(b, 0, kk);
+    }
+
+    __device__ __forceinline__ void forward_from(double& b, int p0_, int kk_) {
+        const int p0 = __builtin_amdgcn_readfirstlane(p0_);
+        const int kk = __builtin_amdgcn_readfirstlane(kk_);
+        if (p0 >= kk) {
+            return;
+        }
+        if (diag_ok) {
+            forward_impl<true>(b, p0, kk);
+        } else {
+            forward_impl<false>(b, p0, kk);
+        }
+    }
+
+    // L^T z = w on positions [0, kk), column-oriented: step j reads row j (L_{j,l} for the
+    // lanes l < j, at stride S), loaded during step j + 1.
+    template <bool FAST>
+    __device__ __forceinline__ void backward_impl(double& b, int kk) {
+        const int l = lane_id();
         const int top = kk > 0 ? kk - 1 : 0;
         const int lr = l < KCAP ? l : 0;
         double row_next = L[lr * S + top];
-        double diag_next = L[top * S + l];
         for (int j = kk - 1; j >= 0; --j) {
             const double rowj = row_next;
-            const double diagj = diag_next;
             const int jn = j > 0 ? j - 1 : 0;
             row_next = L[lr * S + jn];
-            diag_next = L[jn * S + l];
+            const double2 dd = diag_at(j);
             const double wj = readlane(b, j);
-            const double ljj = readlane(diagj, j);
-            const double zj = wj != 0.0 ? wj / ljj : wj;
+            const double zj = divide_by_diag<FAST>(wj, dd);
             const double updated = b - rowj * zj;
             b = l == j ? zj : (l < j ? updated : b);
         }
     }
 
-    // Appends column c; false when its pivot is refused or the factor is full.
+    __device__ __forceinline__ void backward(double& b, int kk_) {
+        const int kk = __builtin_amdgcn_readfirstlane(kk_);
+        if (diag_ok) {
+            backward_impl<true>(b, kk);
+        } else {
+            backward_impl<false>(b, kk);
+        }
+    }
+
+    // Appends column c; false when its pivot is refused or the factor is full.  With y valid on
+    // every position, also extends y to the new one.
     __device__ __forceinline__ bool append(int c) {
         const int l = lane_id();
         if (kf >= KCAP) {
@@ -268,17 +314,34 @@ struct LdpWave {
         if (!(pivot > MIN_RELATIVE_PIVOT * hcc)) {
             return false;
         }
+        const double lkk = sqrt(pivot);
+        const double rkk = division_reciprocal(lkk);
         if (l < kf) {
             Lat(kf, l) = b;
         }
         if (l == kf) {
-            Lat(kf, kf) = sqrt(pivot);
+            Lat(kf, kf) = lkk;
             fcol = c;
+            diag[2 * kf] = lkk;
+            diag[2 * kf + 1] = rkk;
+        }
+        const bool lkk_ok = divisor_in_range(lkk);
+        if (y_valid == kf) {
+            // y_k = (f_c - L_k0 y_0 - L_k1 y_1 - ...) / L_kk, subtracted in order.
+            const double prod = b * y;
+            double v = fc;
+            for (int j = 0; j < kf; ++j) {
+                v = v - readlane(prod, j);
+            }
+            const double yk = divide_nonzero_by(v, lkk, rkk, lkk_ok);
+            y = l == kf ? yk : y;
+            y_valid = kf + 1;
         }
 #pragma unroll
         for (int t = 0; t < NKS; ++t) {
             pos[t] = col(t) == c ? kf : pos[t];
         }
+        diag_ok = diag_ok && lkk_ok;
         kf += 1;
         return true;
     }
@@ -297,8 +360,9 @@ struct LdpWave {
             const double diagonal = Luni(column, column);
             const double above = Luni(column, column + 1);
             const double length = hypot_fixed(diagonal, above);
-            const double cosine = diagonal / length;
-            const double sine = above / length;
+            const double rl = division_reciprocal(length);
+            const double cosine = divide_by(diagonal, length, rl);
+            const double sine = divide_by(above, length, rl);
             if (l >= column && l < kf - 1) {
                 const double left = Lat(l, column);
                 const double right = Lat(l, column + 1);
@@ -306,6 +370,18 @@ struct LdpWave {
                 Lat(l, column + 1) = cosine * right - sine * left;
             }
         }
+        // The rotated diagonal and its reciprocals (each lane its own position).
+        bool out_of_range = false;
+        if (l < kf - 1) {
+            double dl = diag[2 * l];
+            if (l >= p) {
+                dl = Lat(l, l);
+                diag[2 * l] = dl;
+                diag[2 * l + 1] = division_reciprocal(dl);
+            }
+            out_of_range = !divisor_in_range(dl);
+        }
+        diag_ok = ballot(out_of_range) == 0ull;
         const int from = gather(fcol, (l + 1) % WAVE);
         fcol = (l >= p && l < kf - 1) ? from : fcol;
 #pragma unroll
@@ -313,6 +389,7 @@ struct LdpWave {
             pos[t] = pos[t] == p ? -1 : (pos[t] > p ? pos[t] - 1 : pos[t]);
         }
         kf -= 1;
+        y_valid = y_valid < p ? y_valid : p;
     }
 
     // Drops the positions whose column is inactive, last to first (GrowingCholesky::keep).
@@ -338,8 +415,10 @@ struct LdpWave {
     __device__ __forceinline__ void solve() {
         const int l = lane_id();
         const double fv = f_at(l < kf ? fcol : 0);
-        double b = l < kf ? fv : 0.0;
-        forward(b, kf);
+        double b = l < kf ? (l < y_valid ? y : fv) : 0.0;
+        forward_from(b, y_valid, kf);
+        y = b;
+        y_valid = kf;
         backward(b, kf);
 #pragma unroll
         for (int t = 0; t < NKS; ++t) {
@@ -354,6 +433,9 @@ struct LdpWave {
         const int l = lane_id();
         kf = 0;
         fcol = 0;
+        y = 0.0;
+        y_valid = 0;
+        diag_ok = true;
         double norm1 = 0.0;
 #pragma unroll
         for (int t = 0; t < NKS; ++t) {
@@ -707,10 +789,12 @@ __device__ __forceinline__ bool ldp_by_qr(const LsiDev& d, const double (&f)[NKS
 // The fast path over all voxels: the Cholesky LDP; failures go to fallback_list.
 template <int NKS>
 __global__ void __launch_bounds__(WAVE * 4) lsi_kernel(LsiArgs A) {
+    extern __shared__ double lsi_diag[];
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
     const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
     double* slab = A.scratch + gw * A.per_wave;
+    double* diag = lsi_diag + w * LSI_DIAG_DOUBLES;
     const int l = lane_id();
     const LsiDev& d = A.d;
     for (long v = gw; v < A.n_vox; v += n_waves) {
@@ -731,7 +815,7 @@ __global__ void __launch_bounds__(WAVE * 4) lsi_kernel(LsiArgs A) {
         for (int t = 0; t < NKS; ++t) {
             f[t] = raw[t] / scale;
         }
-        LdpWave<NKS> ldp(d, f, slab);
+        LdpWave<NKS> ldp(d, f, slab, diag);
         if (!ldp.run()) {
             if (l == 0) {
                 A.fallback_list[atomicAdd(A.fallback_count, 1)] = static_cast<int>(v);
