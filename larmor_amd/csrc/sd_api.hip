@@ -676,8 +676,29 @@ int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
     fa.status = status;
     fa.vs = VoxelSet{n_voxels, nullptr, nullptr, olist, counters};
     fa.scratch = SolveScratch{scratch, 0};
+    // The correlations K^T s of every voxel by the lane-per-voxel pass (the design's QR-only
+    // solves do not use them).
+    if (fa.design.normal_equations != 0) {
+        double* corr = nullptr;
+        rc = mem.get(&corr, std::max(1L, static_cast<long>(n_voxels) * fa.design.n));
+        if (rc) return rc;
+        CorrArgs ca{};
+        ca.signals = signals;
+        ca.m = static_cast<int>(m);
+        ca.multi = 0;
+        ca.kernel = fa.design.kernel;
+        ca.n = fa.design.n;
+        ca.n_vox = n_voxels;
+        ca.corr = corr;
+        rc = launch_fill("fod_single_corr", correlation_kernel<CORR_COLUMNS>, ca,
+                         static_cast<int>((n_voxels + WAVE - 1) / WAVE), 0, stream);
+        if (rc) return rc;
+        fa.corr = corr;
+    }
+    FodArgs rerun = fa;
+    rerun.corr = nullptr;
     void* rerun_scratch = nullptr;
-    rc = dispatch_fod("fod_single", "fod_single_rerun", ncpl, rpl, fa, fa, main_waves(n_voxels),
+    rc = dispatch_fod("fod_single", "fod_single_rerun", ncpl, rpl, fa, rerun, main_waves(n_voxels),
                       olist, counters, stream, &rerun_scratch);
     mem.adopt(rerun_scratch);
     return rc;
@@ -871,10 +892,33 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
             fod.w_iso = hr->d_w_iso;
             fod.x_out = fods;
             fod.vs = VoxelSet{nv, nullptr, nullptr, olist, ovf_count};
-            void* rerun_scratch = nullptr;
-            rc = dispatch_fod("fod_hr", "fod_hr_rerun", ncpl_hr, rpl, fod, fod, waves, olist,
-                              ovf_count, stream, &rerun_scratch);
-            mem.adopt(rerun_scratch);
+            // The correlations of every voxel by the lane-per-voxel pass, as in the rounds.
+            double* corr_hr = nullptr;
+            rc = mem.get(&corr_hr, std::max(1L, nv * fod.design.n));
+            if (!rc) {
+                CorrArgs ca{};
+                ca.signals = signals;
+                ca.m = static_cast<int>(m);
+                ca.multi = 1;
+                ca.weights = weights;
+                ca.w_iso = hr->d_w_iso;
+                ca.iso = iso;
+                ca.ci = ci;
+                ca.kernel = fod.design.kernel;
+                ca.n = fod.design.n;
+                ca.n_vox = nv;
+                ca.corr = corr_hr;
+                rc = launch_fill("fod_hr_corr", correlation_kernel<CORR_COLUMNS>, ca,
+                                 static_cast<int>((nv + WAVE - 1) / WAVE), 0, stream);
+            }
+            if (!rc) {
+                fod.corr = corr_hr;
+                void* rerun_scratch = nullptr;
+                rc = dispatch_fod("fod_hr", "fod_hr_rerun", ncpl_hr, rpl, fod, fod, waves, olist,
+                                  ovf_count, stream, &rerun_scratch);
+                fod.corr = nullptr;
+                mem.adopt(rerun_scratch);
+            }
         }
     }
     if (!rc) {
