@@ -381,10 +381,12 @@ __device__ __forceinline__ bool nnls_small(const P& C, int rows, int ncols, cons
             }
             r[t] = d.v[t] - s;
         }
+        // Only an inactive column's gradient is ever read (the entering choice), so the
+        // active ones are left at zero.
 #pragma unroll
         for (int c = 0; c < NQ; ++c) {
             out[c] = 0.0;
-            if (c < ncols) {
+            if (c < ncols && !active[c]) {
                 out[c] = wave_sum_f<RPL>(0, rows, [&](int t) { return C.get(c, t) * r[t]; });
             }
         }
