@@ -403,6 +403,25 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     const int l = lane_id();
     const int ca = A.kernel.ca;
     const int ci = A.kernel.ci;
+    // The loop's voxels for the next round, buffered per wave (one per lane) and appended to the
+    // next list with one atomic per buffer instead of one per voxel: every wave of the launch
+    // appends to the same counter.  (The list's order is arbitrary either way; no voxel's result
+    // depends on it.)
+    int pending = 0;
+    int pending_voxel = 0;
+    auto flush_pending = [&]() {
+        if (pending > 0) {
+            int base = 0;
+            if (l == 0) {
+                base = atomicAdd(A.next_count, pending);
+            }
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (l < pending) {
+                A.next_list[base + l] = pending_voxel;
+            }
+            pending = 0;
+        }
+    };
     for (long it = gw; it < count; it += n_waves) {
         const long v = A.list ? A.list[it] : it;
         RowVec<RPL> s;
@@ -464,11 +483,18 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
         }
         if (l == 0) {
             A.rounds[v] = rounds + 1;
-            if (!converged) {
-                const int at = atomicAdd(A.next_count, 1);
-                A.next_list[at] = static_cast<int>(v);
+        }
+        if (!converged) {
+            // Buffered: lane `pending` holds the voxel; a full buffer goes out at once.
+            pending_voxel = l == pending ? static_cast<int>(v) : pending_voxel;
+            pending += 1;
+            if (pending == WAVE) {
+                flush_pending();
             }
         }
+    }
+    if (!FINAL) {
+        flush_pending();
     }
 }
 
