@@ -119,6 +119,88 @@ def _cpu_rate(fn, n, threads):
     return n / (time.perf_counter() - t0), threads
 
 
+def serve_report(n_jobs: int, rates: list[float], slots: int) -> int:
+    """The batched and served forms (--serve).  Workload: n_jobs seeded requests, each a volume's
+    masked voxels (log-uniform 5k-400k) with its own acquisition (10 b0 + 1-4 shells of 30-90
+    directions, b 700-3000) and lmax (6, 8 or 10; lowered when the top shell is too small), mFOD
+    (WM + GM + CSF), float32 normalised signals in host memory as a request would bring them.
+    Every request's design is made resident first (prepare).  Phases, one FodServer throughout:
+    plain per-job (each request submitted when the previous one's results are back), batched
+    (all submitted at once) and served at each Poisson arrival rate.  Prints one JSON line with
+    voxel solves/s and p50/p99 latency per phase."""
+    import torch
+
+    from larmor_amd import synthetic
+    from larmor_amd.sd.model import SDModel
+    from larmor_amd.sd.rf import AnisotropicRF, IsotropicRF
+    from larmor_amd.sd.serving import FodServer, summarize
+    from larmor_amd.types import GradientTable
+
+    cfg = dict(synthetic.BATCHED_CONFIG)
+    cfg["n_jobs"] = n_jobs
+    t = time.perf_counter()
+    jobs = synthetic.batched_jobs(device="cuda", **cfg)
+    n_vox = sum(len(j.signals) for j in jobs)
+    print(f"workload: {n_jobs} requests, {n_vox} voxels, generated in "
+          f"{time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+    model = SDModel([AnisotropicRF([1.7e-3, 0.3e-3, 0.3e-3], 0.0), IsotropicRF(0.8e-3),
+                     IsotropicRF(3.0e-3)], method="mFOD", lmax=8)
+    gtabs = [GradientTable(j.bvals, j.bvecs) for j in jobs]
+    server = FodServer(model, slots=slots, hbm_budget=200e9)
+    t = time.perf_counter()
+    for job, gtab in zip(jobs, gtabs):
+        server.prepare(gtab, lmax=job.lmax)
+    print(f"designs resident: {len(jobs)} in {time.perf_counter() - t:.1f} s", file=sys.stderr,
+          flush=True)
+    smallest = sorted(range(n_jobs), key=lambda i: len(jobs[i].signals))[:4]
+    for f in [server.submit(jobs[i].signals, gtabs[i], lmax=jobs[i].lmax) for i in smallest]:
+        f.result()
+
+    def run(name, arrivals, serial=False):
+        futures = []
+        t0 = time.perf_counter()
+        for offset, i in arrivals:
+            delay = t0 + offset - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            futures.append(server.submit(jobs[i].signals, gtabs[i], lmax=jobs[i].lmax))
+            if serial:
+                futures[-1].result()
+        summary = summarize([f.result() for f in futures])
+        print(f"{name}: {json.dumps(summary)}", file=sys.stderr, flush=True)
+        return summary
+
+    order = list(range(n_jobs))
+    plain = run("plain per-job", [(0.0, i) for i in order], serial=True)
+    batched = run("batched", [(0.0, i) for i in order])
+    served = []
+    for rate in rates:
+        rng = np.random.default_rng(1000 + int(rate))
+        offsets = np.concatenate([[0.0], np.cumsum(rng.exponential(1.0 / rate, n_jobs - 1))])
+        served.append({"rate_jobs_per_s": rate,
+                       **run(f"served at {rate}/s", list(zip(offsets, rng.permutation(n_jobs))))})
+    server.close()
+    result = {
+        "metric": "voxel FOD solves/s (mFOD lmax=8, 3-shell); p50/p99 latency under load",
+        "unit": "voxel solves/s",
+        "dtype": "f64",
+        "data": "synthetic (seeded requests, Rician SNR 25)",
+        "config": {
+            "workload": f"{n_jobs} mFOD requests, voxels log-uniform 5k-400k ({n_vox} in all), "
+                        "each with its own acquisition (10 b0 + 1-4 shells of 30-90 directions, "
+                        "b 700-3000) and lmax 6/8/10, WM + GM 0.8e-3 + CSF 3.0e-3",
+            "slots": slots,
+            "hbm_budget_bytes": 200e9,
+        },
+        "plain_per_job": plain,
+        "batched": batched,
+        "served": served,
+        "batched_vs_plain": batched["voxel_solves_per_s"] / plain["voxel_solves_per_s"],
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def rows_report() -> int:
     """Times each row of the path beside its CPU oracle on a sample (--rows).  Prints one line per
     row: GPU voxels/s on the stated workload, the oracle's voxels/s on its sample with the host's
@@ -282,10 +364,19 @@ def main() -> int:
     parser.add_argument("--rows", action="store_true",
                         help="instead of the JSON line, time every row of the path (single-tissue "
                              "NNLS, GRL, SH projection, CSD, MSMT-CSD, peaks) beside its CPU oracle")
+    parser.add_argument("--serve", action="store_true",
+                        help="instead of the JSON line above, measure the batched and served forms "
+                             "(many requests with their own acquisitions; p50/p99 latency)")
+    parser.add_argument("--serve-jobs", type=int, default=64)
+    parser.add_argument("--serve-rates", default="5,20,50", help="Poisson arrival rates, jobs/s")
+    parser.add_argument("--slots", type=int, default=4, help="requests in flight at once")
     parser.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                         help="only run three device-to-device copies of MIB MiB (the known-size "
                              "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = parser.parse_args()
+    if args.serve:
+        return serve_report(args.serve_jobs, [float(r) for r in args.serve_rates.split(",")],
+                            args.slots)
     if args.copy_probe:
         return copy_probe(args.copy_probe)
     if args.rows:

@@ -164,3 +164,86 @@ def normalized_signals(data: np.ndarray, bvals: np.ndarray, mask: np.ndarray) ->
     out = rows / s0[:, None]
     np.minimum(out, 1.0, out=out)
     return out
+
+
+# The batched and served workloads: many volumes, each with its own acquisition and lmax.
+BATCHED_CONFIG = dict(n_jobs=64, min_voxels=5_000, max_voxels=400_000, n_b0=10,
+                      n_shells=(1, 4), n_dirs=(30, 90), b_range=(700.0, 3000.0),
+                      lmax_choices=(6, 8, 10), snr=25.0, seed=4)
+
+
+@dataclass
+class Job:
+    """One deconvolution request: b0-normalised signals (n_voxels, m) float32, capped at 1, of
+    a volume's masked voxels, with its acquisition and the lmax asked for."""
+
+    signals: np.ndarray
+    bvals: np.ndarray
+    bvecs: np.ndarray
+    lmax: int
+
+
+def tissue_signals(n_vox: int, bvals: np.ndarray, bvecs: np.ndarray, rng: np.random.Generator,
+                   *, snr: float, tissue_alpha=(2.0, 1.0, 0.5), gm_d: float = 0.8e-3,
+                   csf_d: float = 3.0e-3, device: str = "cpu", chunk: int = 32768) -> np.ndarray:
+    """Normalised noisy multi-tissue signals of n_vox voxels, float32 (n_vox, m): the voxel
+    model of `phantom` (1-3 fibres, Dirichlet(1) weights, Dirichlet(tissue_alpha) WM/GM/CSF,
+    Rician noise at snr), divided by each voxel's mean b0 and capped at 1.  The per-voxel draws
+    come from rng; the O(n_vox m) work (attenuations, noise) runs in torch on `device` with a
+    generator seeded from rng, in chunks of `chunk` voxels."""
+    import torch
+
+    n_fib = rng.integers(1, 4, size=n_vox)
+    dirs = random_unit(rng, n_vox * 3).reshape(n_vox, 3, 3)
+    weights = np.zeros((n_vox, 3))
+    for k in (1, 2, 3):
+        sel = n_fib == k
+        weights[sel, :k] = rng.dirichlet(np.ones(k), size=int(sel.sum()))
+    fractions = rng.dirichlet(np.asarray(tissue_alpha), size=n_vox)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(rng.integers(0, 2**62)))
+    b = torch.as_tensor(bvals, dtype=torch.float64, device=device)
+    g = torch.as_tensor(bvecs, dtype=torch.float64, device=device)
+    b0 = torch.as_tensor(bvals <= 50.0, device=device)
+    gm = torch.exp(-b * gm_d)
+    csf = torch.exp(-b * csf_d)
+    sigma = 1.0 / snr
+    l_par, l_perp = 1.7e-3, 0.3e-3
+    out = np.empty((n_vox, len(bvals)), dtype=np.float32)
+    for s in range(0, n_vox, chunk):
+        e = min(n_vox, s + chunk)
+        u = torch.as_tensor(dirs[s:e], device=device)
+        w = torch.as_tensor(weights[s:e], device=device)
+        f = torch.as_tensor(fractions[s:e], device=device)
+        cos = torch.einsum("mj,vfj->vmf", g, u)
+        wm = (torch.exp(-b[None, :, None] * (l_perp + (l_par - l_perp) * cos * cos))
+              * w[:, None, :]).sum(dim=2)
+        sig = f[:, :1] * wm + f[:, 1:2] * gm + f[:, 2:3] * csf
+        re = sig + sigma * torch.randn(sig.shape, generator=gen, dtype=torch.float64, device=device)
+        im = sigma * torch.randn(sig.shape, generator=gen, dtype=torch.float64, device=device)
+        noisy = torch.sqrt(re * re + im * im).to(torch.float32).to(torch.float64)
+        s0 = noisy[:, b0].mean(dim=1, keepdim=True)
+        out[s:e] = torch.clamp(noisy / s0, max=1.0).to(torch.float32).cpu().numpy()
+    return out
+
+
+def batched_jobs(n_jobs: int, *, seed: int, min_voxels: int, max_voxels: int, n_b0: int = 10,
+                 n_shells=(1, 4), n_dirs=(30, 90), b_range=(700.0, 3000.0),
+                 lmax_choices=(6, 8, 10), snr: float = 25.0, device: str = "cpu") -> list[Job]:
+    """Seeded requests: voxel counts log-uniform in [min_voxels, max_voxels]; each job draws its
+    own acquisition, n_b0 b0 rows plus 1-4 shells of 30-90 spiral directions at distinct
+    b-values in b_range (on a 150 s/mm^2 grid, so no two shells fall within the 100 s/mm^2
+    shell tolerance), and an lmax from lmax_choices."""
+    rng = np.random.default_rng(seed)
+    grid = np.arange(b_range[0], b_range[1] + 1.0, 150.0)
+    jobs = []
+    for _ in range(n_jobs):
+        n_vox = int(round(np.exp(rng.uniform(np.log(min_voxels), np.log(max_voxels)))))
+        k = int(rng.integers(n_shells[0], n_shells[1] + 1))
+        bs = np.sort(rng.choice(grid, size=k, replace=False))
+        shells = [(float(bv), int(rng.integers(n_dirs[0], n_dirs[1] + 1))) for bv in bs]
+        lmax = int(rng.choice(np.asarray(lmax_choices)))
+        bvals, bvecs = acquisition(n_b0, shells, rng)
+        signals = tissue_signals(n_vox, bvals, bvecs, rng, snr=snr, device=device)
+        jobs.append(Job(signals=signals, bvals=bvals, bvecs=bvecs, lmax=lmax))
+    return jobs

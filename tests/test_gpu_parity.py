@@ -183,3 +183,29 @@ def test_multi_tissue_low_resolution_widths(n_lr):
     np.testing.assert_array_equal(rounds, tr[:, 0])
     np.testing.assert_array_equal(fods, ref_fods)
     np.testing.assert_array_equal(fr, ref_fr)
+
+
+@pytest.mark.parametrize("shells", [[(700.0, 90), (1000.0, 90), (2000.0, 90), (3000.0, 90)],
+                                    [(700.0, 80), (1000.0, 80), (1500.0, 80), (2000.0, 80),
+                                     (2500.0, 80), (3000.0, 80)]])
+def test_multi_tissue_many_measurements(shells):
+    """Acquisitions of 372 and 490 measurements (six and eight signal rows per lane), as the
+    batched requests and the 4-shell brain-sized workload bring them, bit-identical to the
+    oracle."""
+    ph = synthetic.phantom((6, 6, 4), 10, shells, seed=13, snr=25.0, mask_share=None, tissues=True)
+    signals = synthetic.normalized_signals(ph.data, ph.bvals, ph.mask)
+    rfs = [AnisotropicRF([1.7e-3, 0.3e-3, 0.3e-3], 0.0), IsotropicRF(0.8e-3), IsotropicRF(3e-3)]
+    w = np.ones(len(ph.bvals))
+    hr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=DIRS_300, weights=w)
+    lr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=[get_dirs(45)[:-1]], weights=w)
+    s = np.ascontiguousarray(signals[:120])
+    print("measurements", s.shape, flush=True)
+    ref_fods, ref_fr, tr = oracle.fit_multi_tissue(s, w, lr, hr, 0.0, 0.1, 1e-6, trace=True)
+    lr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(lr.w_k_aniso), 0.0, 1e-6)
+    hr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(hr.w_k_aniso), 0.1, 1e-6)
+    fods, fr, rounds, _ = _core.fit_multi_tissue(
+        s, w, _deconv(lr), lr_solver, _deconv(hr), hr_solver, 50, 1e-7, 1e-2, 1,
+        return_diagnostics=True)
+    np.testing.assert_array_equal(rounds, tr[:, 0])
+    np.testing.assert_array_equal(fods, ref_fods)
+    np.testing.assert_array_equal(fr, ref_fr)

@@ -1,0 +1,153 @@
+"""The served form on the GPU: requests fitted through FodServer while others are in flight give
+the same bits as the plain fit of the same rows alone (the reference's batch-invariance rule,
+tests/sd/test_fit.py:87-119, cpp/tests/test_multi_tissue.cpp:56), under continuous admission and
+under an HBM budget that admits one request at a time; served fractions equal the oracle's."""
+
+import numpy as np
+import pytest
+
+from larmor_amd import synthetic
+from larmor_amd.sd.model import SDModel, _to_sh, compute_max_lmax
+from larmor_amd.sd.rf import AnisotropicRF, IsotropicRF
+from larmor_amd.sd.serving import FodServer, request_bytes
+from larmor_amd.sd.sh import projection_matrix
+from larmor_amd.sd.solvers import L2NNLS
+from larmor_amd.sd.sphere import DIRS_300
+from larmor_amd.types import GradientTable
+
+pytestmark = pytest.mark.gpu
+
+WM = [1.7e-3, 0.3e-3, 0.3e-3]
+
+
+def _model(method: str, multi: bool = True) -> SDModel:
+    rfs = [AnisotropicRF(WM, 0.0)]
+    if multi:
+        rfs += [IsotropicRF(0.8e-3), IsotropicRF(3.0e-3)]
+    return SDModel(rfs, method=method, lmax=8)
+
+
+def _alone(model: SDModel, job: synthetic.Job):
+    """What SDModel.fit does with these rows on its own: _prepare's chunk fit, then the SH
+    projection of each anisotropic block."""
+    import torch
+
+    solver = model.solver if model.solver is not None else model.method.default_solver()
+    lmax = compute_max_lmax(job.bvals, job.lmax)
+    hr, fit_chunk = model._prepare(solver, job.bvals, job.bvecs, 1)
+    dev = torch.from_numpy(job.signals.astype(np.float64)).to("cuda")
+    amplitudes, fractions = fit_chunk(dev)
+    proj = torch.as_tensor(projection_matrix(DIRS_300, lmax, model.basis).T
+                           * (len(DIRS_300) / (4.0 * np.pi)), dtype=torch.float64, device="cuda")
+    sh = [_to_sh(amplitudes[:, b], proj, exact_integral=isinstance(solver, L2NNLS)).cpu().numpy()
+          for b in hr.aniso_blocks]
+    torch.cuda.synchronize()
+    return sh, None if fractions is None else fractions.cpu().numpy()
+
+
+def _jobs(n, seed, lo, hi):
+    jobs = synthetic.batched_jobs(n, seed=seed, min_voxels=lo, max_voxels=hi)
+    # one more request on the first acquisition, to be served from the resident design
+    extra = synthetic.batched_jobs(1, seed=seed + 1000, min_voxels=lo, max_voxels=hi)[0]
+    rng = np.random.default_rng(seed)
+    signals = synthetic.tissue_signals(len(extra.signals), jobs[0].bvals, jobs[0].bvecs, rng,
+                                       snr=25.0)
+    jobs.append(synthetic.Job(signals, jobs[0].bvals, jobs[0].bvecs, jobs[0].lmax))
+    return jobs
+
+
+def _check_same(served, alone):
+    sh, fractions = alone
+    assert len(served.sh) == len(sh)
+    for a, b in zip(served.sh, sh):
+        assert a.shape == b.shape
+        assert np.array_equal(a, b), np.abs(a - b).max()
+    if fractions is None:
+        assert served.fractions is None
+    else:
+        assert np.array_equal(served.fractions, fractions), np.abs(served.fractions - fractions).max()
+
+
+def _serve(model, jobs, **kwargs):
+    print(f"serving {len(jobs)} requests, {sum(len(j.signals) for j in jobs)} voxels, {kwargs}",
+          flush=True)
+    with FodServer(model, **kwargs) as server:
+        first = [server.submit(j.signals, GradientTable(j.bvals, j.bvecs), lmax=j.lmax)
+                 for j in jobs[: len(jobs) // 2]]
+        # the rest arrive while the first ones are on the device
+        rest = [server.submit(j.signals, GradientTable(j.bvals, j.bvecs), lmax=j.lmax)
+                for j in jobs[len(jobs) // 2:]]
+        results = [f.result(timeout=120) for f in first + rest]
+    return server, results
+
+
+def test_served_mfod_equals_alone_and_overlaps():
+    model = _model("mFOD")
+    jobs = _jobs(7, seed=21, lo=500, hi=6000)
+    server, results = _serve(model, jobs, slots=3)
+    assert server.max_in_flight >= 2
+    for job, fit in zip(jobs, results):
+        assert fit.lmax == compute_max_lmax(job.bvals, job.lmax)
+        assert fit.n_voxels == len(job.signals)
+        _check_same(fit, _alone(model, job))
+    summary = server.summary()
+    assert summary["requests"] == len(jobs)
+    assert summary["p99_ms"] >= summary["p50_ms"] > 0
+
+
+def test_served_fractions_equal_oracle():
+    import oracle
+    from larmor_amd.sd.kernel import Kernel
+    from larmor_amd.sd.model import compute_shell_weights, low_resolution_dirs
+
+    model = _model("mFOD")
+    job = synthetic.batched_jobs(1, seed=31, min_voxels=400, max_voxels=400)[0]
+    _, (fit,) = _serve(model, [job], slots=2)
+    w = compute_shell_weights(job.bvals, model.inner_shell_weight)
+    hr = Kernel(model.rfs, job.bvals, job.bvecs, dirs=DIRS_300, weights=w)
+    lr = Kernel(model.rfs, job.bvals, job.bvecs,
+                dirs=low_resolution_dirs(model.rfs, len(job.bvals)), weights=w)
+    _, ref_fractions = oracle.fit_multi_tissue(job.signals.astype(np.float64), w, lr, hr, 0.0, 0.1,
+                                               1e-6)
+    assert np.array_equal(fit.fractions, ref_fractions)
+
+
+def test_served_grl_and_single_tissue_equal_alone():
+    for model in (_model("GRL"), _model("mFOD", multi=False)):
+        jobs = _jobs(3, seed=41, lo=200, hi=800)
+        server, results = _serve(model, jobs, slots=2)
+        for job, fit in zip(jobs, results):
+            _check_same(fit, _alone(model, job))
+
+
+def test_budget_admits_one_at_a_time():
+    model = _model("mFOD")
+    jobs = _jobs(4, seed=51, lo=500, hi=3000)
+    sizes = [request_bytes(model, len(j.signals), len(j.bvals), compute_max_lmax(j.bvals, j.lmax),
+                           True) for j in jobs]
+    budget = 1.5 * max(sizes)  # any two requests exceed it
+    server, results = _serve(model, jobs, slots=4, hbm_budget=budget)
+    assert server.max_in_flight == 1
+    assert server.peak_bytes <= budget
+    for job, fit in zip(jobs, results):
+        _check_same(fit, _alone(model, job))
+    with FodServer(model, hbm_budget=budget) as small:
+        big = synthetic.batched_jobs(1, seed=52, min_voxels=200_000, max_voxels=200_000)[0]
+        with pytest.raises(ValueError, match="more than the server's budget"):
+            small.submit(big.signals, GradientTable(big.bvals, big.bvecs))
+
+
+def test_request_errors():
+    model = _model("mFOD")
+    job = synthetic.batched_jobs(1, seed=61, min_voxels=100, max_voxels=100)[0]
+    gtab = GradientTable(job.bvals, job.bvecs)
+    with FodServer(model) as server:
+        with pytest.raises(ValueError, match="do not match the gradient table"):
+            server.submit(job.signals[:, :-1], gtab)
+        with pytest.raises(TypeError, match="float32 or float64"):
+            server.submit(job.signals.astype(np.int32), gtab)
+        with pytest.raises(ValueError, match="lmax must be even"):
+            server.submit(job.signals, gtab, lmax=7)
+        assert server.submit(job.signals, gtab).result(timeout=60).n_voxels == 100
+    with pytest.raises(RuntimeError, match="closed"):
+        server.submit(job.signals, gtab)

@@ -1,0 +1,94 @@
+"""Host side of the served and batched forms (no GPU): the request memory estimate, the latency
+summary, the seeded batched workload, and that the server refuses to start without a GPU."""
+
+import numpy as np
+import pytest
+
+from larmor_amd import synthetic
+from larmor_amd.sd.model import SDModel, compute_max_lmax
+from larmor_amd.sd.rf import AnisotropicRF, IsotropicRF
+from larmor_amd.sd.serving import ServedFit, request_bytes, summarize
+
+WM = [1.7e-3, 0.3e-3, 0.3e-3]
+
+
+def _model(multi=True):
+    rfs = [AnisotropicRF(WM, 0.0)] + ([IsotropicRF(0.8e-3), IsotropicRF(3e-3)] if multi else [])
+    return SDModel(rfs, method="mFOD")
+
+
+def test_request_bytes_is_affine_in_voxels():
+    model = _model()
+    a = request_bytes(model, 1000, 202, 8, False)
+    b = request_bytes(model, 2000, 202, 8, False)
+    c = request_bytes(model, 3000, 202, 8, False)
+    assert b - a == c - b > 0
+    per_voxel = (b - a) // 1000
+    # signals, HR amplitudes + correlations (300 each), LR FOD + correlations (44 each), SH (45),
+    # fractions, isotropic state, lists: 8*202 + 16*300 + 16*44 + 8*(45 + 3 + 6) + 20 bytes
+    assert per_voxel == 8 * 202 + 16 * 300 + 16 * 44 + 8 * (45 + 3 + 6) + 20
+    assert request_bytes(model, 1000, 202, 8, True) - a == 4 * 202 * 1000
+    single = request_bytes(_model(multi=False), 1000, 202, 8, False)
+    assert single < a
+
+
+def test_summarize_percentiles():
+    fits = [ServedFit([], None, 8, 100 * (i + 1), arrival=0.1 * i, admitted=0.1 * i,
+                      finished=0.1 * i + 0.001 * (i + 1)) for i in range(100)]
+    s = summarize(fits)
+    lat = np.arange(1, 101, dtype=float)
+    assert s["requests"] == 100
+    assert s["voxels"] == 100 * 101 // 2 * 100
+    assert s["p50_ms"] == pytest.approx(np.percentile(lat, 50))
+    assert s["p99_ms"] == pytest.approx(np.percentile(lat, 99))
+    assert s["max_ms"] == pytest.approx(100.0)
+    span = fits[-1].finished - fits[0].arrival
+    assert s["voxel_solves_per_s"] == pytest.approx(s["voxels"] / span)
+    assert summarize([])["requests"] == 0
+
+
+def test_batched_jobs_follow_the_workload():
+    jobs = synthetic.batched_jobs(12, seed=7, min_voxels=50, max_voxels=400)
+    again = synthetic.batched_jobs(12, seed=7, min_voxels=50, max_voxels=400)
+    for j, k in zip(jobs, again):
+        assert np.array_equal(j.signals, k.signals) and np.array_equal(j.bvals, k.bvals)
+    for job in jobs:
+        n, m = job.signals.shape
+        assert 50 <= n <= 400 and job.signals.dtype == np.float32
+        assert job.lmax in (6, 8, 10)
+        assert np.sum(job.bvals == 0) == 10
+        shells = np.unique(job.bvals[job.bvals > 0])
+        assert 1 <= len(shells) <= 4 and np.all(np.diff(shells) > 100)
+        assert shells.min() >= 700 and shells.max() <= 3000
+        for b in shells:
+            assert 30 <= np.sum(job.bvals == b) <= 90
+        assert np.allclose(np.linalg.norm(job.bvecs[job.bvals > 0], axis=1), 1.0)
+        assert np.all(job.signals <= 1.0) and np.all(job.signals > 0)
+        # b0 rows average to 1 unless capped
+        assert np.all(job.signals[:, :10].mean(axis=1) <= 1.0 + 1e-6)
+        assert compute_max_lmax(job.bvals, job.lmax) <= job.lmax
+
+
+def test_server_needs_a_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from larmor_amd.sd.serving import FodServer
+
+    with pytest.raises(RuntimeError, match="needs a HIP GPU"):
+        FodServer(_model())
+
+
+def test_running_records_compare_by_identity():
+    """Requests finish out of order: removing the second of two running records must not
+    compare the first one's tensors with it."""
+    import torch
+
+    from larmor_amd.sd.serving import _Running
+
+    a = _Running(None, None, None, None, 0.0, keep=[torch.zeros(3, 261)])
+    b = _Running(None, None, None, None, 0.0, keep=[torch.zeros(3, 52)])
+    running = [a, b]
+    running.remove(b)
+    assert running == [a] and a != _Running(None, None, None, None, 0.0, keep=a.keep)
