@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define LSD_ABI_VERSION 2
+#define LSD_ABI_VERSION 3
 #define LSD_ERR_INVALID 1 /* invalid argument (the reference raises ValueError) */
 #define LSD_ERR_HIP 2     /* a HIP runtime call failed */
 
@@ -78,6 +78,27 @@ int lsd_solver_create_drl(const double* kernel_t, int64_t c, int64_t m, const do
 /* 1 for a damped Richardson-Lucy solver, 0 for ridge NNLS (the variant of FodSolver's method,
  * cpp/src/sd/solver.hpp:103-105). */
 int lsd_solver_is_drl(const lsd_solver* solver);
+/*
+ * An elastic-net FOD solver for kernel K (host, m x n): each column divided by its root mean
+ * square (a column of zeros kept out), device copies of the scaled columns (column-major) and
+ * scales.  Fits with it run lasso_kernel: per voxel, coordinate descent on
+ * (1 / 2m) ||s - K0 b||^2 + reg (alpha ||b||_1 + (1 - alpha) / 2 ||b||^2), full sweeps each
+ * followed by sweeps over the non-zero coefficients, until a full sweep moves no coefficient by
+ * tol relative to 1 + its size or max_iter sweeps ran (status bit 1, and the solver's
+ * unconverged count); the FOD is b over the scales.
+ * Replaces: FodSolver.lasso(k, reg, alpha, tol, max_iter)  cpp/src/python/sd.cpp:101-123
+ *           (elastic_net_solver), ElasticNetSolver cpp/src/sd/solver.hpp:56-64,
+ *           standardize_columns / solve_elastic_net cpp/src/sd/elastic_net.cpp:25-151.
+ * Errors: "reg must be at least 0", "alpha must be in (0, 1]", "tol must be positive",
+ * "max_iter must be at least 1".
+ */
+int lsd_solver_create_lasso(const double* kernel, int64_t m, int64_t n, double reg, double alpha,
+                            double tol, int64_t max_iter, lsd_solver** out);
+/* 1 for an elastic-net solver. */
+int lsd_solver_is_lasso(const lsd_solver* solver);
+/* FodSolver.n_unconverged (cpp/src/python/sd.cpp:441-443): the solves so far that max_iter
+ * stopped, always 0 except for an elastic-net solver; waits for the device. */
+int lsd_solver_n_unconverged(const lsd_solver* solver, int64_t* out);
 void lsd_solver_destroy(lsd_solver* solver);
 /* FodSolver.n_dirs / n_meas  cpp/src/python/sd.cpp:399-443 */
 int64_t lsd_solver_n_dirs(const lsd_solver* solver);

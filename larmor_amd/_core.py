@@ -81,8 +81,8 @@ def _release(obj, destroy: str) -> None:
 
 class FodSolver:
     """An FOD solver with its matrices resident on the device: ridge NNLS (FodSolver.l2nnls,
-    sd.cpp:89-99) or damped Richardson-Lucy (FodSolver.drl, sd.cpp:127-145).  The reference's
-    `lasso` solver (elastic net) is not part of this package."""
+    sd.cpp:89-99), damped Richardson-Lucy (FodSolver.drl, sd.cpp:127-145) or the elastic net
+    (FodSolver.lasso, sd.cpp:101-123)."""
 
     def __init__(self, handle, m: int, n: int):
         self._handle = handle
@@ -118,9 +118,30 @@ class FodSolver:
         )
         return FodSolver(handle, kt.shape[1], kt.shape[0])
 
+    @staticmethod
+    def lasso(k, reg: float, alpha: float, tol: float, max_iter: int) -> "FodSolver":
+        """Elastic net on the columns of k (m, c) scaled to unit root mean square: b minimises
+        (1 / 2m) ||s - k0 b||^2 + reg (alpha ||b||_1 + (1 - alpha) / 2 ||b||^2) by coordinate
+        descent until no coefficient moves by tol relative to 1 + its size over a full sweep, or
+        max_iter sweeps; the FOD is b over the scales and may be negative (sd.cpp:101-123)."""
+        kernel = _host_matrix(k, "k")
+        lib = _lib.load()
+        handle = ctypes.c_void_p()
+        _lib.check(
+            lib.lsd_solver_create_lasso(
+                _ptr(kernel), kernel.shape[0], kernel.shape[1], float(reg), float(alpha),
+                float(tol), int(max_iter), ctypes.byref(handle),
+            )
+        )
+        return FodSolver(handle, kernel.shape[0], kernel.shape[1])
+
     @property
     def is_drl(self) -> bool:
         return bool(_lib.load().lsd_solver_is_drl(self._handle))
+
+    @property
+    def is_lasso(self) -> bool:
+        return bool(_lib.load().lsd_solver_is_lasso(self._handle))
 
     @property
     def n_dirs(self) -> int:
@@ -132,8 +153,11 @@ class FodSolver:
 
     @property
     def n_unconverged(self) -> int:
-        """Always 0, as in the reference (only the elastic net counts)."""
-        return 0
+        """Solves so far that max_iter stopped; always 0 except for the elastic net.  Waits for
+        the device."""
+        out = ctypes.c_int64()
+        _lib.check(_lib.load().lsd_solver_n_unconverged(self._handle, ctypes.byref(out)))
+        return int(out.value)
 
     @property
     def normal_equations(self) -> bool:

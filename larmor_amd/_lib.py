@@ -8,7 +8,7 @@ import ctypes
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "liblarmor_sd.so"
-ABI_VERSION = 2  # LSD_ABI_VERSION of include/larmor_sd.h these bindings follow
+ABI_VERSION = 3  # LSD_ABI_VERSION of include/larmor_sd.h these bindings follow
 HEADER_PATH = Path(__file__).resolve().parents[1] / "include" / "larmor_sd.h"
 
 _lib = None
@@ -37,6 +37,13 @@ SIGNATURES = {
          ctypes.c_double, ctypes.c_int64, ctypes.c_double, ctypes.POINTER(_vp)],
     ),
     "lsd_solver_is_drl": (ctypes.c_int, [_vp]),
+    "lsd_solver_create_lasso": (
+        ctypes.c_int,
+        [_d, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+         ctypes.c_int64, ctypes.POINTER(_vp)],
+    ),
+    "lsd_solver_is_lasso": (ctypes.c_int, [_vp]),
+    "lsd_solver_n_unconverged": (ctypes.c_int, [_vp, _i64]),
     "lsd_solver_destroy": (None, [_vp]),
     "lsd_solver_n_dirs": (ctypes.c_int64, [_vp]),
     "lsd_solver_n_meas": (ctypes.c_int64, [_vp]),

@@ -6,6 +6,7 @@
 #include "csd_wave.hpp"
 #include "kernels.hpp"
 #include "drl_wave.hpp"
+#include "lasso_wave.hpp"
 #include "lsi_host.hpp"
 #include "lsi_wave.hpp"
 #include "peaks_wave.hpp"
@@ -131,6 +132,15 @@ struct lsd_solver {
     double nu = 0.0;
     int64_t n_iter = 0;
     double init = 0.0;
+    // Elastic net (FodSolver.lasso) instead of ridge NNLS.
+    bool lasso = false;
+    double l1_threshold = 0.0;
+    double shrink = 1.0;
+    int64_t max_sweeps = 0;
+    double* d_columns_t = nullptr;          // (n, m) the scaled columns
+    double* d_scales = nullptr;             // (n)
+    int* d_nonzero = nullptr;               // (n)
+    unsigned long long* d_unconverged = nullptr;
 };
 
 struct lsd_tissue_kernel {
@@ -326,6 +336,30 @@ int run_drl(const char* name, const lsd_solver* solver, DrlArgs base, const DrlB
     a.pred = b.pred;
     return launch_fill(name, drl_kernel<DRL_COLUMNS>, a, static_cast<int>((b.ld + WAVE - 1) / WAVE),
                        0, stream);
+}
+
+// An elastic-net solve: base carries the signal inputs, the list and the outputs; the solver
+// supplies the standardised design and settings.
+int run_lasso(const char* name, const lsd_solver* solver, LassoArgs base, long n_items,
+              hipStream_t stream) {
+    const int ncpl = ncpl_for(solver->n);
+    const int rpl = rpl_for(solver->m);
+    if (ncpl < 0 || rpl < 0) {
+        return fail("problem size " + shape_tuple(solver->m, solver->n) +
+                    " exceeds 512 measurements or 640 directions");
+    }
+    LassoArgs a = base;
+    a.columns_t = solver->d_columns_t;
+    a.scales = solver->d_scales;
+    a.nonzero = solver->d_nonzero;
+    a.c = static_cast<int>(solver->n);
+    a.l1_threshold = solver->l1_threshold;
+    a.shrink = solver->shrink;
+    a.tol = solver->tol;
+    a.max_sweeps = static_cast<long>(solver->max_sweeps);
+    a.unconverged = solver->d_unconverged;
+    return launch_lasso(name, ncpl, rpl, a, static_cast<int>(std::min<long>(n_items, 1L << 30)),
+                        stream);
 }
 
 } // namespace lsd
@@ -534,10 +568,95 @@ int lsd_solver_create_drl(const double* kernel_t, int64_t c, int64_t m, const do
 
 int lsd_solver_is_drl(const lsd_solver* s) { return s->drl ? 1 : 0; }
 
+int lsd_solver_create_lasso(const double* kernel, int64_t m, int64_t n, double reg, double alpha,
+                            double tol, int64_t max_iter, lsd_solver** out) {
+    if (!(reg >= 0.0)) {
+        return fail("reg must be at least 0");
+    }
+    if (!(alpha > 0.0) || !(alpha <= 1.0)) {
+        return fail("alpha must be in (0, 1]");
+    }
+    if (!(tol > 0.0)) {
+        return fail("tol must be positive");
+    }
+    if (max_iter < 1) {
+        return fail("max_iter must be at least 1");
+    }
+    if (m < 1 || n < 0 || (m * n > 0 && kernel == nullptr)) {
+        return fail("kernel must be (m, n) with m >= 1, n >= 0");
+    }
+    auto* s = new lsd_solver();
+    s->m = m;
+    s->n = n;
+    s->tol = tol;
+    s->lasso = true;
+    s->l1_threshold = reg * alpha;
+    s->shrink = 1.0 + reg * (1.0 - alpha);
+    s->max_sweeps = max_iter;
+    // standardize_columns (elastic_net.cpp:25-37): each column over its root mean square, the
+    // squares summed over rows in order; a column of zeros keeps scale 1 and stays out.
+    std::vector<double> cols(static_cast<size_t>(std::max<int64_t>(1, n * m)), 0.0);
+    std::vector<double> scales(static_cast<size_t>(std::max<int64_t>(1, n)), 1.0);
+    std::vector<int> nonzero(static_cast<size_t>(std::max<int64_t>(1, n)), 0);
+    for (int64_t j = 0; j < n; ++j) {
+        double squares = 0.0;
+        for (int64_t r = 0; r < m; ++r) {
+            squares += kernel[r * n + j] * kernel[r * n + j];
+        }
+        const double scale = std::sqrt(squares / static_cast<double>(m));
+        nonzero[j] = scale > 0.0 ? 1 : 0;
+        scales[j] = nonzero[j] ? scale : 1.0;
+        for (int64_t r = 0; r < m; ++r) {
+            cols[j * m + r] = kernel[r * n + j] / scales[j];
+        }
+    }
+    int rc = upload(&s->d_columns_t, cols);
+    if (!rc) rc = upload(&s->d_scales, scales);
+    if (!rc) {
+        const hipError_t e1 = hipMalloc(reinterpret_cast<void**>(&s->d_nonzero),
+                                        sizeof(int) * nonzero.size());
+        const hipError_t e2 =
+            e1 == hipSuccess ? hipMemcpy(s->d_nonzero, nonzero.data(), sizeof(int) * nonzero.size(),
+                                         hipMemcpyHostToDevice)
+                             : e1;
+        const hipError_t e3 =
+            e2 == hipSuccess ? hipMalloc(reinterpret_cast<void**>(&s->d_unconverged),
+                                         sizeof(unsigned long long))
+                             : e2;
+        const hipError_t e4 =
+            e3 == hipSuccess ? hipMemset(s->d_unconverged, 0, sizeof(unsigned long long)) : e3;
+        if (e4 != hipSuccess) rc = hip_fail(e4, "lsd_solver_create_lasso");
+    }
+    if (rc) {
+        lsd_solver_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return 0;
+}
+
+int lsd_solver_is_lasso(const lsd_solver* s) { return s->lasso ? 1 : 0; }
+
+int lsd_solver_n_unconverged(const lsd_solver* s, int64_t* out) {
+    *out = 0;
+    if (!s->lasso) {
+        return 0;
+    }
+    unsigned long long n = 0;
+    LSD_HIP(hipDeviceSynchronize());
+    LSD_HIP(hipMemcpy(&n, s->d_unconverged, sizeof(n), hipMemcpyDeviceToHost));
+    *out = static_cast<int64_t>(n);
+    return 0;
+}
+
 void lsd_solver_destroy(lsd_solver* s) {
     if (!s) return;
     if (s->d_kernel) (void)hipFree(s->d_kernel);
     if (s->d_gram) (void)hipFree(s->d_gram);
+    if (s->d_columns_t) (void)hipFree(s->d_columns_t);
+    if (s->d_scales) (void)hipFree(s->d_scales);
+    if (s->d_nonzero) (void)hipFree(s->d_nonzero);
+    if (s->d_unconverged) (void)hipFree(s->d_unconverged);
     delete s;
 }
 
@@ -644,6 +763,25 @@ int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
         da.x_out = fods;
         da.status = status;
         return run_drl("drl_single", solver, da, b, stream);
+    }
+    if (solver->lasso) {
+        if (n_voxels == 0) {
+            return 0;
+        }
+        StreamAllocs mem(stream);
+        int rc = 0;
+        if (status == nullptr) rc = mem.get(&status, n_voxels);
+        if (rc) return rc;
+        hipLaunchKernelGGL(init_loop_state, dim3((n_voxels + 255) / 256), dim3(256), 0, stream,
+                           nullptr, 0L, nullptr, 0L, nullptr, status, static_cast<long>(n_voxels));
+        LassoArgs la{};
+        la.signals = signals;
+        la.m = static_cast<int>(m);
+        la.multi = 0;
+        la.n_vox = n_voxels;
+        la.x_out = fods;
+        la.status = status;
+        return run_lasso("lasso_single", solver, la, n_voxels, stream);
     }
     const int ncpl = ncpl_for(solver->n);
     const int rpl = rpl_for(m);
@@ -763,7 +901,7 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
     if (!rc) rc = mem.get(&iso, std::max(1L, nv * ci));
     if (!rc) rc = mem.get(&prev, nv * nlf);
     if (!rc) rc = mem.get(&x_lr, nv * lr->ca);
-    if (!rc && !lr_solver->drl) rc = mem.get(&corr_lr, std::max(1L, nv * lr->ca));
+    if (!rc && !lr_solver->drl && !lr_solver->lasso) rc = mem.get(&corr_lr, std::max(1L, nv * lr->ca));
     if (!rc && fods == nullptr) rc = mem.get(&fods, nv * hr->ca);
     if (!rc && rounds == nullptr) rc = mem.get(&rounds, nv);
     if (!rc && status == nullptr) rc = mem.get(&status, nv);
@@ -828,8 +966,22 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
             da.x_out = x_lr;
             da.status = status;
             rc = run_drl("drl_lr", lr_solver, da, drl_lr, stream);
-        }
-        if (!lr_solver->drl) {
+        } else if (lr_solver->lasso) {
+            LassoArgs la{};
+            la.signals = signals;
+            la.m = static_cast<int>(m);
+            la.multi = 1;
+            la.weights = weights;
+            la.w_iso = lr->d_w_iso;
+            la.iso = iso;
+            la.ci = ci;
+            la.list = list;
+            la.list_count = count;
+            la.n_vox = nv;
+            la.x_out = x_lr;
+            la.status = status;
+            rc = run_lasso("lasso_lr", lr_solver, la, nv, stream);
+        } else {
             CorrArgs ca{};
             ca.signals = signals;
             ca.m = static_cast<int>(m);
@@ -885,6 +1037,19 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
             da.x_out = fods;
             da.status = status;
             if (!rc) rc = run_drl("drl_hr", hr_solver, da, drl_hr, stream);
+        } else if (hr_solver->lasso) {
+            LassoArgs la{};
+            la.signals = signals;
+            la.m = static_cast<int>(m);
+            la.multi = 1;
+            la.weights = weights;
+            la.w_iso = hr->d_w_iso;
+            la.iso = iso;
+            la.ci = ci;
+            la.n_vox = nv;
+            la.x_out = fods;
+            la.status = status;
+            rc = run_lasso("lasso_hr", hr_solver, la, nv, stream);
         } else {
             int* ovf_count = counters + 2 * max_rounds;
             fod.design = design_of(hr_solver);

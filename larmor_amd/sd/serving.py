@@ -46,7 +46,7 @@ from larmor_amd.sd.model import (SDModel, _to_sh, compute_max_lmax, compute_shel
                                   low_resolution_dirs)
 from larmor_amd.sd.rf import IsotropicRF
 from larmor_amd.sd.sh import projection_matrix
-from larmor_amd.sd.solvers import (FINAL_FRACTION_TOL_X, L2NNLS, MAX_ROUNDS, Stage,
+from larmor_amd.sd.solvers import (FINAL_FRACTION_TOL_X, L2NNLS, MAX_ROUNDS, Lasso, Stage,
                                    _deconv_kernel)
 from larmor_amd.sd.sphere import DIRS_300
 from larmor_amd.types import GradientTable
@@ -109,7 +109,7 @@ class _Design:
         self.lmax = compute_max_lmax(bvals, lmax)
         self.n_meas = len(bvals)
         self.multi = model.is_multi_tissue
-        self.exact_integral = isinstance(solver, L2NNLS)
+        self.exact_integral = isinstance(solver, (L2NNLS, Lasso))
         if self.multi:
             weights = compute_shell_weights(bvals, model.inner_shell_weight)
             hr = Kernel(model.rfs, bvals, bvecs, dirs=DIRS_300, weights=weights)

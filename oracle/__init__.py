@@ -52,6 +52,16 @@ def _load():
     lib.oracle_fit_single_tissue.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, _d,
                                              ctypes.c_int64, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int, _d]
+    lib.oracle_elastic_net_design.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, _d, _d]
+    lib.oracle_fit_single_tissue_lasso.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, _d,
+                                                   ctypes.c_int64, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_int64, ctypes.c_int, _d, _i64]
+    lib.oracle_fit_multi_tissue_lasso.argtypes = (
+        [_d, ctypes.c_int64, ctypes.c_int64, _d]
+        + 2 * [_d, _d, _d, _d, ctypes.c_int64, _i64, ctypes.c_int64]
+        + [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int64, ctypes.c_int64,
+           ctypes.c_int64, ctypes.c_double, ctypes.c_double, ctypes.c_int, _d, _d, _i64])
     lib.oracle_fit_single_tissue_drl.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, _d,
                                                  ctypes.c_int64, _d, ctypes.c_double,
                                                  ctypes.c_int64, ctypes.c_double, ctypes.c_int,
@@ -379,4 +389,57 @@ def fit_multi_tissue_drl(signals, weights, lr, hr, lr_rl, hr_rl, max_rounds=50,
     args += [ci, max_rounds, fraction_tol, final_tol, n_threads, _p(fods), _p(fractions),
              rd.ctypes.data_as(_i64)]
     _check(_load().oracle_fit_multi_tissue_drl(*args))
+    return (fods, fractions, rd) if rounds else (fods, fractions)
+
+
+def elastic_net_design(kernel):
+    """The standardised design of FodSolver.lasso: (scaled columns (m, c), scales (c,))."""
+    k = _c(kernel)
+    m, c = k.shape
+    cols, scales = np.zeros((m, c)), np.zeros(c)
+    _check(_load().oracle_elastic_net_design(_p(k), m, c, _p(cols), _p(scales)))
+    return cols, scales
+
+
+def fit_single_tissue_lasso(signals, kernel, reg, alpha, tol, max_iter, n_threads=None,
+                            sweeps=False):
+    """Elastic net of each voxel (FodSolver.lasso(kernel, reg, alpha, tol, max_iter)).
+    Returns fods (v, c)[, sweeps (v,): negative when max_iter stopped the voxel]."""
+    s, k = _c(signals), _c(kernel)
+    nv, m = s.shape
+    c = k.shape[1]
+    out = np.zeros((nv, c))
+    sw = np.zeros(nv, dtype=np.int64)
+    _check(_load().oracle_fit_single_tissue_lasso(_p(s), nv, m, _p(k), c, float(reg), float(alpha),
+                                                  float(tol), int(max_iter),
+                                                  n_threads or os.cpu_count() or 1, _p(out),
+                                                  sw.ctypes.data_as(_i64)))
+    return (out, sw) if sweeps else out
+
+
+def fit_multi_tissue_lasso(signals, weights, lr, hr, reg, alpha, tol, max_iter, max_rounds=50,
+                           fraction_tol=1e-7, final_tol=1e-2, n_threads=None, rounds=False):
+    """Multi-tissue fit with FodSolver.lasso at both resolutions (each on its kernel's
+    w_k_aniso).  Returns (fods, fractions[, rounds])."""
+    s = _c(signals)
+    nv, m = s.shape
+    w = _c(weights)
+    args = [_p(s), nv, m, _p(w)]
+    keep = []
+    ci = None
+    for k in (lr, hr):
+        ka, ki, wki, wka = _c(k.k_aniso), _c(k.k_iso), _c(k.w_k_iso), _c(k.w_k_aniso)
+        b = _blocks(k.aniso_blocks)
+        keep += [ka, ki, wki, wka, b]
+        ci = ki.shape[1]
+        args += [_p(ka), _p(ki), _p(wki), _p(wka), ka.shape[1], b.ctypes.data_as(_i64), len(b)]
+    ca_hr = hr.k_aniso.shape[1]
+    nf = len(_blocks(hr.aniso_blocks)) + ci
+    fods = np.zeros((nv, ca_hr))
+    fractions = np.zeros((nv, nf))
+    rd = np.zeros(nv, dtype=np.int64)
+    args += [float(reg), float(alpha), float(tol), int(max_iter), ci, max_rounds, fraction_tol,
+             final_tol, n_threads or os.cpu_count() or 1, _p(fods), _p(fractions),
+             rd.ctypes.data_as(_i64)]
+    _check(_load().oracle_fit_multi_tissue_lasso(*args))
     return (fods, fractions, rd) if rounds else (fods, fractions)

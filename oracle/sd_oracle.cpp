@@ -11,6 +11,7 @@
 //   cpp/src/sd/multi_tissue.cpp fit_multi_tissue_block
 //   cpp/src/sd/solver.cpp      L2NnlsSolver::solve_block, DrlSolver::solve_block
 //   cpp/src/sd/drl.cpp         damped_richardson_lucy (per voxel)
+//   cpp/src/sd/elastic_net.cpp standardize_columns, solve_elastic_net (coordinate descent)
 //   cpp/src/sd/csd.cpp         make_csd_problem, solve_csd (LLT restated as below)
 //   cpp/src/sh/harmonics.cpp   fill_legendre, Legendre derivatives, derivatives
 //   cpp/src/sh/peaks.cpp       climb, voxel_peaks
@@ -877,13 +878,155 @@ Vec drl_voxel(const DrlSolver& d, const Vec& signal, Index* iterations, bool* co
     return f;
 }
 
+// ---- Elastic net by coordinate descent (cpp/src/sd/elastic_net.cpp) ----------------------------
+// The standardised design (elastic_net.cpp:25-37): each column divided by its root mean square
+// sqrt(sum_r k(r, j)^2 / m), the sum sequential over rows; a column of zeros keeps scale 1 and
+// stays out of the fit.  The sums over rows inside the descent (the correlations) are wave sums.
+struct ElasticNet {
+    Mat columns; // (m, c) scaled
+    Vec scales;  // (c)
+    std::vector<char> nonzero;
+    double l1_threshold = 0.0; // penalty * l1_ratio
+    double shrink = 1.0;       // 1 + penalty (1 - l1_ratio)
+    double tol = 0.0;
+    Index max_sweeps = 0;
+};
+
+ElasticNet make_elastic_net(const Mat& k, double penalty, double l1_ratio, double tol,
+                            Index max_sweeps) {
+    ElasticNet e;
+    const Index m = k.rows;
+    const Index c = k.cols;
+    e.columns = k;
+    e.scales.assign(static_cast<size_t>(c), 1.0);
+    e.nonzero.assign(static_cast<size_t>(c), 0);
+    for (Index j = 0; j < c; ++j) {
+        double squares = 0.0;
+        for (Index r = 0; r < m; ++r) {
+            squares += k(r, j) * k(r, j);
+        }
+        const double scale = std::sqrt(squares / static_cast<double>(m));
+        e.nonzero[j] = scale > 0.0 ? 1 : 0;
+        e.scales[j] = e.nonzero[j] ? scale : 1.0;
+        for (Index r = 0; r < m; ++r) {
+            e.columns(r, j) = k(r, j) / e.scales[j];
+        }
+    }
+    e.l1_threshold = penalty * l1_ratio;
+    e.shrink = 1.0 + penalty * (1.0 - l1_ratio);
+    e.tol = tol;
+    e.max_sweeps = max_sweeps;
+    return e;
+}
+
+// elastic_net.cpp:13-21.
+double soft_threshold(double value, double threshold) {
+    if (value > threshold) {
+        return value - threshold;
+    }
+    if (value < -threshold) {
+        return value + threshold;
+    }
+    return 0.0;
+}
+
+// solve_elastic_net (elastic_net.cpp:128-151) with the CoordinateDescent steps (:40-124).
+// *sweeps (optional) receives the sweeps run.
+Vec elastic_net_voxel(const ElasticNet& e, const Vec& data, bool* converged, Index* sweeps_out) {
+    const Index m = e.columns.rows;
+    const Index c = e.columns.cols;
+    const double row_count = static_cast<double>(m);
+    Vec residual = data;
+    Vec coef(static_cast<size_t>(c), 0.0);
+    auto correlation = [&](Index j) {
+        return wave_sum(0, m, [&](Index r) { return e.columns(r, j) * residual[r]; }) / row_count;
+    };
+    // has_nonzero_solution (:51-57): the largest |correlation| of a non-zero column.
+    double largest = 0.0;
+    for (Index j = 0; j < c; ++j) {
+        const double a = e.nonzero[j] ? std::fabs(correlation(j)) : 0.0;
+        largest = std::max(largest, a);
+    }
+    if (!(largest > e.l1_threshold)) {
+        if (converged) *converged = true;
+        if (sweeps_out) *sweeps_out = 0;
+        return coef; // unscaled zeros
+    }
+    // update (:104-116): one coefficient to its minimiser; its change relative to 1 + |old|.
+    auto update = [&](Index j) {
+        const double old_value = coef[j];
+        const double corr = correlation(j) + old_value;
+        const double new_value = soft_threshold(corr, e.l1_threshold) / e.shrink;
+        if (new_value == old_value) {
+            return 0.0;
+        }
+        const double delta = new_value - old_value;
+        for (Index r = 0; r < m; ++r) {
+            residual[r] = residual[r] - delta * e.columns(r, j);
+        }
+        coef[j] = new_value;
+        return std::fabs(delta) / (1.0 + std::fabs(old_value));
+    };
+    Index sweeps = 0;
+    bool done = false;
+    std::vector<Index> active;
+    while (sweeps < e.max_sweeps) {
+        ++sweeps;
+        double change = 0.0;
+        for (Index j = 0; j < c; ++j) {
+            if (e.nonzero[j]) {
+                change = std::max(change, update(j));
+            }
+        }
+        if (change < e.tol) {
+            done = true;
+            break;
+        }
+        active.clear();
+        for (Index j = 0; j < c; ++j) {
+            if (coef[j] != 0.0) {
+                active.push_back(j);
+            }
+        }
+        while (sweeps < e.max_sweeps) {
+            ++sweeps;
+            double inner = 0.0;
+            for (Index j : active) {
+                inner = std::max(inner, update(j));
+            }
+            if (inner < e.tol) {
+                break;
+            }
+        }
+    }
+    for (Index j = 0; j < c; ++j) {
+        coef[j] = coef[j] / e.scales[j];
+    }
+    if (converged) *converged = done;
+    if (sweeps_out) *sweeps_out = sweeps;
+    return coef;
+}
+
 // One stage's FOD solver: ridge NNLS (L2NnlsSolver) or damped Richardson-Lucy (DrlSolver).
 struct Solver {
     RidgeDesign design;
     double tol = 0.0;
     bool drl = false;
     DrlSolver rl;
+    bool lasso = false; // elastic net (ElasticNetSolver) instead
+    ElasticNet net;
 };
+
+// The FOD of one signal by a stage's solver (solver.cpp:9-37).
+Vec stage_fod(const Solver& solver, const Vec& a, NnlsTrace* tr) {
+    if (solver.drl) {
+        return drl_voxel(solver.rl, a, nullptr, nullptr);
+    }
+    if (solver.lasso) {
+        return elastic_net_voxel(solver.net, a, nullptr, nullptr);
+    }
+    return solve_ridge_nnls(solver.design, a, 0.0, solver.tol, tr);
+}
 
 struct MtTrace {
     Index rounds = 0;
@@ -935,9 +1078,7 @@ void fit_multi_tissue_block(const Mat& signals, const Vec& weights, const Tissue
         for (Index v : looping) {
             const Vec a = aniso_signal(v, lr.weighted_isotropic);
             NnlsTrace tr;
-            const Vec fod = lr_solver.drl ? drl_voxel(lr_solver.rl, a, nullptr, nullptr)
-                                          : solve_ridge_nnls(lr_solver.design, a, 0.0,
-                                                             lr_solver.tol, traces ? &tr : nullptr);
+            const Vec fod = stage_fod(lr_solver, a, traces ? &tr : nullptr);
             NnlsTrace ftr;
             const Vec fr = compute_fractions(fod, lr, row(signals, v), true, false, 0.0,
                                              traces ? &ftr : nullptr);
@@ -974,9 +1115,7 @@ void fit_multi_tissue_block(const Mat& signals, const Vec& weights, const Tissue
     for (Index v = 0; v < nv; ++v) {
         const Vec a = aniso_signal(v, hr.weighted_isotropic);
         NnlsTrace tr;
-        const Vec fod = hr_solver.drl ? drl_voxel(hr_solver.rl, a, nullptr, nullptr)
-                                      : solve_ridge_nnls(hr_solver.design, a, 0.0, hr_solver.tol,
-                                                         traces ? &tr : nullptr);
+        const Vec fod = stage_fod(hr_solver, a, traces ? &tr : nullptr);
         if (traces) {
             traces[v].hr = tr;
         }
@@ -2128,6 +2267,79 @@ int oracle_fit_multi_tissue_drl(
         hr_solver.drl = true;
         hr_solver.rl = DrlSolver{from_ptr(hr_kernel_t, hr_ca, m), from_ptr(hr_gram, hr_ca, hr_ca),
                                  hr_nu, hr_n_iter, hr_init};
+        const Vec w(weights, weights + m);
+        const Index nf = hr_n_blocks + ci;
+        parallel_for_blocks(n_voxels, 128, n_threads, [&](Index begin, Index end) {
+            const Index nb = end - begin;
+            const Mat block = from_ptr(signals + begin * m, nb, m);
+            std::vector<MtTrace> tr(rounds ? static_cast<size_t>(nb) : 0);
+            fit_multi_tissue_block(block, w, lr, lr_solver, hr, hr_solver, max_rounds,
+                                   fraction_tol, final_tol, fods + begin * hr_ca,
+                                   fractions + begin * nf, rounds ? tr.data() : nullptr);
+            if (rounds) {
+                for (Index v = 0; v < nb; ++v) rounds[begin + v] = tr[v].rounds;
+            }
+        });
+    });
+}
+
+// The standardised elastic-net design: scaled columns (m, c) and scales (c).
+int oracle_elastic_net_design(const double* kernel, std::int64_t m, std::int64_t c,
+                              double* columns, double* scales) {
+    return guarded([&] {
+        const ElasticNet e = make_elastic_net(from_ptr(kernel, m, c), 0.0, 1.0, 1.0, 1);
+        std::copy(e.columns.v.begin(), e.columns.v.end(), columns);
+        std::copy(e.scales.begin(), e.scales.end(), scales);
+    });
+}
+
+// fit_single_tissue with FodSolver.lasso (ElasticNetSolver::solve_block, solver.cpp:23-35):
+// sweeps (optional, int64 per voxel) holds the sweeps run, negative when max_sweeps stopped it.
+int oracle_fit_single_tissue_lasso(const double* signals, std::int64_t n_voxels, std::int64_t m,
+                                   const double* kernel, std::int64_t c, double penalty,
+                                   double l1_ratio, double tol, std::int64_t max_sweeps,
+                                   int n_threads, double* fods, std::int64_t* sweeps) {
+    return guarded([&] {
+        const ElasticNet e = make_elastic_net(from_ptr(kernel, m, c), penalty, l1_ratio, tol,
+                                              max_sweeps);
+        parallel_for_blocks(n_voxels, 128, n_threads, [&](Index begin, Index end) {
+            for (Index v = begin; v < end; ++v) {
+                const Vec sig(signals + v * m, signals + (v + 1) * m);
+                bool converged = false;
+                Index n = 0;
+                const Vec f = elastic_net_voxel(e, sig, &converged, &n);
+                std::copy(f.begin(), f.end(), fods + v * c);
+                if (sweeps) sweeps[v] = converged ? n : -n;
+            }
+        });
+    });
+}
+
+// fit_multi_tissue with FodSolver.lasso at both resolutions: each stage's weighted anisotropic
+// kernel (m, ca) and its settings.  rounds (optional, int64 per voxel).
+int oracle_fit_multi_tissue_lasso(
+    const double* signals, std::int64_t n_voxels, std::int64_t m, const double* weights,
+    const double* lr_k_aniso, const double* lr_k_iso, const double* lr_w_k_iso,
+    const double* lr_w_k_aniso, std::int64_t lr_ca, const std::int64_t* lr_blocks,
+    std::int64_t lr_n_blocks, const double* hr_k_aniso, const double* hr_k_iso,
+    const double* hr_w_k_iso, const double* hr_w_k_aniso, std::int64_t hr_ca,
+    const std::int64_t* hr_blocks, std::int64_t hr_n_blocks, double penalty, double l1_ratio,
+    double tol, std::int64_t max_sweeps, std::int64_t ci, std::int64_t max_rounds,
+    double fraction_tol, double final_tol, int n_threads, double* fods, double* fractions,
+    std::int64_t* rounds) {
+    return guarded([&] {
+        const TissueKernel lr =
+            make_kernel(lr_k_aniso, lr_k_iso, lr_w_k_iso, m, lr_ca, ci, lr_blocks, lr_n_blocks);
+        const TissueKernel hr =
+            make_kernel(hr_k_aniso, hr_k_iso, hr_w_k_iso, m, hr_ca, ci, hr_blocks, hr_n_blocks);
+        Solver lr_solver;
+        lr_solver.lasso = true;
+        lr_solver.net = make_elastic_net(from_ptr(lr_w_k_aniso, m, lr_ca), penalty, l1_ratio, tol,
+                                         max_sweeps);
+        Solver hr_solver;
+        hr_solver.lasso = true;
+        hr_solver.net = make_elastic_net(from_ptr(hr_w_k_aniso, m, hr_ca), penalty, l1_ratio, tol,
+                                         max_sweeps);
         const Vec w(weights, weights + m);
         const Index nf = hr_n_blocks + ci;
         parallel_for_blocks(n_voxels, 128, n_threads, [&](Index begin, Index end) {

@@ -11,7 +11,7 @@ from larmor_amd.sd.model import SDModel, _to_sh, compute_max_lmax
 from larmor_amd.sd.rf import AnisotropicRF, IsotropicRF
 from larmor_amd.sd.serving import FodServer, request_bytes
 from larmor_amd.sd.sh import projection_matrix
-from larmor_amd.sd.solvers import L2NNLS
+from larmor_amd.sd.solvers import L2NNLS, Lasso
 from larmor_amd.sd.sphere import DIRS_300
 from larmor_amd.types import GradientTable
 
@@ -39,7 +39,7 @@ def _alone(model: SDModel, job: synthetic.Job):
     amplitudes, fractions = fit_chunk(dev)
     proj = torch.as_tensor(projection_matrix(DIRS_300, lmax, model.basis).T
                            * (len(DIRS_300) / (4.0 * np.pi)), dtype=torch.float64, device="cuda")
-    sh = [_to_sh(amplitudes[:, b], proj, exact_integral=isinstance(solver, L2NNLS)).cpu().numpy()
+    sh = [_to_sh(amplitudes[:, b], proj, exact_integral=isinstance(solver, (L2NNLS, Lasso))).cpu().numpy()
           for b in hr.aniso_blocks]
     torch.cuda.synchronize()
     return sh, None if fractions is None else fractions.cpu().numpy()
