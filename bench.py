@@ -25,6 +25,9 @@ durations measured with HIP events on its launch stream.  The algorithmic flops 
 The low-resolution kernels run once per voxel and round, so their unit count is the sum of the
 voxels' rounds.  `traffic` is the kernel's HBM bytes per launch from the committed rocprofv3
 counter summary (null when it has no entry).
+`latency_under_load` is the served form's p50/p99 request latency: 32 requests of the batched
+workload (own acquisitions, 5k-400k voxels each) arriving as a Poisson stream at 5 jobs/s
+(--no-serve leaves it out; --serve reports every phase and rate).
 `cpu_baseline` is the CPU
 FP64 oracle (a C++ restatement of the reference algorithm, `kind: port`) on a bounded random
 sample of the same voxels, on the host's cores.
@@ -119,79 +122,96 @@ def _cpu_rate(fn, n, threads):
     return n / (time.perf_counter() - t0), threads
 
 
-def serve_report(n_jobs: int, rates: list[float], slots: int) -> int:
-    """The batched and served forms (--serve).  Workload: n_jobs seeded requests, each a volume's
-    masked voxels (log-uniform 5k-400k) with its own acquisition (10 b0 + 1-4 shells of 30-90
+class ServeBench:
+    """The batched and served forms' workload: n_jobs seeded requests, each a volume's masked
+    voxels (log-uniform 5k-400k) with its own acquisition (10 b0 + 1-4 shells of 30-90
     directions, b 700-3000) and lmax (6, 8 or 10; lowered when the top shell is too small), mFOD
-    (WM + GM + CSF), float32 normalised signals in host memory as a request would bring them.
-    Every request's design is made resident first (prepare).  Phases, one FodServer throughout:
-    plain per-job (each request submitted when the previous one's results are back), batched
-    (all submitted at once) and served at each Poisson arrival rate.  Prints one JSON line with
-    voxel solves/s and p50/p99 latency per phase."""
-    import torch
+    (WM + GM + CSF), float32 normalised signals in host memory as a request would bring them;
+    one FodServer with every request's design made resident first (prepare), warmed on the four
+    smallest requests."""
 
-    from larmor_amd import synthetic
-    from larmor_amd.sd.model import SDModel
-    from larmor_amd.sd.rf import AnisotropicRF, IsotropicRF
-    from larmor_amd.sd.serving import FodServer, summarize
-    from larmor_amd.types import GradientTable
+    def __init__(self, n_jobs: int, slots: int):
+        from larmor_amd import synthetic
+        from larmor_amd.sd.model import SDModel
+        from larmor_amd.sd.rf import AnisotropicRF, IsotropicRF
+        from larmor_amd.sd.serving import FodServer
+        from larmor_amd.types import GradientTable
 
-    cfg = dict(synthetic.BATCHED_CONFIG)
-    cfg["n_jobs"] = n_jobs
-    t = time.perf_counter()
-    jobs = synthetic.batched_jobs(device="cuda", **cfg)
-    n_vox = sum(len(j.signals) for j in jobs)
-    print(f"workload: {n_jobs} requests, {n_vox} voxels, generated in "
-          f"{time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
-    model = SDModel([AnisotropicRF([1.7e-3, 0.3e-3, 0.3e-3], 0.0), IsotropicRF(0.8e-3),
-                     IsotropicRF(3.0e-3)], method="mFOD", lmax=8)
-    gtabs = [GradientTable(j.bvals, j.bvecs) for j in jobs]
-    server = FodServer(model, slots=slots, hbm_budget=200e9)
-    t = time.perf_counter()
-    for job, gtab in zip(jobs, gtabs):
-        server.prepare(gtab, lmax=job.lmax)
-    print(f"designs resident: {len(jobs)} in {time.perf_counter() - t:.1f} s", file=sys.stderr,
-          flush=True)
-    smallest = sorted(range(n_jobs), key=lambda i: len(jobs[i].signals))[:4]
-    for f in [server.submit(jobs[i].signals, gtabs[i], lmax=jobs[i].lmax) for i in smallest]:
-        f.result()
+        cfg = dict(synthetic.BATCHED_CONFIG)
+        cfg["n_jobs"] = n_jobs
+        t = time.perf_counter()
+        self.jobs = synthetic.batched_jobs(device="cuda", **cfg)
+        self.n_vox = sum(len(j.signals) for j in self.jobs)
+        print(f"serving workload: {n_jobs} requests, {self.n_vox} voxels, generated in "
+              f"{time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
+        model = SDModel([AnisotropicRF([1.7e-3, 0.3e-3, 0.3e-3], 0.0), IsotropicRF(0.8e-3),
+                         IsotropicRF(3.0e-3)], method="mFOD", lmax=8)
+        self.gtabs = [GradientTable(j.bvals, j.bvecs) for j in self.jobs]
+        self.slots = slots
+        self.server = FodServer(model, slots=slots, hbm_budget=200e9)
+        t = time.perf_counter()
+        for job, gtab in zip(self.jobs, self.gtabs):
+            self.server.prepare(gtab, lmax=job.lmax)
+        print(f"designs resident: {n_jobs} in {time.perf_counter() - t:.1f} s", file=sys.stderr,
+              flush=True)
+        smallest = sorted(range(n_jobs), key=lambda i: len(self.jobs[i].signals))[:4]
+        for f in [self._submit(i) for i in smallest]:
+            f.result()
 
-    def run(name, arrivals, serial=False):
+    def _submit(self, i):
+        return self.server.submit(self.jobs[i].signals, self.gtabs[i], lmax=self.jobs[i].lmax)
+
+    def run(self, name, arrivals, serial=False) -> dict:
+        """Submits request i at each (offset seconds, i); returns the latency/throughput summary."""
+        from larmor_amd.sd.serving import summarize
+
         futures = []
         t0 = time.perf_counter()
         for offset, i in arrivals:
             delay = t0 + offset - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            futures.append(server.submit(jobs[i].signals, gtabs[i], lmax=jobs[i].lmax))
+            futures.append(self._submit(i))
             if serial:
                 futures[-1].result()
         summary = summarize([f.result() for f in futures])
         print(f"{name}: {json.dumps(summary)}", file=sys.stderr, flush=True)
         return summary
 
-    order = list(range(n_jobs))
-    plain = run("plain per-job", [(0.0, i) for i in order], serial=True)
-    batched = run("batched", [(0.0, i) for i in order])
-    served = []
-    for rate in rates:
+    def poisson(self, rate: float) -> dict:
+        n = len(self.jobs)
         rng = np.random.default_rng(1000 + int(rate))
-        offsets = np.concatenate([[0.0], np.cumsum(rng.exponential(1.0 / rate, n_jobs - 1))])
-        served.append({"rate_jobs_per_s": rate,
-                       **run(f"served at {rate}/s", list(zip(offsets, rng.permutation(n_jobs))))})
-    server.close()
+        offsets = np.concatenate([[0.0], np.cumsum(rng.exponential(1.0 / rate, n - 1))])
+        return {"rate_jobs_per_s": rate,
+                **self.run(f"served at {rate}/s", list(zip(offsets, rng.permutation(n))))}
+
+    def workload(self) -> str:
+        return (f"{len(self.jobs)} mFOD requests, voxels log-uniform 5k-400k ({self.n_vox} in "
+                "all), each with its own acquisition (10 b0 + 1-4 shells of 30-90 directions, "
+                "b 700-3000) and lmax 6/8/10, WM + GM 0.8e-3 + CSF 3.0e-3, float32 signals in "
+                f"host memory; {self.slots} streams, 200 GB HBM budget")
+
+    def close(self) -> None:
+        self.server.close()
+
+
+def serve_report(n_jobs: int, rates: list[float], slots: int) -> int:
+    """The batched and served forms (--serve) on ServeBench's workload.  Phases: plain per-job
+    (each request submitted when the previous one's results are back), batched (all submitted at
+    once) and served at each Poisson arrival rate.  Prints one JSON line with voxel solves/s and
+    p50/p99 latency per phase."""
+    bench = ServeBench(n_jobs, slots)
+    order = list(range(n_jobs))
+    plain = bench.run("plain per-job", [(0.0, i) for i in order], serial=True)
+    batched = bench.run("batched", [(0.0, i) for i in order])
+    served = [bench.poisson(rate) for rate in rates]
+    bench.close()
     result = {
         "metric": "voxel FOD solves/s (mFOD lmax=8, 3-shell); p50/p99 latency under load",
         "unit": "voxel solves/s",
         "dtype": "f64",
         "data": "synthetic (seeded requests, Rician SNR 25)",
-        "config": {
-            "workload": f"{n_jobs} mFOD requests, voxels log-uniform 5k-400k ({n_vox} in all), "
-                        "each with its own acquisition (10 b0 + 1-4 shells of 30-90 directions, "
-                        "b 700-3000) and lmax 6/8/10, WM + GM 0.8e-3 + CSF 3.0e-3",
-            "slots": slots,
-            "hbm_budget_bytes": 200e9,
-        },
+        "config": {"workload": bench.workload(), "slots": slots, "hbm_budget_bytes": 200e9},
         "plain_per_job": plain,
         "batched": batched,
         "served": served,
@@ -216,7 +236,7 @@ def rows_report() -> int:
     from larmor_amd.sd.peaks import _build_icosphere, _device_sphere, _get_amplitudes
     from larmor_amd.sd.rf import AnisotropicRF, IsotropicRF, ZonalSHRF
     from larmor_amd.sd.sh import projection_matrix
-    from larmor_amd.sd.solvers import (DRL, FINAL_FRACTION_TOL_X, MAX_ROUNDS, Stage,
+    from larmor_amd.sd.solvers import (DRL, FINAL_FRACTION_TOL_X, MAX_ROUNDS, Lasso, Stage,
                                        _deconv_kernel)
     from larmor_amd.sd.sphere import DIRS_300
 
@@ -288,6 +308,25 @@ def rows_report() -> int:
                                                         stage(hr, 600), n_threads=threads),
                     len(small), threads)
     report("GRL (damped Richardson-Lucy)", vol + ", 44 LR / 300 HR directions", n_vox, t, cpu, kt)
+
+    # Elastic net (Lasso, FodSolver.lasso): multi-tissue, equal shell weights, default settings.
+    wl = compute_shell_weights(ph.bvals, 1.0)
+    hrl = Kernel(rfs, ph.bvals, ph.bvecs, dirs=DIRS_300, weights=wl)
+    lrl = Kernel(rfs, ph.bvals, ph.bvecs, dirs=low_resolution_dirs(rfs, len(ph.bvals)),
+                 weights=wl)
+    net = Lasso()
+    lr_n = net._prepare(lrl, ph.bvals, Stage.LOW_RES)
+    hr_n = net._prepare(hrl, ph.bvals, Stage.HIGH_RES)
+    wlt = torch.from_numpy(np.ascontiguousarray(wl)).to("cuda")
+    lkn, hkn = _deconv_kernel(lrl), _deconv_kernel(hrl)
+    t, _, kt = profiled(lambda: _core.fit_multi_tissue(dev, wlt, lkn, lr_n, hkn, hr_n, MAX_ROUNDS,
+                                                       1e-7, FINAL_FRACTION_TOL_X, 1))
+    nsample = sample[:128]
+    cpu = _cpu_rate(lambda: oracle.fit_multi_tissue_lasso(nsample, wl, lrl, hrl, net.reg, net.alpha,
+                                                          net.tol, net.max_iter, n_threads=threads),
+                    len(nsample), threads)
+    report("Lasso (elastic net), multi-tissue", vol + ", 44 LR / 300 HR directions, reg 0.01, "
+           "alpha 0.9", n_vox, t, cpu, kt)
 
     # a8: SH projection of the 300 amplitudes to lmax 8.
     amps = torch.rand((n_vox, 300), dtype=torch.float64, device="cuda")
@@ -363,13 +402,16 @@ def main() -> int:
     parser.add_argument("--cpu-seconds", type=float, default=15.0)
     parser.add_argument("--rows", action="store_true",
                         help="instead of the JSON line, time every row of the path (single-tissue "
-                             "NNLS, GRL, SH projection, CSD, MSMT-CSD, peaks) beside its CPU oracle")
+                             "NNLS, GRL, Lasso, SH projection, CSD, MSMT-CSD, peaks) beside its "
+                             "CPU oracle")
     parser.add_argument("--serve", action="store_true",
                         help="instead of the JSON line above, measure the batched and served forms "
                              "(many requests with their own acquisitions; p50/p99 latency)")
     parser.add_argument("--serve-jobs", type=int, default=64)
     parser.add_argument("--serve-rates", default="5,20,50", help="Poisson arrival rates, jobs/s")
     parser.add_argument("--slots", type=int, default=4, help="requests in flight at once")
+    parser.add_argument("--no-serve", action="store_true",
+                        help="leave the latency-under-load phase out of the JSON line")
     parser.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                         help="only run three device-to-device copies of MIB MiB (the known-size "
                              "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -463,7 +505,7 @@ def main() -> int:
           f"{torch.bincount(status.to(torch.int64)).tolist()}", file=sys.stderr, flush=True)
 
     result = {
-        "metric": "voxel FOD solves/s (mFOD lmax=8, 3-shell)",
+        "metric": "voxel FOD solves/s (mFOD lmax=8, 3-shell); p50/p99 latency under load",
         "value": value,
         "unit": "voxel solves/s",
         "n_gpus": 1,
@@ -498,6 +540,11 @@ def main() -> int:
             "peak_kind": "FP64 (MI355X's vector and matrix FP64 peaks are both 78.6 TFLOP/s)",
         },
     }
+    if not args.no_serve:
+        # p50/p99 latency under load: requests of the served form arriving at 5 jobs/s.
+        serve = ServeBench(32, args.slots)
+        result["latency_under_load"] = {"workload": serve.workload(), **serve.poisson(5.0)}
+        serve.close()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(signals, w, lr, hr, args.cpu_seconds)
     print(json.dumps(result), flush=True)
