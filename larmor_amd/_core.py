@@ -12,6 +12,7 @@ sized to the device.
 """
 
 import ctypes
+import sys
 
 import numpy as np
 
@@ -70,10 +71,14 @@ def _stream_ptr(torch):
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
-def _release(obj, destroy: str) -> None:
-    """Frees a device object; skipped during interpreter shutdown, when modules are gone."""
+def _release(obj, destroy: str, _sys=sys, _libmod=_lib) -> None:
+    """Frees a device object; skipped during interpreter shutdown, when modules (and the HIP
+    runtime's state) may already be gone and the process frees everything anyway.  The modules
+    are bound as defaults because shutdown may clear this module's globals first."""
+    if _sys.is_finalizing():
+        return
     handle = getattr(obj, "_handle", None)
-    lib = getattr(_lib, "_lib", None) if _lib is not None else None
+    lib = getattr(_libmod, "_lib", None)
     if handle and lib is not None:
         getattr(lib, destroy)(handle)
         obj._handle = None
@@ -163,7 +168,7 @@ class FodSolver:
     def normal_equations(self) -> bool:
         return bool(_lib.load().lsd_solver_normal_equations(self._handle))
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "lsd_solver_destroy")
 
 
@@ -193,7 +198,7 @@ class DeconvKernel:
         self.n_isotropic = ki.shape[1]
         self.n_blocks = len(blocks)
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "lsd_tissue_kernel_destroy")
 
 
@@ -309,7 +314,7 @@ class CsdProblem:
         self.n_coefficients = fw.shape[1]
         self.n_meas = fw.shape[0]
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "lsd_csd_destroy")
 
 
@@ -389,7 +394,7 @@ class Sphere:
         self.n_vertices = vert.shape[0]
         self.n_coeffs = 0 if b is None else b.shape[1]
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "lsd_sphere_destroy")
 
 
@@ -510,7 +515,7 @@ class LsiDesign:
         self.n_params = am.shape[1]
         self.n_constraints = gm.shape[0]
 
-    def __del__(self):
+    def __del__(self, _release=_release):
         _release(self, "lsd_lsi_destroy")
 
 
