@@ -178,7 +178,7 @@ int lsd_fit_csd(const double* signals, int64_t n_voxels, int64_t m, const lsd_cs
                 double* fods, uint8_t* converged, int32_t* solves, void* stream);
 
 /*
- * The sphere a peak search climbs from (host arrays): unit vertices (nv x 3, nv <= 704), the 6
+ * The sphere a peak search climbs from (host arrays): unit vertices (nv x 3, nv <= 3072), the 6
  * neighbours of each vertex (nv x 6; a vertex with 5 lists itself as the sixth) and, optionally,
  * the tournier SH basis at the vertices (nv x n_coeffs) for searches that compute the amplitudes
  * on the device.

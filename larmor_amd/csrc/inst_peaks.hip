@@ -7,10 +7,32 @@ namespace lsd {
 
 namespace {
 
+// Spheres up to 704 vertices go four waves to a block (at most 33.8 KB of LDS); larger ones one
+// wave to a block, so that a block's LDS stays at one wave's share (up to 36 KB for 3072
+// vertices) and the grid is sized by the occupancy of one-wave blocks.
+constexpr int PEAKS_BLOCK_LDS_LIMIT = 11 * WAVE * (8 + 4) * WAVES_PER_BLOCK;
+
 template <int LMAX, bool FULL>
 int launch_peaks_t(const PeaksArgs& a, int waves, hipStream_t stream) {
-    const size_t lds = static_cast<size_t>(peaks_lds_bytes_per_wave()) * WAVES_PER_BLOCK;
-    return launch_fill("peaks", peaks_kernel<LMAX, FULL>, a, waves, lds, stream);
+    const size_t per_wave = static_cast<size_t>(peaks_lds_bytes_per_wave(a.nv));
+    if (per_wave * WAVES_PER_BLOCK <= static_cast<size_t>(PEAKS_BLOCK_LDS_LIMIT)) {
+        return launch_fill("peaks", peaks_kernel<LMAX, FULL>, a, waves, per_wave * WAVES_PER_BLOCK,
+                           stream);
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, peaks_kernel<LMAX, FULL>, WAVE,
+                                                     per_wave) == hipSuccess &&
+        per_cu > 0) {
+        int dev = 0;
+        int cus = 0;
+        LSD_HIP(hipGetDevice(&dev));
+        LSD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const long resident = static_cast<long>(per_cu) * cus;
+        if (resident > 0 && resident < waves) {
+            waves = static_cast<int>(resident);
+        }
+    }
+    return launch("peaks", peaks_kernel<LMAX, FULL>, a, waves, per_wave, stream, 1);
 }
 
 template <int LMAX, bool FULL>

@@ -23,7 +23,7 @@
 namespace lsd {
 
 constexpr int SH_TABLE = 21;       // recurrence constants up to lmax 20
-constexpr int PEAK_VERTEX_SLOTS = 11; // search spheres of at most 704 vertices
+constexpr int PEAK_VERTEX_SLOTS = 48; // search spheres of at most 3072 vertices
 constexpr int PEAK_FOUND_SLOTS = 2;   // at most 128 distinct peaks per voxel
 
 // Constant tables, made on the host with the same expressions as the CPU oracle.
@@ -252,22 +252,24 @@ struct PeaksArgs {
     int* overflow;           // voxels with more distinct peaks than the wave keeps
 };
 
-// LDS per wave: the amplitudes (doubles) and the ranked local-maximum list (ints).
-constexpr int peaks_lds_bytes_per_wave() {
-    return PEAK_VERTEX_SLOTS * WAVE * (8 + 4);
+// LDS per wave for a sphere of nv vertices: the amplitudes (doubles) and the ranked
+// local-maximum list (ints), one vertex slot of 64 per lane row.
+inline int peak_vertex_slots(int nv) { return nv > 0 ? (nv + WAVE - 1) / WAVE : 1; }
+inline int peaks_lds_bytes_per_wave(int nv) {
+    return peak_vertex_slots(nv) * WAVE * (8 + 4);
 }
 
 template <int LMAX, bool FULL>
 __global__ void __launch_bounds__(WAVE * 4) peaks_kernel(PeaksArgs A) {
-    constexpr int NVS = PEAK_VERTEX_SLOTS;
+    const int nvs = A.nv > 0 ? (A.nv + WAVE - 1) / WAVE : 1;
     constexpr int NF = PEAK_FOUND_SLOTS;
     extern __shared__ double smem[];
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
     const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
-    char* base = reinterpret_cast<char*>(smem) + w * peaks_lds_bytes_per_wave();
+    char* base = reinterpret_cast<char*>(smem) + static_cast<long>(w) * nvs * WAVE * (8 + 4);
     double* amp_lds = reinterpret_cast<double*>(base);
-    int* order_lds = reinterpret_cast<int*>(base + NVS * WAVE * 8);
+    int* order_lds = reinterpret_cast<int*>(base + nvs * WAVE * 8);
     const int l = lane_id();
     const double nan = __builtin_nan("");
 
@@ -292,7 +294,7 @@ __global__ void __launch_bounds__(WAVE * 4) peaks_kernel(PeaksArgs A) {
         int n_max = 0;
         if (finite_first) {
 #pragma unroll 1
-            for (int t = 0; t < NVS; ++t) {
+            for (int t = 0; t < nvs; ++t) {
                 const int d = l + WAVE * t;
                 const double here = d < A.nv ? amp_lds[d] : 0.0;
                 bool is_max = d < A.nv && here > 0.0;
