@@ -390,6 +390,55 @@ def rows_report() -> int:
                                                vertices, neighbours, 3, 0.1, 0.0,
                                                n_threads=threads), len(psample), threads)
     report("peaks (a11)", f"{n_vox} lmax-8 FODs, 3 peaks, 642-vertex sphere", n_vox, t, cpu, kt)
+
+    # Response estimation (estimate_csd_rf): the CSD phantom above, every voxel searched. The
+    # CPU side runs the same host code with the oracle's CSD fits and peak searches.
+    from larmor_amd.dwi.inputs import signal_rows
+    from larmor_amd.sd import csd_rf
+    from larmor_amd.sd.sphere import get_theta_phi
+    from larmor_amd.types import GradientTable, Volume
+
+    rvol = Volume(cph.data, np.eye(4))
+    rgtab = GradientTable(cph.bvals, cph.bvecs)
+    rmask = np.ones(cph.data.shape[:3], dtype=bool)
+    _lib.profile_enable(True)
+    _lib.profile_collect()
+    t0 = time.perf_counter()
+    est = csd_rf.estimate_csd_rf(rvol, rgtab, rmask, n_jobs=threads)
+    t = time.perf_counter() - t0
+    kt = _lib.profile_collect()
+    _lib.profile_enable(False)
+
+    def oracle_fods(volume, candidates, on_shell, bvecs, response, lmax, _threads):
+        problem = csd_problem(bvecs, response, lmax, CSDSolver())
+        sig = np.maximum(signal_rows(volume, np.nonzero(candidates), slice(None), on_shell), 0.0)
+        return oracle.fit_csd(np.ascontiguousarray(sig), problem.initial, problem.forward,
+                              problem.constraint, problem.norm_weight, problem.threshold,
+                              problem.max_iter, n_threads=threads)[0]
+
+    def oracle_peaks(fods, lmax):
+        verts, nbrs = csd_rf._sphere()
+        th, ph_ = get_theta_phi(verts)
+        amps = np.ascontiguousarray(fods @ csd_rf.build_basis_matrix(th, ph_, lmax).T)
+        _, found = oracle.peaks_chunk(np.ascontiguousarray(fods), lmax, amps, verts, nbrs, 2, 0.0,
+                                      0.0, n_threads=threads)
+        found = np.nan_to_num(found, nan=0.0)
+        found[fods[:, 0] <= 0] = 0.0
+        return found
+
+    product = (csd_rf._candidate_fods, csd_rf._two_largest_peaks)
+    csd_rf._candidate_fods, csd_rf._two_largest_peaks = oracle_fods, oracle_peaks
+    try:
+        t0 = time.perf_counter()
+        ref = csd_rf.estimate_csd_rf(rvol, rgtab, rmask, n_jobs=threads)
+        cpu = (rmask.sum() / (time.perf_counter() - t0), threads)
+    finally:
+        csd_rf._candidate_fods, csd_rf._two_largest_peaks = product
+    same = np.array_equal(est.voxels, ref.voxels) and est.num_iter == ref.num_iter
+    report("CSD response estimate (estimate_csd_rf)",
+           f"{int(rmask.sum())} voxels x {int((cph.bvals > 50).sum())} measurements, 300 voxels, "
+           f"lmax 10, {est.num_iter} iterations (same selection as the oracle pipeline: {same})",
+           int(rmask.sum()), t, cpu, kt)
     return 0
 
 
@@ -402,8 +451,8 @@ def main() -> int:
     parser.add_argument("--cpu-seconds", type=float, default=15.0)
     parser.add_argument("--rows", action="store_true",
                         help="instead of the JSON line, time every row of the path (single-tissue "
-                             "NNLS, GRL, Lasso, SH projection, CSD, MSMT-CSD, peaks) beside its "
-                             "CPU oracle")
+                             "NNLS, GRL, Lasso, SH projection, CSD, MSMT-CSD, peaks, the CSD "
+                             "response estimate) beside its CPU oracle")
     parser.add_argument("--serve", action="store_true",
                         help="instead of the JSON line above, measure the batched and served forms "
                              "(many requests with their own acquisitions; p50/p99 latency)")

@@ -102,3 +102,19 @@ def test_gpu_estimate_equals_the_oracle_pipeline(monkeypatch):
     np.testing.assert_array_equal(gpu.voxels, ref.voxels)
     np.testing.assert_allclose(gpu.rf.coefficients, ref.rf.coefficients, rtol=1e-9,
                                atol=1e-12 * np.abs(ref.rf.coefficients).max())
+
+
+def test_csd_model_without_a_response_estimates_one():
+    from larmor_amd.sd.csd import CSDModel
+
+    cfg = dict(synthetic.CSD_CONFIG)
+    cfg["shape"] = (10, 10, 6)
+    ph = synthetic.phantom(**cfg)
+    volume = Volume(ph.data, np.eye(4))
+    gtab = GradientTable(ph.bvals, ph.bvecs)
+    mask = np.ones(ph.data.shape[:3], dtype=bool)
+    print("csd model without response", ph.data.shape, flush=True)
+    auto = CSDModel().fit(volume, gtab, mask, n_jobs=1)
+    rf = csd_rf.estimate_csd_rf(volume, gtab, mask, n_jobs=1).rf
+    given = CSDModel(rf).fit(volume, gtab, mask, n_jobs=1)
+    np.testing.assert_array_equal(auto.fods[0], given.fods[0])
