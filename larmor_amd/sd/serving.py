@@ -160,6 +160,8 @@ class _Request:
     nbytes: int
     future: Future
     arrival: float
+    n_rows: int = 0  # rows submitted, non-finite ones included
+    finite: np.ndarray | None = None  # which rows were fitted; None when all were
 
 
 @dataclass(eq=False)  # compared by identity
@@ -220,7 +222,8 @@ class FodServer:
     def submit(self, signals, gtab: GradientTable, *, lmax: int | None = None) -> Future:
         """Queues the fit of signals (n_voxels, n_meas): b0-normalised rows, float32 or float64,
         numpy or a CPU tensor.  Returns a Future resolving to a ServedFit.  The rows are copied
-        into pinned host memory on the caller's thread, so the caller may reuse its array."""
+        into pinned host memory on the caller's thread, so the caller may reuse its array; rows
+        with a non-finite value are not fitted and their results are NaN."""
         torch = _torch()
         arrival = time.perf_counter()
         if self._closed.is_set():
@@ -248,11 +251,19 @@ class FodServer:
         if nbytes > self.hbm_budget:
             raise ValueError(f"the request needs about {nbytes} bytes of device memory, more than "
                              f"the server's budget of {self.hbm_budget}")
+        # Voxels with a non-finite signal are left out of the fit and get NaN results, as
+        # SDModel.fit leaves them out (model.py:171-224).
+        finite = torch.isfinite(host).all(dim=1)
+        keep = None
+        if not bool(finite.all()):
+            keep = finite.numpy().copy()
+            host = host[finite]
         pinned = torch.empty(host.shape, dtype=host.dtype, pin_memory=True)
         pinned.copy_(host)
         key = (bvals.tobytes(), bvecs.tobytes(), lmax)
         future: Future = Future()
-        self._incoming.put(_Request(pinned, bvals, bvecs, lmax, key, nbytes, future, arrival))
+        self._incoming.put(_Request(pinned, bvals, bvecs, lmax, key, nbytes, future, arrival,
+                                    n_rows=n, finite=keep))
         return future
 
     def prepare(self, gtab: GradientTable, *, lmax: int | None = None) -> None:
@@ -333,11 +344,13 @@ class FodServer:
             self._running.remove(r)
             req = r.request
             finished = time.perf_counter()
-            fit = ServedFit(
-                sh=[t.numpy() for t in r.sh_host],
-                fractions=None if r.fractions_host is None else r.fractions_host.numpy(),
-                lmax=r.design.lmax, n_voxels=int(req.host.shape[0]), arrival=req.arrival,
-                admitted=r.admitted, finished=finished)
+            sh = [t.numpy() for t in r.sh_host]
+            fractions = None if r.fractions_host is None else r.fractions_host.numpy()
+            if req.finite is not None:
+                sh = [_expand(a, req.finite) for a in sh]
+                fractions = None if fractions is None else _expand(fractions, req.finite)
+            fit = ServedFit(sh=sh, fractions=fractions, lmax=r.design.lmax, n_voxels=req.n_rows,
+                            arrival=req.arrival, admitted=r.admitted, finished=finished)
             r.keep.clear()
             self._running_bytes -= req.nbytes
             self._streams.append(r.stream)
@@ -417,6 +430,14 @@ class FodServer:
                                                pin_memory=True)
                 r.fractions_host.copy_(fractions, non_blocking=True)
             r.event.record(r.stream)
+
+
+def _expand(values: np.ndarray, finite: np.ndarray) -> np.ndarray:
+    """Rows of the fitted voxels placed at their positions among all submitted rows, NaN
+    elsewhere."""
+    out = np.full((len(finite),) + values.shape[1:], np.nan)
+    out[finite] = values
+    return out
 
 
 def summarize(served: list[ServedFit]) -> dict:

@@ -39,7 +39,8 @@ def _alone(model: SDModel, job: synthetic.Job):
     amplitudes, fractions = fit_chunk(dev)
     proj = torch.as_tensor(projection_matrix(DIRS_300, lmax, model.basis).T
                            * (len(DIRS_300) / (4.0 * np.pi)), dtype=torch.float64, device="cuda")
-    sh = [_to_sh(amplitudes[:, b], proj, exact_integral=isinstance(solver, (L2NNLS, Lasso))).cpu().numpy()
+    exact = isinstance(solver, (L2NNLS, Lasso))
+    sh = [_to_sh(amplitudes[:, b], proj, exact_integral=exact).cpu().numpy()
           for b in hr.aniso_blocks]
     torch.cuda.synchronize()
     return sh, None if fractions is None else fractions.cpu().numpy()
@@ -151,3 +152,27 @@ def test_request_errors():
         assert server.submit(job.signals, gtab).result(timeout=60).n_voxels == 100
     with pytest.raises(RuntimeError, match="closed"):
         server.submit(job.signals, gtab)
+
+
+def test_non_finite_rows_and_an_empty_request():
+    """Rows with a non-finite value are left out of the fit and come back NaN (as SDModel.fit
+    leaves such voxels out); the other rows equal the plain fit of the finite rows alone; an
+    empty request is served with empty results."""
+    model = _model("mFOD")
+    job = synthetic.batched_jobs(1, seed=71, min_voxels=300, max_voxels=300)[0]
+    gtab = GradientTable(job.bvals, job.bvecs)
+    sig = job.signals.copy()
+    sig[[3, 100]] = np.nan
+    sig[200, 5] = np.inf
+    finite = np.isfinite(sig).all(axis=1)
+    with FodServer(model) as server:
+        fit = server.submit(sig, gtab, lmax=job.lmax).result(timeout=60)
+        empty = server.submit(sig[:0], gtab, lmax=job.lmax).result(timeout=60)
+    assert fit.n_voxels == 300 and fit.sh[0].shape[0] == 300
+    assert np.isnan(fit.sh[0][~finite]).all() and np.isnan(fit.fractions[~finite]).all()
+    sh, fractions = _alone(model, synthetic.Job(job.signals[finite], job.bvals, job.bvecs,
+                                                job.lmax))
+    assert np.array_equal(fit.sh[0][finite], sh[0])
+    assert np.array_equal(fit.fractions[finite], fractions)
+    assert empty.n_voxels == 0 and empty.sh[0].shape == (0, sh[0].shape[1])
+    assert empty.fractions.shape == (0, 3)

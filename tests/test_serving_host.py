@@ -92,3 +92,13 @@ def test_running_records_compare_by_identity():
     running = [a, b]
     running.remove(b)
     assert running == [a] and a != _Running(None, None, None, None, 0.0, keep=a.keep)
+
+
+def test_expand_places_fitted_rows():
+    from larmor_amd.sd.serving import _expand
+
+    finite = np.array([True, False, True, True, False])
+    out = _expand(np.arange(6.0).reshape(3, 2), finite)
+    assert out.shape == (5, 2)
+    np.testing.assert_array_equal(out[finite], np.arange(6.0).reshape(3, 2))
+    assert np.isnan(out[~finite]).all()
