@@ -439,6 +439,60 @@ def rows_report() -> int:
            f"{int(rmask.sum())} voxels x {int((cph.bvals > 50).sum())} measurements, 300 voxels, "
            f"lmax 10, {est.num_iter} iterations (same selection as the oracle pipeline: {same})",
            int(rmask.sum()), t, cpu, kt)
+
+    # Three-tissue response estimate (estimate_tissue_responses) on the three-tissue phantom at
+    # 64 x 64 x 40; the CPU side runs the same host code with the oracle's LSI fits and peaks.
+    import types
+
+    from larmor_amd.sd import tissue_responses as tresp
+    from larmor_amd.sd.msmt import constraint_matrix as cmat, forward_matrix as fmat
+    from larmor_amd.sd.peaks import Peaks
+
+    tdata, tb, tv, tmask, _, _ = synthetic.three_tissue_phantom(shape=(64, 64, 40))
+    tvol = Volume(tdata, np.diag([2.5, 2.5, 2.5, 1.0]))
+    tgtab = GradientTable(tb, tv)
+    _lib.profile_enable(True)
+    _lib.profile_collect()
+    t0 = time.perf_counter()
+    test = tresp.estimate_tissue_responses(tvol, tgtab, tmask, n_jobs=threads)
+    t = time.perf_counter() - t0
+    kt = _lib.profile_collect()
+    _lib.profile_enable(False)
+
+    def o_prepare(rfs_, lmaxes, bvals, bvecs, dirs):
+        fw = fmat(rfs_, lmaxes, bvals, bvecs)
+        return fw, cmat(lmaxes, dirs), msmt_ridge(fw)
+
+    def o_fit(sig, design_, _threads):
+        return oracle.fit_lsi(np.ascontiguousarray(sig, dtype=np.float64), *design_,
+                              n_threads=threads)
+
+    def o_peaks(cf, num_peaks, n_jobs):
+        lm = int(round((np.sqrt(8 * cf.shape[1] + 1) - 3) / 2))
+        verts, nbrs = _build_icosphere()
+        d_, a_ = oracle.peaks_chunk(np.ascontiguousarray(cf), lm,
+                                    np.ascontiguousarray(_get_amplitudes(cf, lm, False)), verts,
+                                    nbrs, num_peaks, 0.1, 0.0, n_threads=threads)
+        return Peaks(directions=d_, amplitudes=a_)
+
+    saved = (tresp.prepare_msmt, tresp.fit_msmt, tresp.find_peaks, tresp._core)
+    tresp.prepare_msmt, tresp.fit_msmt, tresp.find_peaks = o_prepare, o_fit, o_peaks
+    tresp._core = types.SimpleNamespace(
+        LsiDesign=lambda a_, g_, r_: (a_, g_, r_),
+        fit_lsi=lambda sig, des, n_threads: oracle.fit_lsi(sig, *des))
+    try:
+        t0 = time.perf_counter()
+        tref = tresp.estimate_tissue_responses(tvol, tgtab, tmask, n_jobs=threads)
+        cpu = (tmask.sum() / (time.perf_counter() - t0), threads)
+    finally:
+        tresp.prepare_msmt, tresp.fit_msmt, tresp.find_peaks, tresp._core = saved
+    same = all(np.array_equal(getattr(test, k), getattr(tref, k))
+               for k in ("wm_voxels", "gm_voxels", "csf_voxels"))
+    report("three-tissue response estimate (estimate_tissue_responses)",
+           f"{int(tmask.sum())} masked voxels of 64x64x40 x {len(tb)} measurements (4 b0 + 3 x 30), "
+           f"{int(test.wm_voxels.sum())} WM / {int(test.gm_voxels.sum())} GM / "
+           f"{int(test.csf_voxels.sum())} CSF voxels (same as the oracle pipeline: {same})",
+           int(tmask.sum()), t, cpu, kt)
     return 0
 
 
@@ -451,8 +505,8 @@ def main() -> int:
     parser.add_argument("--cpu-seconds", type=float, default=15.0)
     parser.add_argument("--rows", action="store_true",
                         help="instead of the JSON line, time every row of the path (single-tissue "
-                             "NNLS, GRL, Lasso, SH projection, CSD, MSMT-CSD, peaks, the CSD "
-                             "response estimate) beside its CPU oracle")
+                             "NNLS, GRL, Lasso, SH projection, CSD, MSMT-CSD, peaks, the CSD and "
+                             "three-tissue response estimates) beside its CPU oracle")
     parser.add_argument("--serve", action="store_true",
                         help="instead of the JSON line above, measure the batched and served forms "
                              "(many requests with their own acquisitions; p50/p99 latency)")

@@ -247,3 +247,54 @@ def batched_jobs(n_jobs: int, *, seed: int, min_voxels: int, max_voxels: int, n_
         signals = tissue_signals(n_vox, bvals, bvecs, rng, snr=snr, device=device)
         jobs.append(Job(signals=signals, bvals=bvals, bvecs=bvecs, lmax=lmax))
     return jobs
+
+
+def three_tissue_phantom(shape=(26, 26, 16), shells=(1000.0, 2000.0, 3000.0), seed: int = 0,
+                         s0=(1000.0, 1300.0, 3000.0), noise: float = 15.0):
+    """An ellipsoid with CSF at its centre, white matter around it and grey matter outside, in
+    signal units (S0 of WM, GM, CSF), for the three-tissue response estimate.
+
+    Acquisition: 4 b0 volumes and 30 directions (get_dirs(30)) on each shell, turned about z by
+    0.5 rad from shell to shell.  The white matter fibres run around the z axis; above the middle
+    slice a second population along z crosses them.  Tensors: l_perp 0.3e-3, l_par 1.7e-3; GM
+    D = 0.8e-3, CSF D = 3e-3; the tissues blend over smooth steps of the normalised radius
+    (CSF below 0.3, GM above 0.65, semi-axes 12/26, 12/26, 7/16 of the grid).  Magnitude of
+    Gaussian noise of `noise`, float32, zero outside the ellipsoid.
+
+    Returns (data, bvals, bvecs, mask, fractions {wm, gm, csf}, single_fibre mask).
+    """
+    from larmor_amd.sd.sphere import get_dirs
+
+    dirs = get_dirs(30)
+    bvecs = [np.zeros((4, 3))]
+    for k in range(len(shells)):
+        c, s = np.cos(0.5 * k), np.sin(0.5 * k)
+        bvecs.append(dirs @ np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]]).T)
+    bvals = np.concatenate([np.zeros(4), np.repeat(np.asarray(shells, dtype=np.float64), 30)])
+    bvecs = np.vstack(bvecs)
+    grid = np.stack(np.meshgrid(*[np.arange(n) - (n - 1) / 2 for n in shape], indexing="ij"), -1)
+    axes = np.array(shape, dtype=np.float64) * np.array([12 / 26, 12 / 26, 7 / 16])
+    radius = np.linalg.norm(grid / axes, axis=-1)
+
+    def step(x, edge):
+        return 1 / (1 + np.exp(-(x - edge) / 0.04))
+
+    csf = 1 - step(radius, 0.3)
+    gm = step(radius, 0.65)
+    wm = 1 - csf - gm
+    mask = radius < 1.0
+    around = np.stack([-grid[..., 1], grid[..., 0], np.zeros(shape)], -1)
+    around /= np.linalg.norm(around, axis=-1, keepdims=True).clip(1e-9)
+    crossing = grid[..., 2] > 0
+
+    def fibre(direction):
+        return np.exp(-bvals * (0.3e-3 + 1.4e-3 * (direction @ bvecs.T) ** 2))
+
+    wm_signal = fibre(around)
+    wm_signal[crossing] = 0.5 * wm_signal[crossing] + 0.5 * fibre(np.array([0.0, 0.0, 1.0]))
+    signal = (s0[0] * wm[..., None] * wm_signal + s0[1] * gm[..., None] * np.exp(-bvals * 0.8e-3)
+              + s0[2] * csf[..., None] * np.exp(-bvals * 3e-3))
+    rng = np.random.default_rng(seed)
+    noisy = np.abs(signal + noise * rng.standard_normal(signal.shape)).astype(np.float32)
+    return (noisy * mask[..., None], bvals, bvecs, mask, {"wm": wm, "gm": gm, "csf": csf},
+            ~crossing)

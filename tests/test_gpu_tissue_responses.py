@@ -14,60 +14,22 @@ import numpy as np
 import pytest
 
 import oracle
+from larmor_amd import synthetic
 from larmor_amd.sd import tissue_responses as tr
 from larmor_amd.sd.msmt import constraint_matrix, forward_matrix, msmt_ridge
 from larmor_amd.sd.peaks import Peaks, _build_icosphere, _get_amplitudes
-from larmor_amd.sd.sphere import get_dirs
 from larmor_amd.types import GradientTable, Volume
 
 pytestmark = pytest.mark.gpu
 
-SHAPE = (26, 26, 16)
-SHELLS = (1000.0, 2000.0, 3000.0)
 S0 = {"wm": 1000.0, "gm": 1300.0, "csf": 3000.0}
 
 
-def _gradients():
-    dirs = get_dirs(30)
-    bvecs = [np.zeros((4, 3))]
-    for k in range(len(SHELLS)):
-        c, s = np.cos(0.5 * k), np.sin(0.5 * k)
-        bvecs.append(dirs @ np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]]).T)
-    return GradientTable(np.concatenate([np.zeros(4), np.repeat(SHELLS, 30)]), np.vstack(bvecs))
-
-
-def _step(x, edge):
-    return 1 / (1 + np.exp(-(x - edge) / 0.04))
-
-
 def _phantom():
-    """The reference's phantom: CSF at the centre of an ellipsoid, white matter around it (fibres
-    around z, crossed along z above the middle slice), grey matter outside; Gaussian noise of
-    15 in signal units, magnitude."""
-    gtab = _gradients()
-    grid = np.stack(np.meshgrid(*[np.arange(n) - (n - 1) / 2 for n in SHAPE], indexing="ij"), -1)
-    radius = np.linalg.norm(grid / np.array([12.0, 12.0, 7.0]), axis=-1)
-    csf = 1 - _step(radius, 0.3)
-    gm = _step(radius, 0.65)
-    wm = 1 - csf - gm
-    mask = radius < 1.0
-    around = np.stack([-grid[..., 1], grid[..., 0], np.zeros(SHAPE)], -1)
-    around /= np.linalg.norm(around, axis=-1, keepdims=True).clip(1e-9)
-    crossing = grid[..., 2] > 0
-    b, g = gtab.bvals, gtab.bvecs
-
-    def fibre(direction):
-        return np.exp(-b * (0.3e-3 + 1.4e-3 * (direction @ g.T) ** 2))
-
-    wm_signal = fibre(around)
-    wm_signal[crossing] = 0.5 * wm_signal[crossing] + 0.5 * fibre(np.array([0.0, 0.0, 1.0]))
-    signal = (S0["wm"] * wm[..., None] * wm_signal
-              + S0["gm"] * gm[..., None] * np.exp(-b * 0.8e-3)
-              + S0["csf"] * csf[..., None] * np.exp(-b * 3e-3))
-    rng = np.random.default_rng(0)
-    noisy = np.abs(signal + 15.0 * rng.standard_normal(signal.shape)).astype(np.float32)
-    volume = Volume(noisy * mask[..., None], np.diag([2.5, 2.5, 2.5, 1.0]))
-    return volume, gtab, mask, {"wm": wm, "gm": gm, "csf": csf}, ~crossing
+    """The reference's phantom (tests/sd/test_tissue_responses.py:25-99), 26 x 26 x 16."""
+    data, bvals, bvecs, mask, fractions, single = synthetic.three_tissue_phantom()
+    volume = Volume(data, np.diag([2.5, 2.5, 2.5, 1.0]))
+    return volume, GradientTable(bvals, bvecs), mask, fractions, single
 
 
 @pytest.fixture(scope="module")
