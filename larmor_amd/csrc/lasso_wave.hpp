@@ -56,6 +56,13 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) lasso_kernel(LassoArgs
     const int m = A.m;
     const int c = A.c;
     const double row_count = static_cast<double>(m);
+    // Both divisors are fixed for the launch: their refined reciprocals once, then each quotient
+    // is the division's last three operations (bit-identical to a / d where the operands are in
+    // range, a plain division elsewhere).
+    const double y_rows = division_reciprocal(row_count);
+    const bool rows_ok = divisor_in_range(row_count);
+    const double y_shrink = division_reciprocal(A.shrink);
+    const bool shrink_ok = divisor_in_range(A.shrink);
     const long count = A.list ? static_cast<long>(*A.list_count) : A.n_vox;
     const long stride = static_cast<long>(gridDim.x) * WAVES_PER_BLOCK;
     unsigned long long wave_work = 0;
@@ -106,7 +113,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) lasso_kernel(LassoArgs
                     part = part + col[t] * res[t];
                 }
             }
-            return butterfly_sum(part) / row_count;
+            return divide_nonzero_by(butterfly_sum(part), row_count, y_rows, rows_ok);
         };
         // has_nonzero_solution (elastic_net.cpp:51-57).
         double largest = 0.0;
@@ -129,7 +136,8 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) lasso_kernel(LassoArgs
                 load_column(j, col);
                 const double old_value = coef.get(j);
                 const double corr = correlation(col) + old_value;
-                const double new_value = soft_threshold(corr, A.l1_threshold) / A.shrink;
+                const double new_value = divide_nonzero_by(soft_threshold(corr, A.l1_threshold),
+                                                           A.shrink, y_shrink, shrink_ok);
                 wave_work += 2ull * m;
                 if (new_value == old_value) {
                     return 0.0;
