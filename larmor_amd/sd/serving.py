@@ -46,7 +46,7 @@ from larmor_amd.sd.model import (SDModel, _to_sh, compute_max_lmax, compute_shel
                                   low_resolution_dirs)
 from larmor_amd.sd.rf import IsotropicRF
 from larmor_amd.sd.sh import projection_matrix
-from larmor_amd.sd.solvers import (FINAL_FRACTION_TOL_X, L2NNLS, MAX_ROUNDS, Lasso, Stage,
+from larmor_amd.sd.solvers import (DRL, FINAL_FRACTION_TOL_X, L2NNLS, MAX_ROUNDS, Lasso, Stage,
                                    _deconv_kernel)
 from larmor_amd.sd.sphere import DIRS_300
 from larmor_amd.types import GradientTable
@@ -88,7 +88,8 @@ def request_bytes(model: SDModel, n_voxels: int, n_meas: int, lmax: int,
     """Device bytes one request holds while it runs: its signals (plus the float32 upload when
     the rows come as float32), the fit's per-voxel state and outputs (the high-resolution
     amplitudes and correlations, the low-resolution FODs and correlations, fractions, rounds,
-    status, voxel lists) and the SH coefficients, plus a fixed allowance."""
+    status, voxel lists, and a damped Richardson-Lucy solver's position buffers) and the SH
+    coefficients, plus a fixed allowance."""
     n_aniso = sum(1 for rf in model.rfs if not isinstance(rf, IsotropicRF))
     n_iso = len(model.rfs) - n_aniso
     n_hr = len(DIRS_300) * n_aniso
@@ -97,6 +98,10 @@ def request_bytes(model: SDModel, n_voxels: int, n_meas: int, lmax: int,
     per_voxel += 8 * (2 * n_hr + 2 * n_lr)  # amplitudes + correlations, HR and LR
     per_voxel += 8 * (n_aniso * _n_sh(lmax) + len(model.rfs) + 2 * (1 + n_iso))
     per_voxel += 4 * 5  # rounds, status, the overflow list and the two voxel lists
+    solver = model.solver if model.solver is not None else model.method.default_solver()
+    if isinstance(solver, DRL):
+        # damped Richardson-Lucy keeps, per stage, the signal and four c-length vectors per voxel
+        per_voxel += 8 * (2 * n_meas + 4 * (n_hr + n_lr))
     return int(n_voxels) * per_voxel + _FIXED_BYTES
 
 

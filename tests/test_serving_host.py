@@ -102,3 +102,11 @@ def test_expand_places_fitted_rows():
     assert out.shape == (5, 2)
     np.testing.assert_array_equal(out[finite], np.arange(6.0).reshape(3, 2))
     assert np.isnan(out[~finite]).all()
+
+
+def test_request_bytes_counts_the_drl_buffers():
+    rfs = [AnisotropicRF(WM, 0.0), IsotropicRF(0.8e-3), IsotropicRF(3e-3)]
+    grl = SDModel(rfs, method="GRL")
+    mfod = SDModel(rfs, method="mFOD")
+    extra = (request_bytes(grl, 1000, 202, 8, False) - request_bytes(mfod, 1000, 202, 8, False))
+    assert extra == 1000 * 8 * (2 * 202 + 4 * (300 + 44))
