@@ -15,7 +15,9 @@ memory (`request_bytes`) fits in what the budget has left; the request's upload,
 (the low-resolution rounds, the high-resolution solve, the fractions, the SH projection) and the
 download of its results are enqueued on that stream, and an event marks its end.  Requests that
 arrive while others run are admitted as soon as a stream and memory free up, and their kernels
-overlap on the device with the ones already running.  The scheduler never waits on the device: it
+overlap on the device with the ones already running.  Queued requests on the head request's
+acquisition (same design) are fitted together with it, as one launch sequence over their
+concatenated rows, and their results split back.  The scheduler never waits on the device: it
 polls the events, so the host keeps admitting while the GPU works.
 
 Designs.  The per-acquisition objects (the low- and high-resolution kernels and their Gram
@@ -179,20 +181,25 @@ class _Running:
     keep: list = field(default_factory=list)  # device tensors alive until the event completes
     sh_host: list = field(default_factory=list)
     fractions_host: object = None
+    # The requests fitted together: each with its rows' (start, stop) in the merged fit.
+    members: list = field(default_factory=list)
 
 
 class FodServer:
     """Fits deconvolution requests as they arrive, several at once on one GPU.
 
     model: the SDModel whose fit is served (its responses, method, solver, shell weighting, lmax
-    and basis); slots: requests in flight at once, one HIP stream each (the device runs four
+    and basis); slots: fits in flight at once, one HIP stream each (the device runs four
     hardware queues per process, so more than four share them); hbm_budget: device bytes that
     admitted requests and resident designs may hold together; max_designs: acquisitions kept
-    resident (least recently used ones go first, never one in use).
+    resident (least recently used ones go first, never one in use); merge_voxels: queued requests
+    on the head request's acquisition are fitted with it as one launch sequence up to this many
+    rows in all.
     """
 
     def __init__(self, model: SDModel, *, slots: int = 4, hbm_budget: float = 200e9,
-                 max_designs: int = 256, poll_interval: float = 1e-4) -> None:
+                 max_designs: int = 256, merge_voxels: int = 1 << 18,
+                 poll_interval: float = 1e-4) -> None:
         if slots < 1:
             raise ValueError(f"slots must be at least 1, got {slots}")
         if not hbm_budget > 0:
@@ -217,6 +224,8 @@ class FodServer:
         self._closed = threading.Event()
         self._served: list[ServedFit] = []
         self.max_in_flight = 0
+        self.max_merged = 1
+        self.merge_voxels = int(merge_voxels)
         self.peak_bytes = 0
         self._error: BaseException | None = None
         self._thread = threading.Thread(target=self._loop, name="fod-server", daemon=True)
@@ -347,20 +356,23 @@ class FodServer:
         done = [r for r in self._running if r.event.query()]
         for r in done:
             self._running.remove(r)
-            req = r.request
             finished = time.perf_counter()
-            sh = [t.numpy() for t in r.sh_host]
-            fractions = None if r.fractions_host is None else r.fractions_host.numpy()
-            if req.finite is not None:
-                sh = [_expand(a, req.finite) for a in sh]
-                fractions = None if fractions is None else _expand(fractions, req.finite)
-            fit = ServedFit(sh=sh, fractions=fractions, lmax=r.design.lmax, n_voxels=req.n_rows,
-                            arrival=req.arrival, admitted=r.admitted, finished=finished)
+            all_sh = [t.numpy() for t in r.sh_host]
+            all_fractions = None if r.fractions_host is None else r.fractions_host.numpy()
             r.keep.clear()
-            self._running_bytes -= req.nbytes
             self._streams.append(r.stream)
-            self._served.append(replace(fit, sh=[], fractions=None))  # timings only
-            req.future.set_result(fit)
+            for req, start, stop in r.members:
+                sh = [a[start:stop] for a in all_sh]
+                fractions = None if all_fractions is None else all_fractions[start:stop]
+                if req.finite is not None:
+                    sh = [_expand(a, req.finite) for a in sh]
+                    fractions = None if fractions is None else _expand(fractions, req.finite)
+                fit = ServedFit(sh=sh, fractions=fractions, lmax=r.design.lmax,
+                                n_voxels=req.n_rows, arrival=req.arrival, admitted=r.admitted,
+                                finished=finished)
+                self._running_bytes -= req.nbytes
+                self._served.append(replace(fit, sh=[], fractions=None))  # timings only
+                req.future.set_result(fit)
         return bool(done)
 
     def _design(self, key: tuple, bvals: np.ndarray, bvecs: np.ndarray, lmax: int) -> _Design:
@@ -400,25 +412,64 @@ class FodServer:
             if self._running and used + req.nbytes > self.hbm_budget:
                 break
             self._pending.popleft()
+            batch = self._merge(req, used)
             stream = self._streams.pop()
             running = _Running(req, design, stream, torch.cuda.Event(), time.perf_counter())
+            start = 0
+            for member in batch:
+                stop = start + int(member.host.shape[0])
+                running.members.append((member, start, stop))
+                start = stop
             try:
                 self._launch(torch, running)
             except Exception as exc:  # invalid request caught by the C-ABI checks
                 self._streams.append(stream)
-                req.future.set_exception(exc)
+                for member in batch:
+                    member.future.set_exception(exc)
                 continue
-            self._running_bytes += req.nbytes
+            self._running_bytes += sum(member.nbytes for member in batch)
             self._running.append(running)
             self.max_in_flight = max(self.max_in_flight, len(self._running))
+            self.max_merged = max(self.max_merged, len(batch))
             self.peak_bytes = max(self.peak_bytes, self._running_bytes + self._design_bytes)
             admitted = True
         return admitted
 
+    def _merge(self, head: _Request, used: int) -> list:
+        """The head request and the queued requests on the same acquisition and lmax (same design,
+        same input dtype) that fit with it under merge_voxels rows and the budget: they run as one
+        fit, which gives each voxel the same bits as its request alone (no kernel of the path
+        combines voxels)."""
+        batch = [head]
+        rows = int(head.host.shape[0])
+        nbytes = head.nbytes
+        if self.merge_voxels <= rows:
+            return batch
+        kept = collections.deque()
+        while self._pending:
+            other = self._pending.popleft()
+            n = int(other.host.shape[0])
+            if (other.key == head.key and other.host.dtype == head.host.dtype
+                    and not other.future.cancelled() and rows + n <= self.merge_voxels
+                    and used + nbytes + other.nbytes <= self.hbm_budget):
+                batch.append(other)
+                rows += n
+                nbytes += other.nbytes
+            else:
+                kept.append(other)
+        self._pending = kept
+        return batch
+
     def _launch(self, torch, r: _Running) -> None:
         req, design = r.request, r.design
         with torch.cuda.stream(r.stream):
-            dev = req.host.to("cuda", non_blocking=True)
+            if len(r.members) == 1:
+                dev = req.host.to("cuda", non_blocking=True)
+            else:
+                rows = r.members[-1][2]
+                dev = torch.empty((rows, req.host.shape[1]), dtype=req.host.dtype, device="cuda")
+                for member, start, stop in r.members:
+                    dev[start:stop].copy_(member.host, non_blocking=True)
             r.keep.append(dev)
             if dev.dtype != torch.float64:
                 dev = dev.to(torch.float64)

@@ -186,3 +186,22 @@ def test_non_finite_rows_and_an_empty_request():
     assert np.array_equal(fit.fractions[finite], fractions)
     assert empty.n_voxels == 0 and empty.sh[0].shape == (0, sh[0].shape[1])
     assert empty.fractions.shape == (0, 3)
+
+
+def test_queued_requests_on_one_acquisition_are_fitted_together():
+    """With one stream, requests queue; those on the head's acquisition are merged into one fit,
+    and each still gets the same bits as its plain fit alone."""
+    model = _model("mFOD")
+    base = synthetic.batched_jobs(2, seed=81, min_voxels=300, max_voxels=1500)
+    rng = np.random.default_rng(81)
+    jobs = []
+    for k in range(8):
+        acq = base[k % 2]
+        n = int(rng.integers(100, 1200))
+        jobs.append(synthetic.Job(synthetic.tissue_signals(n, acq.bvals, acq.bvecs, rng, snr=25.0),
+                                  acq.bvals, acq.bvecs, acq.lmax))
+    server, results = _serve(model, jobs, slots=1)
+    assert server.max_merged >= 3
+    for job, fit in zip(jobs, results):
+        assert fit.n_voxels == len(job.signals)
+        _check_same(fit, _alone(model, job))
