@@ -206,6 +206,7 @@ def serve_report(n_jobs: int, rates: list[float], slots: int) -> int:
     batched = bench.run("batched", [(0.0, i) for i in order])
     served = [bench.poisson(rate) for rate in rates]
     bench.close()
+    protocol = shared_protocol_phase(slots)
     result = {
         "metric": "voxel FOD solves/s (mFOD lmax=8, 3-shell); p50/p99 latency under load",
         "unit": "voxel solves/s",
@@ -216,9 +217,51 @@ def serve_report(n_jobs: int, rates: list[float], slots: int) -> int:
         "batched": batched,
         "served": served,
         "batched_vs_plain": batched["voxel_solves_per_s"] / plain["voxel_solves_per_s"],
+        "shared_protocol": protocol,
     }
     print(json.dumps(result), flush=True)
     return 0
+
+
+def shared_protocol_phase(slots: int) -> dict:
+    """64 small requests (2k-20k voxels, log-uniform) on one acquisition (10 b0 + 3 x 64
+    directions at b 1000/2000/3000), lmax 8, Poisson arrivals at 400 jobs/s, served with and
+    without merging queued requests on the same acquisition into one fit."""
+    import torch
+
+    from larmor_amd import synthetic
+    from larmor_amd.sd.model import SDModel
+    from larmor_amd.sd.rf import AnisotropicRF, IsotropicRF
+    from larmor_amd.sd.serving import FodServer, summarize
+    from larmor_amd.types import GradientTable
+
+    rng = np.random.default_rng(7)
+    bvals, bvecs = synthetic.acquisition(10, [(1000.0, 64), (2000.0, 64), (3000.0, 64)], rng)
+    sizes = np.exp(rng.uniform(np.log(2000), np.log(20000), 64)).astype(int)
+    signals = [synthetic.tissue_signals(int(n), bvals, bvecs, rng, snr=25.0, device="cuda")
+               for n in sizes]
+    gtab = GradientTable(bvals, bvecs)
+    offsets = np.concatenate([[0.0], np.cumsum(rng.exponential(1.0 / 400.0, 63))])
+    model = SDModel([AnisotropicRF([1.7e-3, 0.3e-3, 0.3e-3], 0.0), IsotropicRF(0.8e-3),
+                     IsotropicRF(3.0e-3)], method="mFOD", lmax=8)
+    out = {"workload": f"64 requests of 2k-20k voxels ({int(sizes.sum())} in all) on one "
+                       "acquisition (10 b0 + 3 x 64, b 1000/2000/3000), Poisson 400 jobs/s"}
+    for name, merge in (("merged", 1 << 18), ("one_per_fit", 0)):
+        with FodServer(model, slots=slots, merge_voxels=merge) as server:
+            server.prepare(gtab, lmax=8)
+            server.submit(signals[0][:500], gtab).result()
+            futures = []
+            t0 = time.perf_counter()
+            for offset, sig in zip(offsets, signals):
+                delay = t0 + offset - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                futures.append(server.submit(sig, gtab))
+            out[name] = {**summarize([f.result() for f in futures]),
+                         "largest_merge": server.max_merged}
+        print(f"shared protocol, {name}: {json.dumps(out[name])}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    return out
 
 
 def rows_report() -> int:
