@@ -34,7 +34,7 @@ constexpr int NQ = 4;             // most fraction columns (groups + isotropic t
 // LDS factor capacity (active columns) of the fast FOD kernels: 24 for the low-resolution solve
 // (its active sets stay below 24 in all but a handful of voxels, which take the rerun kernel),
 // so the LDS does not cap its occupancy; 32 for the others.
-constexpr int kcap_lds(int ncpl) { return ncpl == 1 ? 24 : 32; }
+constexpr int kcap_lds(int ncpl) { return ncpl == 1 ? 20 : 32; }
 constexpr int WAVES_PER_BLOCK = 4;
 constexpr int RERUN_WAVES = 1024; // waves of a rerun launch (4 per CU; the overflow list is a few percent)
 constexpr int RERUN_WAVES_ALL = 512; // waves when every voxel needs the global-memory kernel
@@ -171,7 +171,7 @@ struct FodArgs {
 // The fast low-resolution instantiation is asked for 7 waves per SIMD, what its LDS factor
 // (5.7 KB per wave) allows: the register budget that leaves costs a few spilled registers and
 // still beats 6 waves.
-constexpr int fod_waves_per_eu(int ncpl, bool global_l) { return (!global_l && ncpl == 1) ? 7 : 1; }
+constexpr int fod_waves_per_eu(int ncpl, bool global_l) { return (!global_l && ncpl == 1) ? 8 : 1; }
 
 template <int NCPL, int RPL, int KCAP, bool GLOBAL_L>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
