@@ -64,11 +64,37 @@ def build_workload(seed_override=None):
     return cfg, signals, w, lr, hr
 
 
+def host_cores() -> int:
+    """The host cores this process may run on: its CPU affinity set, capped by OMP_NUM_THREADS
+    when the job's CPU share is set that way (a shared GPU host grants each one-GPU job a share
+    of its cores)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        cores = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        cores = min(cores, int(share))
+    return max(1, cores)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(signals, w, lr, hr, target_seconds=15.0):
-    """Times the oracle on a random sample sized for about target_seconds of CPU work."""
+    """Times the oracle on a random sample sized for about target_seconds of CPU work, on every
+    host core this process may use."""
     import oracle
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_cores()
     rng = np.random.default_rng(0)
     probe = np.ascontiguousarray(signals[rng.choice(len(signals), size=min(1000, len(signals)),
                                                     replace=False)])
@@ -86,6 +112,8 @@ def cpu_baseline(signals, w, lr, hr, target_seconds=15.0):
         "cores": threads,
         "kind": "port",
         "sample": f"{n} random masked voxels of {len(signals)} ({dt:.1f} s)",
+        "cpu_model": cpu_model(),
+        "host_cpus": os.cpu_count(),
     }
 
 
@@ -283,7 +311,7 @@ def rows_report() -> int:
                                        _deconv_kernel)
     from larmor_amd.sd.sphere import DIRS_300
 
-    threads = os.cpu_count() or 1
+    threads = host_cores()
     rng = np.random.default_rng(0)
     lines = []
 
@@ -607,14 +635,23 @@ def main() -> int:
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    _lib.profile_enable(True)
-    _lib.profile_collect()
-    torch.cuda.synchronize()
+    # The timed steps run with the library's per-launch timing off.
     t0 = time.perf_counter()
     for _ in range(args.steps):
         out = step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    print(f"timed: {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    # Per-kernel times and counted work from separate profiled steps (HIP events around each
+    # launch on its own stream).
+    profiled_steps = max(1, min(args.steps, 3))
+    _lib.profile_enable(True)
+    _lib.profile_collect()
+    t_prof = time.perf_counter()
+    for _ in range(profiled_steps):
+        step()
+    torch.cuda.synchronize()
+    profiled_elapsed = time.perf_counter() - t_prof
     kernel_times = _lib.profile_collect()
     _lib.profile_enable(False)
     sh, fractions, rounds, status = out
@@ -625,7 +662,7 @@ def main() -> int:
     n_hr = hr.w_k_aniso.shape[1]
     # Algorithmic FP64 flops of each kernel per step.
     def counted(name):
-        return kernel_times.get(name, (0.0, 0, 0.0))[2] / args.steps
+        return kernel_times.get(name, (0.0, 0, 0.0))[2] / profiled_steps
 
     step_flops = {
         "fod_lr_corr": total_rounds * 2.0 * m * n_lr,
@@ -637,16 +674,25 @@ def main() -> int:
     }
     dominant = max(step_flops, key=lambda k: kernel_times.get(k, (0.0, 0, 0.0))[0])
     dom_ms, dom_launches, _ = kernel_times[dominant]
-    launches_per_step = dom_launches / args.steps
+    launches_per_step = dom_launches / profiled_steps
     avg_launch_s = dom_ms / dom_launches / 1e3
     flops_per_launch = step_flops[dominant] / launches_per_step
     achieved = flops_per_launch / avg_launch_s / 1e12
     print("kernel ms per step: " + ", ".join(
-        f"{k} {v[0] / args.steps:.2f} ({v[1] // args.steps} launches)"
+        f"{k} {v[0] / profiled_steps:.2f} ({v[1] // profiled_steps} launches)"
         for k, v in kernel_times.items()), file=sys.stderr, flush=True)
     print("counted FP64 work per step: " + ", ".join(
-        f"{k} {v[2] / args.steps / 1e9:.2f} GFLOP" for k, v in kernel_times.items() if v[2] > 0),
-        file=sys.stderr, flush=True)
+        f"{k} {v[2] / profiled_steps / 1e9:.2f} GFLOP" for k, v in kernel_times.items()
+        if v[2] > 0), file=sys.stderr, flush=True)
+    # The whole step against the roofs, with the dense work and bytes per voxel of the path:
+    # F = 2 m n_hr (K^T s, high resolution) + 2 n_hr C (SH projection) + R 4 m n_lr (per round
+    # K^T a and the group column K f), R the mean rounds run; B = 4 m + 4 C + 4 T (an FP32 signal
+    # in, FP32 SH and fractions out).
+    n_coeffs = projection.shape[1]
+    n_tissues = fractions.shape[1]
+    mean_rounds = total_rounds / n_vox
+    dense_per_voxel = 2.0 * m * n_hr + 2.0 * n_hr * n_coeffs + mean_rounds * 4.0 * m * n_lr
+    bytes_per_voxel = 4.0 * (m + n_coeffs + n_tissues)
     print(f"rounds: mean {total_rounds / n_vox:.2f}; status bits: "
           f"{torch.bincount(status.to(torch.int64)).tolist()}", file=sys.stderr, flush=True)
 
@@ -673,7 +719,9 @@ def main() -> int:
             "mean_rounds": total_rounds / n_vox,
         },
         "roofline": {
-            "bound": "mfma",
+            # The dominant kernel issues no MFMA: its active-set solves are chains of dependent
+            # FP64 vector and scalar instructions, so FP64 VALU issue bounds it.
+            "bound": "fp64-valu",
             "achieved": achieved,
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
@@ -683,7 +731,17 @@ def main() -> int:
             "kernel_avg_launch_ms": avg_launch_s * 1e3,
             "launches_per_step": launches_per_step,
             "flops_per_launch": flops_per_launch,
-            "peak_kind": "FP64 (MI355X's vector and matrix FP64 peaks are both 78.6 TFLOP/s)",
+            "peak_kind": "FP64 vector (78.6 TFLOP/s on MI355X)",
+            "profiled_steps": profiled_steps,
+            "profiled_ms_per_step": profiled_elapsed / profiled_steps * 1e3,
+        },
+        "step_roofline": {
+            "dense_flops_per_voxel": dense_per_voxel,
+            "dense_tflops": value * dense_per_voxel / 1e12,
+            "dense_frac_fp64": value * dense_per_voxel / 1e12 / FP64_PEAK_TFLOPS,
+            "hbm_bytes_per_voxel": bytes_per_voxel,
+            "hbm_gbs": value * bytes_per_voxel / 1e9,
+            "hbm_frac": value * bytes_per_voxel / 1e9 / HBM_PEAK_GBS,
         },
     }
     if not args.no_serve:

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define LSD_ABI_VERSION 3
+#define LSD_ABI_VERSION 4
 #define LSD_ERR_INVALID 1 /* invalid argument (the reference raises ValueError) */
 #define LSD_ERR_HIP 2     /* a HIP runtime call failed */
 
@@ -67,7 +67,7 @@ int lsd_solver_create_l2nnls(const double* kernel, int64_t m, int64_t n, double 
  * direction.  Fits with it run drl_kernel: per voxel the damping strength and stopping threshold
  * from the signal, K^T s, then the multiplicative updates until the FOD changes in total by no
  * more than 1e-4 of the signal's largest value (status bit 1 when n_iter stopped the voxel).
- * Replaces: FodSolver.drl(k_t, k_t_k, nu, n_iter, init)  cpp/src/python/sd.cpp:127-145
+ * Replaces: FodSolver.drl(k_t, k_t_k, nu, n_iter, init)  cpp/src/python/sd.cpp:127-146
  *           (drl_solver), DrlSolver cpp/src/sd/solver.hpp:16-33, damped_richardson_lucy
  *           cpp/src/sd/drl.cpp:39-99.
  * Errors: "shapes do not match: k_t (c, m), k_t_k (r, s)", "n_iter must be at least 0".
@@ -86,7 +86,7 @@ int lsd_solver_is_drl(const lsd_solver* solver);
  * followed by sweeps over the non-zero coefficients, until a full sweep moves no coefficient by
  * tol relative to 1 + its size or max_iter sweeps ran (status bit 1, and the solver's
  * unconverged count); the FOD is b over the scales.
- * Replaces: FodSolver.lasso(k, reg, alpha, tol, max_iter)  cpp/src/python/sd.cpp:101-123
+ * Replaces: FodSolver.lasso(k, reg, alpha, tol, max_iter)  cpp/src/python/sd.cpp:102-125
  *           (elastic_net_solver), ElasticNetSolver cpp/src/sd/solver.hpp:56-64,
  *           standardize_columns / solve_elastic_net cpp/src/sd/elastic_net.cpp:25-151.
  * Errors: "reg must be at least 0", "alpha must be in (0, 1]", "tol must be positive",
@@ -100,7 +100,7 @@ int lsd_solver_is_lasso(const lsd_solver* solver);
  * stopped, always 0 except for an elastic-net solver; waits for the device. */
 int lsd_solver_n_unconverged(const lsd_solver* solver, int64_t* out);
 void lsd_solver_destroy(lsd_solver* solver);
-/* FodSolver.n_dirs / n_meas  cpp/src/python/sd.cpp:399-443 */
+/* FodSolver.n_dirs / n_meas  cpp/src/python/sd.cpp:437-440 */
 int64_t lsd_solver_n_dirs(const lsd_solver* solver);
 int64_t lsd_solver_n_meas(const lsd_solver* solver);
 int lsd_solver_normal_equations(const lsd_solver* solver);
@@ -109,7 +109,7 @@ int lsd_solver_normal_equations(const lsd_solver* solver);
  * One resolution of a multi-tissue kernel (host arrays): the unweighted anisotropic columns
  * (m x ca), the unweighted isotropic columns (m x ci), the weighted isotropic columns (m x ci) and
  * the anisotropic blocks as (start, stop) pairs (n_blocks x 2).
- * Replaces: DeconvKernel(k_aniso, k_iso, w_k_iso, aniso_blocks)  cpp/src/python/sd.cpp:240-277
+ * Replaces: DeconvKernel(k_aniso, k_iso, w_k_iso, aniso_blocks)  cpp/src/python/sd.cpp:261-294
  *           (construct_kernel), TissueKernel cpp/src/sd/fractions.hpp:27-38.
  * Errors: "aniso_blocks must be non-empty ranges within 0..ca, got [...]"; at most 4 fraction
  * columns (blocks + isotropic tissues).
@@ -122,7 +122,7 @@ void lsd_tissue_kernel_destroy(lsd_tissue_kernel* kernel);
 /*
  * The FOD of each voxel with one solver: signals (device, n_voxels x m) -> fods (device,
  * n_voxels x n), status (device int32, n_voxels).
- * Replaces: fit_single_tissue(signals, solver, n_threads)  cpp/src/python/sd.cpp:145-166,
+ * Replaces: fit_single_tissue(signals, solver, n_threads)  cpp/src/python/sd.cpp:148-169,
  *           L2NnlsSolver::solve_block cpp/src/sd/solver.cpp:14-21,
  *           solve_ridge_nnls cpp/src/sd/nnls.cpp:294-350.
  * Errors: "signals (r, c) do not have the solver's N measurements".
@@ -138,7 +138,7 @@ int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
  * Outputs (device): fods (n_voxels x hr.ca, may be NULL), fractions (n_voxels x
  * (hr.n_blocks + ci)), rounds (int32, n_voxels: low-resolution rounds run), status (int32).
  * Replaces: fit_multi_tissue(signals, weights, lr, lr_solver, hr, hr_solver, max_outer,
- *           fraction_tol, final_tol_x, n_threads)  cpp/src/python/sd.cpp:279-330,
+ *           fraction_tol, final_tol_x, n_threads)  cpp/src/python/sd.cpp:310-358,
  *           fit_multi_tissue_block cpp/src/sd/multi_tissue.cpp:28-100,
  *           compute_fractions cpp/src/sd/fractions.cpp:45-76.
  * Errors: the reference's shape and kernel/solver mismatch messages; "max_outer must be at
@@ -150,6 +150,15 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m,
                          const lsd_solver* hr_solver, int64_t max_rounds, double fraction_tol,
                          double final_tol, double* fods, double* fractions, int32_t* rounds,
                          int32_t* status, void* stream);
+
+/*
+ * The device bytes of the global-memory rerun slabs one fit allocates besides its per-voxel
+ * buffers, for m measurements and ridge NNLS solves of n_lr (0 for none: a single-tissue fit) and
+ * n_hr directions; -1 for sizes the kernels do not take.  Host-only (no device call).  No
+ * counterpart in the reference, which allocates per thread on the host: it lets a caller budget
+ * device memory (the request server's admission).
+ */
+int64_t lsd_fit_scratch_bytes(int64_t m, int64_t n_lr, int64_t n_hr);
 
 /*
  * A constrained spherical deconvolution problem (host arrays): the linear start initial (c0 x m),
@@ -211,7 +220,7 @@ int lsd_find_peaks(const double* coeffs, int64_t n_voxels, int64_t n_coeffs, int
 /*
  * The value and derivatives of series i at (theta[i], phi[i]) (all device): out (n x 6) =
  * (value, d_theta, d_phi / sin, d_theta_theta, d_theta_phi / sin, d_phi_phi / sin^2).
- * Replaces: sh_derivatives(sh, lmax, theta, phi, full)  cpp/src/python/sh.cpp:67-83,
+ * Replaces: sh_derivatives(sh, lmax, theta, phi, full)  cpp/src/python/sh.cpp:70-83,
  *           derivatives cpp/src/sh/harmonics.cpp:139-185 (batched over series).
  */
 int lsd_sh_derivatives(const double* coeffs, int64_t n, int64_t n_coeffs, int64_t lmax, int full,
@@ -233,7 +242,7 @@ int lsd_sh_climb(const double* coeffs, int64_t n, int64_t n_coeffs, int64_t lmax
  * subject to g x >= 0, prepared for many data vectors (host arrays a (m x n), g (k x g_cols)):
  * the column-pivoted QR of [a; ridge I] and the least-distance form E = g P R^-1 (unit rows),
  * E E^T, made on the host and kept on the device.  n <= 63, k <= 512.
- * Replaces: LsiDesign(a, g, ridge)  cpp/src/python/sd.cpp:360-366,
+ * Replaces: LsiDesign(a, g, ridge)  cpp/src/python/sd.cpp:361-367,
  *           make_lsi_design cpp/src/sd/lsi.cpp:19-69.
  * Errors: "the constraints have C columns but the design has N", "the ridge must be at least 0",
  * "the design does not have full column rank: the measurements cannot determine all N
@@ -252,8 +261,8 @@ int64_t lsd_lsi_n_constraints(const lsd_lsi* design);
  * n_voxels x n), converged (device uint8: 0 when the NNLS stopped at its inner-step limit) and,
  * when not NULL, status (device int32: LSD_STATUS_QR_FALLBACK when the QR-based NNLS finished
  * the voxel).
- * Replaces: fit_lsi(signals, design, n_threads)  cpp/src/python/sd.cpp:368-394,
- *           solve_lsi cpp/src/sd/lsi.cpp:245-296.
+ * Replaces: fit_lsi(signals, design, n_threads)  cpp/src/python/sd.cpp:369-396,
+ *           solve_lsi cpp/src/sd/lsi.cpp:245-288.
  * Errors: "signals (r, c) do not have the design's M measurements".
  */
 int lsd_fit_lsi(const double* signals, int64_t n_voxels, int64_t m, const lsd_lsi* design,
@@ -261,7 +270,7 @@ int lsd_fit_lsi(const double* signals, int64_t n_voxels, int64_t m, const lsd_ls
 
 /*
  * lsd_fit_lsi with the QR-based NNLS for every voxel.
- * Replaces: solve_lsi_by_qr cpp/src/sd/lsi.cpp:298-300 (the fallback, on its own).
+ * Replaces: solve_lsi_by_qr cpp/src/sd/lsi.cpp:290-292 (the fallback, on its own).
  */
 int lsd_fit_lsi_by_qr(const double* signals, int64_t n_voxels, int64_t m, const lsd_lsi* design,
                       double* x, uint8_t* converged, int32_t* status, void* stream);

@@ -8,7 +8,7 @@ import ctypes
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "liblarmor_sd.so"
-ABI_VERSION = 3  # LSD_ABI_VERSION of include/larmor_sd.h these bindings follow
+ABI_VERSION = 4  # LSD_ABI_VERSION of include/larmor_sd.h these bindings follow
 HEADER_PATH = Path(__file__).resolve().parents[1] / "include" / "larmor_sd.h"
 
 _lib = None
@@ -62,6 +62,7 @@ SIGNATURES = {
         [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64,
          ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp, _vp],
     ),
+    "lsd_fit_scratch_bytes": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
     "lsd_csd_create": (
         ctypes.c_int,
         [_d, ctypes.c_int64, ctypes.c_int64, _d, ctypes.c_int64, _d, ctypes.c_int64,

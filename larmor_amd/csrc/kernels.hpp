@@ -31,9 +31,9 @@
 namespace lsd {
 
 constexpr int NQ = 4;             // most fraction columns (groups + isotropic tissues)
-// LDS factor capacity (active columns) of the fast FOD kernels: 24 for the low-resolution solve
-// (its active sets stay below 24 in all but a handful of voxels, which take the rerun kernel),
-// so the LDS does not cap its occupancy; 32 for the others.
+// LDS factor capacity (active columns) of the fast FOD kernels: 20 for the low-resolution solve
+// (its active sets stay below 20 in all but about 0.1% of voxels, which take the rerun kernel),
+// so that 8 waves per SIMD fit in LDS; 32 for the others.
 constexpr int kcap_lds(int ncpl) { return ncpl == 1 ? 20 : 32; }
 constexpr int WAVES_PER_BLOCK = 4;
 constexpr int RERUN_WAVES = 1024; // waves of a rerun launch (4 per CU; the overflow list is a few percent)
@@ -168,9 +168,9 @@ struct FodArgs {
     unsigned long long* work; // profiled launches: the completed solves' FP64 operations
 };
 
-// The fast low-resolution instantiation is asked for 7 waves per SIMD, what its LDS factor
-// (5.7 KB per wave) allows: the register budget that leaves costs a few spilled registers and
-// still beats 6 waves.
+// The fast low-resolution instantiation is asked for 8 waves per SIMD, what its LDS factor
+// (4.2 KB per wave, 20 columns) allows: the register budget that leaves costs a few spilled
+// registers and still beats 7 waves with a 24-column factor.
 constexpr int fod_waves_per_eu(int ncpl, bool global_l) { return (!global_l && ncpl == 1) ? 8 : 1; }
 
 template <int NCPL, int RPL, int KCAP, bool GLOBAL_L>
@@ -543,6 +543,21 @@ constexpr long slab_doubles(bool global_l) {
            (global_l ? static_cast<long>(lds_factor_doubles(KCAP, RPL)) : 0);
 }
 
+// The same slab size for the rerun (global-memory) instantiation of (ncpl, rpl), at run time.
+constexpr long rerun_slab_doubles(int ncpl, int rpl) {
+    const int kcap = ncpl * WAVE;
+    const int rplq = (rpl * WAVE + kcap + WAVE - 1) / WAVE;
+    return static_cast<long>(rplq * WAVE) * (kcap + 1) + static_cast<long>(lds_factor_doubles(kcap, rpl));
+}
+
+// Scratch of the global-memory rerun launches of one kind of solve.  Either the first launch
+// that needs it allocates it (stream-ordered; the caller frees it) and later launches reuse it,
+// or the caller hands in a fixed region (fixed = true) whose size sets the rerun's wave count.
+struct RerunScratch {
+    void* base = nullptr;
+    size_t bytes = 0;
+    bool fixed = false;
+};
 // Opt-in per-launch timing: events recorded on the launch stream around each kernel, and for
 // the kernels that count their algorithmic work, a device counter slot per launch.
 struct ProfiledLaunch {
@@ -618,23 +633,46 @@ inline int launch_fill(const char* name, Kern kern, const Args& args, int waves,
 }
 
 // The fast launch (LDS factor) of a kernel family, then its rerun of the overflowed voxels with
-// the factor in global memory, sized for every column (KG = NCPL * 64 >= n).  The rerun's
-// scratch is allocated into *rerun_scratch when that is null, and otherwise reused: a caller
-// that launches the same instantiation repeatedly (the low-resolution rounds) allocates once.
+// the factor in global memory, sized for every column (KG = NCPL * 64 >= n).  The rerun runs on
+// `want` waves of scratch slabs from rs: allocated into rs when it is empty (a caller that
+// launches the same instantiation repeatedly, the low-resolution rounds, allocates once), or as
+// many whole blocks of slabs as a fixed region holds.
+template <template <int, int, int, bool> class Family, int NCPL, int RPL, class Args>
+int rerun_waves_in(RerunScratch* rs, int want, hipStream_t stream, int* waves) {
+    const size_t per = sizeof(double) * static_cast<size_t>(slab_doubles<RPL, NCPL * WAVE>(true));
+    if (rs->fixed) {
+        const size_t fit = rs->bytes / per / WAVES_PER_BLOCK * WAVES_PER_BLOCK;
+        if (fit < static_cast<size_t>(WAVES_PER_BLOCK)) {
+            return fail("the workspace left for the global-memory solves holds " +
+                        std::to_string(rs->bytes) + " bytes, less than one block of " +
+                        std::to_string(WAVES_PER_BLOCK) + " slabs of " + std::to_string(per) +
+                        " bytes; pass fewer voxels per call");
+        }
+        *waves = static_cast<int>(std::min<size_t>(fit, static_cast<size_t>(want)));
+        return 0;
+    }
+    if (rs->base == nullptr) {
+        const int rc = device_alloc(&rs->base, per * static_cast<size_t>(want), stream);
+        if (rc) return rc;
+        rs->bytes = per * static_cast<size_t>(want);
+    }
+    *waves = static_cast<int>(std::min<size_t>(rs->bytes / per, static_cast<size_t>(want)) /
+                              WAVES_PER_BLOCK * WAVES_PER_BLOCK);
+    return 0;
+}
+
 template <template <int, int, int, bool> class Family, int NCPL, int RPL, class Args>
 int run_pair(const char* name, const char* rerun_name, const Args& fast, Args rerun, int waves,
-             int* d_overflow_list, int* d_overflow_count, hipStream_t stream,
-             void** rerun_scratch) {
+             int* d_overflow_list, int* d_overflow_count, hipStream_t stream, RerunScratch* rs) {
     constexpr int KG = NCPL * WAVE;
     const long per = slab_doubles<RPL, KG>(true);
     if (fast.design.normal_equations == 0) {
         // QR fits throughout: every voxel takes the global-memory kernel.
-        const int all_waves = std::min(waves, RERUN_WAVES_ALL);
-        if (*rerun_scratch == nullptr) {
-            int rc = device_alloc(rerun_scratch, sizeof(double) * per * all_waves, stream);
-            if (rc) return rc;
-        }
-        rerun.scratch = SolveScratch{static_cast<double*>(*rerun_scratch), per};
+        int all_waves = 0;
+        int rc = rerun_waves_in<Family, NCPL, RPL, Args>(rs, std::min(waves, RERUN_WAVES_ALL),
+                                                          stream, &all_waves);
+        if (rc) return rc;
+        rerun.scratch = SolveScratch{static_cast<double*>(rs->base), per};
         rerun.vs.overflow_list = nullptr;
         rerun.vs.overflow_count = nullptr;
         rerun.corr = nullptr;
@@ -645,17 +683,16 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
     const size_t lds = sizeof(double) * lds_factor_doubles(KC, RPL) * WAVES_PER_BLOCK;
     int rc = launch_fill(name, Family<NCPL, RPL, KC, false>::kernel, fast, waves, lds, stream);
     if (rc) return rc;
-    if (*rerun_scratch == nullptr) {
-        rc = device_alloc(rerun_scratch, sizeof(double) * per * RERUN_WAVES, stream);
-        if (rc) return rc;
-    }
-    rerun.scratch = SolveScratch{static_cast<double*>(*rerun_scratch), per};
+    int rerun_waves = 0;
+    rc = rerun_waves_in<Family, NCPL, RPL, Args>(rs, RERUN_WAVES, stream, &rerun_waves);
+    if (rc) return rc;
+    rerun.scratch = SolveScratch{static_cast<double*>(rs->base), per};
     rerun.vs.list = d_overflow_list;
     rerun.vs.list_count = d_overflow_count;
     rerun.vs.overflow_list = nullptr;
     rerun.vs.overflow_count = nullptr;
     rerun.corr = nullptr; // indexed by the fast launch's list positions
-    return launch(rerun_name, Family<NCPL, RPL, KG, true>::kernel, rerun, RERUN_WAVES, 0, stream);
+    return launch(rerun_name, Family<NCPL, RPL, KG, true>::kernel, rerun, rerun_waves, 0, stream);
 }
 
 // Launchers of one (NCPL, RPL) instantiation, defined in its own translation unit.
@@ -663,13 +700,13 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
 #define LSD_LAUNCHER_DECL(NC, RP)                                                              \
     int launch_fod_##NC##_##RP(const char* name, const char* rerun_name, const FodArgs& fast,   \
                                const FodArgs& rerun, int waves, int* ol, int* oc,               \
-                               hipStream_t stream, void** scratch);                            \
+                               hipStream_t stream, RerunScratch* scratch);                     \
     int launch_fraction_##NC##_##RP(const char* name, const FractionArgs& args, int waves,      \
                                     hipStream_t stream);
 #define LSD_LAUNCHER_DEF(NC, RP)                                                               \
     int launch_fod_##NC##_##RP(const char* name, const char* rerun_name, const FodArgs& fast,   \
                                const FodArgs& rerun, int waves, int* ol, int* oc,               \
-                               hipStream_t stream, void** scratch) {                           \
+                               hipStream_t stream, RerunScratch* scratch) {                    \
         return run_pair<fod_family, NC, RP>(name, rerun_name, fast, rerun, waves, ol, oc,       \
                                             stream, scratch);                                  \
     }                                                                                          \

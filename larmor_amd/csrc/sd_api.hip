@@ -64,7 +64,7 @@ int rpl_for(int64_t m) {
 // The launchers of an (NCPL, RPL) instantiation.
 int dispatch_fod(const char* name, const char* rerun_name, int ncpl, int rpl, const FodArgs& fast,
                  const FodArgs& rerun, int waves, int* ol, int* oc, hipStream_t stream,
-                 void** rerun_scratch) {
+                 RerunScratch* rerun_scratch) {
 #define LSD_TRY(NC, RP)                                                                        \
     if (ncpl == NC && rpl == RP)                                                               \
         return launch_fod_##NC##_##RP(name, rerun_name, fast, rerun, waves, ol, oc, stream,     \
@@ -85,33 +85,67 @@ int dispatch_fraction(const char* name, int ncpl, int rpl, const FractionArgs& a
                 std::to_string(rpl) + " row slots");
 }
 
-// Stream-ordered temporaries of one call, freed (stream-ordered) when the call returns.
-class StreamAllocs {
+// The device memory of one call.  By default each buffer is a stream-ordered allocation, freed
+// (stream-ordered) when the call returns.  Given a fixed region instead (the allocation-free
+// entry point), buffers are carved from it in order, 256-byte aligned, and a request beyond its
+// end fails; what is left after the last buffer serves the rerun scratch.
+class Workspace {
 public:
-    explicit StreamAllocs(hipStream_t stream) : stream_(stream) {}
-    StreamAllocs(const StreamAllocs&) = delete;
-    StreamAllocs& operator=(const StreamAllocs&) = delete;
-    ~StreamAllocs() {
+    explicit Workspace(hipStream_t stream) : stream_(stream) {}
+    Workspace(void* base, size_t bytes, hipStream_t stream)
+        : stream_(stream), base_(static_cast<char*>(base)), capacity_(bytes), fixed_(true) {}
+    Workspace(const Workspace&) = delete;
+    Workspace& operator=(const Workspace&) = delete;
+    ~Workspace() {
         for (void* p : held_) {
             (void)hipFreeAsync(p, stream_);
         }
     }
     template <class T>
     int get(T** out, long count) {
+        const size_t bytes = sizeof(T) * static_cast<size_t>(std::max(1L, count));
+        if (fixed_) {
+            const size_t at = align(used_);
+            if (at > capacity_ || bytes > capacity_ - at) {
+                return fail("the fit needs more than the " + std::to_string(capacity_) +
+                            " bytes of the allocation-free workspace; pass fewer voxels per call");
+            }
+            *out = reinterpret_cast<T*>(base_ + at);
+            used_ = at + bytes;
+            return 0;
+        }
         void* p = nullptr;
-        const int rc = device_alloc(&p, sizeof(T) * static_cast<size_t>(std::max(1L, count)),
-                                    stream_);
+        const int rc = device_alloc(&p, bytes, stream_);
         if (rc) return rc;
         held_.push_back(p);
         *out = static_cast<T*>(p);
         return 0;
     }
+    // Frees p with the call's own buffers (the rerun scratch allocated by a launcher).
     void adopt(void* p) {
-        if (p) held_.push_back(p);
+        if (p && !fixed_) held_.push_back(p);
+    }
+    bool fixed() const { return fixed_; }
+    // The region after the last buffer, for the rerun scratch (empty, to be allocated by the
+    // launcher, when the workspace is not fixed).
+    RerunScratch rest() const {
+        RerunScratch r;
+        if (fixed_) {
+            const size_t at = align(used_);
+            r.base = base_ + std::min(at, capacity_);
+            r.bytes = at < capacity_ ? capacity_ - at : 0;
+            r.fixed = true;
+        }
+        return r;
     }
 
 private:
+    static size_t align(size_t at) { return (at + 255) & ~static_cast<size_t>(255); }
     hipStream_t stream_;
+    char* base_ = nullptr;
+    size_t capacity_ = 0;
+    size_t used_ = 0;
+    bool fixed_ = false;
     std::vector<void*> held_;
 };
 
@@ -171,9 +205,9 @@ FractionKernelDev fractions_of(const lsd_tissue_kernel* k) {
 // Grid of the main launch: enough waves to fill the chip, never more than voxels.
 int main_waves(int64_t n_vox) {
     int dev = 0;
-    hipGetDevice(&dev);
+    (void)hipGetDevice(&dev);
     int cus = 0;
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (cus <= 0) {
         cus = 256;
     }
@@ -304,7 +338,7 @@ struct DrlBuffers {
     double* pred = nullptr;
 };
 
-int drl_buffers(const lsd_solver* solver, long ld, StreamAllocs& mem, DrlBuffers& b) {
+int drl_buffers(const lsd_solver* solver, long ld, Workspace& mem, DrlBuffers& b) {
     b.ld = ld;
     const long cl = static_cast<long>(solver->n) * ld;
     int rc = mem.get(&b.a, static_cast<long>(solver->m) * ld);
@@ -395,6 +429,284 @@ struct lsd_csd {
     double* d_constraint_t = nullptr; // (c, k)
     double* d_gram = nullptr;         // (c, c)
 };
+
+// The multi-tissue fit with its per-call buffers taken from mem.  With a fixed workspace the
+// high-resolution correlations are computed inside the FOD kernel (the same in-order sums) rather
+// than held per voxel, and one rerun region after the per-voxel state serves the low- and
+// high-resolution solves in turn (they run one after the other on the stream).
+static int fit_multi_tissue_in(Workspace& mem, const double* signals, int64_t n_voxels, int64_t m,
+                               const double* weights, const lsd_tissue_kernel* lr,
+                               const lsd_solver* lr_solver, const lsd_tissue_kernel* hr,
+                               const lsd_solver* hr_solver, int64_t max_rounds,
+                               double fraction_tol, double final_tol, double* fods,
+                               double* fractions, int32_t* rounds, int32_t* status,
+                               hipStream_t stream) {
+    if (m != hr->m || lr->m != m) {
+        return fail("signals " + shape_tuple(n_voxels, m) + ", weights " + std::to_string(m) +
+                    " and the kernels' " + std::to_string(hr->m) + " measurements do not match");
+    }
+    if (lr->ci != hr->ci) {
+        return fail("the two kernels have different numbers of isotropic tissues");
+    }
+    auto matching = [&](const char* name, const lsd_tissue_kernel* k, const lsd_solver* s) {
+        if (s->n != k->ca || s->m != m) {
+            return fail(std::string("the ") + name + " solver (" + std::to_string(s->n) +
+                        " directions, " + std::to_string(s->m) +
+                        " measurements) does not match its kernel (" + std::to_string(k->ca) +
+                        " directions, " + std::to_string(m) + " measurements)");
+        }
+        return 0;
+    };
+    int rc = matching("low-resolution", lr, lr_solver);
+    if (!rc) rc = matching("high-resolution", hr, hr_solver);
+    if (rc) return rc;
+    if (max_rounds < 0) {
+        return fail("max_outer must be at least 0");
+    }
+    const int ncpl_lr = ncpl_for(lr->ca);
+    const int ncpl_hr = ncpl_for(hr->ca);
+    const int rpl = rpl_for(m);
+    if (ncpl_lr < 0 || ncpl_hr < 0 || rpl < 0) {
+        return fail("problem size exceeds 512 measurements or 640 directions");
+    }
+    if (n_voxels == 0) {
+        return 0;
+    }
+    const long nv = static_cast<long>(n_voxels);
+    const int ci = static_cast<int>(lr->ci);
+    const int nlf = 1 + ci;
+    const int waves = main_waves(n_voxels);
+    // counters: per round its overflow and next-list counts, then the final solve's overflow.
+    const long n_counters = 2 * (max_rounds + 1);
+    int* counters = nullptr;
+    int* olist = nullptr;
+    int* lists = nullptr;
+    double* scratch = nullptr;
+    double* iso = nullptr;
+    double* prev = nullptr;
+    double* x_lr = nullptr;
+    double* corr_lr = nullptr;
+    rc = mem.get(&counters, n_counters);
+    if (!rc) rc = mem.get(&olist, nv);
+    if (!rc) rc = mem.get(&lists, 2 * nv);
+    if (!rc) rc = mem.get(&scratch, 1);
+    if (!rc) rc = mem.get(&iso, std::max(1L, nv * ci));
+    if (!rc) rc = mem.get(&prev, nv * nlf);
+    if (!rc) rc = mem.get(&x_lr, nv * lr->ca);
+    if (!rc && !lr_solver->drl && !lr_solver->lasso) rc = mem.get(&corr_lr, std::max(1L, nv * lr->ca));
+    if (!rc && fods == nullptr) rc = mem.get(&fods, nv * hr->ca);
+    if (!rc && rounds == nullptr) rc = mem.get(&rounds, nv);
+    if (!rc && status == nullptr) rc = mem.get(&status, nv);
+    if (rc) return rc;
+    if (mem.fixed()) {
+        // Refuse before any launch when the rest of the workspace holds less than one block of
+        // rerun slabs for either resolution's solve.
+        const size_t need = sizeof(double) * WAVES_PER_BLOCK *
+                            static_cast<size_t>(std::max(rerun_slab_doubles(ncpl_lr, rpl),
+                                                         rerun_slab_doubles(ncpl_hr, rpl)));
+        if (mem.rest().bytes < need) {
+            return fail("the fit needs more than the " + std::to_string(mem.rest().bytes) +
+                        " bytes left of the allocation-free workspace for its rerun slabs (" +
+                        std::to_string(need) + "); pass fewer voxels per call");
+        }
+    }
+    hipLaunchKernelGGL(zero_ints, dim3((n_counters + 255) / 256), dim3(256), 0, stream, counters,
+                       static_cast<int>(n_counters));
+    hipLaunchKernelGGL(init_loop_state, dim3((nv * nlf + 255) / 256), dim3(256), 0, stream, iso,
+                       nv * ci, prev, nv * nlf, rounds, status, nv);
+
+    FodArgs fod{};
+    fod.signals = signals;
+    fod.m = static_cast<int>(m);
+    fod.multi = 1;
+    fod.weights = weights;
+    fod.iso = iso;
+    fod.ci = ci;
+    fod.status = status;
+    fod.scratch = SolveScratch{scratch, 0};
+    FractionArgs fr{};
+    fr.signals = signals;
+    fr.m = static_cast<int>(m);
+    fr.fraction_tol = fraction_tol;
+    fr.final_tol = final_tol;
+    fr.iso = iso;
+    fr.prev = prev;
+    fr.rounds = rounds;
+    fr.status = status;
+    fr.n_vox = nv;
+
+    DrlBuffers drl_lr;
+    if (lr_solver->drl) {
+        rc = drl_buffers(lr_solver, nv, mem, drl_lr);
+        if (rc) return rc;
+    }
+
+    // The low-resolution rounds: round r fits the voxels of its list (all of them in round 0)
+    // and lists those that go on to round r + 1; lists and counts stay on the device.
+    const int* list = nullptr;
+    const int* count = nullptr;
+    // The rerun scratch: allocated by the first round's rerun and reused after (or the fixed
+    // workspace's rest, shared with the high-resolution solve).
+    RerunScratch lr_rerun_scratch;
+    for (int64_t r = 0; r < max_rounds && !rc; ++r) {
+        int* ovf_count = counters + 2 * r;
+        int* next_count = counters + 2 * r + 1;
+        int* next_list = lists + (r % 2) * nv;
+        fod.design = design_of(lr_solver);
+        fod.tol = lr_solver->tol;
+        fod.w_iso = lr->d_w_iso;
+        fod.x_out = x_lr;
+        fod.vs = VoxelSet{nv, list, count, olist, ovf_count};
+        if (lr_solver->drl) {
+            DrlArgs da{};
+            da.signals = signals;
+            da.m = static_cast<int>(m);
+            da.multi = 1;
+            da.weights = weights;
+            da.w_iso = lr->d_w_iso;
+            da.iso = iso;
+            da.ci = ci;
+            da.list = list;
+            da.list_count = count;
+            da.n_vox = nv;
+            da.x_out = x_lr;
+            da.status = status;
+            rc = run_drl("drl_lr", lr_solver, da, drl_lr, stream);
+        } else if (lr_solver->lasso) {
+            LassoArgs la{};
+            la.signals = signals;
+            la.m = static_cast<int>(m);
+            la.multi = 1;
+            la.weights = weights;
+            la.w_iso = lr->d_w_iso;
+            la.iso = iso;
+            la.ci = ci;
+            la.list = list;
+            la.list_count = count;
+            la.n_vox = nv;
+            la.x_out = x_lr;
+            la.status = status;
+            rc = run_lasso("lasso_lr", lr_solver, la, nv, stream);
+        } else {
+            CorrArgs ca{};
+            ca.signals = signals;
+            ca.m = static_cast<int>(m);
+            ca.multi = 1;
+            ca.weights = weights;
+            ca.w_iso = lr->d_w_iso;
+            ca.iso = iso;
+            ca.ci = ci;
+            ca.kernel = fod.design.kernel;
+            ca.n = fod.design.n;
+            ca.list = list;
+            ca.list_count = count;
+            ca.n_vox = nv;
+            ca.corr = corr_lr;
+            rc = launch_fill("fod_lr_corr", correlation_kernel<CORR_COLUMNS>, ca,
+                             static_cast<int>((nv + WAVE - 1) / WAVE), 0, stream);
+            if (rc) break;
+            fod.corr = corr_lr;
+            if (lr_rerun_scratch.base == nullptr && !lr_rerun_scratch.fixed) {
+                lr_rerun_scratch = mem.rest();
+            }
+            const bool first = lr_rerun_scratch.base == nullptr;
+            rc = dispatch_fod("fod_lr", "fod_lr_rerun", ncpl_lr, rpl, fod, fod, waves, olist,
+                              ovf_count, stream, &lr_rerun_scratch);
+            fod.corr = nullptr;
+            if (first) {
+                mem.adopt(lr_rerun_scratch.base);
+            }
+        }
+        if (rc) break;
+        fr.x = x_lr;
+        fr.kernel = fractions_of(lr);
+        fr.final_fit = 0;
+        fr.list = list;
+        fr.list_count = count;
+        fr.next_list = next_list;
+        fr.next_count = next_count;
+        rc = dispatch_fraction("fraction_loop", ncpl_lr, rpl, fr, waves, stream);
+        list = next_list;
+        count = next_count;
+    }
+    if (!rc) {
+        // The high-resolution FOD of every voxel, then its final fractions.
+        if (hr_solver->drl) {
+            DrlBuffers drl_hr;
+            rc = drl_buffers(hr_solver, nv, mem, drl_hr);
+            DrlArgs da{};
+            da.signals = signals;
+            da.m = static_cast<int>(m);
+            da.multi = 1;
+            da.weights = weights;
+            da.w_iso = hr->d_w_iso;
+            da.iso = iso;
+            da.ci = ci;
+            da.n_vox = nv;
+            da.x_out = fods;
+            da.status = status;
+            if (!rc) rc = run_drl("drl_hr", hr_solver, da, drl_hr, stream);
+        } else if (hr_solver->lasso) {
+            LassoArgs la{};
+            la.signals = signals;
+            la.m = static_cast<int>(m);
+            la.multi = 1;
+            la.weights = weights;
+            la.w_iso = hr->d_w_iso;
+            la.iso = iso;
+            la.ci = ci;
+            la.n_vox = nv;
+            la.x_out = fods;
+            la.status = status;
+            rc = run_lasso("lasso_hr", hr_solver, la, nv, stream);
+        } else {
+            int* ovf_count = counters + 2 * max_rounds;
+            fod.design = design_of(hr_solver);
+            fod.tol = hr_solver->tol;
+            fod.w_iso = hr->d_w_iso;
+            fod.x_out = fods;
+            fod.vs = VoxelSet{nv, nullptr, nullptr, olist, ovf_count};
+            // The correlations of every voxel by the lane-per-voxel pass, as in the rounds (a fixed
+            // workspace leaves them to the FOD kernel).
+            double* corr_hr = nullptr;
+            if (!mem.fixed()) rc = mem.get(&corr_hr, std::max(1L, nv * fod.design.n));
+            if (!rc && corr_hr != nullptr) {
+                CorrArgs ca{};
+                ca.signals = signals;
+                ca.m = static_cast<int>(m);
+                ca.multi = 1;
+                ca.weights = weights;
+                ca.w_iso = hr->d_w_iso;
+                ca.iso = iso;
+                ca.ci = ci;
+                ca.kernel = fod.design.kernel;
+                ca.n = fod.design.n;
+                ca.n_vox = nv;
+                ca.corr = corr_hr;
+                rc = launch_fill("fod_hr_corr", correlation_kernel<CORR_COLUMNS>, ca,
+                                 static_cast<int>((nv + WAVE - 1) / WAVE), 0, stream);
+            }
+            if (!rc) {
+                fod.corr = corr_hr;
+                RerunScratch rerun_scratch = mem.rest();
+                rc = dispatch_fod("fod_hr", "fod_hr_rerun", ncpl_hr, rpl, fod, fod, waves, olist,
+                                  ovf_count, stream, &rerun_scratch);
+                fod.corr = nullptr;
+                mem.adopt(rerun_scratch.base);
+            }
+        }
+    }
+    if (!rc) {
+        fr.x = fods;
+        fr.kernel = fractions_of(hr);
+        fr.final_fit = 1;
+        fr.fractions = fractions;
+        fr.list = nullptr;
+        fr.list_count = nullptr;
+        rc = dispatch_fraction("fraction_final", ncpl_hr, rpl, fr, waves, stream);
+    }
+    return rc;
+}
 
 extern "C" {
 
@@ -746,7 +1058,7 @@ int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
         if (n_voxels == 0) {
             return 0;
         }
-        StreamAllocs mem(stream);
+        Workspace mem(stream);
         int rc = 0;
         if (status == nullptr) rc = mem.get(&status, n_voxels);
         if (rc) return rc;
@@ -768,7 +1080,7 @@ int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
         if (n_voxels == 0) {
             return 0;
         }
-        StreamAllocs mem(stream);
+        Workspace mem(stream);
         int rc = 0;
         if (status == nullptr) rc = mem.get(&status, n_voxels);
         if (rc) return rc;
@@ -792,7 +1104,7 @@ int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
     if (n_voxels == 0) {
         return 0;
     }
-    StreamAllocs mem(stream);
+    Workspace mem(stream);
     int* counters = nullptr;
     int* olist = nullptr;
     double* scratch = nullptr;
@@ -835,10 +1147,10 @@ int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
     }
     FodArgs rerun = fa;
     rerun.corr = nullptr;
-    void* rerun_scratch = nullptr;
+    RerunScratch rerun_scratch = mem.rest();
     rc = dispatch_fod("fod_single", "fod_single_rerun", ncpl, rpl, fa, rerun, main_waves(n_voxels),
                       olist, counters, stream, &rerun_scratch);
-    mem.adopt(rerun_scratch);
+    mem.adopt(rerun_scratch.base);
     return rc;
 }
 
@@ -848,254 +1160,24 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
                          int64_t max_rounds, double fraction_tol, double final_tol, double* fods,
                          double* fractions, int32_t* rounds, int32_t* status, void* stream_) {
     auto stream = static_cast<hipStream_t>(stream_);
-    if (m != hr->m || lr->m != m) {
-        return fail("signals " + shape_tuple(n_voxels, m) + ", weights " + std::to_string(m) +
-                    " and the kernels' " + std::to_string(hr->m) + " measurements do not match");
-    }
-    if (lr->ci != hr->ci) {
-        return fail("the two kernels have different numbers of isotropic tissues");
-    }
-    auto matching = [&](const char* name, const lsd_tissue_kernel* k, const lsd_solver* s) {
-        if (s->n != k->ca || s->m != m) {
-            return fail(std::string("the ") + name + " solver (" + std::to_string(s->n) +
-                        " directions, " + std::to_string(s->m) +
-                        " measurements) does not match its kernel (" + std::to_string(k->ca) +
-                        " directions, " + std::to_string(m) + " measurements)");
-        }
-        return 0;
-    };
-    int rc = matching("low-resolution", lr, lr_solver);
-    if (!rc) rc = matching("high-resolution", hr, hr_solver);
-    if (rc) return rc;
-    if (max_rounds < 0) {
-        return fail("max_outer must be at least 0");
-    }
-    const int ncpl_lr = ncpl_for(lr->ca);
-    const int ncpl_hr = ncpl_for(hr->ca);
+    Workspace mem(stream);
+    return fit_multi_tissue_in(mem, signals, n_voxels, m, weights, lr, lr_solver, hr, hr_solver,
+                               max_rounds, fraction_tol, final_tol, fods, fractions, rounds,
+                               status, stream);
+}
+
+int64_t lsd_fit_scratch_bytes(int64_t m, int64_t n_lr, int64_t n_hr) {
     const int rpl = rpl_for(m);
-    if (ncpl_lr < 0 || ncpl_hr < 0 || rpl < 0) {
-        return fail("problem size exceeds 512 measurements or 640 directions");
+    const int64_t sizes[2] = {n_lr, n_hr};
+    int64_t total = 0;
+    for (const int64_t n : sizes) {
+        if (n <= 0) continue;
+        const int ncpl = ncpl_for(n);
+        if (ncpl < 0 || rpl < 0) return -1;
+        total += static_cast<int64_t>(sizeof(double)) * rerun_slab_doubles(ncpl, rpl) *
+                 std::max(RERUN_WAVES, RERUN_WAVES_ALL);
     }
-    if (n_voxels == 0) {
-        return 0;
-    }
-    const long nv = static_cast<long>(n_voxels);
-    const int ci = static_cast<int>(lr->ci);
-    const int nlf = 1 + ci;
-    const int waves = main_waves(n_voxels);
-    StreamAllocs mem(stream);
-    // counters: per round its overflow and next-list counts, then the final solve's overflow.
-    const long n_counters = 2 * (max_rounds + 1);
-    int* counters = nullptr;
-    int* olist = nullptr;
-    int* lists = nullptr;
-    double* scratch = nullptr;
-    double* iso = nullptr;
-    double* prev = nullptr;
-    double* x_lr = nullptr;
-    double* corr_lr = nullptr;
-    rc = mem.get(&counters, n_counters);
-    if (!rc) rc = mem.get(&olist, nv);
-    if (!rc) rc = mem.get(&lists, 2 * nv);
-    if (!rc) rc = mem.get(&scratch, 1);
-    if (!rc) rc = mem.get(&iso, std::max(1L, nv * ci));
-    if (!rc) rc = mem.get(&prev, nv * nlf);
-    if (!rc) rc = mem.get(&x_lr, nv * lr->ca);
-    if (!rc && !lr_solver->drl && !lr_solver->lasso) rc = mem.get(&corr_lr, std::max(1L, nv * lr->ca));
-    if (!rc && fods == nullptr) rc = mem.get(&fods, nv * hr->ca);
-    if (!rc && rounds == nullptr) rc = mem.get(&rounds, nv);
-    if (!rc && status == nullptr) rc = mem.get(&status, nv);
-    if (rc) return rc;
-    hipLaunchKernelGGL(zero_ints, dim3((n_counters + 255) / 256), dim3(256), 0, stream, counters,
-                       static_cast<int>(n_counters));
-    hipLaunchKernelGGL(init_loop_state, dim3((nv * nlf + 255) / 256), dim3(256), 0, stream, iso,
-                       nv * ci, prev, nv * nlf, rounds, status, nv);
-
-    FodArgs fod{};
-    fod.signals = signals;
-    fod.m = static_cast<int>(m);
-    fod.multi = 1;
-    fod.weights = weights;
-    fod.iso = iso;
-    fod.ci = ci;
-    fod.status = status;
-    fod.scratch = SolveScratch{scratch, 0};
-    FractionArgs fr{};
-    fr.signals = signals;
-    fr.m = static_cast<int>(m);
-    fr.fraction_tol = fraction_tol;
-    fr.final_tol = final_tol;
-    fr.iso = iso;
-    fr.prev = prev;
-    fr.rounds = rounds;
-    fr.status = status;
-    fr.n_vox = nv;
-
-    DrlBuffers drl_lr;
-    if (lr_solver->drl) {
-        rc = drl_buffers(lr_solver, nv, mem, drl_lr);
-        if (rc) return rc;
-    }
-
-    // The low-resolution rounds: round r fits the voxels of its list (all of them in round 0)
-    // and lists those that go on to round r + 1; lists and counts stay on the device.
-    const int* list = nullptr;
-    const int* count = nullptr;
-    void* lr_rerun_scratch = nullptr; // allocated by the first round's rerun, reused after
-    for (int64_t r = 0; r < max_rounds && !rc; ++r) {
-        int* ovf_count = counters + 2 * r;
-        int* next_count = counters + 2 * r + 1;
-        int* next_list = lists + (r % 2) * nv;
-        fod.design = design_of(lr_solver);
-        fod.tol = lr_solver->tol;
-        fod.w_iso = lr->d_w_iso;
-        fod.x_out = x_lr;
-        fod.vs = VoxelSet{nv, list, count, olist, ovf_count};
-        if (lr_solver->drl) {
-            DrlArgs da{};
-            da.signals = signals;
-            da.m = static_cast<int>(m);
-            da.multi = 1;
-            da.weights = weights;
-            da.w_iso = lr->d_w_iso;
-            da.iso = iso;
-            da.ci = ci;
-            da.list = list;
-            da.list_count = count;
-            da.n_vox = nv;
-            da.x_out = x_lr;
-            da.status = status;
-            rc = run_drl("drl_lr", lr_solver, da, drl_lr, stream);
-        } else if (lr_solver->lasso) {
-            LassoArgs la{};
-            la.signals = signals;
-            la.m = static_cast<int>(m);
-            la.multi = 1;
-            la.weights = weights;
-            la.w_iso = lr->d_w_iso;
-            la.iso = iso;
-            la.ci = ci;
-            la.list = list;
-            la.list_count = count;
-            la.n_vox = nv;
-            la.x_out = x_lr;
-            la.status = status;
-            rc = run_lasso("lasso_lr", lr_solver, la, nv, stream);
-        } else {
-            CorrArgs ca{};
-            ca.signals = signals;
-            ca.m = static_cast<int>(m);
-            ca.multi = 1;
-            ca.weights = weights;
-            ca.w_iso = lr->d_w_iso;
-            ca.iso = iso;
-            ca.ci = ci;
-            ca.kernel = fod.design.kernel;
-            ca.n = fod.design.n;
-            ca.list = list;
-            ca.list_count = count;
-            ca.n_vox = nv;
-            ca.corr = corr_lr;
-            rc = launch_fill("fod_lr_corr", correlation_kernel<CORR_COLUMNS>, ca,
-                             static_cast<int>((nv + WAVE - 1) / WAVE), 0, stream);
-            if (rc) break;
-            fod.corr = corr_lr;
-            const bool first = lr_rerun_scratch == nullptr;
-            rc = dispatch_fod("fod_lr", "fod_lr_rerun", ncpl_lr, rpl, fod, fod, waves, olist,
-                              ovf_count, stream, &lr_rerun_scratch);
-            fod.corr = nullptr;
-            if (first) {
-                mem.adopt(lr_rerun_scratch);
-            }
-        }
-        if (rc) break;
-        fr.x = x_lr;
-        fr.kernel = fractions_of(lr);
-        fr.final_fit = 0;
-        fr.list = list;
-        fr.list_count = count;
-        fr.next_list = next_list;
-        fr.next_count = next_count;
-        rc = dispatch_fraction("fraction_loop", ncpl_lr, rpl, fr, waves, stream);
-        list = next_list;
-        count = next_count;
-    }
-    if (!rc) {
-        // The high-resolution FOD of every voxel, then its final fractions.
-        if (hr_solver->drl) {
-            DrlBuffers drl_hr;
-            rc = drl_buffers(hr_solver, nv, mem, drl_hr);
-            DrlArgs da{};
-            da.signals = signals;
-            da.m = static_cast<int>(m);
-            da.multi = 1;
-            da.weights = weights;
-            da.w_iso = hr->d_w_iso;
-            da.iso = iso;
-            da.ci = ci;
-            da.n_vox = nv;
-            da.x_out = fods;
-            da.status = status;
-            if (!rc) rc = run_drl("drl_hr", hr_solver, da, drl_hr, stream);
-        } else if (hr_solver->lasso) {
-            LassoArgs la{};
-            la.signals = signals;
-            la.m = static_cast<int>(m);
-            la.multi = 1;
-            la.weights = weights;
-            la.w_iso = hr->d_w_iso;
-            la.iso = iso;
-            la.ci = ci;
-            la.n_vox = nv;
-            la.x_out = fods;
-            la.status = status;
-            rc = run_lasso("lasso_hr", hr_solver, la, nv, stream);
-        } else {
-            int* ovf_count = counters + 2 * max_rounds;
-            fod.design = design_of(hr_solver);
-            fod.tol = hr_solver->tol;
-            fod.w_iso = hr->d_w_iso;
-            fod.x_out = fods;
-            fod.vs = VoxelSet{nv, nullptr, nullptr, olist, ovf_count};
-            // The correlations of every voxel by the lane-per-voxel pass, as in the rounds.
-            double* corr_hr = nullptr;
-            rc = mem.get(&corr_hr, std::max(1L, nv * fod.design.n));
-            if (!rc) {
-                CorrArgs ca{};
-                ca.signals = signals;
-                ca.m = static_cast<int>(m);
-                ca.multi = 1;
-                ca.weights = weights;
-                ca.w_iso = hr->d_w_iso;
-                ca.iso = iso;
-                ca.ci = ci;
-                ca.kernel = fod.design.kernel;
-                ca.n = fod.design.n;
-                ca.n_vox = nv;
-                ca.corr = corr_hr;
-                rc = launch_fill("fod_hr_corr", correlation_kernel<CORR_COLUMNS>, ca,
-                                 static_cast<int>((nv + WAVE - 1) / WAVE), 0, stream);
-            }
-            if (!rc) {
-                fod.corr = corr_hr;
-                void* rerun_scratch = nullptr;
-                rc = dispatch_fod("fod_hr", "fod_hr_rerun", ncpl_hr, rpl, fod, fod, waves, olist,
-                                  ovf_count, stream, &rerun_scratch);
-                fod.corr = nullptr;
-                mem.adopt(rerun_scratch);
-            }
-        }
-    }
-    if (!rc) {
-        fr.x = fods;
-        fr.kernel = fractions_of(hr);
-        fr.final_fit = 1;
-        fr.fractions = fractions;
-        fr.list = nullptr;
-        fr.list_count = nullptr;
-        rc = dispatch_fraction("fraction_final", ncpl_hr, rpl, fr, waves, stream);
-    }
-    return rc;
+    return total;
 }
 
 int lsd_csd_create(const double* initial, int64_t c0, int64_t m, const double* forward,
@@ -1281,7 +1363,7 @@ int lsd_find_peaks(const double* coeffs, int64_t n_voxels, int64_t n_coeffs, int
     ShConstantsDev k{};
     rc = sh_constants(&k);
     if (rc) return rc;
-    StreamAllocs mem(stream);
+    Workspace mem(stream);
     int* counter = overflow;
     if (!counter) {
         rc = mem.get(&counter, 1);
@@ -1407,7 +1489,7 @@ static int fit_lsi_impl(const double* signals, int64_t n_voxels, int64_t m, cons
     }
     const int waves = main_waves(n_voxels);
     const int qr_waves = std::min<int>(LSI_QR_WAVES, waves);
-    StreamAllocs mem(stream);
+    Workspace mem(stream);
     int* counter = nullptr;
     int* list = nullptr;
     double* factor = nullptr;

@@ -37,12 +37,12 @@ import collections
 import queue
 import threading
 import time
-from concurrent.futures import Future
+from concurrent.futures import Future, InvalidStateError
 from dataclasses import dataclass, field, replace
 
 import numpy as np
 
-from larmor_amd import _core
+from larmor_amd import _core, _lib
 from larmor_amd.sd.kernel import Kernel
 from larmor_amd.sd.model import (SDModel, _to_sh, compute_max_lmax, compute_shell_weights,
                                   low_resolution_dirs)
@@ -53,9 +53,10 @@ from larmor_amd.sd.solvers import (DRL, FINAL_FRACTION_TOL_X, L2NNLS, MAX_ROUNDS
 from larmor_amd.sd.sphere import DIRS_300
 from larmor_amd.types import GradientTable
 
-# Device memory a fit needs beyond its inputs and outputs that no per-voxel term covers: the
-# rerun scratch of the voxels whose active set outgrows the LDS factor, counters, lists' slack.
-_FIXED_BYTES = 64 << 20
+# Device memory a fit needs beyond its inputs and outputs that no per-voxel term covers, besides
+# the global-memory rerun slabs the library reports (lsd_fit_scratch_bytes): counters and the
+# allocator's rounding.
+_FIXED_BYTES = 16 << 20
 
 
 def _torch():
@@ -91,7 +92,8 @@ def request_bytes(model: SDModel, n_voxels: int, n_meas: int, lmax: int,
     the rows come as float32), the fit's per-voxel state and outputs (the high-resolution
     amplitudes and correlations, the low-resolution FODs and correlations, fractions, rounds,
     status, voxel lists, and a damped Richardson-Lucy solver's position buffers) and the SH
-    coefficients, plus a fixed allowance."""
+    coefficients, plus the rerun slabs of the ridge NNLS solves (the library's figure for the
+    measurement count and direction counts) and a fixed allowance."""
     n_aniso = sum(1 for rf in model.rfs if not isinstance(rf, IsotropicRF))
     n_iso = len(model.rfs) - n_aniso
     n_hr = len(DIRS_300) * n_aniso
@@ -101,10 +103,27 @@ def request_bytes(model: SDModel, n_voxels: int, n_meas: int, lmax: int,
     per_voxel += 8 * (n_aniso * _n_sh(lmax) + len(model.rfs) + 2 * (1 + n_iso))
     per_voxel += 4 * 5  # rounds, status, the overflow list and the two voxel lists
     solver = model.solver if model.solver is not None else model.method.default_solver()
+    fixed = _FIXED_BYTES
     if isinstance(solver, DRL):
         # damped Richardson-Lucy keeps, per stage, the signal and four c-length vectors per voxel
         per_voxel += 8 * (2 * n_meas + 4 * (n_hr + n_lr))
-    return int(n_voxels) * per_voxel + _FIXED_BYTES
+    elif isinstance(solver, L2NNLS):
+        # -1 for sizes no kernel takes: such a request fails in the fit with the library's
+        # message, alone, like any request the kernels refuse.
+        fixed += max(0, _lib.load().lsd_fit_scratch_bytes(n_meas, n_lr, n_hr))
+    return int(n_voxels) * per_voxel + fixed
+
+
+def _settle(future: Future, result=None, exc: BaseException | None = None) -> None:
+    """Resolves a future with a result or an exception unless it is done already (its client
+    cancelled it while it queued, or an earlier error settled it); never raises."""
+    try:
+        if exc is not None:
+            future.set_exception(exc)
+        else:
+            future.set_result(result)
+    except InvalidStateError:
+        pass
 
 
 class _Design:
@@ -217,6 +236,7 @@ class FodServer:
         self._incoming: queue.SimpleQueue = queue.SimpleQueue()
         self._pending: collections.deque[_Request] = collections.deque()
         self._running: list[_Running] = []
+        self._completing: _Running | None = None  # the batch _complete is resolving
         self._designs: collections.OrderedDict[tuple, _Design] = collections.OrderedDict()
         self._design_lock = threading.Lock()
         self._running_bytes = 0  # held by running requests (scheduler thread only)
@@ -328,17 +348,21 @@ class FodServer:
                         time.sleep(self.poll_interval)
         except BaseException as exc:  # noqa: BLE001 - every waiting future must hear of it
             self._error = exc
-            for r in self._running:
-                r.request.future.set_exception(exc)
+            batches = list(self._running)
+            if self._completing is not None:
+                batches.append(self._completing)
+            for r in batches:
+                for req in [r.request, *(member for member, _, _ in r.members)]:
+                    _settle(req.future, exc=exc)
             for req in self._pending:
-                req.future.set_exception(exc)
+                _settle(req.future, exc=exc)
             while True:
                 try:
                     req = self._incoming.get_nowait()
                 except queue.Empty:
                     break
                 if req is not None:
-                    req.future.set_exception(exc)
+                    _settle(req.future, exc=exc)
 
     def _drain(self, block: bool = False) -> bool:
         """Moves arrived requests to the queue; returns True once close() was called."""
@@ -355,6 +379,7 @@ class FodServer:
     def _complete(self) -> bool:
         done = [r for r in self._running if r.event.query()]
         for r in done:
+            self._completing = r
             self._running.remove(r)
             finished = time.perf_counter()
             all_sh = [t.numpy() for t in r.sh_host]
@@ -372,7 +397,8 @@ class FodServer:
                                 finished=finished)
                 self._running_bytes -= req.nbytes
                 self._served.append(replace(fit, sh=[], fractions=None))  # timings only
-                req.future.set_result(fit)
+                _settle(req.future, fit)
+            self._completing = None
         return bool(done)
 
     def _design(self, key: tuple, bvals: np.ndarray, bvecs: np.ndarray, lmax: int) -> _Design:
@@ -403,7 +429,7 @@ class FodServer:
                 design = self._design(req.key, req.bvals, req.bvecs, req.lmax)
             except Exception as exc:  # a request the design cannot be built for
                 self._pending.popleft()
-                req.future.set_exception(exc)
+                _settle(req.future, exc=exc)
                 continue
             # Strict arrival order: the head waits for memory, later requests wait behind it.
             # A server with nothing running always admits the head (submit() refused any
@@ -412,6 +438,9 @@ class FodServer:
             if self._running and used + req.nbytes > self.hbm_budget:
                 break
             self._pending.popleft()
+            # From here the client can no longer cancel it (False: it was cancelled meanwhile).
+            if not req.future.set_running_or_notify_cancel():
+                continue
             batch = self._merge(req, used)
             stream = self._streams.pop()
             running = _Running(req, design, stream, torch.cuda.Event(), time.perf_counter())
@@ -425,7 +454,7 @@ class FodServer:
             except Exception as exc:  # invalid request caught by the C-ABI checks
                 self._streams.append(stream)
                 for member in batch:
-                    member.future.set_exception(exc)
+                    _settle(member.future, exc=exc)
                 continue
             self._running_bytes += sum(member.nbytes for member in batch)
             self._running.append(running)
@@ -449,12 +478,15 @@ class FodServer:
         while self._pending:
             other = self._pending.popleft()
             n = int(other.host.shape[0])
+            if other.future.cancelled():
+                continue
             if (other.key == head.key and other.host.dtype == head.host.dtype
-                    and not other.future.cancelled() and rows + n <= self.merge_voxels
+                    and rows + n <= self.merge_voxels
                     and used + nbytes + other.nbytes <= self.hbm_budget):
-                batch.append(other)
-                rows += n
-                nbytes += other.nbytes
+                if other.future.set_running_or_notify_cancel():
+                    batch.append(other)
+                    rows += n
+                    nbytes += other.nbytes
             else:
                 kept.append(other)
         self._pending = kept

@@ -105,6 +105,18 @@ def _load():
     return lib
 
 
+def _default_threads() -> int:
+    """The host cores this process may use (its affinity set, capped by OMP_NUM_THREADS)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        cores = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        cores = min(cores, int(share))
+    return max(1, cores)
+
+
 def _c(a, dtype=np.float64):
     return np.ascontiguousarray(a, dtype=dtype)
 
@@ -202,7 +214,7 @@ def fit_single_tissue(signals, kernel, ridge, tol, n_threads=None):
     s = _c(signals)
     k = _c(kernel)
     out = np.zeros((s.shape[0], k.shape[1]))
-    n_threads = n_threads or os.cpu_count() or 1
+    n_threads = n_threads or _default_threads()
     _check(_load().oracle_fit_single_tissue(_p(s), s.shape[0], s.shape[1], _p(k), k.shape[1],
                                             ridge, tol, n_threads, _p(out)))
     return out
@@ -225,7 +237,7 @@ def fit_multi_tissue(signals, weights, lr, hr, lr_ridge, hr_ridge, tol_x, max_ro
     fods = np.zeros((nv, ca_hr))
     fractions = np.zeros((nv, nf))
     tr = np.zeros((nv, 10), dtype=np.int64)
-    n_threads = n_threads or os.cpu_count() or 1
+    n_threads = n_threads or _default_threads()
     args = [_p(s), nv, m, _p(w)]
     for (ka, ki, wki, wka, b), ridge in zip(mats, (lr_ridge, hr_ridge)):
         args += [_p(ka), _p(ki), _p(wki), _p(wka), ka.shape[1], b.ctypes.data_as(_i64), len(b),
@@ -247,7 +259,7 @@ def fit_csd(signals, initial, forward, constraint, norm_weight, threshold, max_i
     solves = np.zeros(n, dtype=np.int32)
     _check(_load().oracle_fit_csd(
         _p(s), n, m, _p(ini), ini.shape[0], _p(fw), c, _p(con), con.shape[0], float(norm_weight),
-        float(threshold), int(max_iter), n_threads or os.cpu_count() or 1, _p(fods),
+        float(threshold), int(max_iter), n_threads or _default_threads(), _p(fods),
         conv.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
         solves.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
     if return_solves:
@@ -291,7 +303,7 @@ def peaks_chunk(coeffs, lmax, on_sphere, vertices, neighbours, n_peaks, relative
     _check(_load().oracle_peaks_chunk(
         _p(c), n, c.shape[1], int(lmax), int(full), _p(o), v.shape[0], _p(v),
         nb.ctypes.data_as(_i64), int(n_peaks), float(relative_threshold), float(threshold),
-        n_threads or os.cpu_count() or 1, _p(dirs), _p(amps)))
+        n_threads or _default_threads(), _p(dirs), _p(amps)))
     return dirs, amps
 
 
@@ -315,7 +327,7 @@ def fit_lsi(signals, a, g, ridge, n_threads=None, trace=False):
     conv = np.zeros(v, dtype=np.uint8)
     tr = np.zeros((v, 4), dtype=np.int64)
     _check(_load().oracle_fit_lsi(_p(s), v, _p(am), m, n, _p(gm), gm.shape[0], float(ridge),
-                                  n_threads or os.cpu_count() or 1, _p(x),
+                                  n_threads or _default_threads(), _p(x),
                                   conv.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                                   tr.ctypes.data_as(_i64)))
     if trace:
@@ -356,7 +368,7 @@ def fit_single_tissue_drl(signals, kernel_t, gram, nu, n_iter, init, n_threads=N
     c = kt.shape[0]
     out = np.zeros((nv, c))
     its = np.zeros(nv, dtype=np.int64)
-    n_threads = n_threads or os.cpu_count() or 1
+    n_threads = n_threads or _default_threads()
     _check(_load().oracle_fit_single_tissue_drl(_p(s), nv, m, _p(kt), c, _p(g), nu, n_iter, init,
                                                 n_threads, _p(out), its.ctypes.data_as(_i64)))
     return (out, its) if iterations else out
@@ -385,7 +397,7 @@ def fit_multi_tissue_drl(signals, weights, lr, hr, lr_rl, hr_rl, max_rounds=50,
     fods = np.zeros((nv, ca_hr))
     fractions = np.zeros((nv, nf))
     rd = np.zeros(nv, dtype=np.int64)
-    n_threads = n_threads or os.cpu_count() or 1
+    n_threads = n_threads or _default_threads()
     args += [ci, max_rounds, fraction_tol, final_tol, n_threads, _p(fods), _p(fractions),
              rd.ctypes.data_as(_i64)]
     _check(_load().oracle_fit_multi_tissue_drl(*args))
@@ -412,7 +424,7 @@ def fit_single_tissue_lasso(signals, kernel, reg, alpha, tol, max_iter, n_thread
     sw = np.zeros(nv, dtype=np.int64)
     _check(_load().oracle_fit_single_tissue_lasso(_p(s), nv, m, _p(k), c, float(reg), float(alpha),
                                                   float(tol), int(max_iter),
-                                                  n_threads or os.cpu_count() or 1, _p(out),
+                                                  n_threads or _default_threads(), _p(out),
                                                   sw.ctypes.data_as(_i64)))
     return (out, sw) if sweeps else out
 
@@ -439,7 +451,7 @@ def fit_multi_tissue_lasso(signals, weights, lr, hr, reg, alpha, tol, max_iter, 
     fractions = np.zeros((nv, nf))
     rd = np.zeros(nv, dtype=np.int64)
     args += [float(reg), float(alpha), float(tol), int(max_iter), ci, max_rounds, fraction_tol,
-             final_tol, n_threads or os.cpu_count() or 1, _p(fods), _p(fractions),
+             final_tol, n_threads or _default_threads(), _p(fods), _p(fractions),
              rd.ctypes.data_as(_i64)]
     _check(_load().oracle_fit_multi_tissue_lasso(*args))
     return (fods, fractions, rd) if rounds else (fods, fractions)

@@ -126,14 +126,17 @@ def test_budget_admits_one_at_a_time():
     jobs = _jobs(4, seed=51, lo=500, hi=3000)
     sizes = [request_bytes(model, len(j.signals), len(j.bvals), compute_max_lmax(j.bvals, j.lmax),
                            True) for j in jobs]
-    budget = 1.5 * max(sizes)  # any two requests exceed it
+    budget = 2 * min(sizes) - 1  # any two requests exceed it, each alone fits
+    assert budget >= max(sizes)
     server, results = _serve(model, jobs, slots=4, hbm_budget=budget)
     assert server.max_in_flight == 1
     assert server.peak_bytes <= budget
     for job, fit in zip(jobs, results):
         _check_same(fit, _alone(model, job))
-    with FodServer(model, hbm_budget=budget) as small:
-        big = synthetic.batched_jobs(1, seed=52, min_voxels=200_000, max_voxels=200_000)[0]
+    big = synthetic.batched_jobs(1, seed=52, min_voxels=200_000, max_voxels=200_000)[0]
+    need = request_bytes(model, len(big.signals), len(big.bvals),
+                         compute_max_lmax(big.bvals, model.lmax), True)
+    with FodServer(model, hbm_budget=need - 1) as small:
         with pytest.raises(ValueError, match="more than the server's budget"):
             small.submit(big.signals, GradientTable(big.bvals, big.bvecs))
 

@@ -108,5 +108,126 @@ def test_request_bytes_counts_the_drl_buffers():
     rfs = [AnisotropicRF(WM, 0.0), IsotropicRF(0.8e-3), IsotropicRF(3e-3)]
     grl = SDModel(rfs, method="GRL")
     mfod = SDModel(rfs, method="mFOD")
-    extra = (request_bytes(grl, 1000, 202, 8, False) - request_bytes(mfod, 1000, 202, 8, False))
-    assert extra == 1000 * 8 * (2 * 202 + 4 * (300 + 44))
+    def per_1000(model):
+        return request_bytes(model, 2000, 202, 8, False) - request_bytes(model, 1000, 202, 8, False)
+
+    assert per_1000(grl) - per_1000(mfod) == 1000 * 8 * (2 * 202 + 4 * (300 + 44))
+
+
+def test_request_bytes_covers_the_library_rerun_scratch():
+    """The admission estimate holds the rerun slabs the library allocates per fit (both
+    resolutions' global-memory slabs live at once)."""
+    from larmor_amd import _lib
+
+    lib = _lib.load()
+    scratch = lib.lsd_fit_scratch_bytes(202, 44, 300)
+    assert scratch > 2e9  # 1024 slabs of the 300-direction solve alone are ~2.4 GB
+    assert request_bytes(_model(), 0, 202, 8, False) >= scratch
+    assert request_bytes(_model(multi=False), 0, 202, 8, False) >= lib.lsd_fit_scratch_bytes(
+        202, 0, 300)
+    assert lib.lsd_fit_scratch_bytes(600, 44, 300) == -1
+
+
+class _Fut:
+    """Stand-ins for the scheduler's device objects."""
+
+
+def _stub_server(monkeypatch):
+    import collections
+    import queue
+    import threading
+
+    from larmor_amd.sd import serving
+
+    monkeypatch.setattr(serving, "_torch", lambda: _Fut())
+    srv = serving.FodServer.__new__(serving.FodServer)
+    srv._incoming = queue.SimpleQueue()
+    srv._pending = collections.deque()
+    srv._running = []
+    srv._completing = None
+    srv._streams = []
+    srv._closed = threading.Event()
+    srv._error = None
+    srv._served = []
+    srv._running_bytes = 0
+    srv._design_bytes = 0
+    srv.hbm_budget = 1 << 40
+    srv.merge_voxels = 1 << 20
+    srv.max_in_flight = 0
+    srv.max_merged = 1
+    srv.peak_bytes = 0
+    srv.poll_interval = 1e-4
+    return srv
+
+
+def _request(key=b"k", rows=4):
+    from concurrent.futures import Future
+
+    import torch
+
+    from larmor_amd.sd.serving import _Request
+
+    return _Request(torch.zeros((rows, 3)), np.zeros(3), np.zeros((3, 3)), 8, key, 1, Future(),
+                    0.0, n_rows=rows)
+
+
+def test_an_error_while_completing_fails_every_merged_member(monkeypatch):
+    """The scheduler's error handler settles every member of every batch, including the batch it
+    was completing when the error happened and the requests merged into a batch's head."""
+    from larmor_amd.sd.serving import _Running
+
+    srv = _stub_server(monkeypatch)
+
+    class Done:
+        def query(self):
+            return True
+
+    class Boom:
+        def numpy(self):
+            raise RuntimeError("device copy failed")
+
+    heads = [_request(), _request()]
+    members = [_request(), _request()]
+    batches = []
+    for head, member in zip(heads, members):
+        r = _Running(head, None, object(), Done(), 0.0, sh_host=[Boom()])
+        r.members = [(head, 0, 4), (member, 4, 8)]
+        batches.append(r)
+    srv._running = batches
+    srv._closed.set()
+    srv._incoming.put(None)
+    queued = _request()
+    srv._pending.append(queued)
+    srv._loop()
+    for req in heads + members + [queued]:
+        assert req.future.done()
+        with pytest.raises(RuntimeError, match="device copy failed"):
+            req.future.result()
+
+
+def test_cancelled_requests_are_skipped_and_running_ones_cannot_be_cancelled(monkeypatch):
+    from larmor_amd.sd.serving import _settle
+
+    srv = _stub_server(monkeypatch)
+    srv._streams = [object()]
+    srv._design = lambda key, bvals, bvecs, lmax: _Fut()
+    launched = []
+    srv._launch = lambda torch, r: launched.append(r)
+    cancelled_head, head, cancelled_member, member = (_request() for _ in range(4))
+    assert cancelled_head.future.cancel() and cancelled_member.future.cancel()
+    srv._pending.extend([cancelled_head, head, cancelled_member, member])
+
+    class Torch:
+        class cuda:
+            Event = object
+
+    assert srv._admit(Torch)
+    (batch,) = launched
+    assert [m for m, _, _ in batch.members] == [head, member]
+    assert head.future.running() and member.future.running()
+    assert not head.future.cancel()  # admitted: a client can no longer cancel it
+    # settling a cancelled or already settled future does not raise
+    _settle(cancelled_head.future, 1)
+    _settle(head.future, 2)
+    _settle(head.future, exc=RuntimeError("late"))
+    assert head.future.result() == 2
