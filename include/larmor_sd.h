@@ -152,6 +152,31 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m,
                          int32_t* status, void* stream);
 
 /*
+ * lsd_fit_multi_tissue with no device allocation: every per-call buffer (voxel lists, counters,
+ * the low-resolution FODs and correlations, the loop state, the overflow lists and the
+ * global-memory rerun slabs) is carved from a static device workspace of
+ * lsd_noalloc_workspace_bytes() (64 MiB), one per device, held until the process ends.  fods and
+ * fractions must be given; rounds and status may be NULL.  Ridge NNLS solvers only.  The
+ * high-resolution correlations are computed inside the FOD kernel (the same sums, so the results
+ * are bit-identical to lsd_fit_multi_tissue's), and the rerun slabs fill what the per-voxel state
+ * leaves.  A call whose buffers do not fit fails with LSD_ERR_INVALID ("... pass fewer voxels per
+ * call"): at the 202-measurement, 44/300-direction three-tissue shape, 16,384 voxels per call fit.
+ * The workspace keeps the last call's contents; calls on one device share it, so calls that
+ * overlap on one device (other streams, other threads) race: serialise them.
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ */
+int lsd_fit_multi_tissue_noalloc(const double* signals, int64_t n_voxels, int64_t m,
+                                 const double* weights, const lsd_tissue_kernel* lr,
+                                 const lsd_solver* lr_solver, const lsd_tissue_kernel* hr,
+                                 const lsd_solver* hr_solver, int64_t max_rounds,
+                                 double fraction_tol, double final_tol, double* fods,
+                                 double* fractions, int32_t* rounds, int32_t* status,
+                                 void* stream);
+/* The size of lsd_fit_multi_tissue_noalloc's per-device workspace in bytes. */
+int64_t lsd_noalloc_workspace_bytes(void);
+
+/*
  * The device bytes of the global-memory rerun slabs one fit allocates besides its per-voxel
  * buffers, for m measurements and ridge NNLS solves of n_lr (0 for none: a single-tissue fit) and
  * n_hr directions; -1 for sizes the kernels do not take.  Host-only (no device call).  No

@@ -62,6 +62,12 @@ SIGNATURES = {
         [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64,
          ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp, _vp],
     ),
+    "lsd_fit_multi_tissue_noalloc": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64,
+         ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp, _vp],
+    ),
+    "lsd_noalloc_workspace_bytes": (ctypes.c_int64, []),
     "lsd_fit_scratch_bytes": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
     "lsd_csd_create": (
         ctypes.c_int,

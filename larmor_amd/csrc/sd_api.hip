@@ -708,6 +708,30 @@ static int fit_multi_tissue_in(Workspace& mem, const double* signals, int64_t n_
     return rc;
 }
 
+// The workspace of lsd_fit_multi_tissue_noalloc: a static device array, so that the entry
+// point allocates nothing.  Every device the library runs on holds its own copy for the life of
+// the process; host code reaches it only through the per-device address hipGetSymbolAddress
+// returns (the array's name on the host is a stand-in, not a device address).
+constexpr size_t NOALLOC_WORKSPACE_BYTES = size_t{64} << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_WORKSPACE_BYTES];
+
+// The device address of `scratch` on the current device, looked up once per device.
+static int noalloc_workspace(void** out) {
+    static std::mutex mu;
+    static std::map<int, void*> per_device;
+    int dev = 0;
+    LSD_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = per_device.find(dev);
+    if (it == per_device.end()) {
+        void* ptr = nullptr;
+        LSD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        it = per_device.emplace(dev, ptr).first;
+    }
+    *out = it->second;
+    return 0;
+}
+
 extern "C" {
 
 const char* lsd_last_error(void) { return lsd::g_error.c_str(); }
@@ -1165,6 +1189,32 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
                                max_rounds, fraction_tol, final_tol, fods, fractions, rounds,
                                status, stream);
 }
+
+int lsd_fit_multi_tissue_noalloc(const double* signals, int64_t n_voxels, int64_t m,
+                                 const double* weights, const lsd_tissue_kernel* lr,
+                                 const lsd_solver* lr_solver, const lsd_tissue_kernel* hr,
+                                 const lsd_solver* hr_solver, int64_t max_rounds,
+                                 double fraction_tol, double final_tol, double* fods,
+                                 double* fractions, int32_t* rounds, int32_t* status,
+                                 void* stream_) {
+    auto stream = static_cast<hipStream_t>(stream_);
+    const auto plain = [](const lsd_solver* s) { return !s->drl && !s->lasso; };
+    if (!plain(lr_solver) || !plain(hr_solver)) {
+        return fail("the allocation-free fit takes ridge NNLS (l2nnls) solvers only");
+    }
+    if (n_voxels > 0 && (fods == nullptr || fractions == nullptr)) {
+        return fail("the allocation-free fit needs the fods and fractions outputs");
+    }
+    void* base = nullptr;
+    const int rc = noalloc_workspace(&base);
+    if (rc) return rc;
+    Workspace mem(base, NOALLOC_WORKSPACE_BYTES, stream);
+    return fit_multi_tissue_in(mem, signals, n_voxels, m, weights, lr, lr_solver, hr, hr_solver,
+                               max_rounds, fraction_tol, final_tol, fods, fractions, rounds,
+                               status, stream);
+}
+
+int64_t lsd_noalloc_workspace_bytes(void) { return static_cast<int64_t>(NOALLOC_WORKSPACE_BYTES); }
 
 int64_t lsd_fit_scratch_bytes(int64_t m, int64_t n_lr, int64_t n_hr) {
     const int rpl = rpl_for(m);
