@@ -152,6 +152,54 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m,
                          int32_t* status, void* stream);
 
 /*
+ * The correlations K^T a of each voxel, a = max(w s - W_iso p, 0) (all device: signals
+ * n_voxels x m, weights m, iso n_voxels x ci the isotropic fractions p, corr n_voxels x n), with
+ * K the ridge NNLS solver's kernel and W_iso the tissue kernel's weighted isotropic columns:
+ * precision 0 the FP64 pass of the fits (each an in-order sum over the measurements), 1 the
+ * mixed-precision form's BF16 matrix-core pass (n <= 64).
+ * Replaces: the K^T s of solve_ridge_nnls cpp/src/sd/nnls.cpp:302 with anisotropic_signal
+ *           cpp/src/sd/multi_tissue.cpp:20-24 (exposed on their own to test the two passes).
+ */
+int lsd_correlations(const double* signals, int64_t n_voxels, int64_t m, const double* weights,
+                     const lsd_tissue_kernel* kernel, const lsd_solver* solver, const double* iso,
+                     int precision, double* corr, void* stream);
+
+/*
+ * The mixed-precision form of lsd_fit_multi_tissue (same arguments and outputs, FP64 in and out).
+ * The low-resolution rounds run in two phases.  First, the correlations K^T a of each round come
+ * from the BF16 matrix cores: a and K split into BF16 halves (x = hi + lo), the products
+ * hi.hi + (lo.hi + hi.lo) accumulated in FP32 (one refinement step of the split product).  Then,
+ * from where that phase left every voxel, the rounds continue with the exact FP64 correlations
+ * until the fractions settle again (the iterative refinement of the loop).  In both phases a
+ * voxel's NNLS starts from the support of its previous solve, and the Cholesky factor and its
+ * updates, the gradients and the KKT test stay FP64.  The high-resolution solve uses FP64
+ * correlations.  Ridge NNLS solvers with a low-resolution design of at most 64 directions.  The
+ * results match the FP64 form to the loop tolerance, not to the bit; rounds counts the rounds of
+ * both phases.  No counterpart in the reference: the precision form of the benchmark.
+ */
+int lsd_fit_multi_tissue_mixed(const double* signals, int64_t n_voxels, int64_t m,
+                               const double* weights, const lsd_tissue_kernel* lr,
+                               const lsd_solver* lr_solver, const lsd_tissue_kernel* hr,
+                               const lsd_solver* hr_solver, int64_t max_rounds,
+                               double fraction_tol, double final_tol, double* fods,
+                               double* fractions, int32_t* rounds, int32_t* status, void* stream);
+
+/*
+ * The plain FP32 form of lsd_fit_multi_tissue: the same kernels and launch sequence with every
+ * value in single precision (signals, weights and outputs float32; the solvers' and tissue
+ * kernels' matrices are float32 copies of their FP64 ones, made at creation).  Ridge NNLS solvers
+ * only.  Its results are not the FP64 form's: they differ by FP32 rounding, and active sets at
+ * near-ties with it.  No counterpart in the reference (which computes in FP64): the baseline
+ * form of the precision comparison.
+ */
+int lsd_fit_multi_tissue_fp32(const float* signals, int64_t n_voxels, int64_t m,
+                              const float* weights, const lsd_tissue_kernel* lr,
+                              const lsd_solver* lr_solver, const lsd_tissue_kernel* hr,
+                              const lsd_solver* hr_solver, int64_t max_rounds, double fraction_tol,
+                              double final_tol, float* fods, float* fractions, int32_t* rounds,
+                              int32_t* status, void* stream);
+
+/*
  * lsd_fit_multi_tissue with no device allocation: every per-call buffer (voxel lists, counters,
  * the low-resolution FODs and correlations, the loop state, the overflow lists and the
  * global-memory rerun slabs) is carved from a static device workspace of

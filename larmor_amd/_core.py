@@ -257,12 +257,18 @@ def fit_multi_tissue(
     n_threads: int,
     *,
     return_diagnostics: bool = False,
+    form: str = "fp64",
 ):
     """Fits the FODs (n, ca_hr) and fractions (n, n_blocks + ci) of every row of signals.
 
-    Replaces `_core.fit_multi_tissue` (sd.cpp:279-330).  With return_diagnostics, also returns
-    the low-resolution rounds run per voxel and the status bits (include/larmor_sd.h).
+    Replaces `_core.fit_multi_tissue` (sd.cpp:310-358).  With return_diagnostics, also returns
+    the low-resolution rounds run per voxel and the status bits (include/larmor_sd.h).  form
+    "fp64" is the reference's computation (bit-identical to the oracle); "mixed" is the
+    mixed-precision form (lsd_fit_multi_tissue_mixed: BF16 matrix-core correlations, then FP64
+    refinement rounds, warm-started FP64 solves), which matches it to the loop tolerance.
     """
+    if form not in ("fp64", "mixed"):
+        raise ValueError(f"form must be 'fp64' or 'mixed', got {form!r}")
     torch = _torch()
     dev, host = _device_signals(signals, torch)
     _check_f64(weights, "weights")
@@ -281,8 +287,75 @@ def fit_multi_tissue(
     rounds = torch.zeros(n, dtype=torch.int32, device="cuda")
     status = torch.zeros(n, dtype=torch.int32, device="cuda")
     lib = _lib.load()
+    entry = lib.lsd_fit_multi_tissue if form == "fp64" else lib.lsd_fit_multi_tissue_mixed
     _lib.check(
-        lib.lsd_fit_multi_tissue(
+        entry(
+            dev.data_ptr(), n, m, w.data_ptr(), lr._handle, lr_solver._handle, hr._handle,
+            hr_solver._handle, int(max_outer), float(fraction_tol), float(final_tol_x),
+            fods.data_ptr(), fractions.data_ptr(), rounds.data_ptr(), status.data_ptr(),
+            _stream_ptr(torch),
+        )
+    )
+    out = (_out(fods, host), _out(fractions, host))
+    if return_diagnostics:
+        out += (_out(rounds, host), _out(status, host))
+    return out
+
+
+def correlations(signals, weights, kernel: DeconvKernel, solver: FodSolver, iso, *,
+                 precision: str = "fp64"):
+    """K^T a of every row of signals (n, m), a = max(w s - W_iso p, 0) with p the rows of iso
+    (n, ci): "fp64" the fits' FP64 pass, "bf16" the mixed-precision form's matrix-core pass.
+    Returns (n, n_dirs) float64 (numpy for numpy inputs)."""
+    torch = _torch()
+    if precision not in ("fp64", "bf16"):
+        raise ValueError(f"precision must be 'fp64' or 'bf16', got {precision!r}")
+    dev, host = _device_signals(signals, torch)
+    n, m = dev.shape
+    w = torch.as_tensor(np.ascontiguousarray(weights, dtype=np.float64)).to("cuda")
+    p = torch.as_tensor(np.ascontiguousarray(iso, dtype=np.float64)).to("cuda")
+    if p.shape != (n, kernel.n_isotropic) or w.numel() != m:
+        raise ValueError(f"iso must be ({n}, {kernel.n_isotropic}) and weights ({m},)")
+    out = torch.empty((n, solver.n_dirs), dtype=torch.float64, device="cuda")
+    _lib.check(_lib.load().lsd_correlations(
+        dev.data_ptr(), n, m, w.data_ptr(), kernel._handle, solver._handle, p.data_ptr(),
+        0 if precision == "fp64" else 1, out.data_ptr(), _stream_ptr(torch)))
+    return _out(out, host)
+
+
+def fit_multi_tissue_fp32(signals, weights, lr: DeconvKernel, lr_solver: FodSolver,
+                          hr: DeconvKernel, hr_solver: FodSolver, max_outer: int,
+                          fraction_tol: float, final_tol_x: float, n_threads: int, *,
+                          return_diagnostics: bool = False):
+    """The plain FP32 form of fit_multi_tissue: float32 signals (n, m) in, float32 FODs and
+    fractions out, every kernel computing in single precision (the solvers' and kernels' float32
+    copies).  Same arguments and checks as fit_multi_tissue otherwise."""
+    torch = _torch()
+    dtype = getattr(signals, "dtype", None)
+    if dtype not in (np.float32, torch.float32):
+        raise TypeError(f"signals must be a float32 array, got {dtype}")
+    if signals.ndim != 2:
+        raise ValueError(f"signals must be 2-D, got shape {tuple(signals.shape)}")
+    if not _is_c_contiguous(signals):
+        raise ValueError("the array must be C-contiguous")
+    host = isinstance(signals, np.ndarray) or signals.device.type != "cuda"
+    dev = torch.as_tensor(signals).to("cuda")
+    n, m = dev.shape
+    w = torch.as_tensor(np.ascontiguousarray(weights) if isinstance(weights, np.ndarray) else weights)
+    w = w.to(device="cuda", dtype=torch.float32).contiguous()
+    if w.numel() != m or hr.m != m or lr.m != m:
+        raise ValueError(
+            f"signals ({n}, {m}), weights {w.numel()} and the kernels' {hr.m} measurements "
+            "do not match"
+        )
+    _require_threads(n_threads)
+    nf = hr.n_blocks + hr.n_isotropic
+    fods = torch.empty((n, hr.n_anisotropic), dtype=torch.float32, device="cuda")
+    fractions = torch.empty((n, nf), dtype=torch.float32, device="cuda")
+    rounds = torch.zeros(n, dtype=torch.int32, device="cuda")
+    status = torch.zeros(n, dtype=torch.int32, device="cuda")
+    _lib.check(
+        _lib.load().lsd_fit_multi_tissue_fp32(
             dev.data_ptr(), n, m, w.data_ptr(), lr._handle, lr_solver._handle, hr._handle,
             hr_solver._handle, int(max_outer), float(fraction_tol), float(final_tol_x),
             fods.data_ptr(), fractions.data_ptr(), rounds.data_ptr(), status.data_ptr(),

@@ -62,6 +62,20 @@ SIGNATURES = {
         [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64,
          ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp, _vp],
     ),
+    "lsd_correlations": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, ctypes.c_int, _vp, _vp],
+    ),
+    "lsd_fit_multi_tissue_mixed": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64,
+         ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp, _vp],
+    ),
+    "lsd_fit_multi_tissue_fp32": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64,
+         ctypes.c_double, ctypes.c_double, _vp, _vp, _vp, _vp, _vp],
+    ),
     "lsd_fit_multi_tissue_noalloc": (
         ctypes.c_int,
         [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, ctypes.c_int64,

@@ -10,11 +10,11 @@
 
 #include "qr_wave.hpp"
 
-namespace lsd {
+namespace LSD_NS {
 
-constexpr double MEDIAN_TIE_TOLERANCE = 1e-9;
+constexpr real MEDIAN_TIE_TOLERANCE = real(1e-9);
 
-__device__ __forceinline__ bool less_nan_last(double a, double b) {
+__device__ __forceinline__ bool less_nan_last(real a, real b) {
     return isnan(b) ? !isnan(a) : a < b;
 }
 
@@ -29,14 +29,14 @@ constexpr int pow2_ceil(int v) {
 // The median of the n values spread as columns (lane c % 64, slot c / 64), by a bitonic sort of
 // the wave's 64 * NP slots with NaN padding.
 template <int NCPL>
-__device__ __forceinline__ double median_cols(const double (&v)[NCPL], int n) {
+__device__ __forceinline__ real median_cols(const real (&v)[NCPL], int n) {
     constexpr int NP = pow2_ceil(NCPL);
     constexpr int TOTAL = NP * WAVE;
     const int l = lane_id();
-    dvec<NP> a;
+    rvec<NP> a;
 #pragma unroll
     for (int t = 0; t < NP; ++t) {
-        double x = __builtin_nan("");
+        real x = __builtin_nan("");
         if (t < NCPL) {
             x = (l + WAVE * t) < n ? v[t < NCPL ? t : 0] : x;
         }
@@ -54,8 +54,8 @@ __device__ __forceinline__ double median_cols(const double (&v)[NCPL], int n) {
                     if (tp > t) {
                         const int idx = t * WAVE + l;
                         const bool ascending = (idx & kk) == 0;
-                        const double lo = a[t];
-                        const double hi = a[tp];
+                        const real lo = a[t];
+                        const real hi = a[tp];
                         const bool swap = ascending ? less_nan_last(hi, lo) : less_nan_last(lo, hi);
                         a[t] = swap ? hi : lo;
                         a[tp] = swap ? lo : hi;
@@ -64,7 +64,7 @@ __device__ __forceinline__ double median_cols(const double (&v)[NCPL], int n) {
             } else {
 #pragma unroll
                 for (int t = 0; t < NP; ++t) {
-                    const double other = xor_lanes(a[t], j);
+                    const real other = xor_lanes(a[t], j);
                     const int idx = t * WAVE + l;
                     const bool ascending = (idx & kk) == 0;
                     const bool lower = (l & j) == 0;
@@ -79,12 +79,12 @@ __device__ __forceinline__ double median_cols(const double (&v)[NCPL], int n) {
     if (n == 0) {
         return __builtin_nan("");
     }
-    auto at = [&](int i) { return readlane(static_cast<double>(a[i / WAVE]), i % WAVE); };
+    auto at = [&](int i) { return readlane(static_cast<real>(a[i / WAVE]), i % WAVE); };
     const int mid = n / 2;
     if (n % 2 == 1) {
         return at(mid);
     }
-    return (at(mid - 1) + at(mid)) / 2.0;
+    return (at(mid - 1) + at(mid)) / real(2.0);
 }
 
 // Shared matrices of one resolution's fraction fit.
@@ -92,8 +92,8 @@ struct FractionKernelDev {
     int m;
     int ca;                 // anisotropic columns
     int ci;                 // isotropic tissues
-    const double* aniso_t;  // (ca, m): the unweighted anisotropic kernel, transposed
-    const double* iso_t;    // (ci, m): the unweighted isotropic kernel, transposed
+    const real* aniso_t;  // (ca, m): the unweighted anisotropic kernel, transposed
+    const real* iso_t;    // (ci, m): the unweighted isotropic kernel, transposed
     const int* blocks;      // (n_blocks, 2) column ranges
     int n_blocks;
 };
@@ -104,16 +104,16 @@ struct FractionKernelDev {
 // loop and hold them all in registers.
 template <int NG, int RPL>
 struct FractionDesign {
-    dvec<RPL> grp[NG];
+    rvec<RPL> grp[NG];
     int n_groups;
-    const double* iso_lds;
+    const real* iso_lds;
 
-    __device__ __forceinline__ double get(int c, int t) const {
+    __device__ __forceinline__ real get(int c, int t) const {
         if (c < n_groups) {
-            double x = 0.0;
+            real x = real(0.0);
 #pragma unroll
             for (int q = 0; q < NG; ++q) {
-                x = c == q ? static_cast<double>(grp[q][t]) : x;
+                x = c == q ? static_cast<real>(grp[q][t]) : x;
             }
             return x;
         }
@@ -130,23 +130,23 @@ constexpr int GROUP_BATCH = 4; // kernel rows loaded together for a group signal
 // doubles per column of the fit), iso_lds the block's LDS copy of the isotropic columns.
 // Returns false when the NNLS hit its inner-step limit.
 template <int NCPL, int RPL, int NQ, int NG = MAX_GROUPS>
-__device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const double (&fod)[NCPL],
+__device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const real (&fod)[NCPL],
                                                const RowVec<RPL>& s, bool lumped, bool has_tol,
-                                               double tol_in, double (&fr)[NQ], double* work,
-                                               const double* iso_lds) {
+                                               real tol_in, real (&fr)[NQ], real* work,
+                                               const real* iso_lds) {
     const int l = lane_id();
-    const double threshold = median_cols<NCPL>(fod, K.ca) * (1.0 - MEDIAN_TIE_TOLERANCE);
-    double thr[NCPL];
+    const real threshold = median_cols<NCPL>(fod, K.ca) * (real(1.0) - MEDIAN_TIE_TOLERANCE);
+    real thr[NCPL];
 #pragma unroll
     for (int t = 0; t < NCPL; ++t) {
-        thr[t] = fod[t] < threshold ? 0.0 : fod[t];
+        thr[t] = fod[t] < threshold ? real(0.0) : fod[t];
     }
     FractionDesign<NG, RPL> C;
     C.n_groups = lumped ? 1 : K.n_blocks;
     C.iso_lds = iso_lds;
 #pragma unroll
     for (int grp = 0; grp < NG; ++grp) {
-        dvec<RPL> gs = 0.0;
+        rvec<RPL> gs = real(0.0);
         if (grp < C.n_groups) {
             const int begin = lumped ? 0 : K.blocks[2 * grp];
             const int end = lumped ? K.ca : K.blocks[2 * grp + 1];
@@ -156,10 +156,10 @@ __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const
 #pragma unroll
             for (int t2 = 0; t2 < NCPL; ++t2) {
                 const int c0 = l + WAVE * t2;
-                unsigned long long mask = ballot(c0 >= begin && c0 < end && thr[t2] != 0.0);
+                unsigned long long mask = ballot(c0 >= begin && c0 < end && thr[t2] != real(0.0));
                 while (mask) {
-                    double fb[GROUP_BATCH];
-                    double kb[GROUP_BATCH][RPL];
+                    real fb[GROUP_BATCH];
+                    real kb[GROUP_BATCH][RPL];
                     bool valid[GROUP_BATCH];
 #pragma unroll
                     for (int q = 0; q < GROUP_BATCH; ++q) {
@@ -167,11 +167,11 @@ __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const
                         const int b = valid[q] ? __ffsll(static_cast<long long>(mask)) - 1 : 0;
                         mask &= mask - 1;
                         fb[q] = readlane(thr[t2], b);
-                        const double* krow = K.aniso_t + static_cast<long>(WAVE * t2 + b) * K.m;
+                        const real* krow = K.aniso_t + static_cast<long>(WAVE * t2 + b) * K.m;
 #pragma unroll
                         for (int t = 0; t < RPL; ++t) {
                             const int r = l + WAVE * t;
-                            kb[q][t] = (valid[q] && r < K.m) ? krow[r] : 0.0;
+                            kb[q][t] = (valid[q] && r < K.m) ? krow[r] : real(0.0);
                         }
                     }
 #pragma unroll
@@ -189,12 +189,12 @@ __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const
                 }
             }
             if (K.m > 0) {
-                double largest = __builtin_nan("");
+                real largest = __builtin_nan("");
 #pragma unroll
                 for (int t = 0; t < RPL; ++t) {
                     const int r = l + WAVE * t;
                     if (r < K.m) {
-                        largest = fmax(largest, static_cast<double>(gs[t]));
+                        largest = fmax(largest, static_cast<real>(gs[t]));
                     }
                 }
 #pragma unroll
@@ -202,7 +202,7 @@ __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const
                     largest = fmax(largest, xor_lanes(largest, o));
                 }
                 largest = readlane(largest, 0);
-                if (largest > 0.0) {
+                if (largest > real(0.0)) {
 #pragma unroll
                     for (int t = 0; t < RPL; ++t) {
                         gs[t] = gs[t] / largest;
@@ -213,8 +213,8 @@ __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const
         C.grp[grp] = gs;
     }
     const int ncols = C.n_groups + K.ci;
-    const double tol = has_tol ? tol_in : default_tolerance_wave<NQ, RPL>(C, K.m, ncols);
+    const real tol = has_tol ? tol_in : default_tolerance_wave<NQ, RPL>(C, K.m, ncols);
     return nnls_small<NQ, RPL>(C, K.m, ncols, s, tol, 3 * ncols, fr, work);
 }
 
-} // namespace lsd
+} // namespace LSD_NS

@@ -1,0 +1,9 @@
+// Instantiations of the plain FP32 form's FOD and fraction kernels with 1 column slot(s) per lane.
+#define LSD_FP32 1
+#include "kernels.hpp"
+
+namespace lsd32 {
+LSD_LAUNCHER_DEF(1, 4)
+LSD_LAUNCHER_DEF(1, 6)
+LSD_LAUNCHER_DEF(1, 8)
+} // namespace lsd32

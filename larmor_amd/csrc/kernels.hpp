@@ -26,19 +26,23 @@
 
 #include "../../include/larmor_sd.h"
 #include "fractions_wave.hpp"
+#include "launch.hpp"
 #include "ridge_wave.hpp"
 
-namespace lsd {
+namespace LSD_NS {
+
+#ifdef LSD_FP32
+using namespace lsd; // the shared launch helpers
+#endif
 
 constexpr int NQ = 4;             // most fraction columns (groups + isotropic tissues)
 // LDS factor capacity (active columns) of the fast FOD kernels: 20 for the low-resolution solve
 // (its active sets stay below 20 in all but about 0.1% of voxels, which take the rerun kernel),
 // so that 8 waves per SIMD fit in LDS; 32 for the others.
 constexpr int kcap_lds(int ncpl) { return ncpl == 1 ? 20 : 32; }
-constexpr int WAVES_PER_BLOCK = 4;
 constexpr int RERUN_WAVES = 1024; // waves of a rerun launch (4 per CU; the overflow list is a few percent)
 constexpr int RERUN_WAVES_ALL = 512; // waves when every voxel needs the global-memory kernel
-constexpr double FRACTION_SUM_OFFSET = 1e-16;
+constexpr real FRACTION_SUM_OFFSET = real(1e-16);
 
 // Per-wave slab of the FOD kernels: the Cholesky factor (stride kcap + 1, plus one wave of slack
 // for the uniform reads), then its diagonal and the diagonal's reciprocals (2 kcap).  An even
@@ -53,12 +57,12 @@ constexpr int lds_factor_doubles(int kcap, int /*rpl*/) {
 
 // corr_c = sum over rows, in order, of K(r, c) a_r, minus l1 (nnls.cpp:302).
 template <int NCPL, int RPL>
-__device__ __forceinline__ void correlations(const RidgeDesignDev& D, const RowVec<RPL>& a, double l1,
-                             double (&corr)[NCPL]) {
+__device__ __forceinline__ void correlations(const RidgeDesignDev& D, const RowVec<RPL>& a, real l1,
+                             real (&corr)[NCPL]) {
     const int l = lane_id();
 #pragma unroll
     for (int t = 0; t < NCPL; ++t) {
-        corr[t] = 0.0;
+        corr[t] = real(0.0);
     }
 #pragma unroll
     for (int ts = 0; ts < RPL; ++ts) {
@@ -66,8 +70,8 @@ __device__ __forceinline__ void correlations(const RidgeDesignDev& D, const RowV
         const int count = rows_here < WAVE ? rows_here : WAVE;
         for (int ll = 0; ll < count; ++ll) {
             const int r = WAVE * ts + ll;
-            const double ar = readlane(a.v[ts], ll);
-            const double* krow = D.kernel + static_cast<long>(r) * D.n;
+            const real ar = readlane(a.v[ts], ll);
+            const real* krow = D.kernel + static_cast<long>(r) * D.n;
 #pragma unroll
             for (int t = 0; t < NCPL; ++t) {
                 const int c = l + WAVE * t;
@@ -84,42 +88,42 @@ __device__ __forceinline__ void correlations(const RidgeDesignDev& D, const RowV
 }
 
 template <int RPL>
-__device__ __forceinline__ void load_signal(const double* signals, long v, int m, RowVec<RPL>& s) {
+__device__ __forceinline__ void load_signal(const real* signals, long v, int m, RowVec<RPL>& s) {
     const int l = lane_id();
 #pragma unroll
     for (int t = 0; t < RPL; ++t) {
         const int r = l + WAVE * t;
-        s.v[t] = r < m ? signals[v * m + r] : 0.0;
+        s.v[t] = r < m ? signals[v * m + r] : real(0.0);
     }
 }
 
 // max(w s - W_iso p, 0) per row (multi_tissue.cpp:20-24), p the isotropic fractions.
 template <int RPL>
-__device__ __forceinline__ void anisotropic_signal(const RowVec<RPL>& s, const double* weights,
-                                   const double* w_iso, int m, int ci, const double (&p)[NQ],
+__device__ __forceinline__ void anisotropic_signal(const RowVec<RPL>& s, const real* weights,
+                                   const real* w_iso, int m, int ci, const real (&p)[NQ],
                                    RowVec<RPL>& a) {
     const int l = lane_id();
 #pragma unroll
     for (int t = 0; t < RPL; ++t) {
         const int r = l + WAVE * t;
-        double out = 0.0;
+        real out = real(0.0);
         if (r < m) {
-            const double ws = s.v[t] * weights[r];
-            double acc = 0.0;
+            const real ws = s.v[t] * weights[r];
+            real acc = real(0.0);
 #pragma unroll
             for (int tt = 0; tt < NQ; ++tt) {
                 if (tt < ci) {
                     acc = acc + w_iso[static_cast<long>(r) * ci + tt] * p[tt];
                 }
             }
-            out = fmax(ws - acc, 0.0);
+            out = fmax(ws - acc, real(0.0));
         }
         a.v[t] = out;
     }
 }
 
 struct SolveScratch {
-    double* base;     // this launch's slab array
+    real* base;     // this launch's slab array
     long per_wave;    // doubles per wave
 };
 
@@ -151,21 +155,22 @@ __device__ __forceinline__ void record_overflow(const VoxelSet& vs, long v) {
 // multi_tissue.cpp:20-24), its correlations and the ridge NNLS (solver.cpp:14-21,
 // nnls.cpp:294-350); the amplitudes go to x_out (n_vox, n).
 struct FodArgs {
-    const double* signals;    // (n_vox, m)
+    const real* signals;    // (n_vox, m)
     int m;
     int multi;                // 1: anisotropic signal of iso; 0: the signal itself
-    const double* weights;    // (m), multi only
-    const double* w_iso;      // (m, ci), multi only
-    const double* iso;        // (n_vox, ci), multi only
+    const real* weights;    // (m), multi only
+    const real* w_iso;      // (m, ci), multi only
+    const real* iso;        // (n_vox, ci), multi only
     int ci;
     RidgeDesignDev design;
-    double tol;
-    double* x_out;            // (n_vox, n)
+    real tol;
+    real* x_out;            // (n_vox, n)
     int* status;              // per voxel, OR-ed
     VoxelSet vs;
     SolveScratch scratch;
-    const double* corr;       // (list position, n) precomputed correlations, or null
+    const real* corr;       // (list position, n) precomputed correlations, or null
     unsigned long long* work; // profiled launches: the completed solves' FP64 operations
+    unsigned long long* active_mask; // (n_vox) warm-start supports in and out (WARM kernels)
 };
 
 // The fast low-resolution instantiation is asked for 8 waves per SIMD, what its LDS factor
@@ -173,18 +178,20 @@ struct FodArgs {
 // registers and still beats 7 waves with a 24-column factor.
 constexpr int fod_waves_per_eu(int ncpl, bool global_l) { return (!global_l && ncpl == 1) ? 8 : 1; }
 
-template <int NCPL, int RPL, int KCAP, bool GLOBAL_L>
+// WARM (the mixed-precision form's low-resolution solves, NCPL 1): each voxel's solve starts from
+// the support its previous solve left in active_mask and leaves its own there.
+template <int NCPL, int RPL, int KCAP, bool GLOBAL_L, bool WARM = false>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     __attribute__((amdgpu_waves_per_eu(fod_waves_per_eu(NCPL, GLOBAL_L))))
     fod_kernel(FodArgs A) {
     constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
-    extern __shared__ double smem[];
+    extern __shared__ real smem[];
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
     const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
-    double* slab = A.scratch.base + gw * A.scratch.per_wave;
-    double* qrs = slab;
-    double* L = GLOBAL_L ? slab + static_cast<long>(RPLQ * WAVE) * (KCAP + 1)
+    real* slab = A.scratch.base + gw * A.scratch.per_wave;
+    real* qrs = slab;
+    real* L = GLOBAL_L ? slab + static_cast<long>(RPLQ * WAVE) * (KCAP + 1)
                          : smem + w * lds_factor_doubles(KCAP, RPL);
     const long count = voxel_count(A.vs);
     const int l = lane_id();
@@ -192,36 +199,43 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     for (long it = gw; it < count; it += n_waves) {
         const long v = A.vs.list ? A.vs.list[it] : it;
         RowVec<RPL> s;
-        double corr[NCPL];
+        real corr[NCPL];
         if (!GLOBAL_L && A.corr != nullptr) {
             // The correlation pass computed them (the fast kernels carry no QR fits, the only
             // other use of the signal).
-            s.fill(0.0);
+            s.fill(real(0.0));
 #pragma unroll
             for (int t = 0; t < NCPL; ++t) {
                 const int c = l + WAVE * t;
-                corr[t] = c < A.design.n ? A.corr[it * A.design.n + c] : 0.0;
+                corr[t] = c < A.design.n ? A.corr[it * A.design.n + c] : real(0.0);
             }
         } else {
             load_signal<RPL>(A.signals, v, A.m, s);
             if (A.multi) {
-                double p[NQ];
+                real p[NQ];
 #pragma unroll
                 for (int q = 0; q < NQ; ++q) {
-                    p[q] = q < A.ci ? A.iso[v * A.ci + q] : 0.0;
+                    p[q] = q < A.ci ? A.iso[v * A.ci + q] : real(0.0);
                 }
                 RowVec<RPL> a;
                 anisotropic_signal<RPL>(s, A.weights, A.w_iso, A.m, A.ci, p, a);
                 s = a;
             }
-            correlations<NCPL, RPL>(A.design, s, 0.0, corr);
+            correlations<NCPL, RPL>(A.design, s, real(0.0), corr);
         }
-        RidgeNnlsWave<NCPL, KCAP, RPLQ, GLOBAL_L> solver(A.design, A.tol, 3 * A.design.n, 0.0, L,
+        RidgeNnlsWave<NCPL, KCAP, RPLQ, GLOBAL_L> solver(A.design, A.tol, 3 * A.design.n, real(0.0), L,
                                                          qrs);
-        const int st = solver.template run<RPL>(corr, s);
+        const unsigned long long warm = WARM ? A.active_mask[v] : 0ull;
+        const int st = solver.template run<RPL>(corr, s, warm);
         if (st & ST_OVERFLOW) {
             record_overflow(A.vs, v);
             continue;
+        }
+        if constexpr (WARM) {
+            const unsigned long long support = solver.support();
+            if (l == 0) {
+                A.active_mask[v] = support;
+            }
         }
         wave_work += solver.work;
 #pragma unroll
@@ -246,19 +260,19 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
 // broadcast read.  Each correlation is the same in-order sum of K(r, c) a_r, rounded
 // the same way, as the FOD kernels' own (correlations() above).  Columns go in passes of NCB.
 struct CorrArgs {
-    const double* signals;    // (n_vox, m)
+    const real* signals;    // (n_vox, m)
     int m;
     int multi;
-    const double* weights;    // (m), multi only
-    const double* w_iso;      // (m, ci), multi only
-    const double* iso;        // (n_vox, ci), multi only
+    const real* weights;    // (m), multi only
+    const real* w_iso;      // (m, ci), multi only
+    const real* iso;        // (n_vox, ci), multi only
     int ci;
-    const double* kernel;     // (m, n), then at least NCB readable doubles
+    const real* kernel;     // (m, n), then at least NCB readable doubles
     int n;
     const int* list;          // voxels by list position, or null for all
     const int* list_count;
     long n_vox;
-    double* corr;             // (list position, n)
+    real* corr;             // (list position, n)
 };
 
 constexpr int CORR_COLUMNS = 44; // columns per pass of the correlation kernel
@@ -270,8 +284,8 @@ constexpr int CORR_ROWS = 16;    // signal rows staged per LDS tile
 // staged once per block, shared by its waves.
 template <int NCB>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(CorrArgs A) {
-    __shared__ double tile[WAVES_PER_BLOCK][CORR_ROWS][WAVE + 1];
-    __shared__ double ktile[CORR_ROWS][NCB];
+    __shared__ real tile[WAVES_PER_BLOCK][CORR_ROWS][WAVE + 1];
+    __shared__ real ktile[CORR_ROWS][NCB];
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const int l = lane_id();
     const long count = A.list ? static_cast<long>(*A.list_count) : A.n_vox;
@@ -285,16 +299,16 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
         const long it = base + l;
         const bool on = it < count;
         const long v = on ? (A.list ? A.list[it] : it) : 0;
-        double p[NQ];
+        real p[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            p[q] = (on && A.multi && q < A.ci) ? A.iso[v * A.ci + q] : 0.0;
+            p[q] = (on && A.multi && q < A.ci) ? A.iso[v * A.ci + q] : real(0.0);
         }
         for (int c0 = 0; c0 < A.n; c0 += NCB) {
-            double acc[NCB];
+            real acc[NCB];
 #pragma unroll
             for (int j = 0; j < NCB; ++j) {
-                acc[j] = 0.0;
+                acc[j] = real(0.0);
             }
             for (int r0 = 0; r0 < A.m; r0 += CORR_ROWS) {
                 const int rows = min(CORR_ROWS, A.m - r0);
@@ -305,7 +319,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
                     const int rr = e / NCB;
                     const int j = e % NCB;
                     ktile[rr][j] = rr < rows ? A.kernel[static_cast<long>(r0 + rr) * A.n + c0 + j]
-                                             : 0.0;
+                                             : real(0.0);
                 }
 #pragma unroll 4
                 for (int k = 0; k < CORR_ROWS; ++k) {
@@ -314,23 +328,23 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
                     const int rr = e % CORR_ROWS;
                     const long vj = __shfl(v, j);
                     const bool load = base + j < count && rr < rows;
-                    tile[w][rr][j] = load ? A.signals[vj * A.m + r0 + rr] : 0.0;
+                    tile[w][rr][j] = load ? A.signals[vj * A.m + r0 + rr] : real(0.0);
                 }
                 __syncthreads();
                 for (int rr = 0; rr < rows; ++rr) {
                     const int r = r0 + rr;
-                    const double sr = tile[w][rr][l];
-                    double ar = sr;
+                    const real sr = tile[w][rr][l];
+                    real ar = sr;
                     if (A.multi) {
-                        const double ws = sr * A.weights[r];
-                        double wp = 0.0;
+                        const real ws = sr * A.weights[r];
+                        real wp = real(0.0);
 #pragma unroll
                         for (int q = 0; q < NQ; ++q) {
                             if (q < A.ci) {
                                 wp = wp + A.w_iso[static_cast<long>(r) * A.ci + q] * p[q];
                             }
                         }
-                        ar = fmax(ws - wp, 0.0);
+                        ar = fmax(ws - wp, real(0.0));
                     }
                     // Every lane reads the same kernel entries: LDS broadcasts.
 #pragma unroll
@@ -358,19 +372,19 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
 // next round's list unless no fraction moved by fraction_tol; the final fit (multi_tissue.cpp:
 // 92-99) fits the per-block fractions with final_tol and normalises them to sum to 1.
 struct FractionArgs {
-    const double* signals;    // (n_vox, m), unweighted
+    const real* signals;    // (n_vox, m), unweighted
     int m;
-    const double* x;          // (n_vox, ca) FOD amplitudes
+    const real* x;          // (n_vox, ca) FOD amplitudes
     FractionKernelDev kernel;
     int final_fit;
-    double final_tol;
-    double fraction_tol;
-    double* iso;              // (n_vox, ci)     loop rounds
-    double* prev;             // (n_vox, 1 + ci) loop rounds
+    real final_tol;
+    real fraction_tol;
+    real* iso;              // (n_vox, ci)     loop rounds
+    real* prev;             // (n_vox, 1 + ci) loop rounds
     int* rounds;              // (n_vox)         loop rounds
     int* next_list;           // loop rounds: the voxels of the next round
     int* next_count;
-    double* fractions;        // (n_vox, n_blocks + ci), final fit
+    real* fractions;        // (n_vox, n_blocks + ci), final fit
     int* status;
     const int* list;          // voxels to fit, or null for all
     const int* list_count;
@@ -383,20 +397,20 @@ constexpr int fraction_waves_per_eu(int ncpl) { return ncpl == 1 ? 5 : 1; }
 template <int NCPL, int RPL, bool FINAL>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     __attribute__((amdgpu_waves_per_eu(fraction_waves_per_eu(NCPL)))) fraction_kernel(FractionArgs A) {
-    extern __shared__ double smem[];
+    extern __shared__ real smem[];
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
     const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
     // The fit's columns: one lumped group (the loop) or one per block (the final fit), then the
     // isotropic tissues.
     const int qcols = (FINAL ? A.kernel.n_blocks : 1) + A.kernel.ci;
-    double* work = smem + w * (qcols * RPL * WAVE);
+    real* work = smem + w * (qcols * RPL * WAVE);
     // The block's copy of the isotropic columns, after the waves' QR slabs.
-    double* iso_lds = smem + WAVES_PER_BLOCK * (qcols * RPL * WAVE);
+    real* iso_lds = smem + WAVES_PER_BLOCK * (qcols * RPL * WAVE);
     for (int i = threadIdx.x; i < A.kernel.ci * RPL * WAVE; i += blockDim.x) {
         const int c = i / (RPL * WAVE);
         const int r = i % (RPL * WAVE);
-        iso_lds[i] = r < A.m ? A.kernel.iso_t[static_cast<long>(c) * A.m + r] : 0.0;
+        iso_lds[i] = r < A.m ? A.kernel.iso_t[static_cast<long>(c) * A.m + r] : real(0.0);
     }
     __syncthreads();
     const long count = A.list ? static_cast<long>(*A.list_count) : A.n_vox;
@@ -429,31 +443,31 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
         // The loop's previous fractions (lane q holds fraction q) and round count, loaded before
         // the fit so that their latency overlaps it.
         const int nlf = 1 + ci;
-        double prev = 0.0;
+        real prev = real(0.0);
         int rounds = 0;
         if (!FINAL) {
-            prev = l < nlf ? A.prev[v * nlf + l] : 0.0;
+            prev = l < nlf ? A.prev[v * nlf + l] : real(0.0);
             rounds = l == 0 ? A.rounds[v] : 0;
         }
-        double fod[NCPL];
+        real fod[NCPL];
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {
             const int c = l + WAVE * t;
-            fod[t] = c < ca ? A.x[v * ca + c] : 0.0;
+            fod[t] = c < ca ? A.x[v * ca + c] : real(0.0);
         }
-        double fr[NQ];
+        real fr[NQ];
         // The final fit's per-block groups, or the loop's one lumped group.
         const bool ok = FINAL
                             ? fractions_wave<NCPL, RPL, NQ, MAX_GROUPS>(
                                   A.kernel, fod, s, false, true, A.final_tol, fr, work, iso_lds)
-                            : fractions_wave<NCPL, RPL, NQ, 1>(A.kernel, fod, s, true, false, 0.0,
+                            : fractions_wave<NCPL, RPL, NQ, 1>(A.kernel, fod, s, true, false, real(0.0),
                                                                fr, work, iso_lds);
         if (!ok && l == 0) {
             A.status[v] |= ST_NOT_CONVERGED;
         }
         if (FINAL) {
             const int nf = A.kernel.n_blocks + ci;
-            double total = 0.0;
+            real total = real(0.0);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 if (q < nf) {
@@ -469,7 +483,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
             }
             continue;
         }
-        double mine = 0.0;
+        real mine = real(0.0);
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             mine = l == q ? fr[q] : mine;
@@ -503,38 +517,14 @@ template <int NCPL, int RPL, int KCAP, bool G>
 struct fod_family {
     static constexpr auto kernel = fod_kernel<NCPL, RPL, KCAP, G>;
 };
+template <int NCPL, int RPL, int KCAP, bool G>
+struct fod_warm_family {
+    static constexpr auto kernel = fod_kernel<NCPL, RPL, KCAP, G, true>;
+};
 
 // ---------------------------------------------------------------------------------------------
 // Host-side launch helpers (shared by the instantiation units)
 // ---------------------------------------------------------------------------------------------
-
-extern thread_local std::string g_error;
-
-inline int fail(const std::string& message) {
-    g_error = message;
-    return LSD_ERR_INVALID;
-}
-
-inline int hip_fail(hipError_t e, const char* what) {
-    g_error = std::string(what) + ": " + hipGetErrorString(e);
-    return LSD_ERR_HIP;
-}
-
-#define LSD_HIP(call)                                                                          \
-    do {                                                                                       \
-        const hipError_t lsd_e = (call);                                                       \
-        if (lsd_e != hipSuccess) {                                                             \
-            return hip_fail(lsd_e, #call);                                                     \
-        }                                                                                      \
-    } while (0)
-
-inline int device_alloc(void** p, size_t bytes, hipStream_t stream) {
-    if (bytes == 0) {
-        bytes = 8;
-    }
-    LSD_HIP(hipMallocAsync(p, bytes, stream));
-    return 0;
-}
 
 template <int RPL, int KCAP>
 constexpr long slab_doubles(bool global_l) {
@@ -550,87 +540,15 @@ constexpr long rerun_slab_doubles(int ncpl, int rpl) {
     return static_cast<long>(rplq * WAVE) * (kcap + 1) + static_cast<long>(lds_factor_doubles(kcap, rpl));
 }
 
-// Scratch of the global-memory rerun launches of one kind of solve.  Either the first launch
-// that needs it allocates it (stream-ordered; the caller frees it) and later launches reuse it,
-// or the caller hands in a fixed region (fixed = true) whose size sets the rerun's wave count.
-struct RerunScratch {
-    void* base = nullptr;
-    size_t bytes = 0;
-    bool fixed = false;
-};
-// Opt-in per-launch timing: events recorded on the launch stream around each kernel, and for
-// the kernels that count their algorithmic work, a device counter slot per launch.
-struct ProfiledLaunch {
-    const char* name;
-    hipEvent_t start;
-    hipEvent_t stop;
-    unsigned long long* work;
-};
-extern bool g_profiling;
-extern std::vector<ProfiledLaunch> g_launches;
-extern unsigned long long* g_work_slots; // one zeroed counter per profiled launch
-constexpr long WORK_SLOTS = 16384;
 
-// Gives a launch's arguments its work counter (kernels that count none ignore it).
-template <class Args>
-inline bool attach_work(Args&, unsigned long long*) {
-    return false;
-}
 inline bool attach_work(FodArgs& a, unsigned long long* slot) {
     a.work = slot;
     return slot != nullptr;
 }
+#ifndef LSD_FP32
 struct DrlArgs;
 inline bool attach_work(DrlArgs& a, unsigned long long* slot); // drl_wave.hpp
-
-// Launches kern over `waves` waves, waves_per_block to a block with lds_bytes of dynamic LDS.
-template <class Kern, class Args>
-inline int launch(const char* name, Kern kern, const Args& args, int waves, size_t lds_bytes,
-           hipStream_t stream, int waves_per_block = WAVES_PER_BLOCK) {
-    const int blocks = (waves + waves_per_block - 1) / waves_per_block;
-    ProfiledLaunch rec{name, nullptr, nullptr, nullptr};
-    Args a = args;
-    if (g_profiling) {
-        LSD_HIP(hipEventCreate(&rec.start));
-        LSD_HIP(hipEventCreate(&rec.stop));
-        LSD_HIP(hipEventRecord(rec.start, stream));
-        const long at = static_cast<long>(g_launches.size());
-        unsigned long long* slot =
-            (g_work_slots != nullptr && at < WORK_SLOTS) ? g_work_slots + at : nullptr;
-        if (attach_work(a, slot)) {
-            rec.work = slot;
-        }
-    }
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(WAVE * waves_per_block), lds_bytes, stream, a);
-    LSD_HIP(hipGetLastError());
-    if (g_profiling) {
-        LSD_HIP(hipEventRecord(rec.stop, stream));
-        g_launches.push_back(rec);
-    }
-    return 0;
-}
-
-
-// Launches a persistent kernel (one that strides over its items) on at most `waves` waves and
-// no more than the device keeps resident at once: one full-occupancy wave set.
-template <class Kern, class Args>
-inline int launch_fill(const char* name, Kern kern, const Args& args, int waves, size_t lds_bytes,
-                       hipStream_t stream) {
-    int blocks_per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks_per_cu, kern, WAVE * WAVES_PER_BLOCK,
-                                                     lds_bytes) == hipSuccess &&
-        blocks_per_cu > 0) {
-        int dev = 0;
-        int cus = 0;
-        LSD_HIP(hipGetDevice(&dev));
-        LSD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        const long resident = static_cast<long>(blocks_per_cu) * cus * WAVES_PER_BLOCK;
-        if (resident > 0 && resident < waves) {
-            waves = static_cast<int>(resident);
-        }
-    }
-    return launch(name, kern, args, waves, lds_bytes, stream);
-}
+#endif
 
 // The fast launch (LDS factor) of a kernel family, then its rerun of the overflowed voxels with
 // the factor in global memory, sized for every column (KG = NCPL * 64 >= n).  The rerun runs on
@@ -639,7 +557,7 @@ inline int launch_fill(const char* name, Kern kern, const Args& args, int waves,
 // many whole blocks of slabs as a fixed region holds.
 template <template <int, int, int, bool> class Family, int NCPL, int RPL, class Args>
 int rerun_waves_in(RerunScratch* rs, int want, hipStream_t stream, int* waves) {
-    const size_t per = sizeof(double) * static_cast<size_t>(slab_doubles<RPL, NCPL * WAVE>(true));
+    const size_t per = sizeof(real) * static_cast<size_t>(slab_doubles<RPL, NCPL * WAVE>(true));
     if (rs->fixed) {
         const size_t fit = rs->bytes / per / WAVES_PER_BLOCK * WAVES_PER_BLOCK;
         if (fit < static_cast<size_t>(WAVES_PER_BLOCK)) {
@@ -672,7 +590,7 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
         int rc = rerun_waves_in<Family, NCPL, RPL, Args>(rs, std::min(waves, RERUN_WAVES_ALL),
                                                           stream, &all_waves);
         if (rc) return rc;
-        rerun.scratch = SolveScratch{static_cast<double*>(rs->base), per};
+        rerun.scratch = SolveScratch{static_cast<real*>(rs->base), per};
         rerun.vs.overflow_list = nullptr;
         rerun.vs.overflow_count = nullptr;
         rerun.corr = nullptr;
@@ -680,13 +598,13 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
                       stream);
     }
     constexpr int KC = kcap_lds(NCPL);
-    const size_t lds = sizeof(double) * lds_factor_doubles(KC, RPL) * WAVES_PER_BLOCK;
+    const size_t lds = sizeof(real) * lds_factor_doubles(KC, RPL) * WAVES_PER_BLOCK;
     int rc = launch_fill(name, Family<NCPL, RPL, KC, false>::kernel, fast, waves, lds, stream);
     if (rc) return rc;
     int rerun_waves = 0;
     rc = rerun_waves_in<Family, NCPL, RPL, Args>(rs, RERUN_WAVES, stream, &rerun_waves);
     if (rc) return rc;
-    rerun.scratch = SolveScratch{static_cast<double*>(rs->base), per};
+    rerun.scratch = SolveScratch{static_cast<real*>(rs->base), per};
     rerun.vs.list = d_overflow_list;
     rerun.vs.list_count = d_overflow_count;
     rerun.vs.overflow_list = nullptr;
@@ -713,14 +631,17 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
     int launch_fraction_##NC##_##RP(const char* name, const FractionArgs& args, int waves,      \
                                     hipStream_t stream) {                                      \
         const int qcols = (args.final_fit ? args.kernel.n_blocks : 1) + args.kernel.ci;        \
-        const size_t lds = sizeof(double) * (qcols * WAVES_PER_BLOCK + args.kernel.ci) * RP * WAVE; \
+        const size_t lds = sizeof(real) * (qcols * WAVES_PER_BLOCK + args.kernel.ci) * RP * WAVE; \
         return args.final_fit                                                                  \
                    ? launch_fill(name, fraction_kernel<NC, RP, true>, args, waves, lds, stream)   \
                    : launch_fill(name, fraction_kernel<NC, RP, false>, args, waves, lds, stream); \
     }
 
 // Every instantiated (NCPL, RPL): X(NC, RP).
-#ifdef LSD_FAST_BUILD
+#if defined(LSD_FP32)
+// The plain FP32 form: the 44- and 300-direction solves of the multi-tissue fit.
+#define LSD_FOR_EACH_LAUNCHER(X) X(1, 4) X(1, 6) X(1, 8) X(5, 4) X(5, 6) X(5, 8)
+#elif defined(LSD_FAST_BUILD)
 #define LSD_FOR_EACH_LAUNCHER(X) X(1, 4) X(5, 4)
 #else
 #define LSD_FOR_EACH_RPL(X, NC) X(NC, 4) X(NC, 6) X(NC, 8)
@@ -730,4 +651,4 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
 
 LSD_FOR_EACH_LAUNCHER(LSD_LAUNCHER_DECL)
 
-} // namespace lsd
+} // namespace LSD_NS

@@ -18,9 +18,17 @@
 
 #include "wave.hpp"
 
-namespace lsd {
+namespace LSD_NS {
 
 constexpr double DBL_EPS = 2.220446049250313e-16;
+// The machine epsilon and the smallest normal number of the arithmetic type.
+#ifdef LSD_FP32
+constexpr float REAL_EPS = FLT_EPSILON;
+constexpr float REAL_MIN = FLT_MIN;
+#else
+constexpr double REAL_EPS = DBL_EPS;
+constexpr double REAL_MIN = DBL_MIN;
+#endif
 
 // Up to NQ columns of RPL row slots each, in registers.  Column indices are uniform; a column
 // is picked by a compile-time-unrolled select over the NQ column vectors, so the columns stay
@@ -28,29 +36,29 @@ constexpr double DBL_EPS = 2.220446049250313e-16;
 template <int NQ, int RPL>
 struct RegCols {
     static constexpr int kRpl = RPL;
-    dvec<RPL> col[NQ];
+    rvec<RPL> col[NQ];
 
     template <int... Q>
-    __device__ __forceinline__ double get_(int c, int t, std::integer_sequence<int, Q...>) const {
-        double x = 0.0;
-        ((x = c == Q ? static_cast<double>(col[Q][t]) : x), ...);
+    __device__ __forceinline__ real get_(int c, int t, std::integer_sequence<int, Q...>) const {
+        real x = real(0.0);
+        ((x = c == Q ? static_cast<real>(col[Q][t]) : x), ...);
         return x;
     }
     template <int... Q>
-    __device__ __forceinline__ void set_(int c, int t, double x, std::integer_sequence<int, Q...>) {
-        ((col[Q][t] = c == Q ? x : static_cast<double>(col[Q][t])), ...);
+    __device__ __forceinline__ void set_(int c, int t, real x, std::integer_sequence<int, Q...>) {
+        ((col[Q][t] = c == Q ? x : static_cast<real>(col[Q][t])), ...);
     }
-    __device__ __forceinline__ double get(int c, int t) const {
+    __device__ __forceinline__ real get(int c, int t) const {
         return get_(c, t, std::make_integer_sequence<int, NQ>{});
     }
-    __device__ __forceinline__ void set(int c, int t, double x) {
+    __device__ __forceinline__ void set(int c, int t, real x) {
         set_(c, t, x, std::make_integer_sequence<int, NQ>{});
     }
     // Element (row r, column c), both uniform.
-    __device__ __forceinline__ double elem(int c, int r) const {
+    __device__ __forceinline__ real elem(int c, int r) const {
         return readlane(get(c, r / WAVE), r % WAVE);
     }
-    __device__ __forceinline__ void set_elem(int c, int r, double x) {
+    __device__ __forceinline__ void set_elem(int c, int r, real x) {
         if (lane_id() == r % WAVE) {
             set(c, r / WAVE, x);
         }
@@ -58,8 +66,8 @@ struct RegCols {
     __device__ __forceinline__ void swap_cols(int a, int b) {
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
-            const double va = get(a, t);
-            const double vb = get(b, t);
+            const real va = get(a, t);
+            const real vb = get(b, t);
             set(a, t, vb);
             set(b, t, va);
         }
@@ -72,21 +80,21 @@ template <int RPL>
 struct MemCols {
     static constexpr int kRpl = RPL;
     static constexpr int kLd = RPL * WAVE;
-    double* base;
+    real* base;
 
-    __device__ __forceinline__ double get(int c, int t) const {
+    __device__ __forceinline__ real get(int c, int t) const {
         return base[static_cast<long>(c) * kLd + t * WAVE + lane_id()];
     }
-    __device__ __forceinline__ void set(int c, int t, double x) {
+    __device__ __forceinline__ void set(int c, int t, real x) {
         base[static_cast<long>(c) * kLd + t * WAVE + lane_id()] = x;
     }
     // A uniform element, read as a vector load of its 64-row segment plus readlane (never
     // through the scalar cache, as the slab is written in the same kernel).
-    __device__ __forceinline__ double elem(int c, int r) const {
-        const double x = base[static_cast<long>(c) * kLd + (r / WAVE) * WAVE + lane_id()];
+    __device__ __forceinline__ real elem(int c, int r) const {
+        const real x = base[static_cast<long>(c) * kLd + (r / WAVE) * WAVE + lane_id()];
         return readlane(x, r % WAVE);
     }
-    __device__ __forceinline__ void set_elem(int c, int r, double x) {
+    __device__ __forceinline__ void set_elem(int c, int r, real x) {
         if (lane_id() == r % WAVE) {
             base[static_cast<long>(c) * kLd + r] = x;
         }
@@ -94,8 +102,8 @@ struct MemCols {
     __device__ __forceinline__ void swap_cols(int a, int b) {
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
-            const double va = get(a, t);
-            const double vb = get(b, t);
+            const real va = get(a, t);
+            const real vb = get(b, t);
             set(a, t, vb);
             set(b, t, va);
         }
@@ -106,14 +114,14 @@ struct MemCols {
 // evaluated on every slot (its reads stay inside the lane's slots) and the terms outside the
 // range are left out by a select, so the lanes do not diverge.
 template <int RPL, class F>
-__device__ __forceinline__ double wave_sum_f(int lo, int hi, F f) {
+__device__ __forceinline__ real wave_sum_f(int lo, int hi, F f) {
     const int l = lane_id();
-    double s = 0.0;
+    real s = real(0.0);
 #pragma unroll
     for (int t = 0; t < RPL; ++t) {
         const int r = l + WAVE * t;
-        const double term = f(t);
-        const double sum = s + term;
+        const real term = f(t);
+        const real sum = s + term;
         s = (r >= lo && r < hi) ? sum : s;
     }
     return butterfly_sum(s);
@@ -132,84 +140,84 @@ __device__ __forceinline__ void least_squares_wave(Cols& A, int rows, int cols, 
     LaneVec<NSC> nd;
     LaneVec<NSC> hc;
     LaneVec<NSC, int> trans;
-    nu.fill(0.0);
-    nd.fill(0.0);
-    hc.fill(0.0);
+    nu.fill(real(0.0));
+    nd.fill(real(0.0));
+    hc.fill(real(0.0));
     trans.fill(0);
     for (int c = 0; c < cols; ++c) {
-        const double s = wave_sum_f<RPL>(0, rows, [&](int t) {
-            const double x = A.get(c, t);
+        const real s = wave_sum_f<RPL>(0, rows, [&](int t) {
+            const real x = A.get(c, t);
             return x * x;
         });
-        const double nrm = sqrt(s);
+        const real nrm = sqrt(s);
         nd.set(c, nrm);
         nu.set(c, nrm);
     }
-    double max_norm = 0.0;
+    real max_norm = real(0.0);
     for (int c = 0; c < cols; ++c) {
-        const double v = nu.get(c);
+        const real v = nu.get(c);
         if (c == 0 || v > max_norm) {
             max_norm = v;
         }
     }
-    const double scaled = max_norm * DBL_EPS;
-    const double threshold_helper = (scaled * scaled) / static_cast<double>(rows);
-    const double norm_downdate_threshold = sqrt(DBL_EPS);
+    const real scaled = max_norm * REAL_EPS;
+    const real threshold_helper = (scaled * scaled) / static_cast<real>(rows);
+    const real norm_downdate_threshold = sqrt(REAL_EPS);
     int nonzero_pivots = size;
-    double maxpivot = 0.0;
+    real maxpivot = real(0.0);
 
     for (int k = 0; k < size; ++k) {
         int biggest = k;
-        double bv = nu.get(k);
+        real bv = nu.get(k);
         for (int c = k + 1; c < cols; ++c) {
-            const double v = nu.get(c);
+            const real v = nu.get(c);
             if (v > bv) {
                 bv = v;
                 biggest = c;
             }
         }
-        const double biggest_sq = bv * bv;
+        const real biggest_sq = bv * bv;
         if (nonzero_pivots == size &&
-            biggest_sq < threshold_helper * static_cast<double>(rows - k)) {
+            biggest_sq < threshold_helper * static_cast<real>(rows - k)) {
             nonzero_pivots = k;
         }
         trans.set(k, biggest);
         if (k != biggest) {
             A.swap_cols(k, biggest);
-            const double u1 = nu.get(k);
-            const double u2 = nu.get(biggest);
+            const real u1 = nu.get(k);
+            const real u2 = nu.get(biggest);
             nu.set(k, u2);
             nu.set(biggest, u1);
-            const double d1 = nd.get(k);
-            const double d2 = nd.get(biggest);
+            const real d1 = nd.get(k);
+            const real d2 = nd.get(biggest);
             nd.set(k, d2);
             nd.set(biggest, d1);
         }
 
         // Householder vector of column k.
-        const double c0 = A.elem(k, k);
-        const double tail_sq = wave_sum_f<RPL>(k + 1, rows, [&](int t) {
-            const double x = A.get(k, t);
+        const real c0 = A.elem(k, k);
+        const real tail_sq = wave_sum_f<RPL>(k + 1, rows, [&](int t) {
+            const real x = A.get(k, t);
             return x * x;
         });
-        double tau;
-        double beta;
-        if (tail_sq <= DBL_MIN) {
-            tau = 0.0;
+        real tau;
+        real beta;
+        if (tail_sq <= REAL_MIN) {
+            tau = real(0.0);
             beta = c0;
 #pragma unroll
             for (int t = 0; t < RPL; ++t) {
                 const int r = l + WAVE * t;
                 if (r > k && r < rows) {
-                    A.set(k, t, 0.0);
+                    A.set(k, t, real(0.0));
                 }
             }
         } else {
             beta = sqrt(c0 * c0 + tail_sq);
-            if (c0 >= 0.0) {
+            if (c0 >= real(0.0)) {
                 beta = -beta;
             }
-            const double denom = c0 - beta;
+            const real denom = c0 - beta;
 #pragma unroll
             for (int t = 0; t < RPL; ++t) {
                 const int r = l + WAVE * t;
@@ -228,13 +236,13 @@ __device__ __forceinline__ void least_squares_wave(Cols& A, int rows, int cols, 
         // Apply H_k to the columns right of k.
         if (rows - k == 1) {
             for (int c = k + 1; c < cols; ++c) {
-                A.set_elem(c, k, A.elem(c, k) * (1.0 - tau));
+                A.set_elem(c, k, A.elem(c, k) * (real(1.0) - tau));
             }
-        } else if (tau != 0.0) {
+        } else if (tau != real(0.0)) {
             for (int c = k + 1; c < cols; ++c) {
-                double tmp = wave_sum_f<RPL>(k + 1, rows,
+                real tmp = wave_sum_f<RPL>(k + 1, rows,
                                              [&](int t) { return A.get(k, t) * A.get(c, t); });
-                const double top = A.elem(c, k);
+                const real top = A.elem(c, k);
                 tmp = tmp + top;
                 A.set_elem(c, k, top - tau * tmp);
 #pragma unroll
@@ -249,19 +257,19 @@ __device__ __forceinline__ void least_squares_wave(Cols& A, int rows, int cols, 
 
         // Norm downdate (LAWN 176).
         for (int c = k + 1; c < cols; ++c) {
-            const double nuc = nu.get(c);
-            if (nuc != 0.0) {
-                double temp = fabs(A.elem(c, k)) / nuc;
-                temp = (1.0 + temp) * (1.0 - temp);
-                temp = temp < 0.0 ? 0.0 : temp;
-                const double ratio = nuc / nd.get(c);
-                const double temp2 = temp * (ratio * ratio);
+            const real nuc = nu.get(c);
+            if (nuc != real(0.0)) {
+                real temp = fabs(A.elem(c, k)) / nuc;
+                temp = (real(1.0) + temp) * (real(1.0) - temp);
+                temp = temp < real(0.0) ? real(0.0) : temp;
+                const real ratio = nuc / nd.get(c);
+                const real temp2 = temp * (ratio * ratio);
                 if (temp2 <= norm_downdate_threshold) {
-                    const double s = wave_sum_f<RPL>(k + 1, rows, [&](int t) {
-                        const double x = A.get(c, t);
+                    const real s = wave_sum_f<RPL>(k + 1, rows, [&](int t) {
+                        const real x = A.get(c, t);
                         return x * x;
                     });
-                    const double nrm = sqrt(s);
+                    const real nrm = sqrt(s);
                     nd.set(c, nrm);
                     nu.set(c, nrm);
                 } else {
@@ -285,8 +293,8 @@ __device__ __forceinline__ void least_squares_wave(Cols& A, int rows, int cols, 
         perm.set(other, a);
     }
 
-    const double threshold = static_cast<double>(rows > cols ? rows : cols) * DBL_EPS;
-    const double premultiplied = fabs(maxpivot) * threshold;
+    const real threshold = static_cast<real>(rows > cols ? rows : cols) * REAL_EPS;
+    const real premultiplied = fabs(maxpivot) * threshold;
     int rank = 0;
     for (int i = 0; i < nonzero_pivots; ++i) {
         rank += fabs(A.elem(i, i)) > premultiplied ? 1 : 0;
@@ -294,12 +302,12 @@ __device__ __forceinline__ void least_squares_wave(Cols& A, int rows, int cols, 
 
     // Q^T d over the first `rank` reflectors, H_0 first.
     for (int k = 0; k < rank; ++k) {
-        const double tau = hc.get(k);
+        const real tau = hc.get(k);
         if (rows - k == 1) {
-            d.set(k, d.get(k) * (1.0 - tau));
-        } else if (tau != 0.0) {
-            double tmp = wave_sum_f<RPL>(k + 1, rows, [&](int t) { return A.get(k, t) * d.v[t]; });
-            const double top = d.get(k);
+            d.set(k, d.get(k) * (real(1.0) - tau));
+        } else if (tau != real(0.0)) {
+            real tmp = wave_sum_f<RPL>(k + 1, rows, [&](int t) { return A.get(k, t) * d.v[t]; });
+            const real top = d.get(k);
             tmp = tmp + top;
             d.set(k, top - tau * tmp);
 #pragma unroll
@@ -313,8 +321,8 @@ __device__ __forceinline__ void least_squares_wave(Cols& A, int rows, int cols, 
     }
     // R11 b = (Q^T d)[0:rank], column-oriented back substitution.
     for (int i = rank - 1; i >= 0; --i) {
-        double ri = d.get(i);
-        if (ri != 0.0) {
+        real ri = d.get(i);
+        if (ri != real(0.0)) {
             ri = ri / A.elem(i, i);
         }
         d.set(i, ri);
@@ -326,7 +334,7 @@ __device__ __forceinline__ void least_squares_wave(Cols& A, int rows, int cols, 
             }
         }
     }
-    coef.fill(0.0);
+    coef.fill(real(0.0));
     for (int p = 0; p < rank; ++p) {
         coef.set(perm.get(p), d.get(p));
     }
@@ -336,20 +344,20 @@ __device__ __forceinline__ void least_squares_wave(Cols& A, int rows, int cols, 
 // 10 eps max_c sum_r |C(r, c)| max(rows, cols), the column sums being wave sums.  P provides
 // the columns: P.get(c, t) is column c at the lane's row slot t (c uniform).
 template <int NQ, int RPL, class P>
-__device__ __forceinline__ double default_tolerance_wave(const P& C, int rows, int cols) {
-    double norm1 = 0.0;
+__device__ __forceinline__ real default_tolerance_wave(const P& C, int rows, int cols) {
+    real norm1 = real(0.0);
     if (rows * cols > 0) {
 #pragma unroll
         for (int c = 0; c < NQ; ++c) {
             if (c < cols) {
-                const double s = wave_sum_f<RPL>(0, rows, [&](int t) { return fabs(C.get(c, t)); });
+                const real s = wave_sum_f<RPL>(0, rows, [&](int t) { return fabs(C.get(c, t)); });
                 if (c == 0 || s > norm1) {
                     norm1 = s;
                 }
             }
         }
     }
-    return 10.0 * DBL_EPS * norm1 * static_cast<double>(rows > cols ? rows : cols);
+    return real(10.0) * REAL_EPS * norm1 * static_cast<real>(rows > cols ? rows : cols);
 }
 
 // The generic Lawson-Hanson NNLS on up to NQ columns (solve_nnls_with_limit).  C provides the
@@ -358,21 +366,21 @@ __device__ __forceinline__ double default_tolerance_wave(const P& C, int rows, i
 // there.  x receives the solution (uniform).  Returns false when the inner-step limit stopped it.
 template <int NQ, int RPL, class P>
 __device__ __forceinline__ bool nnls_small(const P& C, int rows, int ncols, const RowVec<RPL>& d,
-                                           double tol, int max_inner, double (&x)[NQ],
-                                           double* work) {
+                                           real tol, int max_inner, real (&x)[NQ],
+                                           real* work) {
     bool active[NQ];
-    double g[NQ];
-    double cand[NQ];
+    real g[NQ];
+    real cand[NQ];
 #pragma unroll
     for (int c = 0; c < NQ; ++c) {
         active[c] = false;
-        x[c] = 0.0;
+        x[c] = real(0.0);
     }
-    auto gradient_at = [&](const double (&sol)[NQ], double (&out)[NQ]) {
-        double r[RPL];
+    auto gradient_at = [&](const real (&sol)[NQ], real (&out)[NQ]) {
+        real r[RPL];
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
-            double s = 0.0;
+            real s = real(0.0);
 #pragma unroll
             for (int c = 0; c < NQ; ++c) {
                 if (c < ncols) {
@@ -385,18 +393,18 @@ __device__ __forceinline__ bool nnls_small(const P& C, int rows, int ncols, cons
         // active ones are left at zero.
 #pragma unroll
         for (int c = 0; c < NQ; ++c) {
-            out[c] = 0.0;
+            out[c] = real(0.0);
             if (c < ncols && !active[c]) {
                 out[c] = wave_sum_f<RPL>(0, rows, [&](int t) { return C.get(c, t) * r[t]; });
             }
         }
     };
-    auto fit_on = [&](double (&out)[NQ]) {
+    auto fit_on = [&](real (&out)[NQ]) {
         MemCols<RPL> A{work};
         int ka = 0;
 #pragma unroll
         for (int c = 0; c < NQ; ++c) {
-            out[c] = 0.0;
+            out[c] = real(0.0);
             if (c < ncols && active[c]) {
 #pragma unroll
                 for (int t = 0; t < RPL; ++t) {
@@ -431,7 +439,7 @@ __device__ __forceinline__ bool nnls_small(const P& C, int rows, int ncols, cons
             return false;
         }
         int entering = -1;
-        double best = 0.0;
+        real best = real(0.0);
 #pragma unroll
         for (int c = 0; c < NQ; ++c) {
             if (c < ncols && !active[c] && (entering < 0 || g[c] > best)) {
@@ -451,7 +459,7 @@ __device__ __forceinline__ bool nnls_small(const P& C, int rows, int ncols, cons
             bool any = false;
 #pragma unroll
             for (int c = 0; c < NQ; ++c) {
-                any = any || (c < ncols && active[c] && cand[c] <= 0.0);
+                any = any || (c < ncols && active[c] && cand[c] <= real(0.0));
             }
             if (!any) {
                 break;
@@ -463,13 +471,13 @@ __device__ __forceinline__ bool nnls_small(const P& C, int rows, int ncols, cons
                 }
                 return false;
             }
-            double step = __builtin_inf();
+            real step = __builtin_inff();
             int blocking = 0;
 #pragma unroll
             for (int c = 0; c < NQ; ++c) {
-                if (c < ncols && active[c] && cand[c] <= 0.0) {
-                    const double distance = x[c] - cand[c];
-                    const double s = distance > 0.0 ? x[c] / distance : 0.0;
+                if (c < ncols && active[c] && cand[c] <= real(0.0)) {
+                    const real distance = x[c] - cand[c];
+                    const real s = distance > real(0.0) ? x[c] / distance : real(0.0);
                     if (s < step) {
                         step = s;
                         blocking = c;
@@ -485,7 +493,7 @@ __device__ __forceinline__ bool nnls_small(const P& C, int rows, int ncols, cons
 #pragma unroll
             for (int c = 0; c < NQ; ++c) {
                 if (c == blocking) {
-                    x[c] = 0.0;
+                    x[c] = real(0.0);
                     active[c] = false;
                 }
                 active[c] = active[c] && fabs(x[c]) >= tol;
@@ -501,4 +509,4 @@ __device__ __forceinline__ bool nnls_small(const P& C, int rows, int ncols, cons
     return true;
 }
 
-} // namespace lsd
+} // namespace LSD_NS

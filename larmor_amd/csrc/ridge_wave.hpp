@@ -27,7 +27,7 @@
 
 #include "qr_wave.hpp"
 
-namespace lsd {
+namespace LSD_NS {
 
 // Status bits of a voxel solve.
 constexpr int ST_NOT_CONVERGED = 1; // the inner-step limit ended an NNLS (result kept)
@@ -40,15 +40,15 @@ constexpr int ST_OUTER_GUARD = 8;   // more entering steps than any finite solve
 // cycling on rounding, which would never end, so it stops with the current solution.
 __device__ __forceinline__ int outer_guard(int n) { return 8 * n + 64; }
 
-constexpr double MIN_RELATIVE_PIVOT = 1e-12;
+constexpr real MIN_RELATIVE_PIVOT = real(1e-12);
 
 // The shared matrices of one ridge design.
 struct RidgeDesignDev {
     int m;                // measurements
     int n;                // columns
-    const double* kernel; // (m, n) row-major, the weighted anisotropic kernel
-    const double* gram;   // (n, n) row-major, K^T K + ridge^2 I, bitwise symmetric
-    double ridge;
+    const real* kernel; // (m, n) row-major, the weighted anisotropic kernel
+    const real* gram;   // (n, n) row-major, K^T K + ridge^2 I, bitwise symmetric
+    real ridge;
     int normal_equations; // 1: Cholesky path (with QR fallback); 0: QR only
 };
 
@@ -69,16 +69,16 @@ struct RidgeNnlsWave {
     static constexpr bool PREFETCH = QR;                 // factor in global memory
 
     const RidgeDesignDev& D;
-    double tol;
+    real tol;
     int max_inner;
-    double l1;
-    double* L;   // factor storage, S * KCAP doubles
-    double* qrs; // QR fallback slab: (RPLQ * 64) * KCAP doubles, then RPLQ * 64 for the data
+    real l1;
+    real* L;   // factor storage, S * KCAP doubles
+    real* qrs; // QR fallback slab: (RPLQ * 64) * KCAP doubles, then RPLQ * 64 for the data
 
     // Per-lane state.
-    double x[NCPL];
-    double g[NCPL];
-    double cand[NCPL];
+    real x[NCPL];
+    real g[NCPL];
+    real cand[NCPL];
     bool act[NCPL];
     int pos[NCPL];     // factor position of the column, -1 when not in the factor
     LaneVec<NSK, int> fcol; // column at each factor position
@@ -91,24 +91,24 @@ struct RidgeNnlsWave {
     int y_valid;
     // diag[2 p], diag[2 p + 1]: the factor's diagonal at position p and its division reciprocal
     // (after the factor and its slack); diag_ok: every diagonal in the divisor range.
-    double* diag;
+    real* diag;
     bool diag_ok;
     // The solve's FP64 operations, counted by formula as its steps run (uniform): the
     // algorithmic work that profiled launches report.
     unsigned long long work;
 
-    __device__ RidgeNnlsWave(const RidgeDesignDev& design, double tol_, int max_inner_, double l1_,
-                             double* L_, double* qrs_)
+    __device__ RidgeNnlsWave(const RidgeDesignDev& design, real tol_, int max_inner_, real l1_,
+                             real* L_, real* qrs_)
         : D(design), tol(tol_), max_inner(max_inner_), l1(l1_), L(L_), qrs(qrs_),
           diag(L_ + S * KCAP + WAVE) {}
 
     // L_jj and its reciprocal (one 16-byte read of a uniform address).
-    __device__ __forceinline__ double2 diag_at(int j) const {
-        return *reinterpret_cast<const double2*>(diag + 2 * j);
+    __device__ __forceinline__ real2 diag_at(int j) const {
+        return *reinterpret_cast<const real2*>(diag + 2 * j);
     }
     __device__ __forceinline__ int col(int t) const { return lane_id() + WAVE * t; }
-    __device__ __forceinline__ double& Lat(int i, int j) { return L[j * S + i]; }
-    __device__ __forceinline__ double Luni(int i, int j) const {
+    __device__ __forceinline__ real& Lat(int i, int j) { return L[j * S + i]; }
+    __device__ __forceinline__ real Luni(int i, int j) const {
         // A uniform factor entry, read as a vector access of the lane's neighbour + readlane.
         const int base = j * S + (i / WAVE) * WAVE;
         return readlane(L[base + lane_id()], i % WAVE);
@@ -122,11 +122,11 @@ struct RidgeNnlsWave {
     // a != 0 ? a / L_jj : a from d = (L_jj, its reciprocal); FAST: every diagonal is in the
     // divisor range.
     template <bool FAST>
-    __device__ __forceinline__ double divide_by_diag(double a, double2 d) const {
+    __device__ __forceinline__ real divide_by_diag(real a, real2 d) const {
         if (FAST && dividend_in_range(a)) {
             return divide_in_range(a, d.x, d.y);
         }
-        return a != 0.0 ? a / d.x : a;
+        return a != real(0.0) ? a / d.x : a;
     }
 
     // L w = b on positions [p0, kk) when positions [0, p0) of b already hold their final values
@@ -136,11 +136,11 @@ struct RidgeNnlsWave {
     // the factor column) are issued during step j - 1, so their latency is off the chain of
     // dependent steps; from LDS the plain order issues fewer instructions and is faster.
     template <bool FAST>
-    __device__ __forceinline__ double forward_impl(LaneVec<NSK>& b, int p0, int kk) {
+    __device__ __forceinline__ real forward_impl(LaneVec<NSK>& b, int p0, int kk) {
         const int l = lane_id();
-        double sq = 0.0;
-        double2 d_next = diag_at(0);
-        double l_next[NSK];
+        real sq = real(0.0);
+        real2 d_next = diag_at(0);
+        real l_next[NSK];
         if constexpr (PREFETCH) {
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
@@ -148,8 +148,8 @@ struct RidgeNnlsWave {
             }
         }
         for (int j = 0; j < kk; ++j) {
-            double2 d;
-            double lc[NSK];
+            real2 d;
+            real lc[NSK];
             if constexpr (PREFETCH) {
                 d = d_next;
 #pragma unroll
@@ -169,13 +169,13 @@ struct RidgeNnlsWave {
                     lc[u] = Lat(l + WAVE * u, j);
                 }
             }
-            const double bj = b.get(j);
-            const double wj = j >= p0 ? divide_by_diag<FAST>(bj, d) : bj;
+            const real bj = b.get(j);
+            const real wj = j >= p0 ? divide_by_diag<FAST>(bj, d) : bj;
             sq = sq + wj * wj;
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
                 const int i = l + WAVE * u;
-                const double updated = b.v[u] - lc[u] * wj;
+                const real updated = b.v[u] - lc[u] * wj;
                 const bool below = i > j && i < kk && i >= p0;
                 b.v[u] = i == j ? wj : (below ? updated : b.v[u]);
             }
@@ -183,7 +183,7 @@ struct RidgeNnlsWave {
         return sq;
     }
 
-    __device__ __forceinline__ double forward(LaneVec<NSK>& b, int kk_) {
+    __device__ __forceinline__ real forward(LaneVec<NSK>& b, int kk_) {
         const int kk = __builtin_amdgcn_readfirstlane(kk_);
         return diag_ok ? forward_impl<true>(b, 0, kk) : forward_impl<false>(b, 0, kk);
     }
@@ -208,8 +208,8 @@ struct RidgeNnlsWave {
     __device__ __forceinline__ void backward_impl(LaneVec<NSK>& b, int kk) {
         const int l = lane_id();
         const int top = kk > 0 ? kk - 1 : 0;
-        double2 d_next = diag_at(top);
-        double l_next[NSK];
+        real2 d_next = diag_at(top);
+        real l_next[NSK];
         if constexpr (PREFETCH) {
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
@@ -218,8 +218,8 @@ struct RidgeNnlsWave {
             }
         }
         for (int j = kk - 1; j >= 0; --j) {
-            double2 d;
-            double lc[NSK];
+            real2 d;
+            real lc[NSK];
             if constexpr (PREFETCH) {
                 d = d_next;
 #pragma unroll
@@ -241,12 +241,12 @@ struct RidgeNnlsWave {
                     lc[u] = Lat(j, i < KCAP ? i : 0);
                 }
             }
-            const double wj = b.get(j);
-            const double zj = divide_by_diag<FAST>(wj, d);
+            const real wj = b.get(j);
+            const real zj = divide_by_diag<FAST>(wj, d);
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
                 const int i = l + WAVE * u;
-                const double updated = b.v[u] - lc[u] * zj;
+                const real updated = b.v[u] - lc[u] * zj;
                 b.v[u] = i == j ? zj : (i < j ? updated : b.v[u]);
             }
         }
@@ -262,21 +262,21 @@ struct RidgeNnlsWave {
     }
 
     // The value of a column-spread vector at column c for each lane's factor position.
-    __device__ __forceinline__ double gather_cols(const double (&v)[NCPL], int c) const {
+    __device__ __forceinline__ real gather_cols(const real (&v)[NCPL], int c) const {
         const int src = c % WAVE;
         const int slot = c / WAVE;
-        double out = 0.0;
+        real out = real(0.0);
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {
-            const double w = gather(v[t], src);
+            const real w = gather(v[t], src);
             out = slot == t ? w : out;
         }
         return out;
     }
 
     // The value of a column-spread vector at uniform column c.
-    __device__ __forceinline__ double uniform_col(const double (&v)[NCPL], int c) const {
-        double x = 0.0;
+    __device__ __forceinline__ real uniform_col(const real (&v)[NCPL], int c) const {
+        real x = real(0.0);
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {
             x = (c / WAVE) == t ? v[t] : x;
@@ -286,18 +286,18 @@ struct RidgeNnlsWave {
 
     // Appends column c to the factor; false when its pivot is refused.  With y valid on every
     // position, also extends y to the new one.
-    __device__ __forceinline__ bool append(int c, const double (&corr)[NCPL]) {
+    __device__ __forceinline__ bool append(int c, const real (&corr)[NCPL]) {
         const int l = lane_id();
         LaneVec<NSK> b;
 #pragma unroll
         for (int u = 0; u < NSK; ++u) {
             const int p = l + WAVE * u;
-            b.v[u] = p < k ? D.gram[static_cast<long>(c) * D.n + fcol.v[u]] : 0.0;
+            b.v[u] = p < k ? D.gram[static_cast<long>(c) * D.n + fcol.v[u]] : real(0.0);
         }
-        const double sq = forward(b, k);
+        const real sq = forward(b, k);
         work += static_cast<unsigned long long>(k) * k + 4ull * k + 4ull;
-        const double gcc = D.gram[static_cast<long>(c) * D.n + c];
-        const double pivot = gcc - sq;
+        const real gcc = D.gram[static_cast<long>(c) * D.n + c];
+        const real pivot = gcc - sq;
         if (!(pivot > MIN_RELATIVE_PIVOT * gcc)) {
             return false;
         }
@@ -308,8 +308,8 @@ struct RidgeNnlsWave {
                 Lat(k, p) = b.v[u];
             }
         }
-        const double lkk = sqrt(pivot);
-        const double rkk = division_reciprocal(lkk);
+        const real lkk = sqrt(pivot);
+        const real rkk = division_reciprocal(lkk);
         if (l == k % WAVE) {
             Lat(k, k) = lkk;
             diag[2 * k] = lkk;
@@ -318,13 +318,13 @@ struct RidgeNnlsWave {
         const bool lkk_ok = divisor_in_range(lkk);
         if (y_valid == k) {
             // y_k = (corr_c - L_k0 y_0 - L_k1 y_1 - ...) / L_kk, subtracted in order.
-            const double cc = uniform_col(corr, c);
+            const real cc = uniform_col(corr, c);
             LaneVec<NSK> prod;
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
                 prod.v[u] = b.v[u] * y.v[u];
             }
-            double v = cc;
+            real v = cc;
             for (int j = 0; j < k; ++j) {
                 v = v - prod.get(j);
             }
@@ -351,26 +351,26 @@ struct RidgeNnlsWave {
             for (int u = 0; u < NSK; ++u) {
                 const int i = l + WAVE * u;
                 const bool move = i >= p && i < k - 1 && c <= i + 1;
-                const double v = move ? Lat(i + 1, c) : 0.0;
+                const real v = move ? Lat(i + 1, c) : real(0.0);
                 if (move) {
                     Lat(i, c) = v;
                 }
             }
         }
         for (int column = p; column < k - 1; ++column) {
-            const double diagonal = Luni(column, column);
-            const double above = Luni(column, column + 1);
-            const double length = hypot_fixed(diagonal, above);
+            const real diagonal = Luni(column, column);
+            const real above = Luni(column, column + 1);
+            const real length = hypot_fixed(diagonal, above);
             work += 8ull + 6ull * static_cast<unsigned long long>(k - 1 - column);
-            const double rl = division_reciprocal(length);
-            const double cosine = divide_by(diagonal, length, rl);
-            const double sine = divide_by(above, length, rl);
+            const real rl = division_reciprocal(length);
+            const real cosine = divide_by(diagonal, length, rl);
+            const real sine = divide_by(above, length, rl);
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
                 const int i = l + WAVE * u;
                 if (i >= column && i < k - 1) {
-                    const double left = Lat(i, column);
-                    const double right = Lat(i, column + 1);
+                    const real left = Lat(i, column);
+                    const real right = Lat(i, column + 1);
                     Lat(i, column) = cosine * left + sine * right;
                     Lat(i, column + 1) = cosine * right - sine * left;
                 }
@@ -382,7 +382,7 @@ struct RidgeNnlsWave {
         for (int u = 0; u < NSK; ++u) {
             const int i = l + WAVE * u;
             if (i < k - 1) {
-                double d = diag[2 * i];
+                real d = diag[2 * i];
                 if (i >= p) {
                     d = Lat(i, i);
                     diag[2 * i] = d;
@@ -418,14 +418,14 @@ struct RidgeNnlsWave {
     }
 
     // G_AA z = corr_A from the factor, scattered to the columns (zero elsewhere).
-    __device__ __forceinline__ void solve_factor(const double (&corr)[NCPL]) {
+    __device__ __forceinline__ void solve_factor(const real (&corr)[NCPL]) {
         const int l = lane_id();
         LaneVec<NSK> b;
 #pragma unroll
         for (int u = 0; u < NSK; ++u) {
             const int p = l + WAVE * u;
-            const double v = gather_cols(corr, p < k ? fcol.v[u] : 0);
-            b.v[u] = p < k ? (p < y_valid ? static_cast<double>(y.v[u]) : v) : 0.0;
+            const real v = gather_cols(corr, p < k ? fcol.v[u] : 0);
+            b.v[u] = p < k ? (p < y_valid ? static_cast<real>(y.v[u]) : v) : real(0.0);
         }
         if (y_valid < k) {
             const unsigned long long fresh = static_cast<unsigned long long>(k - y_valid);
@@ -439,13 +439,13 @@ struct RidgeNnlsWave {
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {
             const int p = pos[t] < 0 ? 0 : pos[t];
-            double z = 0.0;
+            real z = real(0.0);
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
-                const double w = gather(b.v[u], p % WAVE);
+                const real w = gather(b.v[u], p % WAVE);
                 z = (p / WAVE) == u ? w : z;
             }
-            cand[t] = pos[t] >= 0 ? z : 0.0;
+            cand[t] = pos[t] >= 0 ? z : real(0.0);
         }
     }
 
@@ -473,12 +473,12 @@ struct RidgeNnlsWave {
         }
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {
-            cand[t] = 0.0;
+            cand[t] = real(0.0);
         }
         if (ka == 0) {
             return true;
         }
-        const bool has_ridge = D.ridge > 0.0;
+        const bool has_ridge = D.ridge > real(0.0);
         const int rows = D.m + (has_ridge ? ka : 0);
         MemCols<RPLQ> A{qrs};
         RowVec<RPLQ> d;
@@ -487,7 +487,7 @@ struct RidgeNnlsWave {
 #pragma unroll
             for (int t = 0; t < RPLQ; ++t) {
                 const int r = l + WAVE * t;
-                double v = 0.0;
+                real v = real(0.0);
                 if (r < D.m) {
                     v = D.kernel[static_cast<long>(r) * D.n + c];
                 } else if (has_ridge && r == D.m + p) {
@@ -499,7 +499,7 @@ struct RidgeNnlsWave {
 #pragma unroll
         for (int t = 0; t < RPLQ; ++t) {
             const int r = l + WAVE * t;
-            double v = 0.0;
+            real v = real(0.0);
             if (r < D.m) {
 #pragma unroll
                 for (int s2 = 0; s2 < RPL; ++s2) {
@@ -517,7 +517,7 @@ struct RidgeNnlsWave {
         least_squares_wave<NSK>(A, rows, ka, d, coef);
         for (int p = 0; p < ka; ++p) {
             const int c = idx.get(p);
-            const double v = coef.get(p);
+            const real v = coef.get(p);
 #pragma unroll
             for (int t = 0; t < NCPL; ++t) {
                 cand[t] = col(t) == c ? v : cand[t];
@@ -528,7 +528,7 @@ struct RidgeNnlsWave {
 
     // fit_on: brings the factor to the active set and solves; false on overflow.
     template <int RPL>
-    __device__ __forceinline__ bool fit_on(const double (&corr)[NCPL], const RowVec<RPL>& signal, int& status) {
+    __device__ __forceinline__ bool fit_on(const real (&corr)[NCPL], const RowVec<RPL>& signal, int& status) {
         const int l = lane_id();
         if (!factor_failed) {
             // Remove the positions whose column left, last to first.
@@ -590,26 +590,46 @@ struct RidgeNnlsWave {
     // Runs the solve; corr = K^T s - l1 per column, signal = s (for the QR fallback).
     // Returns the status bits; x holds the solution.
     template <int RPL>
-    __device__ __forceinline__ int run(const double (&corr)[NCPL], const RowVec<RPL>& signal) {
+    __device__ __forceinline__ int run(const real (&corr)[NCPL], const RowVec<RPL>& signal,
+                                       unsigned long long warm = 0ull) {
         const int l = lane_id();
         int status = 0;
         k = 0;
         y_valid = 0;
-        y.fill(0.0);
+        y.fill(real(0.0));
         diag_ok = true;
         work = 0;
         factor_failed = D.normal_equations == 0;
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {
-            x[t] = 0.0;
+            x[t] = real(0.0);
             g[t] = corr[t];
             act[t] = false;
             pos[t] = -1;
-            cand[t] = 0.0;
+            cand[t] = real(0.0);
         }
         fcol.fill(0);
         int n_inner = 0;
         int n_outer = 0;
+        if (NCPL == 1 && warm != 0ull) {
+            // Warm start from a previous solve's active set (columns < 64): fit on it, dropping
+            // the columns whose coefficient is not positive until the fit is, and start the
+            // Lawson-Hanson loop from that feasible point with its gradient.  The loop then ends
+            // at the same optimum as from x = 0 (to rounding), usually after no entering step.
+            act[0] = col(0) < D.n && ((warm >> col(0)) & 1ull) != 0ull;
+            while (ballot(act[0]) != 0ull) {
+                if (!this->template fit_on<RPL>(corr, signal, status)) {
+                    return status;
+                }
+                const bool drop = act[0] && !(cand[0] > real(0));
+                if (ballot(drop) == 0ull) {
+                    break;
+                }
+                act[0] = act[0] && !drop;
+            }
+            x[0] = act[0] ? cand[0] : real(0);
+            gradient_at_x(corr);
+        }
         while (true) {
             if (++n_outer > outer_guard(D.n)) {
                 status |= ST_OUTER_GUARD | ST_NOT_CONVERGED;
@@ -620,7 +640,7 @@ struct RidgeNnlsWave {
             for (int t = 0; t < NCPL; ++t) {
                 free_col[t] = col(t) < D.n && !act[t];
             }
-            double bv = 0.0;
+            real bv = real(0.0);
             int bi = -1;
             wave_argmax_cols<NCPL>(g, free_col, bv, bi);
             if (bi < 0 || bv <= tol) {
@@ -637,7 +657,7 @@ struct RidgeNnlsWave {
                 bool mine = false;
 #pragma unroll
                 for (int t = 0; t < NCPL; ++t) {
-                    mine = mine || (col(t) < D.n && act[t] && cand[t] <= 0.0);
+                    mine = mine || (col(t) < D.n && act[t] && cand[t] <= real(0.0));
                 }
                 if (!ballot(mine)) {
                     break;
@@ -649,15 +669,15 @@ struct RidgeNnlsWave {
                     }
                     return status | ST_NOT_CONVERGED;
                 }
-                double step[NCPL];
+                real step[NCPL];
                 bool blocking[NCPL];
 #pragma unroll
                 for (int t = 0; t < NCPL; ++t) {
-                    blocking[t] = col(t) < D.n && act[t] && cand[t] <= 0.0;
-                    const double distance = x[t] - cand[t];
-                    step[t] = (blocking[t] && distance > 0.0) ? x[t] / distance : 0.0;
+                    blocking[t] = col(t) < D.n && act[t] && cand[t] <= real(0.0);
+                    const real distance = x[t] - cand[t];
+                    step[t] = (blocking[t] && distance > real(0.0)) ? x[t] / distance : real(0.0);
                 }
-                double sv = 0.0;
+                real sv = real(0.0);
                 int si = -1;
                 wave_argmin_cols<NCPL>(step, blocking, sv, si);
                 work += 3ull * static_cast<unsigned long long>(D.n);
@@ -665,7 +685,7 @@ struct RidgeNnlsWave {
                 for (int t = 0; t < NCPL; ++t) {
                     x[t] = x[t] + sv * (cand[t] - x[t]);
                     if (col(t) == si) {
-                        x[t] = 0.0;
+                        x[t] = real(0.0);
                         act[t] = false;
                     }
                     act[t] = act[t] && fabs(x[t]) >= tol;
@@ -677,31 +697,44 @@ struct RidgeNnlsWave {
 #pragma unroll
             for (int t = 0; t < NCPL; ++t) {
                 x[t] = cand[t];
-                g[t] = corr[t];
             }
-            // Gradient: corr - sum over non-zero x_c, in column order, of x_c G[:, c].
-#pragma unroll
-            for (int t2 = 0; t2 < NCPL; ++t2) {
-                unsigned long long mask = ballot(col(t2) < D.n && x[t2] != 0.0);
-                while (mask) {
-                    const int b = __ffsll(static_cast<long long>(mask)) - 1;
-                    mask &= mask - 1;
-                    const int c = WAVE * t2 + b;
-                    work += 2ull * static_cast<unsigned long long>(D.n);
-                    const double xc = readlane(x[t2], b);
-                    const double* grow = D.gram + static_cast<long>(c) * D.n;
-#pragma unroll
-                    for (int t = 0; t < NCPL; ++t) {
-                        if (col(t) < D.n) {
-                            g[t] = g[t] - xc * grow[col(t)];
-                        }
-                    }
-                }
-            }
+            gradient_at_x(corr);
         }
         (void)l;
         return status;
     }
+
+    // The support of the solution as a column mask (columns < 64, NCPL == 1): a warm start for
+    // the next solve of the same voxel.
+    __device__ __forceinline__ unsigned long long support() const {
+        return ballot(col(0) < D.n && x[0] != real(0));
+    }
+
+    // Gradient: corr - sum over non-zero x_c, in column order, of x_c G[:, c].
+    __device__ __forceinline__ void gradient_at_x(const real (&corr)[NCPL]) {
+#pragma unroll
+        for (int t = 0; t < NCPL; ++t) {
+            g[t] = corr[t];
+        }
+#pragma unroll
+        for (int t2 = 0; t2 < NCPL; ++t2) {
+            unsigned long long mask = ballot(col(t2) < D.n && x[t2] != real(0.0));
+            while (mask) {
+                const int b = __ffsll(static_cast<long long>(mask)) - 1;
+                mask &= mask - 1;
+                const int c = WAVE * t2 + b;
+                work += 2ull * static_cast<unsigned long long>(D.n);
+                const real xc = readlane(x[t2], b);
+                const real* grow = D.gram + static_cast<long>(c) * D.n;
+#pragma unroll
+                for (int t = 0; t < NCPL; ++t) {
+                    if (col(t) < D.n) {
+                        g[t] = g[t] - xc * grow[col(t)];
+                    }
+                }
+            }
+        }
+    }
 };
 
-} // namespace lsd
+} // namespace LSD_NS

@@ -5,6 +5,7 @@
 
 #include "csd_wave.hpp"
 #include "kernels.hpp"
+#include "objects.hpp"
 #include "drl_wave.hpp"
 #include "lasso_wave.hpp"
 #include "lsi_host.hpp"
@@ -22,210 +23,25 @@ bool g_profiling = false;
 std::vector<ProfiledLaunch> g_launches;
 unsigned long long* g_work_slots = nullptr;
 
-__global__ void zero_ints(int* p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        p[i] = 0;
-    }
-}
-
-// Starts the loop state: isotropic fractions 0, previous fractions +inf, no rounds, no status.
-// Clears the per-voxel state of a fit; rounds may be null (single-tissue fits have none).
-__global__ void init_loop_state(double* iso, long n_iso, double* prev, long n_prev, int* rounds,
-                                int* status, long n_vox) {
-    const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n_iso) iso[i] = 0.0;
-    if (i < n_prev) prev[i] = __builtin_inf();
-    if (i < n_vox) {
-        if (rounds != nullptr) rounds[i] = 0;
-        status[i] = 0;
-    }
-}
-
-std::string shape_tuple(int64_t r, int64_t c) {
-    return "(" + std::to_string(r) + ", " + std::to_string(c) + ")";
-}
-
-// Which instantiation covers a problem size.
-int ncpl_for(int64_t n) {
-    if (n <= 64) return 1;
-    if (n <= 192) return 3;
-    if (n <= 320) return 5;
-    if (n <= 640) return 10;
-    return -1;
-}
-int rpl_for(int64_t m) {
-    if (m <= 256) return 4;
-    if (m <= 384) return 6;
-    if (m <= 512) return 8;
-    return -1;
-}
-
-// The launchers of an (NCPL, RPL) instantiation.
-int dispatch_fod(const char* name, const char* rerun_name, int ncpl, int rpl, const FodArgs& fast,
-                 const FodArgs& rerun, int waves, int* ol, int* oc, hipStream_t stream,
-                 RerunScratch* rerun_scratch) {
-#define LSD_TRY(NC, RP)                                                                        \
-    if (ncpl == NC && rpl == RP)                                                               \
-        return launch_fod_##NC##_##RP(name, rerun_name, fast, rerun, waves, ol, oc, stream,     \
-                                      rerun_scratch);
-    LSD_FOR_EACH_LAUNCHER(LSD_TRY)
-#undef LSD_TRY
-    return fail("no kernel instantiation for " + std::to_string(ncpl) + " column slots and " +
-                std::to_string(rpl) + " row slots");
-}
-
-int dispatch_fraction(const char* name, int ncpl, int rpl, const FractionArgs& args, int waves,
-                      hipStream_t stream) {
-#define LSD_TRY(NC, RP)                                                                        \
-    if (ncpl == NC && rpl == RP) return launch_fraction_##NC##_##RP(name, args, waves, stream);
-    LSD_FOR_EACH_LAUNCHER(LSD_TRY)
-#undef LSD_TRY
-    return fail("no kernel instantiation for " + std::to_string(ncpl) + " column slots and " +
-                std::to_string(rpl) + " row slots");
-}
-
-// The device memory of one call.  By default each buffer is a stream-ordered allocation, freed
-// (stream-ordered) when the call returns.  Given a fixed region instead (the allocation-free
-// entry point), buffers are carved from it in order, 256-byte aligned, and a request beyond its
-// end fails; what is left after the last buffer serves the rerun scratch.
-class Workspace {
-public:
-    explicit Workspace(hipStream_t stream) : stream_(stream) {}
-    Workspace(void* base, size_t bytes, hipStream_t stream)
-        : stream_(stream), base_(static_cast<char*>(base)), capacity_(bytes), fixed_(true) {}
-    Workspace(const Workspace&) = delete;
-    Workspace& operator=(const Workspace&) = delete;
-    ~Workspace() {
-        for (void* p : held_) {
-            (void)hipFreeAsync(p, stream_);
-        }
-    }
-    template <class T>
-    int get(T** out, long count) {
-        const size_t bytes = sizeof(T) * static_cast<size_t>(std::max(1L, count));
-        if (fixed_) {
-            const size_t at = align(used_);
-            if (at > capacity_ || bytes > capacity_ - at) {
-                return fail("the fit needs more than the " + std::to_string(capacity_) +
-                            " bytes of the allocation-free workspace; pass fewer voxels per call");
-            }
-            *out = reinterpret_cast<T*>(base_ + at);
-            used_ = at + bytes;
-            return 0;
-        }
-        void* p = nullptr;
-        const int rc = device_alloc(&p, bytes, stream_);
-        if (rc) return rc;
-        held_.push_back(p);
-        *out = static_cast<T*>(p);
-        return 0;
-    }
-    // Frees p with the call's own buffers (the rerun scratch allocated by a launcher).
-    void adopt(void* p) {
-        if (p && !fixed_) held_.push_back(p);
-    }
-    bool fixed() const { return fixed_; }
-    // The region after the last buffer, for the rerun scratch (empty, to be allocated by the
-    // launcher, when the workspace is not fixed).
-    RerunScratch rest() const {
-        RerunScratch r;
-        if (fixed_) {
-            const size_t at = align(used_);
-            r.base = base_ + std::min(at, capacity_);
-            r.bytes = at < capacity_ ? capacity_ - at : 0;
-            r.fixed = true;
-        }
-        return r;
-    }
-
-private:
-    static size_t align(size_t at) { return (at + 255) & ~static_cast<size_t>(255); }
-    hipStream_t stream_;
-    char* base_ = nullptr;
-    size_t capacity_ = 0;
-    size_t used_ = 0;
-    bool fixed_ = false;
-    std::vector<void*> held_;
-};
-
 } // namespace lsd
 
 using namespace lsd;
 
-struct lsd_solver {
-    int64_t m = 0;
-    int64_t n = 0;
-    double ridge = 0.0;
-    double tol = 0.0;
-    int normal_equations = 0;
-    double* d_kernel = nullptr; // K (m, n), then CORR_COLUMNS (>= DRL_COLUMNS) zeros
-    double* d_gram = nullptr;   // (n, n); a dRL solver's then DRL_COLUMNS zeros
-    // Damped Richardson-Lucy (FodSolver.drl) instead of ridge NNLS.
-    bool drl = false;
-    double nu = 0.0;
-    int64_t n_iter = 0;
-    double init = 0.0;
-    // Elastic net (FodSolver.lasso) instead of ridge NNLS.
-    bool lasso = false;
-    double l1_threshold = 0.0;
-    double shrink = 1.0;
-    int64_t max_sweeps = 0;
-    double* d_columns_t = nullptr;          // (n, m) the scaled columns
-    double* d_scales = nullptr;             // (n)
-    int* d_nonzero = nullptr;               // (n)
-    unsigned long long* d_unconverged = nullptr;
-};
-
-struct lsd_tissue_kernel {
-    int64_t m = 0;
-    int64_t ca = 0;
-    int64_t ci = 0;
-    std::vector<int> blocks;     // (n_blocks, 2)
-    double* d_aniso_t = nullptr; // (ca, m)
-    double* d_iso_t = nullptr;   // (ci, m)
-    double* d_w_iso = nullptr;   // (m, ci)
-    int* d_blocks = nullptr;
-};
-
 
 namespace lsd {
 
-RidgeDesignDev design_of(const lsd_solver* s) {
-    return RidgeDesignDev{static_cast<int>(s->m), static_cast<int>(s->n), s->d_kernel, s->d_gram,
-                          s->ridge, s->normal_equations};
-}
-
-FractionKernelDev fractions_of(const lsd_tissue_kernel* k) {
-    return FractionKernelDev{static_cast<int>(k->m), static_cast<int>(k->ca), static_cast<int>(k->ci),
-                             k->d_aniso_t, k->d_iso_t, k->d_blocks,
-                             static_cast<int>(k->blocks.size() / 2)};
-}
-
-// Grid of the main launch: enough waves to fill the chip, never more than voxels.
-int main_waves(int64_t n_vox) {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    int cus = 0;
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (cus <= 0) {
-        cus = 256;
-    }
-    // Up to 8 waves per SIMD; each persistent launch then clamps to what its kernel keeps
-    // resident (launch_fill).
-    const int64_t target = static_cast<int64_t>(cus) * 32;
-    int64_t waves = std::min<int64_t>(target, std::max<int64_t>(n_vox, 1));
-    waves = (waves + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK * WAVES_PER_BLOCK;
-    return static_cast<int>(waves);
-}
-
-
-int upload(double** dst, const std::vector<double>& src) {
-    LSD_HIP(hipMalloc(reinterpret_cast<void**>(dst), std::max<size_t>(8, src.size() * sizeof(double))));
+template <class T>
+int upload(T** dst, const std::vector<T>& src) {
+    LSD_HIP(hipMalloc(reinterpret_cast<void**>(dst), std::max<size_t>(8, src.size() * sizeof(T))));
     if (!src.empty()) {
-        LSD_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice));
+        LSD_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
     }
     return 0;
+}
+
+// The FP32 copy of an FP64 host array, uploaded.
+int upload_f32(float** dst, const std::vector<double>& src) {
+    return upload(dst, std::vector<float>(src.begin(), src.end()));
 }
 
 // CSD launcher (inst_csd.hip).
@@ -398,6 +214,8 @@ int run_lasso(const char* name, const lsd_solver* solver, LassoArgs base, long n
 
 } // namespace lsd
 
+#include "fit_multi.hpp"
+
 struct lsd_sphere {
     int64_t nv = 0;
     int64_t n_coeffs = 0;          // columns of the basis, 0 without one
@@ -429,284 +247,6 @@ struct lsd_csd {
     double* d_constraint_t = nullptr; // (c, k)
     double* d_gram = nullptr;         // (c, c)
 };
-
-// The multi-tissue fit with its per-call buffers taken from mem.  With a fixed workspace the
-// high-resolution correlations are computed inside the FOD kernel (the same in-order sums) rather
-// than held per voxel, and one rerun region after the per-voxel state serves the low- and
-// high-resolution solves in turn (they run one after the other on the stream).
-static int fit_multi_tissue_in(Workspace& mem, const double* signals, int64_t n_voxels, int64_t m,
-                               const double* weights, const lsd_tissue_kernel* lr,
-                               const lsd_solver* lr_solver, const lsd_tissue_kernel* hr,
-                               const lsd_solver* hr_solver, int64_t max_rounds,
-                               double fraction_tol, double final_tol, double* fods,
-                               double* fractions, int32_t* rounds, int32_t* status,
-                               hipStream_t stream) {
-    if (m != hr->m || lr->m != m) {
-        return fail("signals " + shape_tuple(n_voxels, m) + ", weights " + std::to_string(m) +
-                    " and the kernels' " + std::to_string(hr->m) + " measurements do not match");
-    }
-    if (lr->ci != hr->ci) {
-        return fail("the two kernels have different numbers of isotropic tissues");
-    }
-    auto matching = [&](const char* name, const lsd_tissue_kernel* k, const lsd_solver* s) {
-        if (s->n != k->ca || s->m != m) {
-            return fail(std::string("the ") + name + " solver (" + std::to_string(s->n) +
-                        " directions, " + std::to_string(s->m) +
-                        " measurements) does not match its kernel (" + std::to_string(k->ca) +
-                        " directions, " + std::to_string(m) + " measurements)");
-        }
-        return 0;
-    };
-    int rc = matching("low-resolution", lr, lr_solver);
-    if (!rc) rc = matching("high-resolution", hr, hr_solver);
-    if (rc) return rc;
-    if (max_rounds < 0) {
-        return fail("max_outer must be at least 0");
-    }
-    const int ncpl_lr = ncpl_for(lr->ca);
-    const int ncpl_hr = ncpl_for(hr->ca);
-    const int rpl = rpl_for(m);
-    if (ncpl_lr < 0 || ncpl_hr < 0 || rpl < 0) {
-        return fail("problem size exceeds 512 measurements or 640 directions");
-    }
-    if (n_voxels == 0) {
-        return 0;
-    }
-    const long nv = static_cast<long>(n_voxels);
-    const int ci = static_cast<int>(lr->ci);
-    const int nlf = 1 + ci;
-    const int waves = main_waves(n_voxels);
-    // counters: per round its overflow and next-list counts, then the final solve's overflow.
-    const long n_counters = 2 * (max_rounds + 1);
-    int* counters = nullptr;
-    int* olist = nullptr;
-    int* lists = nullptr;
-    double* scratch = nullptr;
-    double* iso = nullptr;
-    double* prev = nullptr;
-    double* x_lr = nullptr;
-    double* corr_lr = nullptr;
-    rc = mem.get(&counters, n_counters);
-    if (!rc) rc = mem.get(&olist, nv);
-    if (!rc) rc = mem.get(&lists, 2 * nv);
-    if (!rc) rc = mem.get(&scratch, 1);
-    if (!rc) rc = mem.get(&iso, std::max(1L, nv * ci));
-    if (!rc) rc = mem.get(&prev, nv * nlf);
-    if (!rc) rc = mem.get(&x_lr, nv * lr->ca);
-    if (!rc && !lr_solver->drl && !lr_solver->lasso) rc = mem.get(&corr_lr, std::max(1L, nv * lr->ca));
-    if (!rc && fods == nullptr) rc = mem.get(&fods, nv * hr->ca);
-    if (!rc && rounds == nullptr) rc = mem.get(&rounds, nv);
-    if (!rc && status == nullptr) rc = mem.get(&status, nv);
-    if (rc) return rc;
-    if (mem.fixed()) {
-        // Refuse before any launch when the rest of the workspace holds less than one block of
-        // rerun slabs for either resolution's solve.
-        const size_t need = sizeof(double) * WAVES_PER_BLOCK *
-                            static_cast<size_t>(std::max(rerun_slab_doubles(ncpl_lr, rpl),
-                                                         rerun_slab_doubles(ncpl_hr, rpl)));
-        if (mem.rest().bytes < need) {
-            return fail("the fit needs more than the " + std::to_string(mem.rest().bytes) +
-                        " bytes left of the allocation-free workspace for its rerun slabs (" +
-                        std::to_string(need) + "); pass fewer voxels per call");
-        }
-    }
-    hipLaunchKernelGGL(zero_ints, dim3((n_counters + 255) / 256), dim3(256), 0, stream, counters,
-                       static_cast<int>(n_counters));
-    hipLaunchKernelGGL(init_loop_state, dim3((nv * nlf + 255) / 256), dim3(256), 0, stream, iso,
-                       nv * ci, prev, nv * nlf, rounds, status, nv);
-
-    FodArgs fod{};
-    fod.signals = signals;
-    fod.m = static_cast<int>(m);
-    fod.multi = 1;
-    fod.weights = weights;
-    fod.iso = iso;
-    fod.ci = ci;
-    fod.status = status;
-    fod.scratch = SolveScratch{scratch, 0};
-    FractionArgs fr{};
-    fr.signals = signals;
-    fr.m = static_cast<int>(m);
-    fr.fraction_tol = fraction_tol;
-    fr.final_tol = final_tol;
-    fr.iso = iso;
-    fr.prev = prev;
-    fr.rounds = rounds;
-    fr.status = status;
-    fr.n_vox = nv;
-
-    DrlBuffers drl_lr;
-    if (lr_solver->drl) {
-        rc = drl_buffers(lr_solver, nv, mem, drl_lr);
-        if (rc) return rc;
-    }
-
-    // The low-resolution rounds: round r fits the voxels of its list (all of them in round 0)
-    // and lists those that go on to round r + 1; lists and counts stay on the device.
-    const int* list = nullptr;
-    const int* count = nullptr;
-    // The rerun scratch: allocated by the first round's rerun and reused after (or the fixed
-    // workspace's rest, shared with the high-resolution solve).
-    RerunScratch lr_rerun_scratch;
-    for (int64_t r = 0; r < max_rounds && !rc; ++r) {
-        int* ovf_count = counters + 2 * r;
-        int* next_count = counters + 2 * r + 1;
-        int* next_list = lists + (r % 2) * nv;
-        fod.design = design_of(lr_solver);
-        fod.tol = lr_solver->tol;
-        fod.w_iso = lr->d_w_iso;
-        fod.x_out = x_lr;
-        fod.vs = VoxelSet{nv, list, count, olist, ovf_count};
-        if (lr_solver->drl) {
-            DrlArgs da{};
-            da.signals = signals;
-            da.m = static_cast<int>(m);
-            da.multi = 1;
-            da.weights = weights;
-            da.w_iso = lr->d_w_iso;
-            da.iso = iso;
-            da.ci = ci;
-            da.list = list;
-            da.list_count = count;
-            da.n_vox = nv;
-            da.x_out = x_lr;
-            da.status = status;
-            rc = run_drl("drl_lr", lr_solver, da, drl_lr, stream);
-        } else if (lr_solver->lasso) {
-            LassoArgs la{};
-            la.signals = signals;
-            la.m = static_cast<int>(m);
-            la.multi = 1;
-            la.weights = weights;
-            la.w_iso = lr->d_w_iso;
-            la.iso = iso;
-            la.ci = ci;
-            la.list = list;
-            la.list_count = count;
-            la.n_vox = nv;
-            la.x_out = x_lr;
-            la.status = status;
-            rc = run_lasso("lasso_lr", lr_solver, la, nv, stream);
-        } else {
-            CorrArgs ca{};
-            ca.signals = signals;
-            ca.m = static_cast<int>(m);
-            ca.multi = 1;
-            ca.weights = weights;
-            ca.w_iso = lr->d_w_iso;
-            ca.iso = iso;
-            ca.ci = ci;
-            ca.kernel = fod.design.kernel;
-            ca.n = fod.design.n;
-            ca.list = list;
-            ca.list_count = count;
-            ca.n_vox = nv;
-            ca.corr = corr_lr;
-            rc = launch_fill("fod_lr_corr", correlation_kernel<CORR_COLUMNS>, ca,
-                             static_cast<int>((nv + WAVE - 1) / WAVE), 0, stream);
-            if (rc) break;
-            fod.corr = corr_lr;
-            if (lr_rerun_scratch.base == nullptr && !lr_rerun_scratch.fixed) {
-                lr_rerun_scratch = mem.rest();
-            }
-            const bool first = lr_rerun_scratch.base == nullptr;
-            rc = dispatch_fod("fod_lr", "fod_lr_rerun", ncpl_lr, rpl, fod, fod, waves, olist,
-                              ovf_count, stream, &lr_rerun_scratch);
-            fod.corr = nullptr;
-            if (first) {
-                mem.adopt(lr_rerun_scratch.base);
-            }
-        }
-        if (rc) break;
-        fr.x = x_lr;
-        fr.kernel = fractions_of(lr);
-        fr.final_fit = 0;
-        fr.list = list;
-        fr.list_count = count;
-        fr.next_list = next_list;
-        fr.next_count = next_count;
-        rc = dispatch_fraction("fraction_loop", ncpl_lr, rpl, fr, waves, stream);
-        list = next_list;
-        count = next_count;
-    }
-    if (!rc) {
-        // The high-resolution FOD of every voxel, then its final fractions.
-        if (hr_solver->drl) {
-            DrlBuffers drl_hr;
-            rc = drl_buffers(hr_solver, nv, mem, drl_hr);
-            DrlArgs da{};
-            da.signals = signals;
-            da.m = static_cast<int>(m);
-            da.multi = 1;
-            da.weights = weights;
-            da.w_iso = hr->d_w_iso;
-            da.iso = iso;
-            da.ci = ci;
-            da.n_vox = nv;
-            da.x_out = fods;
-            da.status = status;
-            if (!rc) rc = run_drl("drl_hr", hr_solver, da, drl_hr, stream);
-        } else if (hr_solver->lasso) {
-            LassoArgs la{};
-            la.signals = signals;
-            la.m = static_cast<int>(m);
-            la.multi = 1;
-            la.weights = weights;
-            la.w_iso = hr->d_w_iso;
-            la.iso = iso;
-            la.ci = ci;
-            la.n_vox = nv;
-            la.x_out = fods;
-            la.status = status;
-            rc = run_lasso("lasso_hr", hr_solver, la, nv, stream);
-        } else {
-            int* ovf_count = counters + 2 * max_rounds;
-            fod.design = design_of(hr_solver);
-            fod.tol = hr_solver->tol;
-            fod.w_iso = hr->d_w_iso;
-            fod.x_out = fods;
-            fod.vs = VoxelSet{nv, nullptr, nullptr, olist, ovf_count};
-            // The correlations of every voxel by the lane-per-voxel pass, as in the rounds (a fixed
-            // workspace leaves them to the FOD kernel).
-            double* corr_hr = nullptr;
-            if (!mem.fixed()) rc = mem.get(&corr_hr, std::max(1L, nv * fod.design.n));
-            if (!rc && corr_hr != nullptr) {
-                CorrArgs ca{};
-                ca.signals = signals;
-                ca.m = static_cast<int>(m);
-                ca.multi = 1;
-                ca.weights = weights;
-                ca.w_iso = hr->d_w_iso;
-                ca.iso = iso;
-                ca.ci = ci;
-                ca.kernel = fod.design.kernel;
-                ca.n = fod.design.n;
-                ca.n_vox = nv;
-                ca.corr = corr_hr;
-                rc = launch_fill("fod_hr_corr", correlation_kernel<CORR_COLUMNS>, ca,
-                                 static_cast<int>((nv + WAVE - 1) / WAVE), 0, stream);
-            }
-            if (!rc) {
-                fod.corr = corr_hr;
-                RerunScratch rerun_scratch = mem.rest();
-                rc = dispatch_fod("fod_hr", "fod_hr_rerun", ncpl_hr, rpl, fod, fod, waves, olist,
-                                  ovf_count, stream, &rerun_scratch);
-                fod.corr = nullptr;
-                mem.adopt(rerun_scratch.base);
-            }
-        }
-    }
-    if (!rc) {
-        fr.x = fods;
-        fr.kernel = fractions_of(hr);
-        fr.final_fit = 1;
-        fr.fractions = fractions;
-        fr.list = nullptr;
-        fr.list_count = nullptr;
-        rc = dispatch_fraction("fraction_final", ncpl_hr, rpl, fr, waves, stream);
-    }
-    return rc;
-}
 
 // The workspace of lsd_fit_multi_tissue_noalloc: a static device array, so that the entry
 // point allocates nothing.  Every device the library runs on holds its own copy for the life of
@@ -857,6 +397,15 @@ int lsd_solver_create_l2nnls(const double* kernel, int64_t m, int64_t n, double 
     std::copy(kernel, kernel + m * n, k.begin());
     int rc = upload(&s->d_kernel, k);
     if (!rc) rc = upload(&s->d_gram, gram);
+    if (!rc) rc = upload_f32(&s->d_kernel_f32, k);
+    if (!rc) rc = upload_f32(&s->d_gram_f32, gram);
+    if (!rc && n >= 1 && n <= MFMA_TILE * MFMA_MAX_TILES && m >= 1) {
+        std::vector<unsigned short> hi;
+        std::vector<unsigned short> lo;
+        s->bf16_tiles = pack_bf16_halves(kernel, static_cast<int>(m), static_cast<int>(n), hi, lo);
+        rc = upload(&s->d_kernel_bf16_hi, hi);
+        if (!rc) rc = upload(&s->d_kernel_bf16_lo, lo);
+    }
     if (rc) {
         lsd_solver_destroy(s);
         return rc;
@@ -989,6 +538,10 @@ void lsd_solver_destroy(lsd_solver* s) {
     if (!s) return;
     if (s->d_kernel) (void)hipFree(s->d_kernel);
     if (s->d_gram) (void)hipFree(s->d_gram);
+    if (s->d_kernel_f32) (void)hipFree(s->d_kernel_f32);
+    if (s->d_kernel_bf16_hi) (void)hipFree(s->d_kernel_bf16_hi);
+    if (s->d_kernel_bf16_lo) (void)hipFree(s->d_kernel_bf16_lo);
+    if (s->d_gram_f32) (void)hipFree(s->d_gram_f32);
     if (s->d_columns_t) (void)hipFree(s->d_columns_t);
     if (s->d_scales) (void)hipFree(s->d_scales);
     if (s->d_nonzero) (void)hipFree(s->d_nonzero);
@@ -1045,6 +598,9 @@ int lsd_tissue_kernel_create(const double* k_aniso, const double* k_iso, const d
     int rc = upload(&k->d_aniso_t, at);
     if (!rc) rc = upload(&k->d_iso_t, it);
     if (!rc) rc = upload(&k->d_w_iso, wi);
+    if (!rc) rc = upload_f32(&k->d_aniso_t_f32, at);
+    if (!rc) rc = upload_f32(&k->d_iso_t_f32, it);
+    if (!rc) rc = upload_f32(&k->d_w_iso_f32, wi);
     if (!rc) {
         const hipError_t e1 = hipMalloc(reinterpret_cast<void**>(&k->d_blocks),
                                         sizeof(int) * k->blocks.size());
@@ -1067,6 +623,9 @@ void lsd_tissue_kernel_destroy(lsd_tissue_kernel* k) {
     if (k->d_aniso_t) (void)hipFree(k->d_aniso_t);
     if (k->d_iso_t) (void)hipFree(k->d_iso_t);
     if (k->d_w_iso) (void)hipFree(k->d_w_iso);
+    if (k->d_aniso_t_f32) (void)hipFree(k->d_aniso_t_f32);
+    if (k->d_iso_t_f32) (void)hipFree(k->d_iso_t_f32);
+    if (k->d_w_iso_f32) (void)hipFree(k->d_w_iso_f32);
     if (k->d_blocks) (void)hipFree(k->d_blocks);
     delete k;
 }
@@ -1188,6 +747,75 @@ int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m, con
     return fit_multi_tissue_in(mem, signals, n_voxels, m, weights, lr, lr_solver, hr, hr_solver,
                                max_rounds, fraction_tol, final_tol, fods, fractions, rounds,
                                status, stream);
+}
+
+int lsd_correlations(const double* signals, int64_t n_voxels, int64_t m, const double* weights,
+                     const lsd_tissue_kernel* kernel, const lsd_solver* solver, const double* iso,
+                     int precision, double* corr, void* stream_) {
+    auto stream = static_cast<hipStream_t>(stream_);
+    if (m != kernel->m || solver->m != m || solver->n != kernel->ca || solver->drl ||
+        solver->lasso) {
+        return fail("signals " + shape_tuple(n_voxels, m) + ", the kernel (" +
+                    std::to_string(kernel->m) + " measurements, " + std::to_string(kernel->ca) +
+                    " directions) and the ridge NNLS solver (" + std::to_string(solver->m) +
+                    ", " + std::to_string(solver->n) + ") do not match");
+    }
+    if (precision != 0 && precision != 1) {
+        return fail("precision must be 0 (FP64) or 1 (BF16 split), got " +
+                    std::to_string(precision));
+    }
+    if (precision == 1 && solver->d_kernel_bf16_hi == nullptr) {
+        return fail("the BF16 correlations take at most 64 directions");
+    }
+    if (n_voxels == 0) {
+        return 0;
+    }
+    if (precision == 1) {
+        CorrMfmaArgs ma{};
+        ma.signals = signals;
+        ma.m = static_cast<int>(m);
+        ma.weights = weights;
+        ma.w_iso = kernel->d_w_iso;
+        ma.iso = iso;
+        ma.ci = static_cast<int>(kernel->ci);
+        ma.k_hi = solver->d_kernel_bf16_hi;
+        ma.k_lo = solver->d_kernel_bf16_lo;
+        ma.n = static_cast<int>(solver->n);
+        ma.chunks = static_cast<int>((m + MFMA_ROWS - 1) / MFMA_ROWS);
+        ma.n_vox = n_voxels;
+        ma.corr = corr;
+        return launch_corr_bf16(ma, solver->bf16_tiles, stream);
+    }
+    CorrArgs ca{};
+    ca.signals = signals;
+    ca.m = static_cast<int>(m);
+    ca.multi = 1;
+    ca.weights = weights;
+    ca.w_iso = kernel->d_w_iso;
+    ca.iso = iso;
+    ca.ci = static_cast<int>(kernel->ci);
+    ca.kernel = solver->d_kernel;
+    ca.n = static_cast<int>(solver->n);
+    ca.n_vox = n_voxels;
+    ca.corr = corr;
+    return launch_fill("correlations", correlation_kernel<CORR_COLUMNS>, ca,
+                       static_cast<int>((n_voxels + WAVE - 1) / WAVE), 0, stream);
+}
+
+int lsd_fit_multi_tissue_mixed(const double* signals, int64_t n_voxels, int64_t m,
+                               const double* weights, const lsd_tissue_kernel* lr,
+                               const lsd_solver* lr_solver, const lsd_tissue_kernel* hr,
+                               const lsd_solver* hr_solver, int64_t max_rounds,
+                               double fraction_tol, double final_tol, double* fods,
+                               double* fractions, int32_t* rounds, int32_t* status,
+                               void* stream_) {
+    auto stream = static_cast<hipStream_t>(stream_);
+    Workspace mem(stream);
+    FitForm form;
+    form.mixed = true;
+    return fit_multi_tissue_in(mem, signals, n_voxels, m, weights, lr, lr_solver, hr, hr_solver,
+                               max_rounds, fraction_tol, final_tol, fods, fractions, rounds,
+                               status, stream, form);
 }
 
 int lsd_fit_multi_tissue_noalloc(const double* signals, int64_t n_voxels, int64_t m,

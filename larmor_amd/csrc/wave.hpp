@@ -19,7 +19,23 @@
 
 #include <cstdint>
 
-namespace lsd {
+// The arithmetic type of the solve kernels: double (namespace lsd), or float (namespace lsd32) in
+// the translation units of the plain FP32 form, which define LSD_FP32 before the first include.
+#ifdef LSD_FP32
+#define LSD_NS lsd32
+#else
+#define LSD_NS lsd
+#endif
+
+namespace LSD_NS {
+
+#ifdef LSD_FP32
+using real = float;
+using real2 = float2;
+#else
+using real = double;
+using real2 = double2;
+#endif
 
 constexpr int WAVE = 64;
 
@@ -37,9 +53,13 @@ __device__ __forceinline__ double readlane(double v, int src) {
 }
 
 __device__ __forceinline__ int readlane(int v, int src) { return __builtin_amdgcn_readlane(v, src); }
+__device__ __forceinline__ float readlane(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
 
 // The value of v in lane src, src per lane.
 __device__ __forceinline__ double gather(double v, int src) { return __shfl(v, src); }
+__device__ __forceinline__ float gather(float v, int src) { return __shfl(v, src); }
 __device__ __forceinline__ int gather(int v, int src) { return __shfl(v, src); }
 
 __device__ __forceinline__ unsigned long long ballot(bool p) { return __ballot(p); }
@@ -80,6 +100,10 @@ __device__ __forceinline__ int xor_lanes(int x, int o) {
     return static_cast<int>(xor_lanes(static_cast<unsigned>(x), o));
 }
 
+__device__ __forceinline__ float xor_lanes(float x, int o) {
+    return __uint_as_float(xor_lanes(__float_as_uint(x), o));
+}
+
 __device__ __forceinline__ double xor_lanes(double x, int o) {
     const auto bits = static_cast<unsigned long long>(__double_as_longlong(x));
     const unsigned lo = xor_lanes(static_cast<unsigned>(bits & 0xffffffffull), o);
@@ -97,7 +121,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // The canonical butterfly of 64 lane partials.  Every lane computes the same sum (each step adds
 // the same two values, in either order); lane 0's, read back, is provably uniform to the
 // compiler, which keeps the control flow that depends on it scalar.
-__device__ __forceinline__ double butterfly_sum(double p) {
+__device__ __forceinline__ real butterfly_sum(real p) {
 #pragma unroll
     for (int o = WAVE / 2; o >= 1; o >>= 1) {
         p = p + xor_lanes(p, o);
@@ -106,10 +130,10 @@ __device__ __forceinline__ double butterfly_sum(double p) {
 }
 
 // max over lanes (exact, order-free; with NaN values, lane 0's result).
-__device__ __forceinline__ double wave_max(double p) {
+__device__ __forceinline__ real wave_max(real p) {
 #pragma unroll
     for (int o = WAVE / 2; o >= 1; o >>= 1) {
-        const double q = xor_lanes(p, o);
+        const real q = xor_lanes(p, o);
         p = q > p ? q : p;
     }
     return readlane(p, 0);
@@ -117,10 +141,10 @@ __device__ __forceinline__ double wave_max(double p) {
 
 // The largest value and the smallest index holding it, over lanes; idx < 0 means "none".
 // (With NaN values the lanes' results could differ; lane 0's is the one returned.)
-__device__ __forceinline__ void wave_argmax_first(double& value, int& idx) {
+__device__ __forceinline__ void wave_argmax_first(real& value, int& idx) {
 #pragma unroll
     for (int o = WAVE / 2; o >= 1; o >>= 1) {
-        const double ov = xor_lanes(value, o);
+        const real ov = xor_lanes(value, o);
         const int oi = xor_lanes(idx, o);
         const bool take = oi >= 0 && (idx < 0 || ov > value || (ov == value && oi < idx));
         value = take ? ov : value;
@@ -132,10 +156,10 @@ __device__ __forceinline__ void wave_argmax_first(double& value, int& idx) {
 }
 
 // The smallest value and the smallest index holding it, over lanes; idx < 0 means "none".
-__device__ __forceinline__ void wave_argmin_first(double& value, int& idx) {
+__device__ __forceinline__ void wave_argmin_first(real& value, int& idx) {
 #pragma unroll
     for (int o = WAVE / 2; o >= 1; o >>= 1) {
-        const double ov = xor_lanes(value, o);
+        const real ov = xor_lanes(value, o);
         const int oi = xor_lanes(idx, o);
         const bool take = oi >= 0 && (idx < 0 || ov < value || (ov == value && oi < idx));
         value = take ? ov : value;
@@ -152,9 +176,9 @@ __device__ __forceinline__ void wave_argmin_first(double& value, int& idx) {
 // without NaN this is the in-order scan's "first strictly larger" choice; a NaN value is never
 // chosen.
 template <int NS>
-__device__ __forceinline__ void wave_argmax_cols(const double (&v)[NS], const bool (&flag)[NS],
-                                                 double& value, int& idx) {
-    double lane_max = -__builtin_inf();
+__device__ __forceinline__ void wave_argmax_cols(const real (&v)[NS], const bool (&flag)[NS],
+                                                 real& value, int& idx) {
+    real lane_max = -__builtin_inff();
     bool any = false;
 #pragma unroll
     for (int t = 0; t < NS; ++t) {
@@ -174,9 +198,9 @@ __device__ __forceinline__ void wave_argmax_cols(const double (&v)[NS], const bo
 
 // The same for the smallest value.
 template <int NS>
-__device__ __forceinline__ void wave_argmin_cols(const double (&v)[NS], const bool (&flag)[NS],
-                                                 double& value, int& idx) {
-    double lane_min = __builtin_inf();
+__device__ __forceinline__ void wave_argmin_cols(const real (&v)[NS], const bool (&flag)[NS],
+                                                 real& value, int& idx) {
+    real lane_min = __builtin_inff();
     bool any = false;
 #pragma unroll
     for (int t = 0; t < NS; ++t) {
@@ -185,7 +209,7 @@ __device__ __forceinline__ void wave_argmin_cols(const double (&v)[NS], const bo
     }
 #pragma unroll
     for (int o = WAVE / 2; o >= 1; o >>= 1) {
-        const double q = xor_lanes(lane_min, o);
+        const real q = xor_lanes(lane_min, o);
         lane_min = q < lane_min ? q : lane_min;
     }
     value = readlane(lane_min, 0);
@@ -205,25 +229,27 @@ template <int N>
 using dvec = double __attribute__((ext_vector_type(N)));
 template <class T, int N>
 using vec_t = T __attribute__((ext_vector_type(N)));
+template <int N>
+using rvec = vec_t<real, N>;
 
 // A vector over rows: row r in lane r % 64, slot r / 64.
 template <int RPL>
 struct RowVec {
-    dvec<RPL> v;
+    rvec<RPL> v;
 
-    __device__ __forceinline__ void fill(double x) {
+    __device__ __forceinline__ void fill(real x) {
 #pragma unroll
         for (int t = 0; t < RPL; ++t) {
             v[t] = x;
         }
     }
     // Row r (uniform r) as a wave-wide value.
-    __device__ __forceinline__ double get(int r) const {
-        const double x = v[r / WAVE];
+    __device__ __forceinline__ real get(int r) const {
+        const real x = v[r / WAVE];
         return readlane(x, r % WAVE);
     }
     // Sets row r (uniform r) to x (uniform x).
-    __device__ __forceinline__ void set(int r, double x) {
+    __device__ __forceinline__ void set(int r, real x) {
         if (lane_id() == r % WAVE) {
             v[r / WAVE] = x;
         }
@@ -232,9 +258,9 @@ struct RowVec {
 
 // The wave sum over rows [begin, end) of the per-row terms held in slots.
 template <int RPL>
-__device__ __forceinline__ double wave_sum_rows(const double (&terms)[RPL], int begin, int end) {
+__device__ __forceinline__ real wave_sum_rows(const real (&terms)[RPL], int begin, int end) {
     const int l = lane_id();
-    double s = 0.0;
+    real s = 0;
 #pragma unroll
     for (int t = 0; t < RPL; ++t) {
         const int r = l + WAVE * t;
@@ -246,7 +272,7 @@ __device__ __forceinline__ double wave_sum_rows(const double (&terms)[RPL], int 
 }
 
 // A vector over columns or positions: index p in lane p % 64, slot p / 64.
-template <int NS, class T = double>
+template <int NS, class T = real>
 struct LaneVec {
     vec_t<T, NS> v;
 
@@ -268,18 +294,30 @@ struct LaneVec {
 };
 
 // hypot as m * sqrt(1 + (s / m)^2), m the larger magnitude; the oracle uses the same formula.
-__device__ __forceinline__ double hypot_fixed(double a, double b) {
-    const double x = fabs(a);
-    const double y = fabs(b);
-    const double big = x > y ? x : y;
-    const double small = x > y ? y : x;
-    if (big == 0.0) {
-        return 0.0;
+__device__ __forceinline__ real hypot_fixed(real a, real b) {
+    const real x = fabs(a);
+    const real y = fabs(b);
+    const real big = x > y ? x : y;
+    const real small = x > y ? y : x;
+    if (big == real(0)) {
+        return real(0);
     }
-    const double r = small / big;
-    return big * sqrt(1.0 + r * r);
+    const real r = small / big;
+    return big * sqrt(real(1) + r * r);
 }
 
+#ifdef LSD_FP32
+// The plain FP32 form divides with the compiler's single-precision division everywhere; these
+// keep the FP64 form's call sites.
+__device__ __forceinline__ bool divisor_in_range(float d) { return d > 0.0f; }
+__device__ __forceinline__ bool dividend_in_range(float) { return true; }
+__device__ __forceinline__ float division_reciprocal(float d) { return 1.0f / d; }
+__device__ __forceinline__ float divide_in_range(float a, float d, float) { return a / d; }
+__device__ __forceinline__ float divide_by(float a, float d, float) { return a / d; }
+__device__ __forceinline__ float divide_nonzero_by(float a, float d, float, bool) {
+    return a != 0.0f ? a / d : a;
+}
+#else
 // Division by a divisor used many times, split in the two halves of gfx950's double-division
 // sequence (V_DIV_SCALE, V_RCP and two Newton steps on the reciprocal; then the quotient a y, the
 // exact residual a - d (a y), the correction by V_DIV_FMAS and V_DIV_FIXUP): the reciprocal is
@@ -330,5 +368,6 @@ __device__ __forceinline__ double divide_nonzero_by(double a, double d, double y
     }
     return a != 0.0 ? a / d : a;
 }
+#endif
 
-} // namespace lsd
+} // namespace LSD_NS
