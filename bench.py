@@ -42,25 +42,53 @@ import time
 import numpy as np
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector
 HBM_PEAK_GBS = 8000.0
 
 
-def build_workload(seed_override=None):
+WORKLOADS = {
+    # name: (config, lmax, description)
+    "mfod3": ("MFOD_CONFIG", 8, "mFOD multi-tissue fit, 96x96x60 volume, ellipsoid mask ~40% "
+              "({n} voxels), 10 b0 + 3x64 directions (b 1000/2000/3000), WM 1-3 fibres + GM 0.8e-3 "
+              "+ CSF 3.0e-3, lmax 8, 300 HR / 44 LR directions"),
+    "mfod4": ("PRECISION_CONFIG", 10, "mFOD multi-tissue fit, the masked voxels of a 140x140x96 "
+              "volume with an ellipsoid mask ~45% ({n} voxels), 12 b0 + 4x90 directions "
+              "(b 700/1000/2000/3000), WM 1-3 fibres + GM 0.8e-3 + CSF 3.0e-3, lmax 10, 300 HR / "
+              "44 LR directions"),
+}
+
+
+def build_workload(seed_override=None, workload="mfod3"):
+    """The signals (normalised, float64), shell weights and the LR / HR kernels of a workload.
+    mfod3 is the seeded phantom volume itself; mfod4's 846,728 masked voxels are drawn directly
+    from the same voxel model (1-3 fibres, Dirichlet fractions, Rician noise) on the device."""
     from larmor_amd import synthetic
     from larmor_amd.sd.kernel import Kernel
     from larmor_amd.sd.model import compute_shell_weights, low_resolution_dirs
     from larmor_amd.sd.rf import AnisotropicRF, IsotropicRF
     from larmor_amd.sd.sphere import DIRS_300
 
-    cfg = dict(synthetic.MFOD_CONFIG)
+    cfg = dict(getattr(synthetic, WORKLOADS[workload][0]))
     if seed_override is not None:
         cfg["seed"] = seed_override
-    ph = synthetic.phantom(**cfg)
-    signals = synthetic.normalized_signals(ph.data, ph.bvals, ph.mask)
+    if workload == "mfod3":
+        ph = synthetic.phantom(**cfg)
+        signals = synthetic.normalized_signals(ph.data, ph.bvals, ph.mask)
+        bvals, bvecs = ph.bvals, ph.bvecs
+    else:
+        import torch
+
+        rng = np.random.default_rng(cfg["seed"])
+        bvals, bvecs = synthetic.acquisition(cfg["n_b0"], cfg["shells"], rng)
+        n_vox = int(synthetic.ellipsoid_mask(cfg["shape"], cfg["mask_share"]).sum())
+        device = "cuda" if torch.cuda.is_available() else "cpu"
+        signals = np.ascontiguousarray(
+            synthetic.tissue_signals(n_vox, bvals, bvecs, rng, snr=cfg["snr"], device=device),
+            dtype=np.float64)
     rfs = [AnisotropicRF([1.7e-3, 0.3e-3, 0.3e-3], 0.0), IsotropicRF(0.8e-3), IsotropicRF(3.0e-3)]
-    w = compute_shell_weights(ph.bvals, 1.0)
-    hr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=DIRS_300, weights=w)
-    lr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=low_resolution_dirs(rfs, len(ph.bvals)), weights=w)
+    w = compute_shell_weights(bvals, 1.0)
+    hr = Kernel(rfs, bvals, bvecs, dirs=DIRS_300, weights=w)
+    lr = Kernel(rfs, bvals, bvecs, dirs=low_resolution_dirs(rfs, len(bvals)), weights=w)
     return cfg, signals, w, lr, hr
 
 
@@ -117,11 +145,23 @@ def cpu_baseline(signals, w, lr, hr, target_seconds=15.0):
     }
 
 
-def measured_traffic(kernel: str):
+TRAFFIC_PROFILES = {
+    ("mfod3", "fp64"): "mfod_3shell_traffic.json",
+    ("mfod3", "mixed"): "mfod_3shell_mixed_traffic.json",
+    ("mfod3", "fp32"): "mfod_3shell_fp32_traffic.json",
+    ("mfod4", "fp64"): "mfod_4shell_traffic.json",
+    ("mfod4", "mixed"): "mfod_4shell_mixed_traffic.json",
+}
+
+
+def measured_traffic(kernel: str, workload: str = "mfod3", form: str = "fp64"):
     """HBM bytes per launch of a kernel from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
-    summary (profiles/mfod_3shell_traffic.json, counters scaled by a 1 GiB copy), or None."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "mfod_3shell_traffic.json")
+    summary of the workload and form (profiles/*_traffic.json, counters scaled by a 1 GiB copy),
+    or None."""
+    name = TRAFFIC_PROFILES.get((workload, form))
+    if name is None:
+        return None
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
     try:
         with open(path) as fh:
             entry = json.load(fh)["kernels"].get(kernel)
@@ -158,7 +198,7 @@ class ServeBench:
     one FodServer with every request's design made resident first (prepare), warmed on the four
     smallest requests."""
 
-    def __init__(self, n_jobs: int, slots: int):
+    def __init__(self, n_jobs: int, slots: int, form: str = "fp64"):
         from larmor_amd import synthetic
         from larmor_amd.sd.model import SDModel
         from larmor_amd.sd.rf import AnisotropicRF, IsotropicRF
@@ -176,7 +216,8 @@ class ServeBench:
                          IsotropicRF(3.0e-3)], method="mFOD", lmax=8)
         self.gtabs = [GradientTable(j.bvals, j.bvecs) for j in self.jobs]
         self.slots = slots
-        self.server = FodServer(model, slots=slots, hbm_budget=200e9)
+        self.form = form
+        self.server = FodServer(model, slots=slots, hbm_budget=200e9, form=form)
         t = time.perf_counter()
         for job, gtab in zip(self.jobs, self.gtabs):
             self.server.prepare(gtab, lmax=job.lmax)
@@ -217,18 +258,18 @@ class ServeBench:
         return (f"{len(self.jobs)} mFOD requests, voxels log-uniform 5k-400k ({self.n_vox} in "
                 "all), each with its own acquisition (10 b0 + 1-4 shells of 30-90 directions, "
                 "b 700-3000) and lmax 6/8/10, WM + GM 0.8e-3 + CSF 3.0e-3, float32 signals in "
-                f"host memory; {self.slots} streams, 200 GB HBM budget")
+                f"host memory; {self.slots} streams, 200 GB HBM budget; {self.form} form")
 
     def close(self) -> None:
         self.server.close()
 
 
-def serve_report(n_jobs: int, rates: list[float], slots: int) -> int:
+def serve_report(n_jobs: int, rates: list[float], slots: int, form: str = "fp64") -> int:
     """The batched and served forms (--serve) on ServeBench's workload.  Phases: plain per-job
     (each request submitted when the previous one's results are back), batched (all submitted at
     once) and served at each Poisson arrival rate.  Prints one JSON line with voxel solves/s and
     p50/p99 latency per phase."""
-    bench = ServeBench(n_jobs, slots)
+    bench = ServeBench(n_jobs, slots, form)
     order = list(range(n_jobs))
     plain = bench.run("plain per-job", [(0.0, i) for i in order], serial=True)
     batched = bench.run("batched", [(0.0, i) for i in order])
@@ -238,7 +279,7 @@ def serve_report(n_jobs: int, rates: list[float], slots: int) -> int:
     result = {
         "metric": "voxel FOD solves/s (mFOD lmax=8, 3-shell); p50/p99 latency under load",
         "unit": "voxel solves/s",
-        "dtype": "f64",
+        "dtype": {"fp64": "f64", "mixed": "bf16+f64"}[form],
         "data": "synthetic (seeded requests, Rician SNR 25)",
         "config": {"workload": bench.workload(), "slots": slots, "hbm_budget_bytes": 200e9},
         "plain_per_job": plain,
@@ -586,13 +627,24 @@ def main() -> int:
     parser.add_argument("--slots", type=int, default=4, help="requests in flight at once")
     parser.add_argument("--no-serve", action="store_true",
                         help="leave the latency-under-load phase out of the JSON line")
+    parser.add_argument("--form", choices=["fp64", "fp32", "mixed"], default="fp64",
+                        help="fp64: the reference's FP64 computation (bit-identical to the oracle; "
+                             "the default line); fp32: the plain FP32 form; mixed: the "
+                             "mixed-precision form (BF16 matrix-core correlations, FP64 "
+                             "refinement rounds and solves)")
+    parser.add_argument("--workload", choices=sorted(WORKLOADS), default="mfod3",
+                        help="mfod3: the 96x96x60 3-shell volume (lmax 8); mfod4: the 140x140x96 "
+                             "4-shell volume (lmax 10)")
     parser.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                         help="only run three device-to-device copies of MIB MiB (the known-size "
                              "kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = parser.parse_args()
     if args.serve:
+        if args.form == "fp32":
+            print("the request server runs the fp64 and mixed forms", file=sys.stderr)
+            return 2
         return serve_report(args.serve_jobs, [float(r) for r in args.serve_rates.split(",")],
-                            args.slots)
+                            args.slots, args.form)
     if args.copy_probe:
         return copy_probe(args.copy_probe)
     if args.rows:
@@ -610,25 +662,35 @@ def main() -> int:
     from larmor_amd.sd.sphere import DIRS_300
 
     t_build = time.perf_counter()
-    cfg, signals, w, lr, hr = build_workload()
+    cfg, signals, w, lr, hr = build_workload(workload=args.workload)
     n_vox, m = signals.shape
     print(f"workload: {n_vox} voxels x {m} measurements, built in "
-          f"{time.perf_counter() - t_build:.1f} s", file=sys.stderr, flush=True)
+          f"{time.perf_counter() - t_build:.1f} s; form {args.form}", file=sys.stderr, flush=True)
 
     dev = torch.from_numpy(signals).to("cuda")
     weights = torch.from_numpy(np.ascontiguousarray(w)).to("cuda")
     lr_k, hr_k = _deconv_kernel(lr), _deconv_kernel(hr)
     lr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(lr.w_k_aniso), 0.0, 1e-6)
     hr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(hr.w_k_aniso), 0.1, 1e-6)
-    lmax = 8
+    lmax = WORKLOADS[args.workload][1]
     projection = torch.as_tensor(
         projection_matrix(DIRS_300, lmax).T * (len(DIRS_300) / (4.0 * np.pi)),
         dtype=torch.float64, device="cuda")
+    if args.form == "fp32":
+        dev = dev.float()
+        weights = weights.float()
+        projection = projection.float()
 
     def step():
+        if args.form == "fp32":
+            # The plain FP32 form; its SH projection is a single-precision library GEMM.
+            fods, fractions, rounds, status = _core.fit_multi_tissue_fp32(
+                dev, weights, lr_k, lr_solver, hr_k, hr_solver, MAX_ROUNDS, 1e-7,
+                FINAL_FRACTION_TOL_X, 1, return_diagnostics=True)
+            return torch.matmul(fods, projection), fractions, rounds, status
         fods, fractions, rounds, status = _core.fit_multi_tissue(
             dev, weights, lr_k, lr_solver, hr_k, hr_solver, MAX_ROUNDS, 1e-7,
-            FINAL_FRACTION_TOL_X, 1, return_diagnostics=True)
+            FINAL_FRACTION_TOL_X, 1, return_diagnostics=True, form=args.form)
         sh = _core.sh_project(fods, projection, exact_integral=True)
         return sh, fractions, rounds, status
 
@@ -664,20 +726,26 @@ def main() -> int:
     def counted(name):
         return kernel_times.get(name, (0.0, 0, 0.0))[2] / profiled_steps
 
+    # The FOD kernels count their own work; the others are priced per voxel (and round).  In the
+    # mixed form a round's correlations come from one of two passes, so those two are not priced.
     step_flops = {
-        "fod_lr_corr": total_rounds * 2.0 * m * n_lr,
-        "fod_lr": counted("fod_lr"),
         "fraction_loop": total_rounds * 2.0 * m * n_lr,
         "fod_hr_corr": n_vox * 2.0 * m * n_hr,
-        "fod_hr": counted("fod_hr"),
         "fraction_final": n_vox * 2.0 * m * n_hr,
     }
-    dominant = max(step_flops, key=lambda k: kernel_times.get(k, (0.0, 0, 0.0))[0])
+    if args.form != "mixed":
+        step_flops["fod_lr_corr"] = total_rounds * 2.0 * m * n_lr
+    for name, (_, _, work) in kernel_times.items():
+        if work > 0:
+            step_flops[name] = counted(name)
+    dominant = max((k for k in step_flops if k in kernel_times),
+                   key=lambda k: kernel_times[k][0])
     dom_ms, dom_launches, _ = kernel_times[dominant]
     launches_per_step = dom_launches / profiled_steps
     avg_launch_s = dom_ms / dom_launches / 1e3
     flops_per_launch = step_flops[dominant] / launches_per_step
     achieved = flops_per_launch / avg_launch_s / 1e12
+    peak = FP32_PEAK_TFLOPS if args.form == "fp32" else FP64_PEAK_TFLOPS
     print("kernel ms per step: " + ", ".join(
         f"{k} {v[0] / profiled_steps:.2f} ({v[1] // profiled_steps} launches)"
         for k, v in kernel_times.items()), file=sys.stderr, flush=True)
@@ -690,6 +758,22 @@ def main() -> int:
     # in, FP32 SH and fractions out).
     n_coeffs = projection.shape[1]
     n_tissues = fractions.shape[1]
+    if args.form != "fp64":
+        # Error against the FP64 form on the same voxels (one more fit, outside the timing).
+        ref_fods, ref_fractions = _core.fit_multi_tissue(
+            torch.from_numpy(signals).to("cuda"), torch.from_numpy(np.ascontiguousarray(w)).to(
+                "cuda"), lr_k, lr_solver, hr_k, hr_solver, MAX_ROUNDS, 1e-7,
+            FINAL_FRACTION_TOL_X, 1)
+        ref_sh = _core.sh_project(ref_fods, projection.double(), exact_integral=True)
+        got_sh = sh.double()
+        sh_err = float(torch.linalg.norm(got_sh - ref_sh) / torch.linalg.norm(ref_sh))
+        frac_err = (fractions.double() - ref_fractions).abs()
+        error_vs_fp64 = {
+            "sh_rel_l2": sh_err,
+            "fractions_max_abs": [float(x) for x in frac_err.max(dim=0).values],
+            "fractions_p99_abs": float(torch.quantile(frac_err.max(dim=1).values.float(), 0.99)),
+        }
+        print(f"error against the FP64 form: {error_vs_fp64}", file=sys.stderr, flush=True)
     mean_rounds = total_rounds / n_vox
     dense_per_voxel = 2.0 * m * n_hr + 2.0 * n_hr * n_coeffs + mean_rounds * 4.0 * m * n_lr
     bytes_per_voxel = 4.0 * (m + n_coeffs + n_tissues)
@@ -707,12 +791,11 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"fp64": "f64", "fp32": "f32", "mixed": "bf16+f64"}[args.form],
         "data": "synthetic (seeded phantom, Rician SNR 25)",
         "config": {
-            "workload": "mFOD multi-tissue fit, 96x96x60 volume, ellipsoid mask ~40% "
-                        f"({n_vox} voxels), 10 b0 + 3x64 directions (b 1000/2000/3000), "
-                        "WM 1-3 fibres + GM 0.8e-3 + CSF 3.0e-3, lmax 8, 300 HR / 44 LR directions",
+            "workload": WORKLOADS[args.workload][2].format(n=n_vox),
+            "form": args.form,
             "voxels": n_vox,
             "measurements": m,
             "lmax": lmax,
@@ -723,15 +806,16 @@ def main() -> int:
             # FP64 vector and scalar instructions, so FP64 VALU issue bounds it.
             "bound": "fp64-valu",
             "achieved": achieved,
-            "peak": FP64_PEAK_TFLOPS,
+            "peak": peak,
             "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": measured_traffic(dominant),
+            "frac": achieved / peak,
+            "traffic": measured_traffic(dominant, args.workload, args.form),
             "kernel": dominant,
             "kernel_avg_launch_ms": avg_launch_s * 1e3,
             "launches_per_step": launches_per_step,
             "flops_per_launch": flops_per_launch,
-            "peak_kind": "FP64 vector (78.6 TFLOP/s on MI355X)",
+            "peak_kind": ("FP32 vector (157.3 TFLOP/s on MI355X)" if args.form == "fp32"
+                          else "FP64 vector (78.6 TFLOP/s on MI355X)"),
             "profiled_steps": profiled_steps,
             "profiled_ms_per_step": profiled_elapsed / profiled_steps * 1e3,
         },
@@ -744,9 +828,14 @@ def main() -> int:
             "hbm_frac": value * bytes_per_voxel / 1e9 / HBM_PEAK_GBS,
         },
     }
-    if not args.no_serve:
+    if args.form != "fp64":
+        result["error_vs_fp64"] = error_vs_fp64
+    if args.workload != "mfod3":
+        result["metric"] = ("voxel FOD solves/s (mFOD lmax=10, 4-shell, 140x140x96); "
+                            "p50/p99 latency under load")
+    if not args.no_serve and args.form != "fp32":
         # p50/p99 latency under load: requests of the served form arriving at 5 jobs/s.
-        serve = ServeBench(32, args.slots)
+        serve = ServeBench(32, args.slots, "mixed" if args.form == "mixed" else "fp64")
         result["latency_under_load"] = {"workload": serve.workload(), **serve.poisson(5.0)}
         serve.close()
     if not args.no_cpu_baseline:

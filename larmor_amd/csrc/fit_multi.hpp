@@ -89,8 +89,9 @@ inline const real* weighted_iso_of(const lsd_tissue_kernel* k) { return k->d_w_i
 // with the correlations K^T a from the BF16 matrix cores (split halves, FP32 accumulation), then,
 // from where the first phase left every voxel, with the exact FP64 correlations (the refinement)
 // until the fractions settle again; both phases start each voxel's NNLS from the support of its
-// previous solve, and the factor, gradients and KKT test stay FP64 throughout.  The
-// high-resolution solve uses FP64 correlations in both forms.
+// previous solve, and the factor, gradients and KKT test stay FP64 throughout; the rounds'
+// fraction fits go through the normal equations of their few columns.  The high-resolution solve
+// and the final fractions are the FP64 form's.
 struct FitForm {
     bool mixed = false;
 };
@@ -106,6 +107,8 @@ int dispatch_fod_warm(const char* name, const char* rerun_name, int rpl, const F
                       const FodArgs& rerun, int waves, int* ol, int* oc, hipStream_t stream,
                       RerunScratch* rerun_scratch);
 int launch_corr_bf16(const CorrMfmaArgs& args, int tiles, hipStream_t stream);
+int dispatch_fraction_normal(const char* name, int rpl, const FractionArgs& args, int waves,
+                             hipStream_t stream);
 #endif
 
 // The multi-tissue fit with its per-call buffers taken from mem.  With a fixed workspace the
@@ -368,7 +371,14 @@ inline int fit_multi_tissue_in(Workspace& mem, const real* signals, int64_t n_vo
         fr.list_count = count;
         fr.next_list = next_list;
         fr.next_count = next_count;
-        rc = dispatch_fraction("fraction_loop", ncpl_lr, rpl, fr, waves, stream);
+#ifndef LSD_FP32
+        if (form.mixed) {
+            rc = dispatch_fraction_normal("fraction_loop_normal", rpl, fr, waves, stream);
+        } else
+#endif
+        {
+            rc = dispatch_fraction("fraction_loop", ncpl_lr, rpl, fr, waves, stream);
+        }
         list = next_list;
         count = next_count;
     }

@@ -129,7 +129,9 @@ constexpr int GROUP_BATCH = 4; // kernel rows loaded together for a group signal
 // receives the fractions, groups first.  work is the wave's LDS slab for the pivoted QR (RPL * 64
 // doubles per column of the fit), iso_lds the block's LDS copy of the isotropic columns.
 // Returns false when the NNLS hit its inner-step limit.
-template <int NCPL, int RPL, int NQ, int NG = MAX_GROUPS>
+// NORMAL (the mixed-precision form's loop fits): the NNLS fits through the normal equations
+// (nnls_normal) instead of the reference's pivoted QR.
+template <int NCPL, int RPL, int NQ, int NG = MAX_GROUPS, bool NORMAL = false>
 __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const real (&fod)[NCPL],
                                                const RowVec<RPL>& s, bool lumped, bool has_tol,
                                                real tol_in, real (&fr)[NQ], real* work,
@@ -214,7 +216,12 @@ __device__ __forceinline__ bool fractions_wave(const FractionKernelDev& K, const
     }
     const int ncols = C.n_groups + K.ci;
     const real tol = has_tol ? tol_in : default_tolerance_wave<NQ, RPL>(C, K.m, ncols);
-    return nnls_small<NQ, RPL>(C, K.m, ncols, s, tol, 3 * ncols, fr, work);
+    if constexpr (NORMAL) {
+        (void)work;
+        return nnls_normal<NQ, RPL>(C, K.m, ncols, s, tol, 3 * ncols, fr);
+    } else {
+        return nnls_small<NQ, RPL>(C, K.m, ncols, s, tol, 3 * ncols, fr, work);
+    }
 }
 
 } // namespace LSD_NS

@@ -28,6 +28,26 @@ int dispatch_fod_warm(const char* name, const char* rerun_name, int rpl, const F
     }
 }
 
+int dispatch_fraction_normal(const char* name, int rpl, const FractionArgs& args, int waves,
+                             hipStream_t stream) {
+    // The loop fit only (one lumped group): no QR slab, the block's isotropic columns in LDS.
+    const size_t lds_per_row_slot = sizeof(double) * args.kernel.ci * WAVE;
+    switch (rpl) {
+    case 4:
+        return launch_fill(name, fraction_kernel<1, 4, false, true>, args, waves,
+                           lds_per_row_slot * 4, stream);
+    case 6:
+        return launch_fill(name, fraction_kernel<1, 6, false, true>, args, waves,
+                           lds_per_row_slot * 6, stream);
+    case 8:
+        return launch_fill(name, fraction_kernel<1, 8, false, true>, args, waves,
+                           lds_per_row_slot * 8, stream);
+    default:
+        return fail("no normal-equation fraction kernel for " + std::to_string(rpl) +
+                    " row slots");
+    }
+}
+
 int launch_corr_bf16(const CorrMfmaArgs& args, int tiles, hipStream_t stream) {
     const int waves = static_cast<int>(std::min<long>((args.n_vox + MFMA_TILE - 1) / MFMA_TILE,
                                                       1L << 30));

@@ -176,13 +176,13 @@ struct FodArgs {
 // The fast low-resolution instantiation is asked for 8 waves per SIMD, what its LDS factor
 // (4.2 KB per wave, 20 columns) allows: the register budget that leaves costs a few spilled
 // registers and still beats 7 waves with a 24-column factor.
-constexpr int fod_waves_per_eu(int ncpl, bool global_l) { return (!global_l && ncpl == 1) ? 8 : 1; }
+constexpr int fod_waves_per_eu(int ncpl, bool global_l, int /*rpl*/) { return (!global_l && ncpl == 1) ? 8 : 1; }
 
 // WARM (the mixed-precision form's low-resolution solves, NCPL 1): each voxel's solve starts from
 // the support its previous solve left in active_mask and leaves its own there.
 template <int NCPL, int RPL, int KCAP, bool GLOBAL_L, bool WARM = false>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
-    __attribute__((amdgpu_waves_per_eu(fod_waves_per_eu(NCPL, GLOBAL_L))))
+    __attribute__((amdgpu_waves_per_eu(fod_waves_per_eu(NCPL, GLOBAL_L, RPL))))
     fod_kernel(FodArgs A) {
     constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
     extern __shared__ real smem[];
@@ -391,12 +391,18 @@ struct FractionArgs {
     long n_vox;
 };
 
-// The low-resolution fraction kernel is asked for 5 waves per SIMD.
-constexpr int fraction_waves_per_eu(int ncpl) { return ncpl == 1 ? 5 : 1; }
+// The low-resolution fraction kernel is asked for 5 waves per SIMD with 4 row slots.
+// With more row slots the per-wave QR slab and the block's isotropic columns cap the resident
+// waves at 3 per SIMD (6 slots) and 2 (8 slots), so asking for more would only spill registers.
+// The normal-equation fits hold no slab; 4 waves per SIMD leave them registers without spills.
+constexpr int fraction_waves_per_eu(int ncpl, int rpl, bool normal) {
+    return ncpl != 1 ? 1 : normal ? 4 : (rpl <= 4 ? 5 : (rpl <= 6 ? 3 : 2));
+}
 
-template <int NCPL, int RPL, bool FINAL>
+// NORMAL: the mixed-precision form's loop fits through the normal equations (no QR slab).
+template <int NCPL, int RPL, bool FINAL, bool NORMAL = false>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
-    __attribute__((amdgpu_waves_per_eu(fraction_waves_per_eu(NCPL)))) fraction_kernel(FractionArgs A) {
+    __attribute__((amdgpu_waves_per_eu(fraction_waves_per_eu(NCPL, RPL, NORMAL)))) fraction_kernel(FractionArgs A) {
     extern __shared__ real smem[];
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
@@ -404,9 +410,10 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     // The fit's columns: one lumped group (the loop) or one per block (the final fit), then the
     // isotropic tissues.
     const int qcols = (FINAL ? A.kernel.n_blocks : 1) + A.kernel.ci;
-    real* work = smem + w * (qcols * RPL * WAVE);
+    const int slab = NORMAL ? 0 : qcols * RPL * WAVE;
+    real* work = smem + w * slab;
     // The block's copy of the isotropic columns, after the waves' QR slabs.
-    real* iso_lds = smem + WAVES_PER_BLOCK * (qcols * RPL * WAVE);
+    real* iso_lds = smem + WAVES_PER_BLOCK * slab;
     for (int i = threadIdx.x; i < A.kernel.ci * RPL * WAVE; i += blockDim.x) {
         const int c = i / (RPL * WAVE);
         const int r = i % (RPL * WAVE);
@@ -460,8 +467,8 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
         const bool ok = FINAL
                             ? fractions_wave<NCPL, RPL, NQ, MAX_GROUPS>(
                                   A.kernel, fod, s, false, true, A.final_tol, fr, work, iso_lds)
-                            : fractions_wave<NCPL, RPL, NQ, 1>(A.kernel, fod, s, true, false, real(0.0),
-                                                               fr, work, iso_lds);
+                            : fractions_wave<NCPL, RPL, NQ, 1, NORMAL>(A.kernel, fod, s, true, false,
+                                                                       real(0.0), fr, work, iso_lds);
         if (!ok && l == 0) {
             A.status[v] |= ST_NOT_CONVERGED;
         }

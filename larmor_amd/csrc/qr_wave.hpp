@@ -509,4 +509,209 @@ __device__ __forceinline__ bool nnls_small(const P& C, int rows, int ncols, cons
     return true;
 }
 
+// The generic Lawson-Hanson NNLS on up to NQ columns through the normal equations: C^T C and
+// C^T d are wave sums formed once; each least-squares fit on the active columns is a Cholesky
+// solve of that Gram matrix's active block (uniform values), the gradient C^T d - (C^T C) x.  A
+// column whose pivot is not above MIN_NORMAL_PIVOT of its diagonal is left out of the fit (its
+// coefficient 0), as the pivoted QR leaves out columns past the numerical rank.  The steps, tie
+// rules and limits are nnls_small's; only the fits differ, by rounding (the mixed-precision form's
+// fraction fits: a few well-conditioned columns).
+constexpr double MIN_NORMAL_PIVOT = 1e-12;
+
+template <int NQ, int RPL, class P>
+__device__ __forceinline__ bool nnls_normal(const P& C, int rows, int ncols, const RowVec<RPL>& d,
+                                            real tol, int max_inner, real (&x)[NQ]) {
+    real gram[NQ][NQ];
+    real rhs[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        rhs[i] = real(0);
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+            gram[i][j] = real(0);
+        }
+        if (i < ncols) {
+            rhs[i] = wave_sum_f<RPL>(0, rows, [&](int t) { return C.get(i, t) * d.v[t]; });
+#pragma unroll
+            for (int j = 0; j <= i; ++j) {
+                const real gij = wave_sum_f<RPL>(0, rows, [&](int t) {
+                    return C.get(i, t) * C.get(j, t);
+                });
+                gram[i][j] = gij;
+                gram[j][i] = gij;
+            }
+        }
+    }
+    bool active[NQ];
+    real g[NQ];
+    real cand[NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+        active[c] = false;
+        x[c] = real(0);
+        g[c] = c < ncols ? rhs[c] : real(0);
+    }
+    auto gradient_at = [&]() {
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            real v = rhs[i];
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) {
+                v = v - gram[i][j] * x[j];
+            }
+            g[i] = i < ncols ? v : real(0);
+        }
+    };
+    // cand = the least-squares fit on the active columns (0 elsewhere), by a Cholesky factor of
+    // the active block built in column order.
+    auto fit_on = [&]() {
+        real l[NQ][NQ];
+        bool used[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            used[i] = false;
+            cand[i] = real(0);
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) {
+                l[i][j] = real(0);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+            if (!(j < ncols && active[j])) {
+                continue;
+            }
+            real pivot = gram[j][j];
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) {
+                if (k < j && used[k]) {
+                    pivot = pivot - l[j][k] * l[j][k];
+                }
+            }
+            if (!(pivot > real(MIN_NORMAL_PIVOT) * gram[j][j])) {
+                continue;
+            }
+            const real ljj = sqrt(pivot);
+            l[j][j] = ljj;
+            used[j] = true;
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                if (i > j && i < ncols && active[i]) {
+                    real v = gram[i][j];
+#pragma unroll
+                    for (int k = 0; k < NQ; ++k) {
+                        if (k < j && used[k]) {
+                            v = v - l[i][k] * l[j][k];
+                        }
+                    }
+                    l[i][j] = v / ljj;
+                }
+            }
+        }
+        real y[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            y[i] = real(0);
+            if (used[i]) {
+                real v = rhs[i];
+#pragma unroll
+                for (int k = 0; k < NQ; ++k) {
+                    if (k < i && used[k]) {
+                        v = v - l[i][k] * y[k];
+                    }
+                }
+                y[i] = v / l[i][i];
+            }
+        }
+#pragma unroll
+        for (int i = NQ - 1; i >= 0; --i) {
+            if (used[i]) {
+                real v = y[i];
+#pragma unroll
+                for (int k = 0; k < NQ; ++k) {
+                    if (k > i && used[k]) {
+                        v = v - l[k][i] * cand[k];
+                    }
+                }
+                cand[i] = v / l[i][i];
+            }
+        }
+    };
+    int n_inner = 0;
+    int n_outer = 0;
+    while (true) {
+        if (++n_outer > 8 * ncols + 64) {
+            return false;
+        }
+        int entering = -1;
+        real best = real(0);
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            if (c < ncols && !active[c] && (entering < 0 || g[c] > best)) {
+                entering = c;
+                best = g[c];
+            }
+        }
+        if (entering < 0 || best <= tol) {
+            break;
+        }
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            active[c] = active[c] || c == entering;
+        }
+        fit_on();
+        while (true) {
+            bool any = false;
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) {
+                any = any || (c < ncols && active[c] && cand[c] <= real(0));
+            }
+            if (!any) {
+                break;
+            }
+            if (++n_inner > max_inner) {
+#pragma unroll
+                for (int c = 0; c < NQ; ++c) {
+                    x[c] = cand[c];
+                }
+                return false;
+            }
+            real step = __builtin_inff();
+            int blocking = 0;
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) {
+                if (c < ncols && active[c] && cand[c] <= real(0)) {
+                    const real distance = x[c] - cand[c];
+                    const real sv = distance > real(0) ? x[c] / distance : real(0);
+                    if (sv < step) {
+                        step = sv;
+                        blocking = c;
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) {
+                if (c < ncols) {
+                    x[c] = x[c] + step * (cand[c] - x[c]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) {
+                if (c == blocking) {
+                    x[c] = real(0);
+                    active[c] = false;
+                }
+                active[c] = active[c] && fabs(x[c]) >= tol;
+            }
+            fit_on();
+        }
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            x[c] = cand[c];
+        }
+        gradient_at();
+    }
+    return true;
+}
+
 } // namespace LSD_NS

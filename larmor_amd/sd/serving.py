@@ -130,8 +130,10 @@ class _Design:
     """The device-resident objects of one (gradient table, lmax): what SDModel._prepare builds
     for a fit (model.py:289-319), kept for every request on that acquisition."""
 
-    def __init__(self, model: SDModel, solver, bvals: np.ndarray, bvecs: np.ndarray, lmax: int):
+    def __init__(self, model: SDModel, solver, bvals: np.ndarray, bvecs: np.ndarray, lmax: int,
+                 form: str = "fp64"):
         torch = _torch()
+        self.form = form
         self.lmax = compute_max_lmax(bvals, lmax)
         self.n_meas = len(bvals)
         self.multi = model.is_multi_tissue
@@ -168,7 +170,8 @@ class _Design:
         if self.multi:
             fods, fractions = _core.fit_multi_tissue(
                 signals, self.weights, self.lr_kernel, self.lr_solver, self.hr_kernel,
-                self.hr_solver, MAX_ROUNDS, self.fraction_tol, FINAL_FRACTION_TOL_X, 1)
+                self.hr_solver, MAX_ROUNDS, self.fraction_tol, FINAL_FRACTION_TOL_X, 1,
+                form=self.form)
         else:
             fods, fractions = _core.fit_single_tissue(signals, self.solver, 1), None
         sh = [_to_sh(fods[:, block], self.projection, exact_integral=self.exact_integral)
@@ -213,12 +216,16 @@ class FodServer:
     admitted requests and resident designs may hold together; max_designs: acquisitions kept
     resident (least recently used ones go first, never one in use); merge_voxels: queued requests
     on the head request's acquisition are fitted with it as one launch sequence up to this many
-    rows in all.
+    rows in all; form: the multi-tissue fit's form, "fp64" (the reference's computation, results
+    bit-identical to the plain fit) or "mixed" (the mixed-precision form, lsd_fit_multi_tissue_mixed).
     """
 
     def __init__(self, model: SDModel, *, slots: int = 4, hbm_budget: float = 200e9,
                  max_designs: int = 256, merge_voxels: int = 1 << 18,
-                 poll_interval: float = 1e-4) -> None:
+                 poll_interval: float = 1e-4, form: str = "fp64") -> None:
+        if form not in ("fp64", "mixed"):
+            raise ValueError(f"form must be 'fp64' or 'mixed', got {form!r}")
+        self.form = form
         if slots < 1:
             raise ValueError(f"slots must be at least 1, got {slots}")
         if not hbm_budget > 0:
@@ -413,7 +420,7 @@ class FodServer:
                 if victim is None:
                     break
                 self._design_bytes -= self._designs.pop(victim).bytes
-            design = _Design(self.model, self.solver, bvals, bvecs, lmax)
+            design = _Design(self.model, self.solver, bvals, bvecs, lmax, self.form)
             self._designs[key] = design
             self._design_bytes += design.bytes
             return design
