@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define LSD_ABI_VERSION 4
+#define LSD_ABI_VERSION 5
 #define LSD_ERR_INVALID 1 /* invalid argument (the reference raises ValueError) */
 #define LSD_ERR_HIP 2     /* a HIP runtime call failed */
 
@@ -232,6 +232,37 @@ int64_t lsd_noalloc_workspace_bytes(void);
  * device memory (the request server's admission).
  */
 int64_t lsd_fit_scratch_bytes(int64_t m, int64_t n_lr, int64_t n_hr);
+
+/*
+ * One non-negative least-squares problem, min ||c x - d|| with x >= 0 (device arrays: c (m, n)
+ * row-major, d (d_len = m), x (n); stream-ordered on `stream`).  A coefficient enters the
+ * solution only when its gradient exceeds tol_x; has_tol 0 takes 10 eps norm(c, 1) max(m, n),
+ * each column's absolute sum a wave sum.  Lawson-Hanson with a pivoted-QR least-squares fit per
+ * active set and at most 3 n inner steps; status (nullable, device int32) receives the status
+ * bits (1: the inner-step limit stopped it, the last candidate kept).  m <= 512, n <= 640.
+ * Replaces: lsqnonneg        cpp/src/python/sd.cpp:209-229 (module.def 471-479),
+ *           solve_nnls        cpp/src/sd/nnls.cpp:258-262,
+ *           solve_nnls_with_limit nnls.cpp:264-278.
+ * Errors: "c must be (m, n) and d (m,): got (r, c) and N".
+ */
+int lsd_lsqnonneg(const double* c, int64_t m, int64_t n, const double* d, int64_t d_len,
+                  double tol_x, int has_tol, double* x, int32_t* status, void* stream);
+
+/*
+ * min ||c x - d||^2 + ridge^2 ||x||^2 + 2 l1 sum(x) with x >= 0 (device arrays as
+ * lsd_lsqnonneg): the ridge NNLS of the FOD fits on one problem, its correlations c^T d - l1,
+ * its Gram matrix c^T c + ridge^2 I (each entry a sequential sum over rows, made on the device)
+ * and the normal equations when ridge is 0 or sum(c^2) + ridge^2 <= 1e8 ridge^2, pivoted QR on
+ * [c_A; ridge I] otherwise.  Default tolerance as lsd_lsqnonneg.
+ * Replaces: ridge_l1_nnls     cpp/src/python/sd.cpp:231-258 (module.def 481-491),
+ *           make_ridge_design cpp/src/sd/nnls.cpp:280-292,
+ *           solve_ridge_nnls  nnls.cpp:294-350.
+ * Errors: "c must be (m, n) and d (m,): ...", "ridge and l1 must be finite and >= 0.",
+ *         "l1 > 0 needs ridge > 0.".
+ */
+int lsd_ridge_l1_nnls(const double* c, int64_t m, int64_t n, const double* d, int64_t d_len,
+                      double ridge, double l1, double tol_x, int has_tol, double* x,
+                      int32_t* status, void* stream);
 
 /*
  * A constrained spherical deconvolution problem (host arrays): the linear start initial (c0 x m),

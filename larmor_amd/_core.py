@@ -244,6 +244,55 @@ def fit_single_tissue(signals, solver: FodSolver, n_threads: int, *, return_stat
     return _out(fods, host)
 
 
+def _single_problem(c, d, torch):
+    """(c, d) on the device for one NNLS problem: c (m, n) and d (m,), float64, c C-contiguous."""
+    _check_f64(c, "c")
+    _check_f64(d, "d")
+    if c.ndim != 2 or d.ndim != 1:
+        raise TypeError(f"c must be 2-D and d 1-D, got shapes {tuple(c.shape)} and "
+                        f"{tuple(d.shape)}")
+    if not _is_c_contiguous(c):
+        raise ValueError("the array must be C-contiguous")
+    dev_c, host = _as_device(c, torch)
+    dev_d, _ = _as_device(d, torch)
+    return dev_c, dev_d, host
+
+
+def _solve_single(entry: str, c, d, penalties: tuple, tol_x, return_status: bool):
+    torch = _torch()
+    dev_c, dev_d, host = _single_problem(c, d, torch)
+    m, n = dev_c.shape
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    tol = 0.0 if tol_x is None else float(tol_x)
+    _lib.check(
+        getattr(_lib.load(), entry)(
+            dev_c.data_ptr(), m, n, dev_d.data_ptr(), dev_d.shape[0], *penalties, tol,
+            0 if tol_x is None else 1, x.data_ptr(), status.data_ptr(), _stream_ptr(torch),
+        )
+    )
+    if return_status:
+        return _out(x, host), int(status.item())
+    return _out(x, host)
+
+
+def lsqnonneg(c, d, tol_x: float | None = None, *, return_status: bool = False):
+    """Solves min ||c x - d|| with x >= 0 for c (m, n) and d (m,) (sd.cpp:209-229).  A
+    coefficient enters only when its gradient exceeds tol_x; None takes
+    10 eps norm(c, 1) max(m, n).  return_status also gives the status bits (1: the inner-step
+    limit stopped the solve)."""
+    return _solve_single("lsd_lsqnonneg", c, d, (), tol_x, return_status)
+
+
+def ridge_l1_nnls(c, d, ridge: float, l1: float, tol_x: float | None = None, *,
+                  return_status: bool = False):
+    """Solves min ||c x - d||^2 + ridge^2 ||x||^2 + 2 l1 sum(x) with x >= 0 (sd.cpp:231-258);
+    ridge = l1 = 0 is plain NNLS by the normal equations.  Raises ValueError for a negative or
+    non-finite ridge or l1, or l1 > 0 with ridge 0."""
+    return _solve_single("lsd_ridge_l1_nnls", c, d, (float(ridge), float(l1)), tol_x,
+                         return_status)
+
+
 def fit_multi_tissue(
     signals,
     weights,

@@ -8,7 +8,7 @@ import ctypes
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "liblarmor_sd.so"
-ABI_VERSION = 4  # LSD_ABI_VERSION of include/larmor_sd.h these bindings follow
+ABI_VERSION = 5  # LSD_ABI_VERSION of include/larmor_sd.h these bindings follow
 HEADER_PATH = Path(__file__).resolve().parents[1] / "include" / "larmor_sd.h"
 
 _lib = None
@@ -83,6 +83,16 @@ SIGNATURES = {
     ),
     "lsd_noalloc_workspace_bytes": (ctypes.c_int64, []),
     "lsd_fit_scratch_bytes": (ctypes.c_int64, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
+    "lsd_lsqnonneg": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int64, ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_double, ctypes.c_int,
+         _vp, _vp, _vp],
+    ),
+    "lsd_ridge_l1_nnls": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int64, ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_double,
+         ctypes.c_double, ctypes.c_double, ctypes.c_int, _vp, _vp, _vp],
+    ),
     "lsd_csd_create": (
         ctypes.c_int,
         [_d, ctypes.c_int64, ctypes.c_int64, _d, ctypes.c_int64, _d, ctypes.c_int64,

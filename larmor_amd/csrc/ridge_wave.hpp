@@ -62,7 +62,11 @@ constexpr int qr_rows_per_lane(int max_m) {
 // a voxel whose factor refuses a pivot, or whose design needs QR fits throughout, is handed to
 // the rerun kernel (reported as ST_OVERFLOW), which carries them, so the fast kernels keep their
 // registers for the common path.
-template <int NCPL, int KCAP, int RPLQ, bool QR = true>
+// RESID: the gradient is C^T (d - C x) over the measurements (the generic NNLS,
+// solve_nnls_with_limit nnls.cpp:264-278: the residual's in-column-order sum, then each column's
+// wave sum) instead of corr - G x; such a design has no Gram matrix and fits by pivoted QR
+// (normal_equations 0, ridge 0).
+template <int NCPL, int KCAP, int RPLQ, bool QR = true, bool RESID = false>
 struct RidgeNnlsWave {
     static constexpr int NSK = (KCAP + WAVE - 1) / WAVE; // factor-position slots
     static constexpr int S = KCAP + 1;                   // factor column stride
@@ -628,7 +632,7 @@ struct RidgeNnlsWave {
                 act[0] = act[0] && !drop;
             }
             x[0] = act[0] ? cand[0] : real(0);
-            gradient_at_x(corr);
+            gradient_at_x<RPL>(corr, signal);
         }
         while (true) {
             if (++n_outer > outer_guard(D.n)) {
@@ -698,7 +702,7 @@ struct RidgeNnlsWave {
             for (int t = 0; t < NCPL; ++t) {
                 x[t] = cand[t];
             }
-            gradient_at_x(corr);
+            gradient_at_x<RPL>(corr, signal);
         }
         (void)l;
         return status;
@@ -710,8 +714,60 @@ struct RidgeNnlsWave {
         return ballot(col(0) < D.n && x[0] != real(0));
     }
 
-    // Gradient: corr - sum over non-zero x_c, in column order, of x_c G[:, c].
-    __device__ __forceinline__ void gradient_at_x(const real (&corr)[NCPL]) {
+    // Gradient: corr - sum over non-zero x_c, in column order, of x_c G[:, c]; or, RESID,
+    // C^T (d - C x) of the inactive columns (the entering choice reads no other).
+    template <int RPL>
+    __device__ __forceinline__ void gradient_at_x(const real (&corr)[NCPL], const RowVec<RPL>& d) {
+        if constexpr (RESID) {
+            const int l = lane_id();
+            // s = sum over every column in order of C(r, c) x_c, then the residual d - s, row by
+            // row.
+            real fit[RPL];
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                fit[t] = real(0);
+            }
+            for (int c = 0; c < D.n; ++c) {
+                const real xc = uniform_col(x, c);
+#pragma unroll
+                for (int t = 0; t < RPL; ++t) {
+                    const int r = l + WAVE * t;
+                    const real krc = r < D.m ? D.kernel[static_cast<long>(r) * D.n + c] : real(0);
+                    fit[t] = fit[t] + krc * xc;
+                }
+            }
+            real residual[RPL];
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+                residual[t] = d.v[t] - fit[t];
+            }
+#pragma unroll
+            for (int t2 = 0; t2 < NCPL; ++t2) {
+                g[t2] = real(0);
+            }
+            for (int c = 0; c < D.n; ++c) {
+                const int slot = c / WAVE;
+                bool inactive = false;
+#pragma unroll
+                for (int t2 = 0; t2 < NCPL; ++t2) {
+                    inactive = (t2 == slot) ? !act[t2] : inactive;
+                }
+                if (!ballot(inactive && l == c % WAVE)) {
+                    continue;
+                }
+                const real gc = wave_sum_f<RPL>(0, D.m, [&](int t) {
+                    const int r = l + WAVE * t;
+                    const real krc = r < D.m ? D.kernel[static_cast<long>(r) * D.n + c] : real(0);
+                    return krc * residual[t];
+                });
+#pragma unroll
+                for (int t2 = 0; t2 < NCPL; ++t2) {
+                    g[t2] = (t2 == slot && l == c % WAVE) ? gc : g[t2];
+                }
+            }
+            return;
+        }
+        (void)d;
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {
             g[t] = corr[t];
