@@ -265,6 +265,27 @@ int lsd_ridge_l1_nnls(const double* c, int64_t m, int64_t n, const double* d, in
                       int32_t* status, void* stream);
 
 /*
+ * The log-signal fits of n_voxels voxels on one design (device arrays): each row of log_s
+ * (n_voxels, n) fitted on the prepared design -- its columns scaled to unit norm, transposed xt
+ * (p, n), their pseudo-inverse pinv (p, n) and the column norms scales (p) -- by the ordinary
+ * fit pinv y and, with weighted != 0, refits weighted by the predicted signal squared
+ * (exp(2 (x.p - max x.p))) by LU with partial pivoting, until no predicted log signal moves by
+ * more than tol, at most max_reweightings of them.  A refit whose normal matrix has a zero pivot
+ * or a non-finite result leaves the ordinary fit.  params (n_voxels, p) are the scaled
+ * parameters divided by scales; refits (nullable, device int32) the refits kept, -1 when the
+ * ordinary fit was kept.  n <= 512, p <= 8; no constraints.
+ * Replaces: fit_log_signal        cpp/src/python/dwi.cpp:41-82 (module.def 317-331),
+ *           fit_log_signal_block  cpp/src/dwi/log_fit.cpp:159-185,
+ *           reweighted_fit        log_fit.cpp:28-62 (the design prepared by the caller as
+ *           make_log_signal_design log_fit.cpp:134-149 prepares it).
+ * Errors: "x must be (n, p) with n >= p >= 1: ...", "max_iter must be at least 0".
+ */
+int lsd_fit_log_signal(const double* xt, const double* pinv, const double* scales, int64_t n,
+                       int64_t p, const double* log_s, int64_t n_voxels, int weighted,
+                       int64_t max_reweightings, double tol, double* params, int32_t* refits,
+                       void* stream);
+
+/*
  * A constrained spherical deconvolution problem (host arrays): the linear start initial (c0 x m),
  * the forward model F (m x c), the weighted SH basis on the constraint directions C (k x c),
  * with the Gram matrix F^T F + norm_weight I made here.  c must be 6, 15, 28 or 45 (lmax 2-8);

@@ -351,6 +351,64 @@ def fit_multi_tissue(
     return out
 
 
+def log_signal_design(x):
+    """(xt, pinv, scales) of a log-signal design x (n, p): its columns scaled to unit norm (a
+    zero column kept, its norm taken as 1), transposed (p, n); their pseudo-inverse (p, n); the
+    norms (make_log_signal_design, cpp/src/dwi/log_fit.cpp:134-149)."""
+    x = np.asarray(x, dtype=np.float64)
+    scales = np.sqrt(np.einsum("ij,ij->j", x, x))
+    scales[scales == 0.0] = 1.0
+    unit = x / scales
+    return (np.ascontiguousarray(unit.T), np.ascontiguousarray(np.linalg.pinv(unit)),
+            np.ascontiguousarray(scales))
+
+
+def fit_log_signal(x, g, log_s, weighted: bool, max_iter: int, tol: float, n_threads: int, *,
+                   return_refits: bool = False):
+    """Fits each row of log_s (v, n) by least squares on the design x (n, p): the ordinary fit,
+    then, with weighted, refits weighted by the predicted signal squared until no predicted log
+    signal moves by more than tol, at most max_iter of them (cpp/src/python/dwi.cpp:41-82).
+    Returns (params (v, p), constrained (v,)).  Only unconstrained designs are built: g must
+    have no rows.  return_refits also gives the refits each voxel kept (-1: the ordinary fit)."""
+    _check_f64(x, "x")
+    _check_f64(g, "g")
+    _check_f64(log_s, "log_s")
+    for name, a in (("x", x), ("g", g), ("log_s", log_s)):
+        if a.ndim != 2:
+            raise TypeError(f"{name} must be 2-D, got shape {tuple(a.shape)}")
+        if not _is_c_contiguous(a):
+            raise ValueError("the array must be C-contiguous")
+    n, p = x.shape
+    if n < p or g.shape[1] != p or log_s.shape[1] != n:
+        raise ValueError(
+            f"x must be (n, p) with n >= p, g (m, p) and log_s (v, n): got x ({n}, {p}), g "
+            f"({g.shape[0]}, {g.shape[1]}), log_s ({log_s.shape[0]}, {log_s.shape[1]})"
+        )
+    if max_iter < 0:
+        raise ValueError("max_iter must be at least 0")
+    _require_threads(n_threads)
+    if g.shape[0] != 0:
+        raise ValueError("only unconstrained log-signal fits are built: g must have 0 rows")
+    torch = _torch()
+    host_x = x if isinstance(x, np.ndarray) else x.detach().cpu().numpy()
+    xt, pinv, scales = (torch.from_numpy(a).to("cuda") for a in log_signal_design(host_x))
+    dev, host = _device_signals(log_s, torch)
+    v = dev.shape[0]
+    params = torch.empty((v, p), dtype=torch.float64, device="cuda")
+    refits = torch.zeros(v, dtype=torch.int32, device="cuda")
+    _lib.check(
+        _lib.load().lsd_fit_log_signal(
+            xt.data_ptr(), pinv.data_ptr(), scales.data_ptr(), n, p, dev.data_ptr(), v,
+            int(bool(weighted)), int(max_iter), float(tol), params.data_ptr(), refits.data_ptr(),
+            _stream_ptr(torch),
+        )
+    )
+    constrained = np.zeros(v, dtype=bool)
+    if return_refits:
+        return _out(params, host), constrained, _out(refits, host)
+    return _out(params, host), constrained
+
+
 def correlations(signals, weights, kernel: DeconvKernel, solver: FodSolver, iso, *,
                  precision: str = "fp64"):
     """K^T a of every row of signals (n, m), a = max(w s - W_iso p, 0) with p the rows of iso

@@ -2,17 +2,23 @@
 
 Mirrors the response-function classes of larmor's `larmor.sd.rf` (src/larmor/sd/rf.py):
 `ResponseFunction`, `IsotropicRF`, `AnisotropicRF`, `NoddiRF` and `ZonalSHRF`, with the same
-constructor arguments, attributes and `signal(bvals, bvecs, dirs)` -> (n_meas, n_cols) contract.
-Response estimation from data (`estimate_wm_rf` and friends) is not part of this package.
+constructor arguments, attributes and `signal(bvals, bvecs, dirs)` -> (n_meas, n_cols) contract,
+and the white matter response estimated from data, `estimate_wm_rf`, with the response list of an
+`SDModel` fit, `build_response_functions` (src/larmor/sd/rf.py:166-331).
 """
 
 from abc import ABC, abstractmethod
 from dataclasses import dataclass, field
 from typing import ClassVar
 
+from collections.abc import Sequence
+
 import numpy as np
 from scipy.special import erf, erfi, eval_legendre
 
+from larmor_amd._log import LarmorLogger
+from larmor_amd.dwi.fitting import fit_log_signal
+from larmor_amd.dwi.inputs import as_mask, signal_chunks
 from larmor_amd.dwi.shells import SHELL_TOLERANCE
 from larmor_amd.sd.sphere import get_theta_phi
 
@@ -293,3 +299,111 @@ class ZonalSHRF(ResponseFunction):
         for j, degree in enumerate(degrees):
             out += weights[:, j, None] * eval_legendre(degree, cosines)
         return out
+
+
+# The white matter response from the data (src/larmor/sd/rf.py:157-242), after MRIToolkit's
+# Eigenval_IsoK_WM_FromData: a diffusion tensor, with an isotropic kurtosis term when two shells
+# lie above KURTOSIS_MIN_B, is fitted to every usable mask voxel by weighted linear least squares
+# of the log signal, ln S = ln S0 - b g^T D g + (b / 1000)^2 p_K / 6; the voxels of fractional
+# anisotropy above WM_MIN_FA are single-fibre white matter.  The response takes the medians of
+# their eigenvalues, the two smaller averaged into one (an axially symmetric tensor), and the
+# mean of their kurtosis values K = 1e-6 p_K / MD^2 that lie inside KURTOSIS_RANGE.
+WM_MIN_FA = 0.7
+KURTOSIS_MIN_B = 500.0
+KURTOSIS_RANGE = (0.0, 4.0)
+
+
+def _wm_design(gtab, with_kurtosis: bool) -> np.ndarray:
+    """Columns ln S0, the six tensor elements (the b-matrix, negated) and, with kurtosis,
+    (b / 1000)^2 / 6."""
+    columns = [np.ones((len(gtab.bvals), 1)), -gtab.b_matrix]
+    if with_kurtosis:
+        columns.append(((gtab.bvals / 1000.0) ** 2 / 6.0)[:, None])
+    return np.hstack(columns)
+
+
+def _eigenvalues(elements: np.ndarray) -> np.ndarray:
+    """Eigenvalues, largest first, (n, 3), of the symmetric tensors given by their elements
+    (xx, xy, xz, yy, yz, zz) as columns of elements (6, n)."""
+    xx, xy, xz, yy, yz, zz = elements
+    tensors = np.stack([np.stack([xx, xy, xz], -1), np.stack([xy, yy, yz], -1),
+                        np.stack([xz, yz, zz], -1)], -2)
+    return np.linalg.eigvalsh(tensors)[:, ::-1]
+
+
+def _fractional_anisotropy(ev: np.ndarray) -> np.ndarray:
+    """sqrt(3/2) |ev - mean| / |ev| per row; 0 for a zero tensor."""
+    spread = ev - ev.mean(axis=1, keepdims=True)
+    num = np.sqrt(1.5 * np.einsum("ij,ij->i", spread, spread))
+    den = np.sqrt(np.einsum("ij,ij->i", ev, ev))
+    fa = np.zeros_like(num)
+    np.divide(num, den, out=fa, where=den > 0)
+    return fa
+
+
+def estimate_wm_rf(volume, gtab, mask) -> AnisotropicRF:
+    """The white matter response of the mask's single-fibre voxels (FA > 0.7): the median
+    eigenvalues [l1, l23, l23] and the mean voxel kurtosis in (0, 4) (0 with fewer than two
+    shells above b = 500).  Raises ValueError when the mask does not match the volume or no
+    usable voxel has FA > 0.7."""
+    inside = as_mask(volume, mask)
+    with_kurtosis = gtab.count_shells(above=KURTOSIS_MIN_B) >= 2
+    design = _wm_design(gtab, with_kurtosis)
+    fits = [np.empty((design.shape[1], 0))]
+    for _, s in signal_chunks(volume, np.nonzero(inside)):
+        usable = np.all(np.isfinite(s) & (s > 0), axis=1)
+        fits.append(fit_log_signal(design, s[usable], weighted=True)[0])
+    params = np.concatenate(fits, axis=1)
+    ev = _eigenvalues(params[1:7])
+    single_fibre = _fractional_anisotropy(ev) > WM_MIN_FA
+    n_selected = int(np.count_nonzero(single_fibre))
+    if n_selected == 0:
+        raise ValueError(
+            f"No usable voxel in the mask has FA > {WM_MIN_FA} to estimate the white "
+            "matter response from."
+        )
+    median = np.median(ev[single_fibre], axis=0)
+    axial, radial = float(median[0]), float(np.mean(median[1:]))
+    kurtosis = 0.0
+    if with_kurtosis:
+        voxel_k = 1e-6 * params[7] / ev.mean(axis=1) ** 2
+        low, high = KURTOSIS_RANGE
+        kept = voxel_k[single_fibre & (voxel_k > low) & (voxel_k < high)]
+        if kept.size:
+            kurtosis = float(kept.mean())
+        else:
+            LarmorLogger.warning(
+                "No voxel with FA > %s has a kurtosis in (%s, %s); the white matter "
+                "response gets kurtosis 0.", WM_MIN_FA, low, high)
+    LarmorLogger.info(
+        f"Estimated WM eigenvalues: [{axial:.2e}, {radial:.2e}, {radial:.2e}], "
+        f"kurtosis {kurtosis:.3f}, from {n_selected} voxels."
+    )
+    return AnisotropicRF(eigen_vals=[axial, radial, radial], mean_kurtosis=kurtosis)
+
+
+def build_response_functions(volume, gtab, mask, *, wm_eigen_vals: Sequence[float] | None = None,
+                             mean_kurtosis: float | None = None,
+                             gm_diffusivity: float | None = None,
+                             csf_diffusivity: float | None = None,
+                             gm_noddi: bool = False) -> list[ResponseFunction]:
+    """The responses of an SDModel fit, white matter first, then grey matter (NODDI with
+    gm_noddi, isotropic with gm_diffusivity) and CSF (csf_diffusivity), each left out when not
+    asked for (src/larmor/sd/rf.py:271-331).  White matter comes from wm_eigen_vals, or from
+    the data by estimate_wm_rf; mean_kurtosis, when given, replaces its kurtosis (0 for given
+    eigenvalues otherwise)."""
+    if gm_noddi and gm_diffusivity is not None:
+        raise ValueError("Give either a grey matter diffusivity or gm_noddi, not both.")
+    if wm_eigen_vals is None:
+        wm = estimate_wm_rf(volume, gtab, mask)
+        if mean_kurtosis is not None:
+            wm.mean_kurtosis = mean_kurtosis
+    else:
+        wm = AnisotropicRF(eigen_vals=list(wm_eigen_vals),
+                           mean_kurtosis=0.0 if mean_kurtosis is None else mean_kurtosis)
+    responses: list[ResponseFunction] = [wm]
+    if gm_noddi:
+        responses.append(NoddiRF())
+    responses.extend(IsotropicRF(diffusivity=d) for d in (gm_diffusivity, csf_diffusivity)
+                     if d is not None)
+    return responses

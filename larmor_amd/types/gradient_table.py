@@ -87,3 +87,12 @@ class GradientTable:
 
     def count_shells(self, *, above: float = 0.0) -> int:
         return int(np.count_nonzero(self.shells > above))
+
+    @property
+    def b_matrix(self) -> np.ndarray:
+        """(N, 6): the distinct entries of b g g^T, xx, xy, xz, yy, yz, zz, the off-diagonal ones
+        doubled (they appear twice in g^T D g), so that g^T D g b = b_matrix . (Dxx, Dxy, ...)."""
+        g = self.bvecs
+        pairs = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
+        return np.stack([(1.0 if i == j else 2.0) * self.bvals * g[:, i] * g[:, j]
+                         for i, j in pairs], axis=1)

@@ -101,6 +101,11 @@ def _load():
     lib.oracle_peaks_chunk.argtypes = [_d, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                        ctypes.c_int, _d, ctypes.c_int64, _d, _i64, ctypes.c_int64,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_int, _d, _d]
+    lib.oracle_exp_nonpositive.restype = ctypes.c_double
+    lib.oracle_exp_nonpositive.argtypes = [ctypes.c_double]
+    lib.oracle_fit_log_signal.argtypes = [_d, _d, _d, ctypes.c_int64, ctypes.c_int64, _d,
+                                          ctypes.c_int64, ctypes.c_int, ctypes.c_int64,
+                                          ctypes.c_double, _d, ctypes.POINTER(ctypes.c_int32)]
     _lib = lib
     return lib
 
@@ -455,3 +460,31 @@ def fit_multi_tissue_lasso(signals, weights, lr, hr, reg, alpha, tol, max_iter, 
              rd.ctypes.data_as(_i64)]
     _check(_load().oracle_fit_multi_tissue_lasso(*args))
     return (fods, fractions, rd) if rounds else (fods, fractions)
+
+
+def exp_nonpositive(x: float) -> float:
+    """The fits' e^x for x <= 0 (Cody-Waite reduction and a degree-13 Taylor series)."""
+    return _load().oracle_exp_nonpositive(float(x))
+
+
+def log_signal_design(x):
+    """(xt, pinv, scales): the unit-column design transposed (p, n), its pseudo-inverse (p, n)
+    and the column norms (1 for a zero column), as the fits take them."""
+    x = np.asarray(x, dtype=np.float64)
+    scales = np.linalg.norm(x, axis=0)
+    scales[scales == 0] = 1.0
+    scaled = x / scales
+    return (np.ascontiguousarray(scaled.T), np.ascontiguousarray(np.linalg.pinv(scaled)),
+            np.ascontiguousarray(scales))
+
+
+def fit_log_signal(xt, pinv, scales, log_s, weighted=True, max_reweightings=20, tol=1e-6):
+    """Parameters (n_vox, p) and refits kept (n_vox,) of the log-signal fits."""
+    xt, pinv, scales, y = _c(xt), _c(pinv), _c(scales), _c(log_s)
+    p, n = xt.shape
+    out = np.zeros((y.shape[0], p))
+    refits = np.zeros(y.shape[0], dtype=np.int32)
+    _check(_load().oracle_fit_log_signal(_p(xt), _p(pinv), _p(scales), n, p, _p(y), y.shape[0],
+                                         int(weighted), max_reweightings, tol, _p(out),
+                                         refits.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+    return out, refits

@@ -2059,6 +2059,156 @@ int guarded(F f) {
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Log-signal fit (log_fit.cpp:28-62 reweighted_fit, 159-185 fit_log_signal_block), without
+// constraints, on a prepared design: the unit-column design transposed xt (p, n), its
+// pseudo-inverse (p, n) and the column norms.  The weighted linear least squares of Veraart et
+// al. (NeuroImage 81, 2013): the ordinary fit, then refits weighted by the predicted signal
+// squared, until no predicted log signal moves by more than tol.
+// ---------------------------------------------------------------------------------------------
+
+// e^x for x <= 0: Cody-Waite reduction x = k ln2 + r (ln2 in a 32-bit head and its remainder)
+// and the degree-13 Taylor series of e^r by Horner's rule, scaled by 2^k.
+double exp_nonpositive(double x) {
+    if (!(x > -746.0)) {
+        return x != x ? x : 0.0;
+    }
+    const double k = std::rint(x * 0x1.71547652b82fep+0);
+    const double r = (x - k * 0x1.62e42fee00000p-1) - k * 0x1.a39ef35793c76p-33;
+    static const double inverse_factorials[] = {
+        0x1.6124613a86d09p-33, 0x1.1eed8eff8d898p-29, 0x1.ae64567f544e4p-26,
+        0x1.27e4fb7789f5cp-22, 0x1.71de3a556c734p-19, 0x1.a01a01a01a01ap-16,
+        0x1.a01a01a01a01ap-13, 0x1.6c16c16c16c17p-10, 0x1.1111111111111p-7,
+        0x1.5555555555555p-5,  0x1.5555555555555p-3,  0.5, 1.0, 1.0};
+    double e = inverse_factorials[0];
+    for (int i = 1; i < 14; ++i) {
+        e = e * r + inverse_factorials[i];
+    }
+    return std::ldexp(e, static_cast<int>(k));
+}
+
+// Solves A q = b by LU with partial pivoting (the first row of largest magnitude as pivot);
+// false when a pivot is 0.  aug is (p, p + 1) row-major, [A | b].
+bool lu_solve_augmented(std::vector<double>& aug, Index p, Vec& q) {
+    auto at = [&](Index i, Index j) -> double& { return aug[static_cast<size_t>(i * (p + 1) + j)]; };
+    for (Index k = 0; k < p; ++k) {
+        Index best = k;
+        double best_abs = std::fabs(at(k, k));
+        for (Index i = k + 1; i < p; ++i) {
+            if (std::fabs(at(i, k)) > best_abs) {
+                best = i;
+                best_abs = std::fabs(at(i, k));
+            }
+        }
+        const double pivot = at(best, k);
+        if (pivot == 0.0) {
+            return false;
+        }
+        if (best != k) {
+            for (Index j = 0; j <= p; ++j) {
+                std::swap(at(k, j), at(best, j));
+            }
+        }
+        for (Index i = k + 1; i < p; ++i) {
+            const double lik = at(i, k) / pivot;
+            for (Index j = k + 1; j <= p; ++j) {
+                at(i, j) = at(i, j) - lik * at(k, j);
+            }
+        }
+    }
+    q.assign(static_cast<size_t>(p), 0.0);
+    for (Index k = p - 1; k >= 0; --k) {
+        double s = at(k, p);
+        for (Index j = k + 1; j < p; ++j) {
+            s = s - at(k, j) * q[j];
+        }
+        q[k] = s / at(k, k);
+    }
+    return true;
+}
+
+bool all_finite(const Vec& v) {
+    for (double x : v) {
+        if (!std::isfinite(x)) {
+            return false;
+        }
+    }
+    return true;
+}
+
+// One voxel; returns the refits kept (-1 when a refit failed and the ordinary fit stands).
+int fit_log_signal_voxel(const double* xt, const double* pinv, const double* scales, Index n,
+                         Index p, const double* y, bool weighted, Index max_reweightings,
+                         double tol, double* params) {
+    Vec ordinary(static_cast<size_t>(p));
+    for (Index a = 0; a < p; ++a) {
+        ordinary[a] = wave_sum(0, n, [&](Index r) { return pinv[a * n + r] * y[r]; });
+    }
+    Vec cur = ordinary;
+    int refits = 0;
+    auto predict = [&](const Vec& q) {
+        Vec out(static_cast<size_t>(n));
+        for (Index r = 0; r < n; ++r) {
+            double s = 0.0;
+            for (Index a = 0; a < p; ++a) {
+                s = s + xt[a * n + r] * q[a];
+            }
+            out[r] = s;
+        }
+        return out;
+    };
+    if (weighted && all_finite(ordinary)) {
+        Vec pred = predict(cur);
+        for (Index it = 0; it < max_reweightings; ++it) {
+            double top = -std::numeric_limits<double>::infinity();
+            for (Index r = 0; r < n; ++r) {
+                top = pred[r] > top ? pred[r] : top;
+            }
+            Vec w(static_cast<size_t>(n));
+            for (Index r = 0; r < n; ++r) {
+                w[r] = exp_nonpositive(2.0 * (pred[r] - top));
+            }
+            std::vector<double> aug(static_cast<size_t>(p * (p + 1)));
+            for (Index a = 0; a < p; ++a) {
+                Vec wx(static_cast<size_t>(n));
+                for (Index r = 0; r < n; ++r) {
+                    wx[r] = w[r] * xt[a * n + r];
+                }
+                aug[a * (p + 1) + p] = wave_sum(0, n, [&](Index r) { return wx[r] * y[r]; });
+                for (Index b = a; b < p; ++b) {
+                    const double s = wave_sum(0, n, [&](Index r) { return wx[r] * xt[b * n + r]; });
+                    aug[a * (p + 1) + b] = s;
+                    aug[b * (p + 1) + a] = s;
+                }
+            }
+            Vec next;
+            if (!lu_solve_augmented(aug, p, next) || !all_finite(next)) {
+                refits = -1;
+                break;
+            }
+            const Vec next_pred = predict(next);
+            double change = 0.0;
+            for (Index r = 0; r < n; ++r) {
+                const double d = std::fabs(next_pred[r] - pred[r]);
+                change = d > change ? d : change;
+            }
+            cur = next;
+            pred = next_pred;
+            refits = static_cast<int>(it + 1);
+            if (!(change > tol)) {
+                break;
+            }
+        }
+        if (refits < 0) {
+            cur = ordinary;
+        }
+    }
+    for (Index a = 0; a < p; ++a) {
+        params[a] = cur[a] / scales[a];
+    }
+    return refits;
+}
+
 } // namespace
 
 extern "C" {
@@ -2481,6 +2631,21 @@ int oracle_solve_lsi_by_qr(const double* signal, const double* a, std::int64_t m
         const LsiOut o = solve_lsi(d, Vec(signal, signal + m), nullptr, true);
         std::copy(o.x.begin(), o.x.end(), x);
         *converged = o.converged ? 1 : 0;
+    });
+}
+
+double oracle_exp_nonpositive(double x) { return exp_nonpositive(x); }
+
+// The log-signal fits of n_vox voxels (log_s (n_vox, n)) on a prepared design.
+int oracle_fit_log_signal(const double* xt, const double* pinv, const double* scales,
+                          std::int64_t n, std::int64_t p, const double* log_s, std::int64_t n_vox,
+                          int weighted, std::int64_t max_reweightings, double tol, double* params,
+                          std::int32_t* refits) {
+    return guarded([&] {
+        for (std::int64_t v = 0; v < n_vox; ++v) {
+            refits[v] = fit_log_signal_voxel(xt, pinv, scales, n, p, log_s + v * n, weighted != 0,
+                                             max_reweightings, tol, params + v * p);
+        }
     });
 }
 
