@@ -234,6 +234,40 @@ int64_t lsd_noalloc_workspace_bytes(void);
 int64_t lsd_fit_scratch_bytes(int64_t m, int64_t n_lr, int64_t n_hr);
 
 /*
+ * One volume of a batched multi-tissue fit: its signals (n_voxels, m) and shell weights (m) on
+ * the device, and its own low- and high-resolution kernels and ridge NNLS solvers.
+ */
+typedef struct lsd_batch_job {
+    const double* signals;
+    int64_t n_voxels;
+    int64_t m;
+    const double* weights;
+    const lsd_tissue_kernel* lr;
+    const lsd_solver* lr_solver;
+    const lsd_tissue_kernel* hr;
+    const lsd_solver* hr_solver;
+} lsd_batch_job;
+
+/*
+ * The multi-tissue fit of 1 to 64 volumes in one launch sequence: each low-resolution round is
+ * one correlation, one FOD and one fraction launch over every volume's voxels still in the loop,
+ * then one high-resolution solve and one final fraction launch.  The outputs hold the volumes'
+ * voxels one volume after the other: fods (sum n_voxels, n_hr), fractions (sum n_voxels,
+ * n_blocks + ci), rounds and status (nullable) (sum n_voxels).  Each volume's rows are
+ * bit-identical to lsd_fit_multi_tissue on that volume alone.  The volumes share their direction
+ * counts (at most 64 low-, 320 high-resolution), isotropic tissues and block counts and their
+ * row-slot class (m <= 256, <= 384 or <= 512); the solvers are ridge NNLS (l2nnls).
+ * No counterpart in the reference, which fits one volume per call with its voxels spread over
+ * threads (cpp/src/common/parallel.hpp:25-88): for callers with many volumes (the request
+ * server's batches of acquisitions).
+ * Errors: "a batch holds 1 to 64 jobs, got N", "job i: ...", "max_outer must be at least 0".
+ */
+int lsd_fit_multi_tissue_batched(const lsd_batch_job* jobs, int64_t n_jobs, int64_t max_rounds,
+                                 double fraction_tol, double final_tol, double* fods,
+                                 double* fractions, int32_t* rounds, int32_t* status,
+                                 void* stream);
+
+/*
  * One non-negative least-squares problem, min ||c x - d|| with x >= 0 (device arrays: c (m, n)
  * row-major, d (d_len = m), x (n); stream-ordered on `stream`).  A coefficient enters the
  * solution only when its gradient exceeds tol_x; has_tol 0 takes 10 eps norm(c, 1) max(m, n),

@@ -351,6 +351,72 @@ def fit_multi_tissue(
     return out
 
 
+class _BatchJob(ctypes.Structure):
+    """lsd_batch_job (include/larmor_sd.h)."""
+
+    _fields_ = [("signals", ctypes.c_void_p), ("n_voxels", ctypes.c_int64),
+                ("m", ctypes.c_int64), ("weights", ctypes.c_void_p), ("lr", ctypes.c_void_p),
+                ("lr_solver", ctypes.c_void_p), ("hr", ctypes.c_void_p),
+                ("hr_solver", ctypes.c_void_p)]
+
+
+MAX_BATCH_JOBS = 64
+
+
+def batch_class(m: int) -> int:
+    """The row-slot class of m measurements: the jobs of one batched fit share it."""
+    for limit in (256, 384, 512):
+        if m <= limit:
+            return limit
+    raise ValueError(f"more than 512 measurements ({m})")
+
+
+def fit_multi_tissue_batched(jobs, max_outer: int, fraction_tol: float, final_tol_x: float, *,
+                             return_diagnostics: bool = False):
+    """The multi-tissue fit of several volumes in one launch sequence (lsd_fit_multi_tissue_batched).
+
+    jobs: (signals, weights, lr, lr_solver, hr, hr_solver) per volume, signals (n_j, m_j) float64
+    and weights (m_j,) as CUDA tensors, the kernels and solvers of the volume's own acquisition.
+    Returns device tensors holding the volumes' voxels one volume after the other: FODs
+    (sum n_j, ca_hr) and fractions (sum n_j, n_blocks + ci), and with return_diagnostics the
+    rounds and status bits.  Each volume's rows equal fit_multi_tissue of that volume alone."""
+    torch = _torch()
+    if not 1 <= len(jobs) <= MAX_BATCH_JOBS:
+        raise ValueError(f"a batch holds 1 to {MAX_BATCH_JOBS} jobs, got {len(jobs)}")
+    table = (_BatchJob * len(jobs))()
+    keep = []
+    total = 0
+    for slot, (signals, weights, lr, lr_solver, hr, hr_solver) in zip(table, jobs):
+        _check_f64(signals, "signals")
+        if not (signals.is_cuda and signals.ndim == 2 and signals.is_contiguous()):
+            raise ValueError("each job's signals must be a C-contiguous 2-D CUDA tensor")
+        w = weights.to(device="cuda", dtype=torch.float64).contiguous()
+        keep.extend([signals, w])
+        n, m = signals.shape
+        slot.signals, slot.n_voxels, slot.m = signals.data_ptr(), n, m
+        slot.weights = w.data_ptr()
+        slot.lr, slot.lr_solver = lr._handle, lr_solver._handle
+        slot.hr, slot.hr_solver = hr._handle, hr_solver._handle
+        total += n
+    hr0 = jobs[0][4]
+    nf = hr0.n_blocks + hr0.n_isotropic
+    fods = torch.empty((total, hr0.n_anisotropic), dtype=torch.float64, device="cuda")
+    fractions = torch.empty((total, nf), dtype=torch.float64, device="cuda")
+    rounds = torch.zeros(total, dtype=torch.int32, device="cuda")
+    status = torch.zeros(total, dtype=torch.int32, device="cuda")
+    _lib.check(
+        _lib.load().lsd_fit_multi_tissue_batched(
+            ctypes.cast(table, ctypes.c_void_p), len(jobs), int(max_outer), float(fraction_tol),
+            float(final_tol_x), fods.data_ptr(), fractions.data_ptr(), rounds.data_ptr(),
+            status.data_ptr(), _stream_ptr(torch),
+        )
+    )
+    out = (fods, fractions)
+    if return_diagnostics:
+        out += (rounds, status)
+    return out
+
+
 def log_signal_design(x):
     """(xt, pinv, scales) of a log-signal design x (n, p): its columns scaled to unit norm (a
     zero column kept, its norm taken as 1), transposed (p, n); their pseudo-inverse (p, n); the

@@ -93,6 +93,11 @@ SIGNATURES = {
         [_vp, ctypes.c_int64, ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_double,
          ctypes.c_double, ctypes.c_double, ctypes.c_int, _vp, _vp, _vp],
     ),
+    "lsd_fit_multi_tissue_batched": (
+        ctypes.c_int,
+        [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp,
+         _vp, _vp],
+    ),
     "lsd_fit_log_signal": (
         ctypes.c_int,
         [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int,

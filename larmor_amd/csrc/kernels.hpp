@@ -150,6 +150,83 @@ __device__ __forceinline__ void record_overflow(const VoxelSet& vs, long v) {
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Batched launches: the voxels of up to MAX_JOBS volumes, each with its own measurements and
+// design, in one launch per step of the fit.  The batch numbers its voxels job after job (job j's
+// voxels are first_j .. first_j + n_vox_j - 1); per-voxel state is indexed by that number, and a
+// job's list of voxels for a round lives in its own region (first_j ..) of the list buffer, with
+// its own count.  A wave finds the job of a work item with one ballot: lane j holds job j's start
+// in the launch's job-major item space (a wave prefix sum of the jobs' item counts).
+// ---------------------------------------------------------------------------------------------
+constexpr int MAX_JOBS = WAVE;
+
+// One job's view of a stage (the low-resolution rounds or the high-resolution solve).
+struct JobStage {
+    long first;            // its first voxel in the batch numbering
+    long n_vox;
+    const real* signals;   // (n_vox, m)
+    int m;
+    const real* weights;   // (m)
+    const real* w_iso;     // (m, ci)
+    RidgeDesignDev design; // the stage's ridge design
+    real tol;
+    FractionKernelDev frac;
+};
+
+struct JobTable {
+    const JobStage* jobs; // device array, n_jobs entries
+    int n_jobs;
+    const int* counts;    // per-job list counts of this launch (null: every voxel of every job)
+    int by_voxel;         // 1: the launch walks a flat list of batch voxel numbers (a rerun)
+};
+
+// Inclusive wave prefix sum (lane l: the sum over lanes 0..l).
+__device__ __forceinline__ int wave_inclusive_sum(int x) {
+    const int l = lane_id();
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        x = l >= o ? x + y : x;
+    }
+    return x;
+}
+
+// The launch's item space: each lane's job start (exclusive prefix of the jobs' counts, in units
+// of `unit` items rounded up) and the total.
+struct JobSpace {
+    int start; // lane j: job j's first unit
+    int total; // units over all jobs (uniform)
+};
+
+__device__ __forceinline__ JobSpace job_space(const JobTable& T, int unit) {
+    const int l = lane_id();
+    int count = 0;
+    if (l < T.n_jobs) {
+        count = T.counts != nullptr ? T.counts[l] : static_cast<int>(T.jobs[l].n_vox);
+    }
+    const int units = (count + unit - 1) / unit;
+    const int incl = wave_inclusive_sum(units);
+    JobSpace js;
+    js.start = incl - units;
+    js.total = __builtin_amdgcn_readfirstlane(__shfl(incl, WAVE - 1));
+    return js;
+}
+
+// The job holding unit u (uniform) and u's offset in it.
+__device__ __forceinline__ int job_of_unit(const JobTable& T, const JobSpace& js, int u, int& local) {
+    const unsigned long long before = ballot(lane_id() < T.n_jobs && js.start <= u);
+    const int j = __builtin_amdgcn_readfirstlane(__popcll(before) - 1);
+    local = __builtin_amdgcn_readfirstlane(u - __shfl(js.start, j));
+    return j;
+}
+
+// The job of batch voxel v (uniform).
+__device__ __forceinline__ int job_of_voxel(const JobTable& T, long v) {
+    const int l = lane_id();
+    const bool before = l < T.n_jobs && T.jobs[l < T.n_jobs ? l : 0].first <= v;
+    return __builtin_amdgcn_readfirstlane(__popcll(ballot(before)) - 1);
+}
+
 // The FOD solve of a list of voxels: the signal (single-tissue) or the anisotropic signal
 // max(w s - W_iso p, 0) of each voxel's isotropic fractions p (multi-tissue,
 // multi_tissue.cpp:20-24), its correlations and the ridge NNLS (solver.cpp:14-21,
@@ -171,6 +248,7 @@ struct FodArgs {
     const real* corr;       // (list position, n) precomputed correlations, or null
     unsigned long long* work; // profiled launches: the completed solves' FP64 operations
     unsigned long long* active_mask; // (n_vox) warm-start supports in and out (WARM kernels)
+    JobTable jobs;            // BATCHED kernels: the batch's jobs (vs.list: the lists' base)
 };
 
 // The fast low-resolution instantiation is asked for 8 waves per SIMD, what its LDS factor
@@ -180,7 +258,9 @@ constexpr int fod_waves_per_eu(int ncpl, bool global_l, int /*rpl*/) { return (!
 
 // WARM (the mixed-precision form's low-resolution solves, NCPL 1): each voxel's solve starts from
 // the support its previous solve left in active_mask and leaves its own there.
-template <int NCPL, int RPL, int KCAP, bool GLOBAL_L, bool WARM = false>
+// BATCHED: the voxels of a batch's jobs (A.jobs), each with its own signals, measurements and
+// design; precomputed correlations are indexed by batch voxel number.
+template <int NCPL, int RPL, int KCAP, bool GLOBAL_L, bool WARM = false, bool BATCHED = false>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     __attribute__((amdgpu_waves_per_eu(fod_waves_per_eu(NCPL, GLOBAL_L, RPL))))
     fod_kernel(FodArgs A) {
@@ -193,24 +273,38 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     real* qrs = slab;
     real* L = GLOBAL_L ? slab + static_cast<long>(RPLQ * WAVE) * (KCAP + 1)
                          : smem + w * lds_factor_doubles(KCAP, RPL);
-    const long count = voxel_count(A.vs);
+    long count = 0;
+    JobSpace js{};
+    if constexpr (BATCHED) {
+        if (A.jobs.by_voxel) {
+            count = *A.vs.list_count;
+        } else {
+            js = job_space(A.jobs, 1);
+            count = js.total;
+        }
+    } else {
+        count = voxel_count(A.vs);
+    }
     const int l = lane_id();
     unsigned long long wave_work = 0;
-    for (long it = gw; it < count; it += n_waves) {
-        const long v = A.vs.list ? A.vs.list[it] : it;
+    // One voxel's solve with its design (the launch's, or its job's, read where it is: the
+    // kernel arguments or the job table) and its signal row.
+    auto solve_voxel = [&](long it, long v, long row, const RidgeDesignDev& design, real tol,
+                           const real* signals, int m, const real* weights, const real* w_iso) {
         RowVec<RPL> s;
         real corr[NCPL];
         if (!GLOBAL_L && A.corr != nullptr) {
             // The correlation pass computed them (the fast kernels carry no QR fits, the only
             // other use of the signal).
             s.fill(real(0.0));
+            const long at = BATCHED ? v : it;
 #pragma unroll
             for (int t = 0; t < NCPL; ++t) {
                 const int c = l + WAVE * t;
-                corr[t] = c < A.design.n ? A.corr[it * A.design.n + c] : real(0.0);
+                corr[t] = c < design.n ? A.corr[at * design.n + c] : real(0.0);
             }
         } else {
-            load_signal<RPL>(A.signals, v, A.m, s);
+            load_signal<RPL>(signals, row, m, s);
             if (A.multi) {
                 real p[NQ];
 #pragma unroll
@@ -218,18 +312,18 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
                     p[q] = q < A.ci ? A.iso[v * A.ci + q] : real(0.0);
                 }
                 RowVec<RPL> a;
-                anisotropic_signal<RPL>(s, A.weights, A.w_iso, A.m, A.ci, p, a);
+                anisotropic_signal<RPL>(s, weights, w_iso, m, A.ci, p, a);
                 s = a;
             }
-            correlations<NCPL, RPL>(A.design, s, real(0.0), corr);
+            correlations<NCPL, RPL>(design, s, real(0.0), corr);
         }
-        RidgeNnlsWave<NCPL, KCAP, RPLQ, GLOBAL_L> solver(A.design, A.tol, 3 * A.design.n, real(0.0), L,
+        RidgeNnlsWave<NCPL, KCAP, RPLQ, GLOBAL_L> solver(design, tol, 3 * design.n, real(0.0), L,
                                                          qrs);
         const unsigned long long warm = WARM ? A.active_mask[v] : 0ull;
         const int st = solver.template run<RPL>(corr, s, warm);
         if (st & ST_OVERFLOW) {
             record_overflow(A.vs, v);
-            continue;
+            return;
         }
         if constexpr (WARM) {
             const unsigned long long support = solver.support();
@@ -241,12 +335,33 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
 #pragma unroll
         for (int t = 0; t < NCPL; ++t) {
             const int c = l + WAVE * t;
-            if (c < A.design.n) {
-                A.x_out[v * A.design.n + c] = solver.x[t];
+            if (c < design.n) {
+                A.x_out[v * design.n + c] = solver.x[t];
             }
         }
         if (l == 0 && st != 0) {
             A.status[v] |= st;
+        }
+    };
+    for (long it = gw; it < count; it += n_waves) {
+        if constexpr (BATCHED) {
+            long v = 0;
+            int j = 0;
+            if (A.jobs.by_voxel) {
+                v = __builtin_amdgcn_readfirstlane(A.vs.list[it]);
+                j = job_of_voxel(A.jobs, v);
+            } else {
+                int local = 0;
+                j = job_of_unit(A.jobs, js, static_cast<int>(it), local);
+                const long first = A.jobs.jobs[j].first;
+                v = A.vs.list ? __builtin_amdgcn_readfirstlane(A.vs.list[first + local])
+                              : first + local;
+            }
+            const JobStage& J = A.jobs.jobs[j];
+            solve_voxel(it, v, v - J.first, J.design, J.tol, J.signals, J.m, J.weights, J.w_iso);
+        } else {
+            const long v = A.vs.list ? A.vs.list[it] : it;
+            solve_voxel(it, v, v, A.design, A.tol, A.signals, A.m, A.weights, A.w_iso);
         }
     }
     if (A.work != nullptr && l == 0 && wave_work != 0) {
@@ -272,7 +387,8 @@ struct CorrArgs {
     const int* list;          // voxels by list position, or null for all
     const int* list_count;
     long n_vox;
-    real* corr;             // (list position, n)
+    real* corr;             // (list position, n); BATCHED: (batch voxel, n)
+    JobTable jobs;            // BATCHED: the batch's jobs (list: the lists' base; n common)
 };
 
 constexpr int CORR_COLUMNS = 44; // columns per pass of the correlation kernel
@@ -281,24 +397,57 @@ constexpr int CORR_ROWS = 16;    // signal rows staged per LDS tile
 // Each wave takes 64 consecutive list positions.  Their signals are staged through the wave's
 // LDS tile CORR_ROWS rows at a time with coalesced loads (each load instruction covers 16 rows
 // of 4 voxels) and read back one voxel per lane; the matching CORR_ROWS x NCB chunk of K is
-// staged once per block, shared by its waves.
-template <int NCB>
+// staged once per block, shared by its waves.  BATCHED: a block's 256 positions are those of one
+// job (each job's list is cut into blocks of its own), whose signals, measurements and design it
+// reads.
+template <int NCB, bool BATCHED = false>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(CorrArgs A) {
     __shared__ real tile[WAVES_PER_BLOCK][CORR_ROWS][WAVE + 1];
     __shared__ real ktile[CORR_ROWS][NCB];
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const int l = lane_id();
-    const long count = A.list ? static_cast<long>(*A.list_count) : A.n_vox;
-    const long block_voxels = static_cast<long>(WAVES_PER_BLOCK) * WAVE;
-    const long stride = static_cast<long>(gridDim.x) * block_voxels;
+    constexpr long block_voxels = static_cast<long>(WAVES_PER_BLOCK) * WAVE;
+    long all = 0;
+    long n_units = 0;
+    JobSpace js{};
+    if constexpr (BATCHED) {
+        js = job_space(A.jobs, static_cast<int>(block_voxels));
+        n_units = js.total;
+    } else {
+        all = A.list ? static_cast<long>(*A.list_count) : A.n_vox;
+        n_units = (all + block_voxels - 1) / block_voxels;
+    }
     // The block's waves move through the column passes and row chunks together (the kernel
     // chunk in LDS is shared), so a wave past the list still takes part in every barrier.
-    for (long block_base = static_cast<long>(blockIdx.x) * block_voxels; block_base < count;
-         block_base += stride) {
+    for (long u = blockIdx.x; u < n_units; u += gridDim.x) {
+        long count = all;
+        long block_base = u * block_voxels;
+        const int* list = A.list;
+        long first = 0;
+        const real* signals = A.signals;
+        const real* kernel = A.kernel;
+        const real* weights = A.weights;
+        const real* w_iso = A.w_iso;
+        int m = A.m;
+        if constexpr (BATCHED) {
+            int local_block = 0;
+            const int j = job_of_unit(A.jobs, js, static_cast<int>(u), local_block);
+            const JobStage& J = A.jobs.jobs[j];
+            first = J.first;
+            count = A.jobs.counts != nullptr ? A.jobs.counts[j] : J.n_vox;
+            block_base = local_block * block_voxels;
+            list = A.list != nullptr ? A.list + first : nullptr;
+            signals = J.signals;
+            kernel = J.design.kernel;
+            weights = J.weights;
+            w_iso = J.w_iso;
+            m = J.m;
+        }
         const long base = block_base + static_cast<long>(w) * WAVE;
         const long it = base + l;
         const bool on = it < count;
-        const long v = on ? (A.list ? A.list[it] : it) : 0;
+        const long v = on ? (list ? list[it] : first + it) : first;
+        const long row = v - first;
         real p[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
@@ -310,15 +459,15 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
             for (int j = 0; j < NCB; ++j) {
                 acc[j] = real(0.0);
             }
-            for (int r0 = 0; r0 < A.m; r0 += CORR_ROWS) {
-                const int rows = min(CORR_ROWS, A.m - r0);
+            for (int r0 = 0; r0 < m; r0 += CORR_ROWS) {
+                const int rows = min(CORR_ROWS, m - r0);
                 // The kernel chunk K[r0 .. r0 + rows, c0 .. c0 + NCB), loaded once per block.
                 // Columns past n read the next row (or the padding after the last) and are
                 // not stored.
                 for (int e = threadIdx.x; e < CORR_ROWS * NCB; e += blockDim.x) {
                     const int rr = e / NCB;
                     const int j = e % NCB;
-                    ktile[rr][j] = rr < rows ? A.kernel[static_cast<long>(r0 + rr) * A.n + c0 + j]
+                    ktile[rr][j] = rr < rows ? kernel[static_cast<long>(r0 + rr) * A.n + c0 + j]
                                              : real(0.0);
                 }
 #pragma unroll 4
@@ -326,9 +475,9 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
                     const int e = k * WAVE + l;
                     const int j = e / CORR_ROWS;
                     const int rr = e % CORR_ROWS;
-                    const long vj = __shfl(v, j);
+                    const long vj = __shfl(row, j);
                     const bool load = base + j < count && rr < rows;
-                    tile[w][rr][j] = load ? A.signals[vj * A.m + r0 + rr] : real(0.0);
+                    tile[w][rr][j] = load ? signals[vj * m + r0 + rr] : real(0.0);
                 }
                 __syncthreads();
                 for (int rr = 0; rr < rows; ++rr) {
@@ -336,12 +485,12 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
                     const real sr = tile[w][rr][l];
                     real ar = sr;
                     if (A.multi) {
-                        const real ws = sr * A.weights[r];
+                        const real ws = sr * weights[r];
                         real wp = real(0.0);
 #pragma unroll
                         for (int q = 0; q < NQ; ++q) {
                             if (q < A.ci) {
-                                wp = wp + A.w_iso[static_cast<long>(r) * A.ci + q] * p[q];
+                                wp = wp + w_iso[static_cast<long>(r) * A.ci + q] * p[q];
                             }
                         }
                         ar = fmax(ws - wp, real(0.0));
@@ -358,7 +507,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
 #pragma unroll
                 for (int j = 0; j < NCB; ++j) {
                     if (c0 + j < A.n) {
-                        A.corr[it * A.n + c0 + j] = acc[j];
+                        A.corr[(BATCHED ? v : it) * A.n + c0 + j] = acc[j];
                     }
                 }
             }
@@ -389,6 +538,8 @@ struct FractionArgs {
     const int* list;          // voxels to fit, or null for all
     const int* list_count;
     long n_vox;
+    JobTable jobs;            // BATCHED: the batch's jobs (list, next_list: the lists' base;
+                              // next_count: per-job counts; kernel: the common column counts)
 };
 
 // The low-resolution fraction kernel is asked for 5 waves per SIMD with 4 row slots.
@@ -400,7 +551,9 @@ constexpr int fraction_waves_per_eu(int ncpl, int rpl, bool normal) {
 }
 
 // NORMAL: the mixed-precision form's loop fits through the normal equations (no QR slab).
-template <int NCPL, int RPL, bool FINAL, bool NORMAL = false>
+// BATCHED: the voxels of a batch's jobs, each wave keeping its own copy of its current job's
+// isotropic columns and appending to the job's next list.
+template <int NCPL, int RPL, bool FINAL, bool NORMAL = false, bool BATCHED = false>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     __attribute__((amdgpu_waves_per_eu(fraction_waves_per_eu(NCPL, RPL, NORMAL)))) fraction_kernel(FractionArgs A) {
     extern __shared__ real smem[];
@@ -412,41 +565,89 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     const int qcols = (FINAL ? A.kernel.n_blocks : 1) + A.kernel.ci;
     const int slab = NORMAL ? 0 : qcols * RPL * WAVE;
     real* work = smem + w * slab;
-    // The block's copy of the isotropic columns, after the waves' QR slabs.
-    real* iso_lds = smem + WAVES_PER_BLOCK * slab;
-    for (int i = threadIdx.x; i < A.kernel.ci * RPL * WAVE; i += blockDim.x) {
-        const int c = i / (RPL * WAVE);
-        const int r = i % (RPL * WAVE);
-        iso_lds[i] = r < A.m ? A.kernel.iso_t[static_cast<long>(c) * A.m + r] : real(0.0);
+    // The block's copy of the isotropic columns, after the waves' QR slabs (BATCHED: the wave's).
+    real* iso_lds = smem + WAVES_PER_BLOCK * slab + (BATCHED ? w * A.kernel.ci * RPL * WAVE : 0);
+    long count = 0;
+    JobSpace js{};
+    if constexpr (BATCHED) {
+        js = job_space(A.jobs, 1);
+        count = js.total;
+    } else {
+        for (int i = threadIdx.x; i < A.kernel.ci * RPL * WAVE; i += blockDim.x) {
+            const int c = i / (RPL * WAVE);
+            const int r = i % (RPL * WAVE);
+            iso_lds[i] = r < A.m ? A.kernel.iso_t[static_cast<long>(c) * A.m + r] : real(0.0);
+        }
+        __syncthreads();
+        count = A.list ? static_cast<long>(*A.list_count) : A.n_vox;
     }
-    __syncthreads();
-    const long count = A.list ? static_cast<long>(*A.list_count) : A.n_vox;
     const int l = lane_id();
     const int ca = A.kernel.ca;
     const int ci = A.kernel.ci;
     // The loop's voxels for the next round, buffered per wave (one per lane) and appended to the
     // next list with one atomic per buffer instead of one per voxel: every wave of the launch
     // appends to the same counter.  (The list's order is arbitrary either way; no voxel's result
-    // depends on it.)
+    // depends on it.)  BATCHED: the buffer holds one job's voxels and goes to that job's list.
     int pending = 0;
     int pending_voxel = 0;
+    int pending_job = 0;
     auto flush_pending = [&]() {
         if (pending > 0) {
+            int* next_count = A.next_count;
+            int* next_list = A.next_list;
+            if constexpr (BATCHED) {
+                next_count += pending_job;
+                next_list += A.jobs.jobs[pending_job].first;
+            }
             int base = 0;
             if (l == 0) {
-                base = atomicAdd(A.next_count, pending);
+                base = atomicAdd(next_count, pending);
             }
             base = __builtin_amdgcn_readfirstlane(base);
             if (l < pending) {
-                A.next_list[base + l] = pending_voxel;
+                next_list[base + l] = pending_voxel;
             }
             pending = 0;
         }
     };
+    int staged_job = -1;
     for (long it = gw; it < count; it += n_waves) {
-        const long v = A.list ? A.list[it] : it;
+        long v = 0;
+        long row = 0;
+        FractionKernelDev K = A.kernel;
+        const real* signals = A.signals;
+        int m = A.m;
+        if constexpr (BATCHED) {
+            int local = 0;
+            const int j = job_of_unit(A.jobs, js, static_cast<int>(it), local);
+            const JobStage& J = A.jobs.jobs[j];
+            v = A.list ? __builtin_amdgcn_readfirstlane(A.list[J.first + local]) : J.first + local;
+            row = v - J.first;
+            K = J.frac;
+            signals = J.signals;
+            m = J.m;
+            if (j != staged_job) {
+                if (!FINAL && pending > 0) {
+                    flush_pending();
+                }
+#pragma unroll
+                for (int t = 0; t < RPL; ++t) {
+                    const int r = l + WAVE * t;
+                    for (int c = 0; c < ci; ++c) {
+                        iso_lds[c * (RPL * WAVE) + r] =
+                            r < m ? K.iso_t[static_cast<long>(c) * m + r] : real(0.0);
+                    }
+                }
+                wave_lds_sync();
+                staged_job = j;
+            }
+            pending_job = j;
+        } else {
+            v = A.list ? A.list[it] : it;
+            row = v;
+        }
         RowVec<RPL> s;
-        load_signal<RPL>(A.signals, v, A.m, s);
+        load_signal<RPL>(signals, row, m, s);
         // The loop's previous fractions (lane q holds fraction q) and round count, loaded before
         // the fit so that their latency overlaps it.
         const int nlf = 1 + ci;
@@ -466,8 +667,8 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
         // The final fit's per-block groups, or the loop's one lumped group.
         const bool ok = FINAL
                             ? fractions_wave<NCPL, RPL, NQ, MAX_GROUPS>(
-                                  A.kernel, fod, s, false, true, A.final_tol, fr, work, iso_lds)
-                            : fractions_wave<NCPL, RPL, NQ, 1, NORMAL>(A.kernel, fod, s, true, false,
+                                  K, fod, s, false, true, A.final_tol, fr, work, iso_lds)
+                            : fractions_wave<NCPL, RPL, NQ, 1, NORMAL>(K, fod, s, true, false,
                                                                        real(0.0), fr, work, iso_lds);
         if (!ok && l == 0) {
             A.status[v] |= ST_NOT_CONVERGED;
@@ -527,6 +728,10 @@ struct fod_family {
 template <int NCPL, int RPL, int KCAP, bool G>
 struct fod_warm_family {
     static constexpr auto kernel = fod_kernel<NCPL, RPL, KCAP, G, true>;
+};
+template <int NCPL, int RPL, int KCAP, bool G>
+struct fod_batched_family {
+    static constexpr auto kernel = fod_kernel<NCPL, RPL, KCAP, G, false, true>;
 };
 
 // ---------------------------------------------------------------------------------------------
