@@ -138,6 +138,9 @@ class _Design:
         self.n_meas = len(bvals)
         self.multi = model.is_multi_tissue
         self.exact_integral = isinstance(solver, (L2NNLS, Lasso))
+        # Designs with the same batch key can share one batched fit (lsd_fit_multi_tissue_batched:
+        # ridge NNLS, the FP64 form, equal direction counts and row-slot class); None: fitted alone.
+        self.batch_key = None
         if self.multi:
             weights = compute_shell_weights(bvals, model.inner_shell_weight)
             hr = Kernel(model.rfs, bvals, bvecs, dirs=DIRS_300, weights=weights)
@@ -153,6 +156,9 @@ class _Design:
             self.blocks = hr.aniso_blocks
             matrices = [hr.k, lr.k, hr.w_k_aniso, lr.w_k_aniso]
             grams = hr.k_aniso.shape[1] ** 2 + lr.k_aniso.shape[1] ** 2
+            if isinstance(solver, L2NNLS) and form == "fp64" and self.n_meas <= 512:
+                self.batch_key = (_core.batch_class(self.n_meas), lr.k_aniso.shape[1],
+                                  hr.k_aniso.shape[1], len(hr.aniso_blocks), lr.k_iso.shape[1])
         else:
             kernel = Kernel(model.rfs, bvals, bvecs, dirs=DIRS_300)
             self.solver = solver._prepare(kernel, bvals, Stage.SINGLE)
@@ -174,9 +180,12 @@ class _Design:
                 form=self.form)
         else:
             fods, fractions = _core.fit_single_tissue(signals, self.solver, 1), None
-        sh = [_to_sh(fods[:, block], self.projection, exact_integral=self.exact_integral)
-              for block in self.blocks]
-        return sh, fractions
+        return self.project(fods), fractions
+
+    def project(self, fods):
+        """The SH coefficients of each anisotropic block of device amplitudes (n, 300 blocks)."""
+        return [_to_sh(fods[:, block], self.projection, exact_integral=self.exact_integral)
+                for block in self.blocks]
 
 
 @dataclass(eq=False)  # compared by identity
@@ -191,20 +200,46 @@ class _Request:
     arrival: float
     n_rows: int = 0  # rows submitted, non-finite ones included
     finite: np.ndarray | None = None  # which rows were fitted; None when all were
+    # Filled as its parts are fitted: the results of the fitted rows, the parts still out, when
+    # its first part was admitted, the lmax of its design.
+    sh: list | None = None
+    fractions: np.ndarray | None = None
+    parts_left: int = 1
+    admitted: float | None = None
+    fit_lmax: int = 0
+
+
+@dataclass(eq=False)  # compared by identity
+class _Part:
+    """Rows [start, stop) of a request's pinned signals: the unit the scheduler fits."""
+
+    request: _Request
+    start: int
+    stop: int
+    nbytes: int
+
+
+@dataclass(eq=False)  # compared by identity
+class _Member:
+    """A part in a running fit: its design and its rows [start, stop) in the fit's outputs, with
+    the pinned host copies of its results."""
+
+    part: _Part
+    design: _Design
+    start: int
+    stop: int
+    sh_host: list = field(default_factory=list)
+    fractions_host: object = None
 
 
 @dataclass(eq=False)  # compared by identity
 class _Running:
-    request: _Request
-    design: _Design
     stream: object
     event: object
     admitted: float
-    keep: list = field(default_factory=list)  # device tensors alive until the event completes
-    sh_host: list = field(default_factory=list)
-    fractions_host: object = None
-    # The requests fitted together: each with its rows' (start, stop) in the merged fit.
     members: list = field(default_factory=list)
+    keep: list = field(default_factory=list)  # device tensors alive until the event completes
+    batched: bool = False  # one job per member (lsd_fit_multi_tissue_batched)
 
 
 class FodServer:
@@ -214,15 +249,19 @@ class FodServer:
     and basis); slots: fits in flight at once, one HIP stream each (the device runs four
     hardware queues per process, so more than four share them); hbm_budget: device bytes that
     admitted requests and resident designs may hold together; max_designs: acquisitions kept
-    resident (least recently used ones go first, never one in use); merge_voxels: queued requests
-    on the head request's acquisition are fitted with it as one launch sequence up to this many
-    rows in all; form: the multi-tissue fit's form, "fp64" (the reference's computation, results
-    bit-identical to the plain fit) or "mixed" (the mixed-precision form, lsd_fit_multi_tissue_mixed).
+    resident (least recently used ones go first, never one in use); merge_voxels: the rows one
+    fit may hold -- queued parts on the head's acquisition, or with batching on any acquisition
+    of the same batch key, join the head's fit up to it; batch_jobs: parts (each with its own
+    acquisition) one batched fit may hold (1: no batched fits); slice_voxels: requests with more
+    rows are cut into parts of at most this many, fitted as separate work items, so a large
+    request does not hold the device while small ones wait (0: no cuts); form: the multi-tissue
+    fit's form, "fp64" (the reference's computation, results bit-identical to the plain fit) or
+    "mixed" (the mixed-precision form, lsd_fit_multi_tissue_mixed; never batched).
     """
 
     def __init__(self, model: SDModel, *, slots: int = 4, hbm_budget: float = 200e9,
-                 max_designs: int = 256, merge_voxels: int = 1 << 18,
-                 poll_interval: float = 1e-4, form: str = "fp64") -> None:
+                 max_designs: int = 256, merge_voxels: int = 1 << 18, batch_jobs: int = 1,
+                 slice_voxels: int = 0, poll_interval: float = 1e-4, form: str = "fp64") -> None:
         if form not in ("fp64", "mixed"):
             raise ValueError(f"form must be 'fp64' or 'mixed', got {form!r}")
         self.form = form
@@ -232,6 +271,10 @@ class FodServer:
             raise ValueError(f"hbm_budget must be positive, got {hbm_budget}")
         if max_designs < 1:
             raise ValueError(f"max_designs must be at least 1, got {max_designs}")
+        if not 1 <= batch_jobs <= _core.MAX_BATCH_JOBS:
+            raise ValueError(f"batch_jobs must be in 1..{_core.MAX_BATCH_JOBS}, got {batch_jobs}")
+        if slice_voxels < 0:
+            raise ValueError(f"slice_voxels must be at least 0, got {slice_voxels}")
         torch = _torch()
         self.model = model
         self.solver = model.solver if model.solver is not None else model.method.default_solver()
@@ -241,18 +284,20 @@ class FodServer:
         self.poll_interval = poll_interval
         self._streams = [torch.cuda.Stream() for _ in range(slots)]
         self._incoming: queue.SimpleQueue = queue.SimpleQueue()
-        self._pending: collections.deque[_Request] = collections.deque()
+        self._pending: collections.deque[_Part] = collections.deque()
         self._running: list[_Running] = []
-        self._completing: _Running | None = None  # the batch _complete is resolving
+        self._completing: _Running | None = None  # the fit _complete is resolving
         self._designs: collections.OrderedDict[tuple, _Design] = collections.OrderedDict()
         self._design_lock = threading.Lock()
-        self._running_bytes = 0  # held by running requests (scheduler thread only)
+        self._running_bytes = 0  # held by running parts (scheduler thread only)
         self._design_bytes = 0  # held by resident designs (under the design lock)
         self._closed = threading.Event()
         self._served: list[ServedFit] = []
         self.max_in_flight = 0
         self.max_merged = 1
         self.merge_voxels = int(merge_voxels)
+        self.batch_jobs = int(batch_jobs)
+        self.slice_voxels = int(slice_voxels)
         self.peak_bytes = 0
         self._error: BaseException | None = None
         self._thread = threading.Thread(target=self._loop, name="fod-server", daemon=True)
@@ -288,7 +333,8 @@ class FodServer:
             raise ValueError(f"signals ({n}, {m}) do not match the gradient table's {len(bvals)} "
                              "volumes")
         fit_lmax = compute_max_lmax(bvals, lmax)
-        nbytes = request_bytes(self.model, n, m, fit_lmax, host.dtype == torch.float32)
+        f32 = host.dtype == torch.float32
+        nbytes = request_bytes(self.model, n, m, fit_lmax, f32)
         if nbytes > self.hbm_budget:
             raise ValueError(f"the request needs about {nbytes} bytes of device memory, more than "
                              f"the server's budget of {self.hbm_budget}")
@@ -303,8 +349,17 @@ class FodServer:
         pinned.copy_(host)
         key = (bvals.tobytes(), bvecs.tobytes(), lmax)
         future: Future = Future()
-        self._incoming.put(_Request(pinned, bvals, bvecs, lmax, key, nbytes, future, arrival,
-                                    n_rows=n, finite=keep))
+        req = _Request(pinned, bvals, bvecs, lmax, key, nbytes, future, arrival, n_rows=n,
+                       finite=keep)
+        rows = int(pinned.shape[0])
+        n_parts = 1
+        if self.slice_voxels and rows > self.slice_voxels:
+            n_parts = -(-rows // self.slice_voxels)
+        bounds = [rows * k // n_parts for k in range(n_parts + 1)]
+        req.parts_left = n_parts
+        parts = [_Part(req, a, b, request_bytes(self.model, b - a, m, fit_lmax, f32) if n_parts > 1
+                       else nbytes) for a, b in zip(bounds[:-1], bounds[1:])]
+        self._incoming.put(parts)
         return future
 
     def prepare(self, gtab: GradientTable, *, lmax: int | None = None) -> None:
@@ -355,33 +410,33 @@ class FodServer:
                         time.sleep(self.poll_interval)
         except BaseException as exc:  # noqa: BLE001 - every waiting future must hear of it
             self._error = exc
-            batches = list(self._running)
+            fits = list(self._running)
             if self._completing is not None:
-                batches.append(self._completing)
-            for r in batches:
-                for req in [r.request, *(member for member, _, _ in r.members)]:
-                    _settle(req.future, exc=exc)
-            for req in self._pending:
-                _settle(req.future, exc=exc)
+                fits.append(self._completing)
+            for r in fits:
+                for member in r.members:
+                    _settle(member.part.request.future, exc=exc)
+            for part in self._pending:
+                _settle(part.request.future, exc=exc)
             while True:
                 try:
-                    req = self._incoming.get_nowait()
+                    parts = self._incoming.get_nowait()
                 except queue.Empty:
                     break
-                if req is not None:
-                    _settle(req.future, exc=exc)
+                for part in parts or ():
+                    _settle(part.request.future, exc=exc)
 
     def _drain(self, block: bool = False) -> bool:
-        """Moves arrived requests to the queue; returns True once close() was called."""
+        """Moves arrived requests' parts to the queue; returns True once close() was called."""
         while True:
             try:
-                req = self._incoming.get(timeout=0.05) if block else self._incoming.get_nowait()
+                parts = self._incoming.get(timeout=0.05) if block else self._incoming.get_nowait()
             except queue.Empty:
                 return self._closed.is_set()
             block = False
-            if req is None:
+            if parts is None:
                 continue
-            self._pending.append(req)
+            self._pending.extend(parts)
 
     def _complete(self) -> bool:
         done = [r for r in self._running if r.event.query()]
@@ -389,20 +444,37 @@ class FodServer:
             self._completing = r
             self._running.remove(r)
             finished = time.perf_counter()
-            all_sh = [t.numpy() for t in r.sh_host]
-            all_fractions = None if r.fractions_host is None else r.fractions_host.numpy()
+            results = [([t.numpy() for t in m.sh_host],
+                        None if m.fractions_host is None else m.fractions_host.numpy())
+                       for m in r.members]
             r.keep.clear()
             self._streams.append(r.stream)
-            for req, start, stop in r.members:
-                sh = [a[start:stop] for a in all_sh]
-                fractions = None if all_fractions is None else all_fractions[start:stop]
+            for member, (sh, fractions) in zip(r.members, results):
+                part, req = member.part, member.part.request
+                self._running_bytes -= part.nbytes
+                if req.future.done():  # an earlier part failed
+                    continue
+                if req.sh is None:
+                    rows = int(req.host.shape[0])
+                    req.sh = [np.empty((rows,) + a.shape[1:], dtype=a.dtype) for a in sh]
+                    if fractions is not None:
+                        req.fractions = np.empty((rows,) + fractions.shape[1:], dtype=fractions.dtype)
+                    req.fit_lmax = member.design.lmax
+                for whole, a in zip(req.sh, sh):
+                    whole[part.start:part.stop] = a
+                if fractions is not None:
+                    req.fractions[part.start:part.stop] = fractions
+                req.parts_left -= 1
+                if req.parts_left > 0:
+                    continue
+                sh_all, fr_all = req.sh, req.fractions
                 if req.finite is not None:
-                    sh = [_expand(a, req.finite) for a in sh]
-                    fractions = None if fractions is None else _expand(fractions, req.finite)
-                fit = ServedFit(sh=sh, fractions=fractions, lmax=r.design.lmax,
-                                n_voxels=req.n_rows, arrival=req.arrival, admitted=r.admitted,
+                    sh_all = [_expand(a, req.finite) for a in sh_all]
+                    fr_all = None if fr_all is None else _expand(fr_all, req.finite)
+                fit = ServedFit(sh=sh_all, fractions=fr_all, lmax=req.fit_lmax,
+                                n_voxels=req.n_rows, arrival=req.arrival, admitted=req.admitted,
                                 finished=finished)
-                self._running_bytes -= req.nbytes
+                req.sh, req.fractions = None, None
                 self._served.append(replace(fit, sh=[], fractions=None))  # timings only
                 _settle(req.future, fit)
             self._completing = None
@@ -414,7 +486,7 @@ class FodServer:
             if design is not None:
                 self._designs.move_to_end(key)
                 return design
-            in_use = {id(r.design) for r in list(self._running)}
+            in_use = {id(m.design) for r in list(self._running) for m in r.members}
             while len(self._designs) >= self.max_designs:
                 victim = next((k for k, d in self._designs.items() if id(d) not in in_use), None)
                 if victim is None:
@@ -425,105 +497,169 @@ class FodServer:
             self._design_bytes += design.bytes
             return design
 
+    def _start(self, part: _Part) -> bool:
+        """Marks a part's request as running when its first part is admitted; False when the
+        request is settled already (cancelled by its client, or failed in an earlier part)."""
+        req = part.request
+        if req.admitted is None:
+            # From here the client can no longer cancel it (False: it was cancelled meanwhile).
+            if not req.future.set_running_or_notify_cancel():
+                return False
+            req.admitted = time.perf_counter()
+            return True
+        return not req.future.done()
+
+    def _part_design(self, part: _Part):
+        """The part's design, or None (its request settled with the error) when it cannot be
+        built."""
+        req = part.request
+        try:
+            return self._design(req.key, req.bvals, req.bvecs, req.lmax)
+        except Exception as exc:  # a request the design cannot be built for
+            _settle(req.future, exc=exc)
+            return None
+
     def _admit(self, torch) -> bool:
         admitted = False
         while self._pending and self._streams:
-            req = self._pending[0]
-            if req.future.cancelled():
+            part = self._pending[0]
+            if part.request.future.cancelled() or (part.request.admitted is not None
+                                                   and part.request.future.done()):
                 self._pending.popleft()
                 continue
-            try:
-                design = self._design(req.key, req.bvals, req.bvecs, req.lmax)
-            except Exception as exc:  # a request the design cannot be built for
+            design = self._part_design(part)
+            if design is None:
                 self._pending.popleft()
-                _settle(req.future, exc=exc)
                 continue
-            # Strict arrival order: the head waits for memory, later requests wait behind it.
-            # A server with nothing running always admits the head (submit() refused any
-            # request larger than the whole budget).
+            # Strict arrival order: the head waits for memory, later parts wait behind it.  A
+            # server with nothing running always admits the head (submit() refused any request
+            # larger than the whole budget).
             used = self._running_bytes + self._design_bytes
-            if self._running and used + req.nbytes > self.hbm_budget:
+            if self._running and used + part.nbytes > self.hbm_budget:
                 break
             self._pending.popleft()
-            # From here the client can no longer cancel it (False: it was cancelled meanwhile).
-            if not req.future.set_running_or_notify_cancel():
+            if not self._start(part):
                 continue
-            batch = self._merge(req, used)
+            members = self._gather(part, design, used)
             stream = self._streams.pop()
-            running = _Running(req, design, stream, torch.cuda.Event(), time.perf_counter())
+            running = _Running(stream, torch.cuda.Event(), time.perf_counter())
+            running.batched = (self.batch_jobs > 1 and len(members) > 1
+                               and design.batch_key is not None)
             start = 0
-            for member in batch:
-                stop = start + int(member.host.shape[0])
-                running.members.append((member, start, stop))
+            for p, d in members:
+                stop = start + (p.stop - p.start)
+                running.members.append(_Member(p, d, start, stop))
                 start = stop
             try:
                 self._launch(torch, running)
             except Exception as exc:  # invalid request caught by the C-ABI checks
                 self._streams.append(stream)
-                for member in batch:
-                    _settle(member.future, exc=exc)
+                for p, _ in members:
+                    _settle(p.request.future, exc=exc)
                 continue
-            self._running_bytes += sum(member.nbytes for member in batch)
+            self._running_bytes += sum(p.nbytes for p, _ in members)
             self._running.append(running)
             self.max_in_flight = max(self.max_in_flight, len(self._running))
-            self.max_merged = max(self.max_merged, len(batch))
+            self.max_merged = max(self.max_merged, len(members))
             self.peak_bytes = max(self.peak_bytes, self._running_bytes + self._design_bytes)
             admitted = True
         return admitted
 
-    def _merge(self, head: _Request, used: int) -> list:
-        """The head request and the queued requests on the same acquisition and lmax (same design,
-        same input dtype) that fit with it under merge_voxels rows and the budget: they run as one
-        fit, which gives each voxel the same bits as its request alone (no kernel of the path
-        combines voxels)."""
-        batch = [head]
-        rows = int(head.host.shape[0])
+    def _gather(self, head: _Part, design: _Design, used: int) -> list:
+        """The head part and the queued parts that join its fit: with batching, parts of any
+        acquisition whose design has the head's batch key (one job each, up to batch_jobs);
+        otherwise parts on the head's acquisition and lmax with the same input dtype (one fit over
+        their concatenated rows).  Both up to merge_voxels rows and within the budget.  Either
+        way each voxel gets the same bits as its request alone: no kernel of the path combines
+        voxels."""
+        members = [(head, design)]
+        rows = head.stop - head.start
         nbytes = head.nbytes
+        batching = self.batch_jobs > 1 and design.batch_key is not None
+        limit = self.batch_jobs if batching else None
         if self.merge_voxels <= rows:
-            return batch
+            return members
         kept = collections.deque()
         while self._pending:
             other = self._pending.popleft()
-            n = int(other.host.shape[0])
-            if other.future.cancelled():
+            req = other.request
+            if req.future.cancelled() or (req.admitted is not None and req.future.done()):
                 continue
-            if (other.key == head.key and other.host.dtype == head.host.dtype
-                    and rows + n <= self.merge_voxels
-                    and used + nbytes + other.nbytes <= self.hbm_budget):
-                if other.future.set_running_or_notify_cancel():
-                    batch.append(other)
-                    rows += n
-                    nbytes += other.nbytes
-            else:
+            n = other.stop - other.start
+            joins = False
+            other_design = None
+            if rows + n <= self.merge_voxels and used + nbytes + other.nbytes <= self.hbm_budget \
+                    and (limit is None or len(members) < limit):
+                if batching:
+                    other_design = self._part_design(other)
+                    if other_design is None:
+                        continue
+                    joins = other_design.batch_key == design.batch_key
+                else:
+                    joins = (req.key == head.request.key
+                             and req.host.dtype == head.request.host.dtype)
+                    other_design = design
+            if joins and self._start(other):
+                members.append((other, other_design))
+                rows += n
+                nbytes += other.nbytes
+            elif not joins:
                 kept.append(other)
         self._pending = kept
-        return batch
+        return members
 
     def _launch(self, torch, r: _Running) -> None:
-        req, design = r.request, r.design
         with torch.cuda.stream(r.stream):
-            if len(r.members) == 1:
-                dev = req.host.to("cuda", non_blocking=True)
+            if r.batched:
+                jobs = []
+                for m in r.members:
+                    req, d = m.part.request, m.design
+                    dev = req.host[m.part.start:m.part.stop].to("cuda", non_blocking=True)
+                    r.keep.append(dev)
+                    if dev.dtype != torch.float64:
+                        dev = dev.to(torch.float64)
+                        r.keep.append(dev)
+                    jobs.append((dev, d.weights, d.lr_kernel, d.lr_solver, d.hr_kernel,
+                                 d.hr_solver))
+                d0 = r.members[0].design
+                fods, fractions = _core.fit_multi_tissue_batched(
+                    jobs, MAX_ROUNDS, d0.fraction_tol, FINAL_FRACTION_TOL_X)
+                r.keep.extend([fods, fractions])
+                outputs = [(m.design.project(fods[m.start:m.stop]), fractions[m.start:m.stop])
+                           for m in r.members]
             else:
-                rows = r.members[-1][2]
-                dev = torch.empty((rows, req.host.shape[1]), dtype=req.host.dtype, device="cuda")
-                for member, start, stop in r.members:
-                    dev[start:stop].copy_(member.host, non_blocking=True)
-            r.keep.append(dev)
-            if dev.dtype != torch.float64:
-                dev = dev.to(torch.float64)
+                d = r.members[0].design
+                if len(r.members) == 1:
+                    p = r.members[0].part
+                    dev = p.request.host[p.start:p.stop].to("cuda", non_blocking=True)
+                else:
+                    first = r.members[0].part.request.host
+                    dev = torch.empty((r.members[-1].stop, first.shape[1]), dtype=first.dtype,
+                                      device="cuda")
+                    for m in r.members:
+                        dev[m.start:m.stop].copy_(m.part.request.host[m.part.start:m.part.stop],
+                                                  non_blocking=True)
                 r.keep.append(dev)
-            sh, fractions = design.enqueue(dev)
-            r.keep.extend(sh)
-            for t in sh:
-                host = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
-                host.copy_(t, non_blocking=True)
-                r.sh_host.append(host)
-            if fractions is not None:
-                r.keep.append(fractions)
-                r.fractions_host = torch.empty(fractions.shape, dtype=fractions.dtype,
-                                               pin_memory=True)
-                r.fractions_host.copy_(fractions, non_blocking=True)
+                if dev.dtype != torch.float64:
+                    dev = dev.to(torch.float64)
+                    r.keep.append(dev)
+                sh, fractions = d.enqueue(dev)
+                r.keep.extend(sh)
+                if fractions is not None:
+                    r.keep.append(fractions)
+                outputs = [([t[m.start:m.stop] for t in sh],
+                            None if fractions is None else fractions[m.start:m.stop])
+                           for m in r.members]
+            for m, (sh, fractions) in zip(r.members, outputs):
+                for t in sh:
+                    r.keep.append(t)
+                    host = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+                    host.copy_(t, non_blocking=True)
+                    m.sh_host.append(host)
+                if fractions is not None:
+                    m.fractions_host = torch.empty(fractions.shape, dtype=fractions.dtype,
+                                                   pin_memory=True)
+                    m.fractions_host.copy_(fractions, non_blocking=True)
             r.event.record(r.stream)
 
 

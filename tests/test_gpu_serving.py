@@ -208,3 +208,20 @@ def test_queued_requests_on_one_acquisition_are_fitted_together():
     for job, fit in zip(jobs, results):
         assert fit.n_voxels == len(job.signals)
         _check_same(fit, _alone(model, job))
+
+
+def test_batched_and_sliced_serving_equals_alone():
+    """With batched fits, queued requests on different acquisitions share one launch sequence
+    (lsd_fit_multi_tissue_batched), and requests above slice_voxels are cut into parts fitted as
+    separate jobs; each result still has the bits of the plain fit of its rows alone."""
+    model = _model("mFOD")
+    jobs = [j for j in synthetic.batched_jobs(14, seed=91, min_voxels=200, max_voxels=4000,
+                                              n_shells=(1, 3), n_dirs=(30, 64))
+            if len(j.bvals) > 45][:8]
+    server, results = _serve(model, jobs, slots=1, batch_jobs=16, slice_voxels=900,
+                             merge_voxels=1 << 20)
+    assert server.max_merged >= 3
+    for job, fit in zip(jobs, results):
+        assert fit.n_voxels == len(job.signals)
+        assert fit.lmax == compute_max_lmax(job.bvals, job.lmax)
+        _check_same(fit, _alone(model, job))

@@ -87,11 +87,11 @@ def test_running_records_compare_by_identity():
 
     from larmor_amd.sd.serving import _Running
 
-    a = _Running(None, None, None, None, 0.0, keep=[torch.zeros(3, 261)])
-    b = _Running(None, None, None, None, 0.0, keep=[torch.zeros(3, 52)])
+    a = _Running(None, None, 0.0, keep=[torch.zeros(3, 261)])
+    b = _Running(None, None, 0.0, keep=[torch.zeros(3, 52)])
     running = [a, b]
     running.remove(b)
-    assert running == [a] and a != _Running(None, None, None, None, 0.0, keep=a.keep)
+    assert running == [a] and a != _Running(None, None, 0.0, keep=a.keep)
 
 
 def test_expand_places_fitted_rows():
@@ -153,6 +153,8 @@ def _stub_server(monkeypatch):
     srv._design_bytes = 0
     srv.hbm_budget = 1 << 40
     srv.merge_voxels = 1 << 20
+    srv.batch_jobs = 1
+    srv.slice_voxels = 0
     srv.max_in_flight = 0
     srv.max_merged = 1
     srv.peak_bytes = 0
@@ -174,7 +176,7 @@ def _request(key=b"k", rows=4):
 def test_an_error_while_completing_fails_every_merged_member(monkeypatch):
     """The scheduler's error handler settles every member of every batch, including the batch it
     was completing when the error happened and the requests merged into a batch's head."""
-    from larmor_amd.sd.serving import _Running
+    from larmor_amd.sd.serving import _Member, _Part, _Running
 
     srv = _stub_server(monkeypatch)
 
@@ -190,14 +192,15 @@ def test_an_error_while_completing_fails_every_merged_member(monkeypatch):
     members = [_request(), _request()]
     batches = []
     for head, member in zip(heads, members):
-        r = _Running(head, None, object(), Done(), 0.0, sh_host=[Boom()])
-        r.members = [(head, 0, 4), (member, 4, 8)]
+        r = _Running(object(), Done(), 0.0)
+        r.members = [_Member(_Part(head, 0, 4, 1), None, 0, 4, sh_host=[Boom()]),
+                     _Member(_Part(member, 0, 4, 1), None, 4, 8, sh_host=[Boom()])]
         batches.append(r)
     srv._running = batches
     srv._closed.set()
     srv._incoming.put(None)
     queued = _request()
-    srv._pending.append(queued)
+    srv._pending.append(_Part(queued, 0, 4, 1))
     srv._loop()
     for req in heads + members + [queued]:
         assert req.future.done()
@@ -206,7 +209,7 @@ def test_an_error_while_completing_fails_every_merged_member(monkeypatch):
 
 
 def test_cancelled_requests_are_skipped_and_running_ones_cannot_be_cancelled(monkeypatch):
-    from larmor_amd.sd.serving import _settle
+    from larmor_amd.sd.serving import _Part, _settle
 
     srv = _stub_server(monkeypatch)
     srv._streams = [object()]
@@ -215,7 +218,7 @@ def test_cancelled_requests_are_skipped_and_running_ones_cannot_be_cancelled(mon
     srv._launch = lambda torch, r: launched.append(r)
     cancelled_head, head, cancelled_member, member = (_request() for _ in range(4))
     assert cancelled_head.future.cancel() and cancelled_member.future.cancel()
-    srv._pending.extend([cancelled_head, head, cancelled_member, member])
+    srv._pending.extend(_Part(r, 0, 4, 1) for r in (cancelled_head, head, cancelled_member, member))
 
     class Torch:
         class cuda:
@@ -223,7 +226,7 @@ def test_cancelled_requests_are_skipped_and_running_ones_cannot_be_cancelled(mon
 
     assert srv._admit(Torch)
     (batch,) = launched
-    assert [m for m, _, _ in batch.members] == [head, member]
+    assert [m.part.request for m in batch.members] == [head, member]
     assert head.future.running() and member.future.running()
     assert not head.future.cancel()  # admitted: a client can no longer cancel it
     # settling a cancelled or already settled future does not raise
