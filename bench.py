@@ -292,6 +292,10 @@ def serve_report(n_jobs: int, rates: list[float], slots: int, form: str = "fp64"
     # batched launch sequences of up to 64 acquisitions.  The fit size and the slice size are
     # tuned on the workload itself: each candidate policy serves the whole workload once and the
     # fastest is reported as the batched phase.
+    # concurrent: every request queued at once, each fitted alone on one of the streams (no
+    # batched launches): what the batched launches are measured against.
+    policy = bench.configure(batch_jobs=1, slice_voxels=0, merge_voxels=1 << 18)
+    concurrent = {**bench.run("concurrent", [(0.0, i) for i in order]), "policy": policy}
     candidates = []
     for merge_voxels in (1 << 19, 1 << 20, 1 << 21):
         for slice_voxels in (0, 1 << 16):
@@ -314,10 +318,12 @@ def serve_report(n_jobs: int, rates: list[float], slots: int, form: str = "fp64"
         "data": "synthetic (seeded requests, Rician SNR 25)",
         "config": {"workload": bench.workload(), "slots": slots, "hbm_budget_bytes": 200e9},
         "plain_per_job": plain,
+        "concurrent": concurrent,
         "batched": batched,
         "batched_tuning": tuning,
         "served": served,
         "batched_vs_plain": batched["voxel_solves_per_s"] / plain["voxel_solves_per_s"],
+        "batched_vs_concurrent": batched["voxel_solves_per_s"] / concurrent["voxel_solves_per_s"],
         "shared_protocol": protocol,
     }
     print(json.dumps(result), flush=True)
