@@ -254,7 +254,12 @@ struct FodArgs {
 // The fast low-resolution instantiation is asked for 8 waves per SIMD, what its LDS factor
 // (4.2 KB per wave, 20 columns) allows: the register budget that leaves costs a few spilled
 // registers and still beats 7 waves with a 24-column factor.
-constexpr int fod_waves_per_eu(int ncpl, bool global_l, int /*rpl*/) { return (!global_l && ncpl == 1) ? 8 : 1; }
+#ifndef LSD_FOD_LR_WAVES
+#define LSD_FOD_LR_WAVES 8
+#endif
+constexpr int fod_waves_per_eu(int ncpl, bool global_l, int /*rpl*/) {
+    return (!global_l && ncpl == 1) ? LSD_FOD_LR_WAVES : 1;
+}
 
 // WARM (the mixed-precision form's low-resolution solves, NCPL 1): each voxel's solve starts from
 // the support its previous solve left in active_mask and leaves its own there.
@@ -546,8 +551,11 @@ struct FractionArgs {
 // With more row slots the per-wave QR slab and the block's isotropic columns cap the resident
 // waves at 3 per SIMD (6 slots) and 2 (8 slots), so asking for more would only spill registers.
 // The normal-equation fits hold no slab; 4 waves per SIMD leave them registers without spills.
+#ifndef LSD_FRACTION_LR_WAVES
+#define LSD_FRACTION_LR_WAVES 5
+#endif
 constexpr int fraction_waves_per_eu(int ncpl, int rpl, bool normal) {
-    return ncpl != 1 ? 1 : normal ? 4 : (rpl <= 4 ? 5 : (rpl <= 6 ? 3 : 2));
+    return ncpl != 1 ? 1 : normal ? 4 : (rpl <= 4 ? LSD_FRACTION_LR_WAVES : (rpl <= 6 ? 3 : 2));
 }
 
 // NORMAL: the mixed-precision form's loop fits through the normal equations (no QR slab).
