@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define LSD_ABI_VERSION 5
+#define LSD_ABI_VERSION 6
 #define LSD_ERR_INVALID 1 /* invalid argument (the reference raises ValueError) */
 #define LSD_ERR_HIP 2     /* a HIP runtime call failed */
 
@@ -443,6 +443,95 @@ int lsd_fit_lsi_by_qr(const double* signals, int64_t n_voxels, int64_t m, const 
  */
 int lsd_sh_project(const double* amplitudes, int64_t n, int64_t k, const double* projection,
                    int64_t c, int exact_integral, double* coeffs, void* stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Streamline tracking through a field of FODs (the consumer of the deconvolution's output).
+ */
+
+/* Tracking algorithms of lsd_track_params.algorithm. */
+#define LSD_TRACK_DET 0  /* deterministic: climb to the nearest FOD peak (SD_STREAM) */
+#define LSD_TRACK_PROB 1 /* probabilistic: rejection sampling in a cone (iFOD1) */
+#define LSD_TRACK_TEST 2 /* the fixed-rule algorithm that tests the step loop */
+
+/* End codes of lsd_track_batch's status (larmor's EndStatus). */
+#define LSD_END_NONE 0
+#define LSD_END_MAX_LENGTH 1
+#define LSD_END_LEFT_VOLUME 2
+#define LSD_END_LEFT_MASK 3
+#define LSD_END_LOW_FOD 4
+#define LSD_END_SHARP_TURN 5
+#define LSD_END_NO_DIRECTION 6
+
+/*
+ * The algorithm of a tracking call.  det and prob: cutoff (FOD amplitude), max_angle_deg (det:
+ * the largest turn per step; prob: the cone's half-angle), prob's trials per step and attempts per
+ * seed, and full (the FOD holds all SH degrees).  test: test_rule 0 straight, 1 refuse with
+ * test_stop (1..6), 2 quarter turn about z, 3 wander (three draws per step); its first-direction
+ * search draws a random unit vector and fails where the seed's x > test_refuses_past_x.
+ * Replaces: DetParams, ProbParams, TestAlgorithm  cpp/src/python/tracking.cpp:180-261, 409-478.
+ */
+typedef struct lsd_track_params {
+    int32_t algorithm;
+    int32_t full;
+    double cutoff;
+    double max_angle_deg;
+    int64_t trials;
+    int64_t attempts;
+    int32_t test_rule;
+    int32_t test_stop;
+    double test_refuses_past_x;
+} lsd_track_params;
+
+/* A tracking call's sampling counts, summed over its streamlines (prob only; 0 otherwise). */
+typedef struct lsd_track_stats {
+    uint64_t sampled_steps;
+    uint64_t accepted_index_sum;
+    uint64_t truncations;
+    double max_truncation;
+    uint64_t calibration_directions;
+} lsd_track_stats;
+
+/*
+ * The rejection-sampling calibration of prob for an FOD of degree lmax (all degrees if full) and
+ * a cone of half-angle max_angle_rad: the grid's size in *n (grid, host, receives min(n, cap)
+ * rows of 3 unit directions around +z; may be NULL with cap 0) and the bound's ratio.
+ * Replaces: calibrate, direction_grid  cpp/src/tracking/calibrate.cpp:41-108.
+ * Errors: "the maximum angle must be above 0.001 rad (0.057 degrees) and at most 180, got X".
+ */
+int lsd_track_calibrate(int64_t lmax, int full, double max_angle_rad, double* grid, int64_t cap,
+                        int64_t* n, double* ratio);
+
+/*
+ * Tracks every seed into its row of positions.  fod (device, nx x ny x nz x nc float32, channel
+ * fastest, tournier basis); fod_inv_affine (host, 4 x 4 world-to-voxel); mask (device, mx x my x
+ * mz float32, or NULL) with mask_inv_affine (host); seeds (device, n x 3 world mm), directions
+ * (device, n x 3, or NULL for the algorithm's first-direction search), indices (device int64,
+ * n, the global seed numbers that key each streamline's random numbers with run_seed);
+ * positions (device float32, positions_rows x (max_steps + 1) x 3, positions_rows >= n): row i
+ * receives seed i's points, rows past n and points past a row's count are not written.
+ * Outputs (device): counts (n, 0 where the seed did not start), status (n x 2 uint8: the
+ * LSD_END_* of the row's start and end), followed (n x 3, the direction given or found).
+ * stats (host, may be NULL) receives the sampling counts; it makes the call wait for the
+ * stream.  A streamline's points do not depend on the batch, the grid or the other seeds.
+ * GPU limits: lmax <= 12 (symmetric) or 8 (full).
+ * Replaces: track_batch(fod, fod_inv_affine, mask, seeds, directions, indices, run_seed,
+ *           step_size, max_steps, bidirectional, algorithm, positions, n_threads, full)
+ *           cpp/src/python/tracking.cpp:264-404; track_batch, track_seed, track_one_way
+ *           cpp/src/tracking/tracker.cpp:42-139; det.cpp, prob.cpp, testing.cpp, grid.cpp,
+ *           rng.cpp.
+ * Errors: "the cutoff must be non-negative and finite, got X", "the maximum angle must be in
+ * (0, 180], got X", "trials and attempts must be at least 1, got T and A", "need a positive step
+ * size and max_steps, got S and M", "lmax must be at most 20, and even unless full, got L",
+ * "lmax L has N coefficients, got C".
+ */
+int lsd_track_batch(const float* fod, int64_t nx, int64_t ny, int64_t nz, int64_t nc,
+                    const double* fod_inv_affine, const float* mask, int64_t mx, int64_t my,
+                    int64_t mz, const double* mask_inv_affine, const double* seeds,
+                    const double* directions, const int64_t* indices, int64_t n_seeds,
+                    uint64_t run_seed, double step_size, int64_t max_steps, int bidirectional,
+                    const lsd_track_params* params, float* positions, int64_t positions_rows,
+                    int64_t* counts, uint8_t* status, double* followed, lsd_track_stats* stats,
+                    void* stream);
 
 #ifdef __cplusplus
 }

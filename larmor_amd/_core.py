@@ -809,3 +809,232 @@ def sh_project(amplitudes, projection, *, exact_integral: bool = False):
                                           int(bool(exact_integral)), out.data_ptr(),
                                           _stream_ptr(torch)))
     return _out(out, host)
+
+
+# ---------------------------------------------------------------------------------------------
+# Streamline tracking (cpp/src/python/tracking.cpp).
+
+def _number_text(value: float) -> str:
+    """The binding's number text: fixed notation, NaN and inf spelled out."""
+    if value != value:
+        return "NaN"
+    if value in (float("inf"), float("-inf")):
+        return "inf" if value > 0 else "-inf"
+    text = np.format_float_positional(float(value), trim="-")
+    return text
+
+
+def _require_cutoff(cutoff: float) -> None:
+    if not (np.isfinite(cutoff) and cutoff >= 0.0):
+        raise ValueError(f"the cutoff must be non-negative and finite, got {_number_text(cutoff)}")
+
+
+class DetParams:
+    """Parameters of deterministic tracking (cpp/src/python/tracking.cpp:180-193, 409-426):
+    cutoff, the FOD amplitude below which a streamline stops, and max_angle, the largest turn
+    between steps in degrees.  lmax comes from the FOD."""
+
+    def __init__(self, cutoff: float, max_angle: float):
+        cutoff, max_angle = float(cutoff), float(max_angle)
+        _require_cutoff(cutoff)
+        if not (0.0 < max_angle <= 180.0):
+            raise ValueError(f"the maximum angle must be in (0, 180], got {_number_text(max_angle)}")
+        self.cutoff = cutoff
+        self.max_angle = max_angle
+
+    def _params(self, full: bool) -> "_lib.TrackParams":
+        return _lib.TrackParams(algorithm=0, full=int(full), cutoff=self.cutoff,
+                                max_angle_deg=self.max_angle, trials=1, attempts=1)
+
+
+class ProbParams:
+    """Parameters of probabilistic tracking (cpp/src/python/tracking.cpp:195-224, 428-450):
+    cutoff, the cone's half-angle max_angle in degrees, trials candidates per step and attempts
+    random directions per seed.  The rejection-sampling bound is calibrated from the FOD's lmax."""
+
+    def __init__(self, cutoff: float, max_angle: float, trials: int = 1000, attempts: int = 1000):
+        cutoff, max_angle = float(cutoff), float(max_angle)
+        _require_cutoff(cutoff)
+        if not (max_angle * (np.pi / 180.0) > 0.001 and max_angle <= 180.0):
+            raise ValueError("the maximum angle must be above 0.001 rad (0.057 degrees) and at "
+                             f"most 180, got {_number_text(max_angle)}")
+        if trials < 1 or attempts < 1:
+            raise ValueError(f"trials and attempts must be at least 1, got {trials} and {attempts}")
+        self.cutoff = cutoff
+        self.max_angle = max_angle
+        self.trials = int(trials)
+        self.attempts = int(attempts)
+
+    def _params(self, full: bool) -> "_lib.TrackParams":
+        return _lib.TrackParams(algorithm=1, full=int(full), cutoff=self.cutoff,
+                                max_angle_deg=self.max_angle, trials=self.trials,
+                                attempts=self.attempts)
+
+
+_TEST_RULES = {"straight": 0, "refuse": 1, "quarter_turn": 2, "wander": 3}
+
+
+class TestAlgorithm:
+    """The fixed-rule algorithm that tests the step loop (cpp/src/python/tracking.cpp:117-138,
+    452-478): "straight", "quarter_turn" (90 degrees about z per step), "wander" (a random walk of
+    three draws per step), or "refuse" with stop, an EndStatus from 1 to 6.  Its first-direction
+    search draws a random unit vector and fails where the seed's x > search_refuses_past_x."""
+
+    __test__ = False  # not a pytest class
+
+    def __init__(self, step: str, stop: int | None = None,
+                 search_refuses_past_x: float = float("inf")):
+        if stop is not None and not (0 <= int(stop) <= 255):
+            raise OverflowError("out of range integral type conversion attempted")
+        if stop is None and step in ("straight", "quarter_turn", "wander"):
+            pass
+        elif stop is not None and step == "refuse":
+            if not 1 <= int(stop) <= 6:
+                raise ValueError(f"a stop is an EndStatus from 1 to 6, got {int(stop)}")
+        else:
+            stop_text = f"Some({int(stop)})" if stop is not None else "None"
+            raise ValueError("expected straight, quarter_turn, wander, or refuse with a stop; got "
+                             f'"{step}" with stop {stop_text}')
+        self.step = step
+        self.stop = None if stop is None else int(stop)
+        self.search_refuses_past_x = float(search_refuses_past_x)
+
+    def _params(self, full: bool) -> "_lib.TrackParams":
+        return _lib.TrackParams(algorithm=2, full=int(full), cutoff=0.0, max_angle_deg=180.0,
+                                trials=1, attempts=1, test_rule=_TEST_RULES[self.step],
+                                test_stop=self.stop or 0,
+                                test_refuses_past_x=self.search_refuses_past_x)
+
+
+def calibrate_tracking(lmax: int, full: bool, max_angle_rad: float):
+    """The rejection-sampling calibration of prob (cpp/src/tracking/calibrate.cpp:63-108): the
+    grid of directions around +z (n, 3) and the bound's ratio."""
+    lib = _lib.load()
+    n = ctypes.c_int64()
+    ratio = ctypes.c_double()
+    _lib.check(lib.lsd_track_calibrate(int(lmax), int(bool(full)), float(max_angle_rad), None, 0,
+                                       ctypes.byref(n), ctypes.byref(ratio)))
+    grid = np.zeros((n.value, 3))
+    _lib.check(lib.lsd_track_calibrate(int(lmax), int(bool(full)), float(max_angle_rad),
+                                       _ptr(grid), n.value, ctypes.byref(n), ctypes.byref(ratio)))
+    return grid, ratio.value
+
+
+def _tuple_text(shape) -> str:
+    return "(" + ", ".join(str(x) for x in shape) + ")"
+
+
+def _inverse_affine(m, name: str) -> np.ndarray:
+    _check_f64(m, name)
+    a = np.asarray(m.cpu().numpy() if hasattr(m, "cpu") else m)
+    if a.ndim != 2 or a.shape != (4, 4):
+        raise ValueError(f"{name} must be (4, 4), got {_tuple_text(a.shape[:2])}")
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _require_dtype(array, dtype, name: str) -> None:
+    torch_types = {np.float32: "float32", np.float64: "float64", np.int64: "int64"}
+    if isinstance(array, np.ndarray):
+        if array.dtype != dtype:
+            raise TypeError(f"{name} must be {np.dtype(dtype).name}, got {array.dtype}")
+        return
+    if str(array.dtype).replace("torch.", "") != torch_types[dtype]:
+        raise TypeError(f"{name} must be {np.dtype(dtype).name}, got {array.dtype}")
+
+
+def track_batch(fod, fod_inv_affine, mask, seeds, directions, indices, run_seed: int,
+                step_size: float, max_steps: int, bidirectional: bool, algorithm, positions,
+                n_threads: int, full: bool = False):
+    """Tracks a batch of seeds into the rows of positions (cpp/src/python/tracking.cpp:264-404).
+
+    fod (X, Y, Z, C) float32 tournier, all degrees if full; mask None or ((X, Y, Z) float32,
+    (4, 4) world-to-voxel); seeds (n, 3) float64 world mm, directions (n, 3) or None for the
+    algorithm's first-direction search, indices (n,) int64 keying each streamline's random
+    numbers; positions (batch >= n, max_steps + 1, 3) float32, filled in place (rows past n and
+    points past a row's count are left as they are).  Arrays may be numpy (copied to the device
+    and back) or torch CUDA tensors (used in place).  Returns counts (n,) int64, status (n, 2)
+    uint8, followed (n, 3) float64 and the sampling counts -- torch tensors when positions is one.
+    """
+    if not isinstance(algorithm, (DetParams, ProbParams, TestAlgorithm)):
+        raise TypeError("expected DetParams, ProbParams or TestAlgorithm for algorithm, got "
+                        f"{type(algorithm).__name__}")
+    torch = _torch()
+    _require_dtype(fod, np.float32, "fod")
+    if not _is_c_contiguous(fod):
+        raise ValueError("the array must be C-contiguous")
+    if len(fod.shape) != 4:
+        raise ValueError(f"fod must be (X, Y, Z, C), got {_tuple_text(fod.shape)}")
+    fod_inv = _inverse_affine(fod_inv_affine, "fod_inv_affine")
+    mask_values, mask_inv = None, None
+    if mask is not None:
+        mask_values, mask_inv_in = mask
+        _require_dtype(mask_values, np.float32, "mask")
+        if not _is_c_contiguous(mask_values):
+            raise ValueError("the array must be C-contiguous")
+        mask_inv = _inverse_affine(mask_inv_in, "mask inverse affine")
+    _require_dtype(seeds, np.float64, "seeds")
+    if directions is not None:
+        _require_dtype(directions, np.float64, "directions")
+    _require_dtype(indices, np.int64, "indices")
+    n = seeds.shape[0]
+    bad_dirs = directions is not None and (len(directions.shape) != 2 or directions.shape[0] != n
+                                           or directions.shape[1] != 3)
+    if len(seeds.shape) != 2 or seeds.shape[1] != 3 or bad_dirs or indices.shape[0] != n:
+        dir_text = "None" if directions is None else f"Some({_tuple_text(directions.shape[:2])})"
+        raise ValueError("expected seeds and directions (n, 3) and indices (n,): got seeds "
+                         f"{_tuple_text(seeds.shape[:2])}, directions {dir_text}, indices "
+                         f"{indices.shape[0]}")
+    idx_min = (int(indices.min()) if isinstance(indices, np.ndarray) else int(indices.min().item())) \
+        if n else 0
+    if idx_min < 0:
+        raise ValueError("seed indices must be non-negative")
+    if not (np.isfinite(step_size) and step_size > 0 and max_steps > 0):
+        raise ValueError(f"need a positive step size and max_steps, got {_number_text(step_size)} "
+                         f"and {max_steps}")
+    _require_dtype(positions, np.float32, "positions")
+    if not _is_c_contiguous(positions):
+        raise ValueError("positions must be C-contiguous")
+    pshape = tuple(positions.shape)
+    if len(pshape) != 3 or pshape[0] < n or pshape[1] != max_steps + 1 or pshape[2] != 3:
+        raise ValueError(f"positions must be (at least {n}, {max_steps + 1}, 3), got "
+                         f"{_tuple_text(pshape)}")
+    _require_threads(n_threads)
+
+    params = algorithm._params(full)
+    d_fod, _ = _as_device(fod, torch)
+    d_mask = None
+    if mask_values is not None:
+        d_mask, _ = _as_device(mask_values, torch)
+    d_seeds, _ = _as_device(seeds, torch)
+    d_dirs = None if directions is None else _as_device(directions, torch)[0]
+    d_idx, _ = _as_device(indices, torch)
+    on_host = isinstance(positions, np.ndarray)
+    d_pos = torch.empty((n, max_steps + 1, 3), dtype=torch.float32, device="cuda") if on_host \
+        else positions
+    counts = torch.empty(n, dtype=torch.int64, device="cuda")
+    status = torch.empty((n, 2), dtype=torch.uint8, device="cuda")
+    followed = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+    stats = _lib.TrackStats()
+    fs = fod.shape
+    ms = (0, 0, 0) if mask_values is None else tuple(mask_values.shape)
+    _lib.check(_lib.load().lsd_track_batch(
+        d_fod.data_ptr(), fs[0], fs[1], fs[2], fs[3], _ptr(fod_inv),
+        None if d_mask is None else d_mask.data_ptr(), ms[0], ms[1], ms[2],
+        None if mask_inv is None else _ptr(mask_inv),
+        d_seeds.data_ptr(), None if d_dirs is None else d_dirs.data_ptr(), d_idx.data_ptr(), n,
+        int(run_seed) & 0xFFFFFFFFFFFFFFFF, float(step_size), int(max_steps),
+        int(bool(bidirectional)), ctypes.byref(params), d_pos.data_ptr(), pshape[0],
+        counts.data_ptr(), status.data_ptr(), followed.data_ptr(), ctypes.byref(stats),
+        _stream_ptr(torch)))
+    stats_out = {"sampled_steps": stats.sampled_steps,
+                 "accepted_index_sum": stats.accepted_index_sum,
+                 "truncations": stats.truncations, "max_truncation": stats.max_truncation,
+                 "calibration_directions": stats.calibration_directions}
+    if not on_host:
+        return counts, status, followed, stats_out
+    counts_h = counts.cpu().numpy()
+    if n:
+        rows = d_pos.cpu().numpy()
+        written = np.arange(max_steps + 1)[None, :, None] < counts_h[:, None, None]
+        np.copyto(positions[:n], rows, where=written)
+    return counts_h, status.cpu().numpy(), followed.cpu().numpy(), stats_out

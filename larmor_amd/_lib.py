@@ -8,7 +8,7 @@ import ctypes
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "liblarmor_sd.so"
-ABI_VERSION = 5  # LSD_ABI_VERSION of include/larmor_sd.h these bindings follow
+ABI_VERSION = 6  # LSD_ABI_VERSION of include/larmor_sd.h these bindings follow
 HEADER_PATH = Path(__file__).resolve().parents[1] / "include" / "larmor_sd.h"
 
 _lib = None
@@ -151,6 +151,50 @@ SIGNATURES = {
         [_vp, ctypes.c_int64, ctypes.c_int64, _vp, ctypes.c_int64, ctypes.c_int, _vp, _vp],
     ),
 }
+
+
+
+
+class TrackParams(ctypes.Structure):
+    """lsd_track_params of include/larmor_sd.h."""
+
+    _fields_ = [
+        ("algorithm", ctypes.c_int32),
+        ("full", ctypes.c_int32),
+        ("cutoff", ctypes.c_double),
+        ("max_angle_deg", ctypes.c_double),
+        ("trials", ctypes.c_int64),
+        ("attempts", ctypes.c_int64),
+        ("test_rule", ctypes.c_int32),
+        ("test_stop", ctypes.c_int32),
+        ("test_refuses_past_x", ctypes.c_double),
+    ]
+
+
+class TrackStats(ctypes.Structure):
+    """lsd_track_stats of include/larmor_sd.h."""
+
+    _fields_ = [
+        ("sampled_steps", ctypes.c_uint64),
+        ("accepted_index_sum", ctypes.c_uint64),
+        ("truncations", ctypes.c_uint64),
+        ("max_truncation", ctypes.c_double),
+        ("calibration_directions", ctypes.c_uint64),
+    ]
+
+
+SIGNATURES["lsd_track_calibrate"] = (
+    ctypes.c_int,
+    [ctypes.c_int64, ctypes.c_int, ctypes.c_double, _d, ctypes.c_int64, _i64, _d],
+)
+SIGNATURES["lsd_track_batch"] = (
+    ctypes.c_int,
+    [_vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _d,
+     _vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _d,
+     _vp, _vp, _vp, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double, ctypes.c_int64,
+     ctypes.c_int, ctypes.POINTER(TrackParams), _vp, ctypes.c_int64, _vp, _vp, _vp,
+     ctypes.POINTER(TrackStats), _vp],
+)
 
 
 class LibraryMissing(RuntimeError):

@@ -42,7 +42,8 @@ def _torch_lib_dir() -> str:
 def sources() -> list[Path]:
     """The API units (FP64 and the plain FP32 form) and one unit per (kernel family, column slots)
     instantiation set."""
-    return [CSRC / "sd_api.hip", CSRC / "fp32_api.hip", *sorted(CSRC.glob("inst_*.hip"))]
+    return [CSRC / "sd_api.hip", CSRC / "fp32_api.hip", CSRC / "track_api.hip",
+            *sorted(CSRC.glob("inst_*.hip"))]
 
 
 def _compile(src: Path, obj: Path, extra: list[str]) -> tuple[Path, int, str]:

@@ -488,3 +488,135 @@ def fit_log_signal(xt, pinv, scales, log_s, weighted=True, max_reweightings=20, 
                                          int(weighted), max_reweightings, tol, _p(out),
                                          refits.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
     return out, refits
+
+
+# ---------------------------------------------------------------------------------------------
+# Streamline tracking (track_oracle.cpp).
+
+class TrackParams(ctypes.Structure):
+    """The algorithm's settings (the layout of lsd_track_params in include/larmor_sd.h):
+    algorithm 0 det, 1 prob, 2 the fixed-rule test algorithm (rule 0 straight, 1 refuse with
+    test_stop, 2 quarter turn, 3 wander; its search refuses seeds with x > test_refuses_past_x)."""
+
+    _fields_ = [
+        ("algorithm", ctypes.c_int32),
+        ("full", ctypes.c_int32),
+        ("cutoff", ctypes.c_double),
+        ("max_angle_deg", ctypes.c_double),
+        ("trials", ctypes.c_int64),
+        ("attempts", ctypes.c_int64),
+        ("test_rule", ctypes.c_int32),
+        ("test_stop", ctypes.c_int32),
+        ("test_refuses_past_x", ctypes.c_double),
+    ]
+
+
+_u64 = ctypes.POINTER(ctypes.c_uint64)
+_f = ctypes.POINTER(ctypes.c_float)
+
+
+def _track_lib():
+    lib = _load()
+    if getattr(lib, "_track_ready", False):
+        return lib
+    lib.oracle_track_last_error.restype = ctypes.c_char_p
+    lib.oracle_rng_uniforms.restype = ctypes.c_uint64
+    lib.oracle_rng_uniforms.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                        ctypes.c_int64, _d]
+    lib.oracle_sh_amplitude.argtypes = [_d, ctypes.c_int64, ctypes.c_int, _d, ctypes.c_int64, _d]
+    lib.oracle_rotate_to.argtypes = [_d, _d, ctypes.c_int64, _d]
+    lib.oracle_direction_grid.restype = ctypes.c_int64
+    lib.oracle_direction_grid.argtypes = [ctypes.c_double, ctypes.c_double, _d, ctypes.c_int64]
+    lib.oracle_calibrate.restype = ctypes.c_int64
+    lib.oracle_calibrate.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_double, _d,
+                                     ctypes.c_int64, _d]
+    lib.oracle_track_batch.argtypes = [
+        _f, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _d,
+        _f, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _d,
+        _d, _d, _i64, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double, ctypes.c_int64,
+        ctypes.c_int, ctypes.POINTER(TrackParams), ctypes.c_int, _f, _i64,
+        ctypes.POINTER(ctypes.c_uint8), _d, _u64, _d, _i64]
+    lib._track_ready = True
+    return lib
+
+
+def rng_uniforms(run_seed: int, index: int, n: int, search: bool = False):
+    """(key, n uniform draws) of streamline `index` in a run seeded `run_seed`."""
+    out = np.zeros(n)
+    key = _track_lib().oracle_rng_uniforms(int(run_seed), int(index), int(search), int(n), _p(out))
+    return int(key), out
+
+
+def sh_amplitude(coeffs, lmax, dirs, full=False):
+    """The series' amplitudes along the unit vectors dirs (n, 3)."""
+    c, d = _c(coeffs), _c(dirs).reshape(-1, 3)
+    out = np.zeros(d.shape[0])
+    _track_lib().oracle_sh_amplitude(_p(c), int(lmax), int(full), _p(d), d.shape[0], _p(out))
+    return out
+
+
+def rotate_to(reference, dirs):
+    """dirs (n, 3) turned by the rotation that takes +z to the unit vector reference."""
+    r, d = _c(reference), _c(dirs).reshape(-1, 3)
+    out = np.zeros_like(d)
+    _track_lib().oracle_rotate_to(_p(r), _p(d), d.shape[0], _p(out))
+    return out
+
+
+def direction_grid(max_angle, spacing):
+    lib = _track_lib()
+    n = lib.oracle_direction_grid(float(max_angle), float(spacing), None, 0)
+    out = np.zeros((n, 3))
+    lib.oracle_direction_grid(float(max_angle), float(spacing), _p(out), n)
+    return out
+
+
+def calibrate(lmax, full, max_angle):
+    """(grid (n, 3), ratio) of the rejection sampling for a cone of half-angle max_angle (rad)."""
+    lib = _track_lib()
+    ratio = np.zeros(1)
+    n = lib.oracle_calibrate(int(lmax), int(full), float(max_angle), None, 0, _p(ratio))
+    out = np.zeros((n, 3))
+    lib.oracle_calibrate(int(lmax), int(full), float(max_angle), _p(out), n, _p(ratio))
+    return out, float(ratio[0])
+
+
+def track_batch(fod, fod_inv, mask, seeds, directions, indices, run_seed, step_size, max_steps,
+                bidirectional, params: TrackParams, n_threads=None):
+    """Tracks every seed; returns (positions (n, max_steps + 1, 3) float32, counts, status (n, 2)
+    uint8, followed (n, 3), stats dict).  mask is None or (values (X, Y, Z) float32, inverse
+    affine); rows past a streamline's points are zero."""
+    f = _c(fod, np.float32)
+    fi = _c(fod_inv)
+    s = _c(seeds).reshape(-1, 3)
+    n = s.shape[0]
+    d = None if directions is None else _c(directions).reshape(-1, 3)
+    idx = _c(indices, np.int64)
+    if mask is not None:
+        mv, mi = _c(mask[0], np.float32), _c(mask[1])
+        mshape = mv.shape
+    else:
+        mv, mi, mshape = None, None, (0, 0, 0)
+    pos = np.zeros((n, max_steps + 1, 3), dtype=np.float32)
+    counts = np.zeros(n, dtype=np.int64)
+    status = np.zeros((n, 2), dtype=np.uint8)
+    followed = np.zeros((n, 3))
+    sc = np.zeros(3, dtype=np.uint64)
+    mt = np.zeros(1)
+    ncal = np.zeros(1, dtype=np.int64)
+    lib = _track_lib()
+    rc = lib.oracle_track_batch(
+        f.ctypes.data_as(_f), *f.shape[:3], f.shape[3], _p(fi),
+        None if mv is None else mv.ctypes.data_as(_f), *mshape,
+        None if mi is None else _p(mi), _p(s), None if d is None else _p(d),
+        idx.ctypes.data_as(_i64), n, int(run_seed), float(step_size), int(max_steps),
+        int(bool(bidirectional)), ctypes.byref(params), n_threads or _default_threads(),
+        pos.ctypes.data_as(_f), counts.ctypes.data_as(_i64),
+        status.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _p(followed),
+        sc.ctypes.data_as(_u64), _p(mt), ncal.ctypes.data_as(_i64))
+    if rc != 0:
+        raise ValueError(lib.oracle_track_last_error().decode())
+    stats = {"sampled_steps": int(sc[0]), "accepted_index_sum": int(sc[1]),
+             "truncations": int(sc[2]), "max_truncation": float(mt[0]),
+             "calibration_directions": int(ncal[0])}
+    return pos, counts, status, followed, stats

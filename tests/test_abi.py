@@ -18,7 +18,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert _lib.load().lsd_abi_version() == _lib.ABI_VERSION == 5
+    assert _lib.load().lsd_abi_version() == _lib.ABI_VERSION == 6
 
 
 def test_solver_argument_errors_match_reference():
