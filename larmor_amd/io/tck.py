@@ -13,6 +13,8 @@ import numpy as np
 from larmor_amd.types.tractogram import Tractogram
 
 MAGIC = "mrtrix tracks"
+# Header keys the writers set themselves.
+_RESERVED = {"count", "datatype", "file"}
 
 
 def read_tck_header(path) -> tuple[dict, int]:
@@ -67,6 +69,8 @@ def write_tck(tractogram: Tractogram, path, extra: dict | None = None) -> None:
     """Writes Float32LE TCK with a count line and any extra header fields."""
     lines = [MAGIC]
     for key, value in (extra or {}).items():
+        if key in _RESERVED:
+            continue
         for part in str(value).split("\n"):
             lines.append(f"{key}: {part}")
     lines.append(f"count: {len(tractogram)}")
@@ -91,3 +95,58 @@ def write_tck(tractogram: Tractogram, path, extra: dict | None = None) -> None:
     with open(path, "wb") as f:
         f.write(header)
         f.write(rows.tobytes())
+
+
+class TckWriter:
+    """Writes a TCK file batch by batch: the header with a fixed-width count, each batch's rows
+    appended before the end marker, and the count filled in at close."""
+
+    COUNT_WIDTH = 20
+
+    def __init__(self, path, extra: dict | None = None):
+        self.path = Path(path)
+        self.count = 0
+        lines = [MAGIC]
+        for key, value in (extra or {}).items():
+            if key in _RESERVED:
+                continue
+            for part in str(value).split("\n"):
+                lines.append(f"{key}: {part}")
+        self._count_prefix = "\n".join(lines) + "\ncount: "
+        head = self._count_prefix + "0".zfill(self.COUNT_WIDTH) + "\ndatatype: Float32LE\nfile: . "
+        offset = len(head) + len("\nEND\n") + 1
+        while len(f"{head}{offset}\nEND\n") > offset:
+            offset += 1
+        header = f"{head}{offset}\nEND\n".encode("latin-1")
+        self._file = open(self.path, "wb")
+        self._file.write(header + b"\0" * (offset - len(header)))
+        self._file.write(np.full(3, np.inf, dtype="<f4").tobytes())
+
+    def write(self, tractogram: Tractogram) -> None:
+        n = len(tractogram)
+        if n == 0:
+            return
+        pts = np.asarray(tractogram.points, dtype="<f4")
+        rows = np.empty((len(pts) + n, 3), dtype="<f4")
+        idx = tractogram.offsets[1:] - 1 + np.arange(1, n + 1)
+        mask = np.ones(len(rows), dtype=bool)
+        mask[idx] = False
+        rows[mask] = pts
+        rows[idx] = np.nan
+        self._file.seek(-12, 2)
+        self._file.write(rows.tobytes())
+        self._file.write(np.full(3, np.inf, dtype="<f4").tobytes())
+        self.count += n
+
+    def close(self) -> None:
+        if self._file.closed:
+            return
+        self._file.seek(len(self._count_prefix.encode("latin-1")))
+        self._file.write(str(self.count).zfill(self.COUNT_WIDTH).encode("latin-1"))
+        self._file.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

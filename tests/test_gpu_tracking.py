@@ -256,3 +256,75 @@ def test_gpu_limits_raise():
     with pytest.raises(ValueError, match="the GPU tracking kernels take lmax <= 12"):
         _core.track_batch(fod, np.eye(4), None, seeds, None, np.zeros(1, np.int64), 1, 1.0, 10,
                           True, _core.DetParams(0.1, 30.0), pos, 1)
+
+
+# ---------------------------------------------------------------------------------------------
+# The Tracker end to end (the reference's tests/tractography/test_det_tckgen.py and
+# test_prob_tckgen.py drive it the same way).
+
+
+def _fixture_volumes():
+    from larmor_amd.types import Volume
+
+    z = fixture()
+    return z, Volume(z["fod"], z["fod_affine"]), Volume(z["mask"], z["mask_affine"])
+
+
+def test_tracker_det_matches_tckgen():
+    from larmor_amd.tractography import EndStatus, Seeds, Tracker
+
+    z, fod, mask = _fixture_volumes()
+    refs, k, seeds = det_reference(z, "det_ref")
+    run = Tracker("det", step_size=STEP, max_angle=30.0, cutoff=CUTOFF, min_length=5.0,
+                  max_length=80.0).track(fod, None, Seeds(seeds, np.tile([1.0, 0, 0], (32, 1))),
+                                         mask=mask, seed=1, n_jobs=1)
+    t = run.tractogram
+    assert run.num_streamlines == 32 and run.num_seeds == 32
+    assert run.num_rejected == {"no_start": 0, "too_short": 0}
+    for i, (r, j) in enumerate(zip(refs, k)):
+        p = t.points[t.offsets[i]:t.offsets[i + 1]].astype(np.float64)
+        at = int(np.flatnonzero(np.all(p == seeds[i].astype(np.float32), axis=1))[0])
+        for x, y in zip(halves(p, at), halves(r, int(j))):
+            assert len(x) == len(y)
+            np.testing.assert_allclose(x, y, rtol=0, atol=POS_TOL)
+    assert set(np.unique(t.streamline_data["end_status"]).tolist()) == {
+        EndStatus.LEFT_VOLUME, EndStatus.LEFT_MASK, EndStatus.LOW_FOD, EndStatus.SHARP_TURN}
+    assert run.parameters["max_steps"] == 400
+
+
+def test_tracker_prob_sampling_is_tckgens():
+    from larmor_amd.tractography import Seeds, Tracker
+
+    z, fod, mask = _fixture_volumes()
+    seeds = z["prob_ref_seeds"][:, 2:5]
+    run = Tracker("prob", cutoff=CUTOFF, min_length=5.0, max_length=80.0).track(
+        fod, None, Seeds(seeds, np.tile([1.0, 0, 0], (len(seeds), 1))), mask=mask, seed=1)
+    assert run.num_streamlines == len(seeds)
+    s = run.sampling
+    means = z["prob_info_mean_samples_per_step"]
+    assert s["calibration_directions"] == 7
+    assert abs(s["mean_samples_per_step"] - means.mean()) < 4 * means.std(ddof=1)
+
+
+def test_tracker_repeats_a_run_from_its_seed_file(tmp_path):
+    from larmor_amd.io.tck import read_tck
+    from larmor_amd.tractography import MaskSeeds, Tracker, read_seeds
+    from larmor_amd.types import Volume
+
+    fod_data, affine = fibre_field(lmax=8)
+    fod = Volume(fod_data, affine)
+    mask = Volume(np.ones(fod_data.shape[:3], np.float32), affine)
+    tracker = Tracker("prob", step_size=0.5, max_angle=20.0, cutoff="auto", min_length=3.0,
+                      max_length=40.0)
+    run = tracker.track(fod, tmp_path / "t.tck", MaskSeeds(mask), num_streamlines=700,
+                        mask=mask, seed=12345, output_seeds=tmp_path / "s.txt")
+    assert run.num_streamlines == 700
+    assert run.num_seeds == 700 + sum(run.num_rejected.values())
+    first = read_tck(tmp_path / "t.tck")
+    assert len(first) == 700
+    seeds = read_seeds(tmp_path / "s.txt")
+    assert len(seeds) == 700 and seeds.indices is not None and seeds.directions is not None
+    again = tracker.track(fod, None, seeds, mask=mask, seed=12345)
+    t = again.tractogram
+    assert np.array_equal(t.offsets, first.offsets)
+    assert np.array_equal(t.points, first.points)
