@@ -646,6 +646,108 @@ def rows_report() -> int:
     return 0
 
 
+def track_report(n_seeds: int, cpu_seconds: float) -> int:
+    """Streamline tracking (--track): det and prob over a brain-sized synthetic FOD field
+    (96 x 96 x 60 voxels of 2 mm, lmax 8, float32; coherent arcs, a vertical and an
+    anterior-posterior bundle, crossing), n_seeds seeds drawn in a 40% ellipsoid mask without
+    directions (each searches its own), the tracker's defaults at this voxel size (step 0.2 mm,
+    max length 200 mm = 1000 steps, det's angle 8.8 degrees, prob's 15) and cutoff 0.1.  The FOD,
+    the mask, the seeds and the points buffer are resident in HBM; one call tracks every seed.
+    Prints one JSON line per algorithm: streamlines/s and steps/s on the GPU, the kernel's mean
+    launch time from HIP events, and the oracle on a bounded sample of the same seeds on the
+    host's cores."""
+    import math
+
+    import torch
+
+    import oracle
+    from larmor_amd import _core, _lib, synthetic
+    from larmor_amd.tractography.seed import MaskSeeds
+    from larmor_amd.types import Volume
+
+    fod, affine, mask = synthetic.tracking_field()
+    inv = np.linalg.inv(affine)
+    mask_f = mask.astype(np.float32)
+    seeds = MaskSeeds(Volume(mask_f, affine)).draw(n_seeds, np.random.default_rng(1)).positions
+    idx = np.arange(n_seeds, dtype=np.int64)
+    d_fod = torch.from_numpy(fod).to("cuda")
+    d_mask = (torch.from_numpy(mask_f).to("cuda"), inv)
+    d_seeds = torch.from_numpy(seeds).to("cuda")
+    d_idx = torch.from_numpy(idx).to("cuda")
+    step, max_steps, cutoff = 0.2, 1000, 0.1
+    det_angle = math.degrees(2 * math.asin(step / (2 * 0.65 * 2.0)))
+    d_pos = torch.empty((n_seeds, max_steps + 1, 3), dtype=torch.float32, device="cuda")
+    threads = host_cores()
+    rng = np.random.default_rng(0)
+    for name, algo, ref in [
+        ("det", _core.DetParams(cutoff, det_angle),
+         oracle.TrackParams(algorithm=0, cutoff=cutoff, max_angle_deg=det_angle)),
+        ("prob", _core.ProbParams(cutoff, 15.0),
+         oracle.TrackParams(algorithm=1, cutoff=cutoff, max_angle_deg=15.0, trials=1000,
+                            attempts=1000)),
+    ]:
+        def run(n=n_seeds):
+            return _core.track_batch(d_fod, inv, d_mask, d_seeds[:n], None, d_idx[:n], 1, step,
+                                     max_steps, True, algo, d_pos, 1)
+
+        run(min(n_seeds, 4096))  # warm-up (module load, calibration)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        counts, _, _, stats = run()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        _lib.profile_enable(True)
+        _lib.profile_collect()
+        run()
+        torch.cuda.synchronize()
+        kt = _lib.profile_collect()
+        _lib.profile_enable(False)
+        c = counts.cpu().numpy()
+        started = int((c > 0).sum())
+        steps = int((c[c > 0] - 1).sum())
+        # The oracle on a sample sized for about cpu_seconds of host work.
+        probe = rng.choice(n_seeds, size=min(200, n_seeds), replace=False)
+        t1 = time.perf_counter()
+        oracle.track_batch(fod, inv, (mask_f, inv), seeds[probe], None, idx[probe], 1, step,
+                           max_steps, True, ref, n_threads=threads)
+        rate = len(probe) / (time.perf_counter() - t1)
+        n_cpu = int(min(n_seeds, max(500, rate * cpu_seconds)))
+        sample = rng.choice(n_seeds, size=n_cpu, replace=False)
+        t1 = time.perf_counter()
+        r = oracle.track_batch(fod, inv, (mask_f, inv), seeds[sample], None, idx[sample], 1,
+                               step, max_steps, True, ref, n_threads=threads)
+        cpu_dt = time.perf_counter() - t1
+        same = float(np.mean(r[1] == c[sample]))
+        started_cpu = r[1] > 0
+        ms, launches = kt.get("track", (0.0, 0, 0.0))[:2]
+        line = {
+            "metric": f"streamlines/s ({name} tracking)",
+            "value": n_seeds / dt,
+            "unit": "seeds/s",
+            "started_per_s": started / dt,
+            "steps_per_s": steps / dt,
+            "ms": dt * 1e3,
+            "kernel_ms": ms / max(launches, 1),
+            "dtype": "f64 (float32 FOD and points)",
+            "config": {"workload": "synthetic FOD field 96x96x60, 2 mm, lmax 8, 40% ellipsoid "
+                                   "mask; seeds without directions",
+                       "seeds": n_seeds, "started": started, "steps": steps,
+                       "mean_points": float(c[c > 0].mean()) if started else 0.0,
+                       "step_mm": step, "max_steps": max_steps, "cutoff": cutoff,
+                       "max_angle": det_angle if name == "det" else 15.0,
+                       "sampling": stats},
+            "cpu_baseline": {"value": n_cpu / cpu_dt, "unit": "seeds/s", "cores": threads,
+                             "kind": "port",
+                             "sample": f"{n_cpu} random seeds of {n_seeds} ({cpu_dt:.1f} s)",
+                             "cpu_model": cpu_model(),
+                             "same_counts_as_gpu": same,
+                             "mean_points": float(r[1][started_cpu].mean()),
+                             "gpu_mean_points_on_sample": float(c[sample][started_cpu].mean())},
+        }
+        print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__)
     parser.add_argument("--gpus", type=int, default=1)
@@ -657,6 +759,10 @@ def main() -> int:
                         help="instead of the JSON line, time every row of the path (single-tissue "
                              "NNLS, GRL, Lasso, SH projection, CSD, MSMT-CSD, peaks, the CSD and "
                              "three-tissue response estimates) beside its CPU oracle")
+    parser.add_argument("--track", action="store_true",
+                        help="instead of the JSON line, time det and prob streamline tracking "
+                             "over a brain-sized synthetic FOD field beside the CPU oracle")
+    parser.add_argument("--track-seeds", type=int, default=200_000)
     parser.add_argument("--serve", action="store_true",
                         help="instead of the JSON line above, measure the batched and served forms "
                              "(many requests with their own acquisitions; p50/p99 latency)")
@@ -687,6 +793,8 @@ def main() -> int:
         return copy_probe(args.copy_probe)
     if args.rows:
         return rows_report()
+    if args.track:
+        return track_report(args.track_seeds, args.cpu_seconds)
     if args.gpus != 1:
         print("bench.py measures one GPU; the path shards voxels across GPUs with no exchange, "
               "and runs on more than one GPU are not part of this build.", file=sys.stderr)

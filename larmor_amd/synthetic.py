@@ -298,3 +298,56 @@ def three_tissue_phantom(shape=(26, 26, 16), shells=(1000.0, 2000.0, 3000.0), se
     noisy = np.abs(signal + noise * rng.standard_normal(signal.shape)).astype(np.float32)
     return (noisy * mask[..., None], bvals, bvecs, mask, {"wm": wm, "gm": gm, "csf": csf},
             ~crossing)
+
+
+def lobe_zonal(lmax: int, sharpness: float = 8.0) -> np.ndarray:
+    """Zonal SH coefficients z_l (even l) of the axially symmetric lobe exp(-k (1 - cos^2 t)) about
+    +z, by Gauss-Legendre quadrature of its projection on the normalised Legendre functions."""
+    from scipy.special import eval_legendre
+
+    t, w = np.polynomial.legendre.leggauss(200)
+    f = np.exp(-sharpness * (1.0 - t * t))
+    z = np.zeros(lmax + 1)
+    for ell in range(0, lmax + 1, 2):
+        z[ell] = 2.0 * np.pi * np.sum(w * f * np.sqrt((2 * ell + 1) / (4 * np.pi))
+                                      * eval_legendre(ell, t))
+    return z
+
+
+def tracking_field(shape=(96, 96, 60), lmax: int = 8, voxel: float = 2.0, share: float = 0.4):
+    """A brain-sized synthetic FOD field with coherent bundles, for tracking: circular arcs about
+    the volume's vertical axis everywhere (a commissure-like sheet), a vertical bundle in a central
+    slab and an anterior-posterior bundle in a horizontal slab, crossing where they overlap; each
+    fibre a lobe exp(-8 (1 - (u.a)^2)) placed on the tournier basis by rotating its zonal
+    coefficients (c_lm = z_l sqrt(4 pi / (2l + 1)) Y_lm(a)), plus an isotropic floor of 0.05.
+    Returns (fod (X, Y, Z, C) float32, voxel-to-world affine, ellipsoid mask of `share`)."""
+    from larmor_amd.sd.sh import build_basis_matrix, get_sh_indices
+
+    ls, _ = get_sh_indices(lmax)
+    z = lobe_zonal(lmax)
+    scale = z[ls] * np.sqrt(4 * np.pi / (2 * ls + 1))
+    x, y, zz = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij")
+    cx, cy, cz = [(n - 1) / 2.0 for n in shape]
+
+    def coefficients(dirs):
+        d = dirs.reshape(-1, 3)
+        d = d / np.linalg.norm(d, axis=1, keepdims=True)
+        b = build_basis_matrix(np.arccos(np.clip(d[:, 2], -1, 1)), np.arctan2(d[:, 1], d[:, 0]),
+                               lmax)
+        return b * scale
+
+    tangent = np.stack([-(y - cy), x - cx, 0.15 * (zz - cz)], -1)
+    tangent[np.linalg.norm(tangent, axis=-1) == 0] = [1.0, 0.0, 0.0]
+    vertical = np.stack([0.1 * (x - cx) / shape[0], 0.1 * (y - cy) / shape[1],
+                         np.ones(shape)], -1)
+    ap = np.stack([np.zeros(shape), np.ones(shape), 0.2 * np.sin((x - cx) / 10.0)], -1)
+    w_vert = (np.abs(x - cx) < shape[0] / 8).astype(np.float64) * 0.8
+    w_ap = (np.abs(zz - cz) < shape[2] / 8).astype(np.float64) * 0.6
+    c = coefficients(tangent)
+    c += w_vert.reshape(-1, 1) * coefficients(vertical)
+    c += w_ap.reshape(-1, 1) * coefficients(ap)
+    c[:, 0] += 0.05 * np.sqrt(4 * np.pi)
+    affine = np.diag([voxel, voxel, voxel, 1.0])
+    affine[:3, 3] = -voxel * np.array([cx, cy, cz])
+    fod = np.ascontiguousarray(c.reshape(*shape, -1), dtype=np.float32)
+    return fod, affine, ellipsoid_mask(shape, share)
