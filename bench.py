@@ -305,6 +305,26 @@ def serve_report(n_jobs: int, rates: list[float], slots: int, form: str = "fp64"
             candidates.append({**run, "policy": policy, "largest_batch": bench.server.max_merged})
     batched = max(candidates, key=lambda c: c["voxel_solves_per_s"])
     tuning = [{**c["policy"], "voxel_solves_per_s": c["voxel_solves_per_s"]} for c in candidates]
+    # Kernel autotuning per bucket (row class, low-resolution directions): the low-resolution
+    # factor capacity timed on a sample of the bucket's first request, then the best batched
+    # policy again with every fit at its bucket's choice.
+    autotuned = None
+    if form == "fp64":
+        t = time.perf_counter()
+        for job, gtab in zip(bench.jobs, bench.gtabs):
+            bench.server.autotune(gtab, job.signals, lmax=job.lmax)
+        tune_s = time.perf_counter() - t
+        policy = bench.configure(**batched["policy"])
+        autotuned = {**bench.run(f"batched, autotuned {policy}", [(0.0, i) for i in order]),
+                     "policy": policy, "largest_batch": bench.server.max_merged,
+                     "tuning_seconds": tune_s, "table": bench.server.tuning.as_rows()}
+        if autotuned["voxel_solves_per_s"] < batched["voxel_solves_per_s"]:
+            # Not faster on this run: the served phases keep the default shapes.
+            from larmor_amd.sd import tuning as lr_tuning
+
+            bench.server.tuning = None
+            for rows in (256, 384, 512):
+                lr_tuning.set_capacity(rows, 0)
     # served: Poisson arrivals; requests cut into slices of 64k rows, fits of up to 512k rows.
     policy = bench.configure(batch_jobs=64, slice_voxels=1 << 16, merge_voxels=1 << 19)
     served = [{**bench.poisson(rate), "policy": policy, "largest_batch": bench.server.max_merged}
@@ -321,6 +341,7 @@ def serve_report(n_jobs: int, rates: list[float], slots: int, form: str = "fp64"
         "concurrent": concurrent,
         "batched": batched,
         "batched_tuning": tuning,
+        "batched_autotuned": autotuned,
         "served": served,
         "batched_vs_plain": batched["voxel_solves_per_s"] / plain["voxel_solves_per_s"],
         "batched_vs_concurrent": batched["voxel_solves_per_s"] / concurrent["voxel_solves_per_s"],

@@ -417,6 +417,82 @@ def fit_multi_tissue_batched(jobs, max_outer: int, fraction_tol: float, final_to
     return out
 
 
+
+class BatchedFitGraph:
+    """A batched multi-tissue fit (lsd_fit_multi_tissue_batched) captured once as a HIP graph and
+    replayed on new signals of the same shapes.
+
+    The fit's launch sequence has no host synchronisation (every round's voxel list lives on the
+    device), so it captures whole, its stream-ordered scratch becoming the graph's own memory.
+    The graph holds device copies of each job's signals and weights and the outputs; `replay`
+    copies the new signals in (or takes them already there), replays, and returns the output
+    tensors (overwritten by the next replay).  Each replay equals the eager batched fit of the
+    same signals bit for bit."""
+
+    def __init__(self, jobs, max_outer: int, fraction_tol: float, final_tol_x: float):
+        torch = _torch()
+        if not 1 <= len(jobs) <= MAX_BATCH_JOBS:
+            raise ValueError(f"a batch holds 1 to {MAX_BATCH_JOBS} jobs, got {len(jobs)}")
+        self.signals = []
+        self._keep = []
+        self._table = (_BatchJob * len(jobs))()
+        total = 0
+        for slot, (signals, weights, lr, lr_solver, hr, hr_solver) in zip(self._table, jobs):
+            _check_f64(signals, "signals")
+            dev = torch.as_tensor(signals, dtype=torch.float64, device="cuda").contiguous().clone()
+            w = torch.as_tensor(weights, dtype=torch.float64, device="cuda").contiguous().clone()
+            if dev.ndim != 2:
+                raise ValueError("each job's signals must be 2-D (voxels, measurements)")
+            self.signals.append(dev)
+            self._keep.extend([w, lr, lr_solver, hr, hr_solver])
+            slot.signals, slot.n_voxels, slot.m = dev.data_ptr(), dev.shape[0], dev.shape[1]
+            slot.weights = w.data_ptr()
+            slot.lr, slot.lr_solver = lr._handle, lr_solver._handle
+            slot.hr, slot.hr_solver = hr._handle, hr_solver._handle
+            total += dev.shape[0]
+        hr0 = jobs[0][4]
+        nf = hr0.n_blocks + hr0.n_isotropic
+        self.fods = torch.empty((total, hr0.n_anisotropic), dtype=torch.float64, device="cuda")
+        self.fractions = torch.empty((total, nf), dtype=torch.float64, device="cuda")
+        self.rounds = torch.empty(total, dtype=torch.int32, device="cuda")
+        self.status = torch.empty(total, dtype=torch.int32, device="cuda")
+        self._args = (int(max_outer), float(fraction_tol), float(final_tol_x))
+        lib = _lib.load()
+
+        def run():
+            return lib.lsd_fit_multi_tissue_batched(
+                ctypes.cast(self._table, ctypes.c_void_p), len(jobs), *self._args,
+                self.fods.data_ptr(), self.fractions.data_ptr(), self.rounds.data_ptr(),
+                self.status.data_ptr(), _stream_ptr(torch))
+
+        # One eager run on a side stream first (argument checks, first-use set-up such as the
+        # device constants), then the capture.
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            _lib.check(run())
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            rc = run()
+        _lib.check(rc)
+
+    def replay(self, signals=None):
+        """Fits `signals` (one array per job, the captured shapes; None: the signals already in
+        self.signals) and returns (fods, fractions, rounds, status) as device tensors."""
+        if signals is not None:
+            if len(signals) != len(self.signals):
+                raise ValueError(f"expected {len(self.signals)} jobs' signals, got {len(signals)}")
+            for dst, src in zip(self.signals, signals):
+                if tuple(src.shape) != tuple(dst.shape):
+                    raise ValueError(f"signals of shape {tuple(src.shape)} do not match the "
+                                     f"captured {tuple(dst.shape)}")
+                dst.copy_(_torch().as_tensor(src, dtype=dst.dtype).to(dst.device),
+                          non_blocking=True)
+        self.graph.replay()
+        return self.fods, self.fractions, self.rounds, self.status
+
 def log_signal_design(x):
     """(xt, pinv, scales) of a log-signal design x (n, p): its columns scaled to unit norm (a
     zero column kept, its norm taken as 1), transposed (p, n); their pseudo-inverse (p, n); the

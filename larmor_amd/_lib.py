@@ -183,6 +183,8 @@ class TrackStats(ctypes.Structure):
     ]
 
 
+SIGNATURES["lsd_tune_lr_capacity"] = (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64])
+SIGNATURES["lsd_tuned_lr_capacity"] = (ctypes.c_int64, [ctypes.c_int64])
 SIGNATURES["lsd_track_calibrate"] = (
     ctypes.c_int,
     [ctypes.c_int64, ctypes.c_int, ctypes.c_double, _d, ctypes.c_int64, _i64, _d],

@@ -51,14 +51,11 @@ int upload_jobs(const std::vector<JobStage>& host, JobStage* dst, hipStream_t st
 template <int NCPL, int RPL>
 int fod_pair_batched(const char* name, const char* rerun_name, const FodArgs& fast, int waves,
                      int* olist, int* ocount, hipStream_t stream, RerunScratch* rs) {
-    constexpr int KC = kcap_lds(NCPL);
     constexpr int KG = NCPL * WAVE;
-    const size_t lds = sizeof(real) * lds_factor_doubles(KC, RPL) * WAVES_PER_BLOCK;
     FodArgs a = fast;
     a.vs.overflow_list = olist;
     a.vs.overflow_count = ocount;
-    int rc = launch_fill(name, fod_batched_family<NCPL, RPL, KC, false>::kernel, a, waves, lds,
-                         stream);
+    int rc = launch_fast<fod_batched_family, NCPL, RPL>(name, a, waves, stream);
     if (rc) return rc;
     int rerun_waves = 0;
     rc = rerun_waves_in<fod_batched_family, NCPL, RPL, FodArgs>(rs, RERUN_WAVES, stream,

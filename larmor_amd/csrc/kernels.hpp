@@ -253,12 +253,14 @@ struct FodArgs {
 
 // The fast low-resolution instantiation is asked for 8 waves per SIMD, what its LDS factor
 // (4.2 KB per wave, 20 columns) allows: the register budget that leaves costs a few spilled
-// registers and still beats 7 waves with a 24-column factor.
+// registers and still beats 7 waves with a 24-column factor.  The capacities a fit may be tuned
+// to (LR_CAPACITIES) are built each for the waves its factor allows.
 #ifndef LSD_FOD_LR_WAVES
 #define LSD_FOD_LR_WAVES 8
 #endif
-constexpr int fod_waves_per_eu(int ncpl, bool global_l, int /*rpl*/) {
-    return (!global_l && ncpl == 1) ? LSD_FOD_LR_WAVES : 1;
+constexpr int lr_waves_for(int kcap) { return kcap <= 20 ? LSD_FOD_LR_WAVES : (kcap <= 24 ? 7 : 6); }
+constexpr int fod_waves_per_eu(int ncpl, int kcap, bool global_l) {
+    return (!global_l && ncpl == 1) ? lr_waves_for(kcap) : 1;
 }
 
 // WARM (the mixed-precision form's low-resolution solves, NCPL 1): each voxel's solve starts from
@@ -267,7 +269,7 @@ constexpr int fod_waves_per_eu(int ncpl, bool global_l, int /*rpl*/) {
 // design; precomputed correlations are indexed by batch voxel number.
 template <int NCPL, int RPL, int KCAP, bool GLOBAL_L, bool WARM = false, bool BATCHED = false>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
-    __attribute__((amdgpu_waves_per_eu(fod_waves_per_eu(NCPL, GLOBAL_L, RPL))))
+    __attribute__((amdgpu_waves_per_eu(fod_waves_per_eu(NCPL, KCAP, GLOBAL_L))))
     fod_kernel(FodArgs A) {
     constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
     extern __shared__ real smem[];
@@ -728,19 +730,52 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     }
 }
 
-// Kernel families, for the launch helpers.
+// Kernel families, for the launch helpers.  TUNABLE: the fast low-resolution launch (one column
+// slot) takes the factor capacity tuned for its row class (lr_capacity) instead of kcap_lds.
 template <int NCPL, int RPL, int KCAP, bool G>
 struct fod_family {
     static constexpr auto kernel = fod_kernel<NCPL, RPL, KCAP, G>;
+    static constexpr bool TUNABLE = true;
 };
 template <int NCPL, int RPL, int KCAP, bool G>
 struct fod_warm_family {
     static constexpr auto kernel = fod_kernel<NCPL, RPL, KCAP, G, true>;
+    static constexpr bool TUNABLE = false;
 };
 template <int NCPL, int RPL, int KCAP, bool G>
 struct fod_batched_family {
     static constexpr auto kernel = fod_kernel<NCPL, RPL, KCAP, G, false, true>;
+    static constexpr bool TUNABLE = true;
 };
+
+// The fast launch (LDS factor of KC columns) of a family.
+template <template <int, int, int, bool> class Family, int NCPL, int RPL, int KC, class Args>
+int launch_fast_at(const char* name, const Args& fast, int waves, hipStream_t stream) {
+    const size_t lds = sizeof(real) * lds_factor_doubles(KC, RPL) * WAVES_PER_BLOCK;
+    return launch_fill(name, Family<NCPL, RPL, KC, false>::kernel, fast, waves, lds, stream);
+}
+
+// The fast launch at the family's capacity: for a tunable one-slot solve the capacity tuned for
+// the row class (16, 20, 24 or 32 columns; a voxel whose active set outgrows it finishes in the
+// rerun with the same arithmetic, so every capacity gives the same bits), else kcap_lds.
+template <template <int, int, int, bool> class Family, int NCPL, int RPL, class Args>
+int launch_fast(const char* name, const Args& fast, int waves, hipStream_t stream) {
+#if !defined(LSD_FP32)
+    if constexpr (NCPL == 1 && Family<1, RPL, 20, false>::TUNABLE) {
+        switch (lsd::lr_capacity(RPL)) {
+        case 16:
+            return launch_fast_at<Family, NCPL, RPL, 16>(name, fast, waves, stream);
+        case 24:
+            return launch_fast_at<Family, NCPL, RPL, 24>(name, fast, waves, stream);
+        case 32:
+            return launch_fast_at<Family, NCPL, RPL, 32>(name, fast, waves, stream);
+        default:
+            break;
+        }
+    }
+#endif
+    return launch_fast_at<Family, NCPL, RPL, kcap_lds(NCPL)>(name, fast, waves, stream);
+}
 
 // ---------------------------------------------------------------------------------------------
 // Host-side launch helpers (shared by the instantiation units)
@@ -817,9 +852,7 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
         return launch(rerun_name, Family<NCPL, RPL, KG, true>::kernel, rerun, all_waves, 0,
                       stream);
     }
-    constexpr int KC = kcap_lds(NCPL);
-    const size_t lds = sizeof(real) * lds_factor_doubles(KC, RPL) * WAVES_PER_BLOCK;
-    int rc = launch_fill(name, Family<NCPL, RPL, KC, false>::kernel, fast, waves, lds, stream);
+    int rc = launch_fast<Family, NCPL, RPL>(name, fast, waves, stream);
     if (rc) return rc;
     int rerun_waves = 0;
     rc = rerun_waves_in<Family, NCPL, RPL, Args>(rs, RERUN_WAVES, stream, &rerun_waves);

@@ -20,6 +20,10 @@ constexpr int WAVES_PER_BLOCK = 4;
 
 extern thread_local std::string g_error;
 
+// The low-resolution solve's LDS factor capacity for the row class rpl (4, 6 or 8 rows per lane),
+// 20 columns unless lsd_tune_lr_capacity set another (sd_api.hip).
+int lr_capacity(int rpl);
+
 inline int fail(const std::string& message) {
     g_error = message;
     return LSD_ERR_INVALID;

@@ -12,6 +12,7 @@
 #include "lsi_wave.hpp"
 #include "peaks_wave.hpp"
 
+#include <atomic>
 #include <cmath>
 #include <map>
 #include <mutex>
@@ -20,6 +21,17 @@ namespace lsd {
 
 thread_local std::string g_error;
 bool g_profiling = false;
+
+// The tuned low-resolution factor capacity per row class (RPL 4, 6, 8); 0 means the default.
+static std::atomic<int> g_lr_capacity[3] = {0, 0, 0};
+
+static int rpl_slot(int rpl) { return rpl == 4 ? 0 : (rpl == 6 ? 1 : (rpl == 8 ? 2 : -1)); }
+
+int lr_capacity(int rpl) {
+    const int slot = rpl_slot(rpl);
+    const int v = slot < 0 ? 0 : g_lr_capacity[slot].load(std::memory_order_relaxed);
+    return v == 0 ? 20 : v;
+}
 std::vector<ProfiledLaunch> g_launches;
 unsigned long long* g_work_slots = nullptr;
 
@@ -843,6 +855,24 @@ int lsd_fit_multi_tissue_noalloc(const double* signals, int64_t n_voxels, int64_
 }
 
 int64_t lsd_noalloc_workspace_bytes(void) { return static_cast<int64_t>(NOALLOC_WORKSPACE_BYTES); }
+
+int lsd_tune_lr_capacity(int64_t m, int64_t capacity) {
+    const int rpl = rpl_for(m);
+    if (rpl < 0) {
+        return fail("problem size exceeds 512 measurements");
+    }
+    if (capacity != 0 && capacity != 16 && capacity != 20 && capacity != 24 && capacity != 32) {
+        return fail("the low-resolution factor capacity is 16, 20, 24 or 32 columns (0: the "
+                    "default), got " + std::to_string(capacity));
+    }
+    g_lr_capacity[rpl_slot(rpl)].store(static_cast<int>(capacity), std::memory_order_relaxed);
+    return 0;
+}
+
+int64_t lsd_tuned_lr_capacity(int64_t m) {
+    const int rpl = rpl_for(m);
+    return rpl < 0 ? -1 : lr_capacity(rpl);
+}
 
 int64_t lsd_fit_scratch_bytes(int64_t m, int64_t n_lr, int64_t n_hr) {
     const int rpl = rpl_for(m);

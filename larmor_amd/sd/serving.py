@@ -150,6 +150,7 @@ class _Design:
             self.hr_solver = solver._prepare(hr, bvals, Stage.HIGH_RES)
             self.lr_kernel = _deconv_kernel(lr)
             self.hr_kernel = _deconv_kernel(hr)
+            self.n_lr = lr.k_aniso.shape[1]
             self.weights = torch.as_tensor(np.ascontiguousarray(lr.weights, dtype=np.float64),
                                            device="cuda")
             self.fraction_tol = model.method._fraction_tolerance
@@ -261,7 +262,8 @@ class FodServer:
 
     def __init__(self, model: SDModel, *, slots: int = 4, hbm_budget: float = 200e9,
                  max_designs: int = 256, merge_voxels: int = 1 << 18, batch_jobs: int = 1,
-                 slice_voxels: int = 0, poll_interval: float = 1e-4, form: str = "fp64") -> None:
+                 slice_voxels: int = 0, poll_interval: float = 1e-4, form: str = "fp64",
+                 tuning=None) -> None:
         if form not in ("fp64", "mixed"):
             raise ValueError(f"form must be 'fp64' or 'mixed', got {form!r}")
         self.form = form
@@ -298,6 +300,9 @@ class FodServer:
         self.merge_voxels = int(merge_voxels)
         self.batch_jobs = int(batch_jobs)
         self.slice_voxels = int(slice_voxels)
+        # A larmor_amd.sd.tuning.TuningTable: each multi-tissue fit runs with its bucket's tuned
+        # kernel shapes (None: the defaults).
+        self.tuning = tuning
         self.peak_bytes = 0
         self._error: BaseException | None = None
         self._thread = threading.Thread(target=self._loop, name="fod-server", daemon=True)
@@ -362,13 +367,30 @@ class FodServer:
         self._incoming.put(parts)
         return future
 
-    def prepare(self, gtab: GradientTable, *, lmax: int | None = None) -> None:
+    def prepare(self, gtab: GradientTable, *, lmax: int | None = None):
         """Builds and keeps resident the design of an acquisition ahead of its requests (on the
-        caller's thread), so that the first request on it does not wait for the build."""
+        caller's thread), so that the first request on it does not wait for the build; returns
+        it."""
         lmax = self.model.lmax if lmax is None else int(lmax)
         bvals = np.asarray(gtab.bvals, dtype=np.float64)
         bvecs = np.asarray(gtab.bvecs, dtype=np.float64)
-        self._design((bvals.tobytes(), bvecs.tobytes(), lmax), bvals, bvecs, lmax)
+        return self._design((bvals.tobytes(), bvecs.tobytes(), lmax), bvals, bvecs, lmax)
+
+    def autotune(self, gtab: GradientTable, signals, *, lmax: int | None = None,
+                 sample: int = 16384):
+        """Tunes the kernel shapes of the bucket of an acquisition (its row class and
+        low-resolution directions) on a sample of `signals`, once per bucket, into the server's
+        tuning table (made on first use); call it before submitting, on the caller's thread."""
+        from larmor_amd.sd import tuning
+
+        if self.tuning is None:
+            self.tuning = tuning.TuningTable()
+        d = self.prepare(gtab, lmax=lmax)
+        if not d.multi or self.tuning.bucket(d.n_meas, d.n_lr) in self.tuning.choice:
+            return None
+        return tuning.autotune(signals, d.weights, d.lr_kernel, d.lr_solver, d.hr_kernel,
+                               d.hr_solver, fraction_tol=d.fraction_tol, sample=sample,
+                               table=self.tuning)
 
     def close(self, wait: bool = True) -> None:
         """Stops taking requests; with wait, returns once every queued request is served."""
@@ -609,6 +631,9 @@ class FodServer:
         return members
 
     def _launch(self, torch, r: _Running) -> None:
+        d_head = r.members[0].design
+        if self.tuning is not None and d_head.multi:
+            self.tuning.use(d_head.n_meas, d_head.n_lr)
         with torch.cuda.stream(r.stream):
             if r.batched:
                 jobs = []

@@ -133,3 +133,68 @@ def test_batch_checks():
         _core.fit_multi_tissue_batched([tuple(swapped)], 50, FRACTION_TOL, FINAL_FRACTION_TOL_X)
     with pytest.raises(ValueError, match="max_outer"):
         _core.fit_multi_tissue_batched(entries[:1], -1, FRACTION_TOL, FINAL_FRACTION_TOL_X)
+
+
+def test_captured_graph_replays_equal_the_eager_fit():
+    """The batched fit captured as a HIP graph: replayed three times on new signals, with eager
+    work and synchronisations between the replays, every replay equals the eager batched fit of
+    the same signals bit for bit."""
+    import torch
+
+    jobs = [j for j in _jobs(31, 15, 300, 3000) if 45 < len(j.bvals) <= 256][:4]
+    designs = [_design(j.bvals, j.bvecs) for j in jobs]
+    entries = [(torch.from_numpy(j.signals.astype(np.float64)).to("cuda"),
+                torch.as_tensor(d[0], device="cuda"), *d[1:]) for j, d in zip(jobs, designs)]
+    graph = _core.BatchedFitGraph(entries, MAX_ROUNDS, FRACTION_TOL, FINAL_FRACTION_TOL_X)
+    rng = np.random.default_rng(0)
+    for replay in range(3):
+        # New signals each time: the jobs' rows shuffled and scaled.
+        signals = [e[0][torch.from_numpy(rng.permutation(e[0].shape[0])).to("cuda")]
+                   * (1.0 + 0.01 * replay) for e in entries]
+        eager = _core.fit_multi_tissue_batched(
+            [(s, *e[1:]) for s, e in zip(signals, entries)], MAX_ROUNDS, FRACTION_TOL,
+            FINAL_FRACTION_TOL_X, return_diagnostics=True)
+        eager = [t.cpu().numpy() for t in eager]
+        _ = torch.randn(512, 512, device="cuda") @ torch.randn(512, 512, device="cuda")
+        torch.cuda.synchronize()
+        out = [t.cpu().numpy() for t in graph.replay(signals)]
+        for a, b in zip(out, eager):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_every_tuned_capacity_gives_the_same_bits_and_the_tuner_keeps_the_fastest():
+    """The low-resolution factor capacity (16, 20, 24 or 32 columns) changes which voxels finish
+    in the global-memory solve, never a bit of the results; the tuner times each on the bucket's
+    sample, keeps the fastest and puts it in force."""
+    import torch
+
+    from larmor_amd.sd import tuning
+
+    jobs = [j for j in _jobs(41, 12, 3000, 20000) if 45 < len(j.bvals) <= 256][:2]
+    designs = [_design(j.bvals, j.bvecs) for j in jobs]
+    m = len(jobs[0].bvals)
+    try:
+        ref = None
+        for cap in tuning.CAPACITIES:
+            tuning.set_capacity(m, cap)
+            assert tuning.capacity_in_force(m) == cap
+            single = _fit_alone(jobs[0].signals.astype(np.float64), designs[0])
+            batch = _batched(jobs, designs)
+            got = [np.asarray(x) for x in single] + batch
+            if ref is None:
+                ref = got
+            for a, b in zip(got, ref):
+                np.testing.assert_array_equal(a, b)
+        table = tuning.TuningTable()
+        w, lr, lrs, hr, hrs = designs[0]
+        chosen = tuning.autotune(torch.from_numpy(jobs[0].signals.astype(np.float64)).to("cuda"),
+                                 w, lr, lrs, hr, hrs, fraction_tol=FRACTION_TOL,
+                                 final_tol_x=FINAL_FRACTION_TOL_X, sample=4096, repeats=1,
+                                 table=table)
+        times = table.timings[table.bucket(m, lr.n_anisotropic)]
+        print(f"tuned capacity {chosen} for m={m}: ms {times}")
+        assert chosen == min(times, key=times.get) and tuning.capacity_in_force(m) == chosen
+    finally:
+        tuning.set_capacity(m, 0)
+    with pytest.raises(ValueError, match="16, 20, 24 or 32 columns"):
+        tuning.set_capacity(m, 40)
