@@ -377,7 +377,7 @@ class FodServer:
         return self._design((bvals.tobytes(), bvecs.tobytes(), lmax), bvals, bvecs, lmax)
 
     def autotune(self, gtab: GradientTable, signals, *, lmax: int | None = None,
-                 sample: int = 16384):
+                 sample: int = 65536):
         """Tunes the kernel shapes of the bucket of an acquisition (its row class and
         low-resolution directions) on a sample of `signals`, once per bucket, into the server's
         tuning table (made on first use); call it before submitting, on the caller's thread."""

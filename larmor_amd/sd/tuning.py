@@ -68,7 +68,7 @@ def capacity_in_force(m: int) -> int:
 
 
 def autotune(signals, weights, lr, lr_solver, hr, hr_solver, *, max_rounds: int = 50,
-             fraction_tol: float = 1e-7, final_tol_x: float = 1e-2, sample: int = 16384,
+             fraction_tol: float = 1e-7, final_tol_x: float = 1e-2, sample: int = 65536,
              repeats: int = 2, capacities=CAPACITIES, table: TuningTable = TABLE,
              seed: int = 0) -> int:
     """Times the multi-tissue fit of a random sample of `signals` (device or host rows of one
