@@ -322,11 +322,14 @@ int lsd_fit_log_signal(const double* xt, const double* pinv, const double* scale
 /*
  * A constrained spherical deconvolution problem (host arrays): the linear start initial (c0 x m),
  * the forward model F (m x c), the weighted SH basis on the constraint directions C (k x c),
- * with the Gram matrix F^T F + norm_weight I made here.  c must be 6, 15, 28 or 45 (lmax 2-8);
- * k <= 512.
+ * with the Gram matrix F^T F + norm_weight I made here.  1 <= c <= 496 (lmax up to 30: lmax 2-8
+ * run with the system in registers, larger c with it in LDS, or in global memory past ~190
+ * coefficients); k <= 512.
  * Replaces: fit_csd's problem set-up  cpp/src/python/sd.cpp:171-192,
  *           make_csd_problem cpp/src/sd/csd.cpp:11-21.
- * Errors: "shapes do not match: ...", "max_iter must be at least 0".
+ * Errors: "shapes do not match: ...", "max_iter must be at least 0", "CSD on the GPU takes 1 to
+ * 496 coefficients (lmax up to 30), got c", "CSD on the GPU takes at most 512 constraint
+ * directions, got k".
  */
 typedef struct lsd_csd lsd_csd;
 int lsd_csd_create(const double* initial, int64_t c0, int64_t m, const double* forward,

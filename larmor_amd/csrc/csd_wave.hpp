@@ -20,6 +20,9 @@
 // measurements, coefficients or constrained directions runs in increasing index order; the
 // forward solve subtracts earlier positions first, the backward solve later positions first, a
 // zero right-hand entry stays zero.
+//
+// csd_kernel holds a row per lane and so stops at 64 coefficients with a compile-time N;
+// csd_wide_kernel below takes any N at run time with the system in memory.
 #pragma once
 
 #include "wave.hpp"
@@ -42,7 +45,14 @@ struct CsdArgs {
     double* fods;               // (n_vox, N)
     unsigned char* converged;   // (n_vox)
     int* solves;                // (n_vox), or null
+    int n;                      // coefficients (csd_wide_kernel)
+    double* sys_slabs;          // csd_wide_kernel with the system in global memory: one packed
+                                // triangle per workgroup
 };
+
+// The most coefficients a CSD fit takes: lmax 30 (496 coefficients), whose system's triangle
+// (~1 MB) goes to global memory.
+constexpr int CSD_MAX_COEFFICIENTS = 496;
 
 // Doubles of the per-wave LDS copy of the factor: the packed lower triangle.
 constexpr int csd_lds_doubles(int n) { return n * (n + 1) / 2; }
@@ -197,6 +207,197 @@ __global__ void __launch_bounds__(WAVE * 4) csd_kernel(CsdArgs A) {
                 A.solves[v] = solves;
             }
         }
+    }
+}
+
+// The wide form: any number of coefficients N, one voxel per single-wave workgroup.  The packed
+// lower triangle of the system (column-major, csd_lds_doubles(N) doubles) lives in the
+// workgroup's LDS, or, when it does not fit there (GLOBAL_SYS), in the workgroup's own slab of
+// global memory; the FOD, F^T s, the solve's right-hand side and the constrained directions'
+// list live in LDS.  Each step distributes rows over the lanes and gives every entry the same
+// terms in the same order as csd_kernel and the oracle:
+//  - system: entry (i, j), i >= j, is G_ij plus c_d[i] c_d[j] over the constrained d in
+//    increasing d;
+//  - Cholesky, right-looking: at step j the column below the diagonal is divided by
+//    sqrt(A_jj), then every trailing entry (i, q) loses L_ij L_qj, so entry (i, q) receives its
+//    terms in increasing j, as the left-looking oracle subtracts them;
+//  - forward and backward solves column-oriented (earlier, then later positions first).
+// Between steps the wave synchronises its memory: LDS at wavefront scope, the global slab at
+// workgroup scope (the workgroup is the one wave).
+template <bool GLOBAL_SYS>
+__device__ __forceinline__ void csd_wide_sync() {
+    if constexpr (GLOBAL_SYS) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    } else {
+        wave_lds_sync();
+    }
+}
+
+// LDS bytes of a csd_wide_kernel workgroup: the triangle (unless in global memory), three
+// vectors of N doubles and k direction indices.
+inline size_t csd_wide_lds_bytes(int n, int k, bool global_sys) {
+    const size_t tri = global_sys ? 0 : sizeof(double) * static_cast<size_t>(csd_lds_doubles(n));
+    return tri + sizeof(double) * 3 * static_cast<size_t>(n) + sizeof(int) * static_cast<size_t>(k > 0 ? k : 1);
+}
+
+template <int NK, bool GLOBAL_SYS>
+__global__ void __launch_bounds__(WAVE) csd_wide_kernel(CsdArgs A) {
+    extern __shared__ double smem[];
+    const int n = A.n;
+    const long tri = static_cast<long>(n) * (n + 1) / 2;
+    double* sys = GLOBAL_SYS ? A.sys_slabs + static_cast<long>(blockIdx.x) * tri : smem;
+    double* f = GLOBAL_SYS ? smem : smem + tri;
+    double* proj = f + n;
+    double* b = proj + n;
+    int* list = reinterpret_cast<int*>(b + n);
+    const int l = lane_id();
+    auto column = [n](int i) { return static_cast<long>(i) * n - static_cast<long>(i) * (i - 1) / 2; };
+
+    for (long v = blockIdx.x; v < A.n_vox; v += gridDim.x) {
+        const double* s = A.signals + v * A.m;
+        for (int i = l; i < n; i += WAVE) {
+            double p = 0.0;
+            double x = 0.0;
+            for (int r = 0; r < A.m; ++r) {
+                const double sr = s[r];
+                p = p + A.forward[static_cast<long>(r) * n + i] * sr;
+                if (i < A.c0) {
+                    x = x + A.initial_t[static_cast<long>(r) * A.c0 + i] * sr;
+                }
+            }
+            proj[i] = p;
+            f[i] = x;
+        }
+        wave_lds_sync();
+        bool first = true;
+        bool converged = A.max_iter == 0;
+        int solves = 0;
+        unsigned long long prev[NK];
+#pragma unroll
+        for (int t = 0; t < NK; ++t) {
+            prev[t] = 0ull;
+        }
+        for (int it = 0; it < A.max_iter; ++it) {
+            double a[NK];
+#pragma unroll
+            for (int t = 0; t < NK; ++t) {
+                a[t] = 0.0;
+            }
+            for (int j = 0; j < n; ++j) {
+                const double fj = f[j];
+#pragma unroll
+                for (int t = 0; t < NK; ++t) {
+                    const int d = l + WAVE * t;
+                    if (d < A.k) {
+                        a[t] = a[t] + A.constraint_t[static_cast<long>(j) * A.k + d] * fj;
+                    }
+                }
+            }
+            bool same = true;
+            int n_c = 0;
+            unsigned long long mask[NK];
+#pragma unroll
+            for (int t = 0; t < NK; ++t) {
+                const int d = l + WAVE * t;
+                const bool on = d < A.k && a[t] < A.threshold;
+                mask[t] = ballot(on);
+                same = same && mask[t] == prev[t];
+                if (on) {
+                    list[n_c + __popcll(mask[t] & ((1ull << l) - 1ull))] = d;
+                }
+                n_c += __popcll(mask[t]);
+            }
+            if (!first && same) {
+                converged = true;
+                break;
+            }
+            first = false;
+            wave_lds_sync();
+
+            for (int j = 0; j < n; ++j) {
+                const long cj = column(j);
+                for (int i = j + l; i < n; i += WAVE) {
+                    double acc = A.gram[static_cast<long>(i) * n + j];
+                    for (int q = 0; q < n_c; ++q) {
+                        const double* cd = A.constraint + static_cast<long>(list[q]) * n;
+                        acc = acc + cd[i] * cd[j];
+                    }
+                    sys[cj + (i - j)] = acc;
+                }
+            }
+            csd_wide_sync<GLOBAL_SYS>();
+
+            for (int j = 0; j < n; ++j) {
+                const long cj = column(j);
+                const double ljj = sqrt(sys[cj]);
+                for (int i = j + 1 + l; i < n; i += WAVE) {
+                    sys[cj + (i - j)] = sys[cj + (i - j)] / ljj;
+                }
+                if (l == 0) {
+                    sys[cj] = ljj;
+                }
+                csd_wide_sync<GLOBAL_SYS>();
+                for (int q = j + 1; q < n; ++q) {
+                    const double lqj = sys[cj + (q - j)];
+                    const long cq = column(q);
+                    for (int i = q + l; i < n; i += WAVE) {
+                        sys[cq + (i - q)] = sys[cq + (i - q)] - sys[cj + (i - j)] * lqj;
+                    }
+                }
+                csd_wide_sync<GLOBAL_SYS>();
+            }
+
+            // L y = F^T s, column-oriented.
+            for (int i = l; i < n; i += WAVE) {
+                b[i] = proj[i];
+            }
+            wave_lds_sync();
+            for (int j = 0; j < n; ++j) {
+                const long cj = column(j);
+                const double bj = b[j];
+                const double yj = bj != 0.0 ? bj / sys[cj] : bj;
+                for (int i = j + 1 + l; i < n; i += WAVE) {
+                    b[i] = b[i] - sys[cj + (i - j)] * yj;
+                }
+                if (l == 0) {
+                    b[j] = yj;
+                }
+                wave_lds_sync();
+            }
+            // L^T f = y, column-oriented from the last position.
+            for (int j = n - 1; j >= 0; --j) {
+                const double yj = b[j];
+                const double xj = yj != 0.0 ? yj / sys[column(j)] : yj;
+                for (int i = l; i < j; i += WAVE) {
+                    b[i] = b[i] - sys[column(i) + (j - i)] * xj;
+                }
+                if (l == 0) {
+                    b[j] = xj;
+                }
+                wave_lds_sync();
+            }
+            for (int i = l; i < n; i += WAVE) {
+                f[i] = b[i];
+            }
+            wave_lds_sync();
+            solves += 1;
+#pragma unroll
+            for (int t = 0; t < NK; ++t) {
+                prev[t] = mask[t];
+            }
+        }
+        for (int i = l; i < n; i += WAVE) {
+            A.fods[v * n + i] = f[i];
+        }
+        if (l == 0) {
+            A.converged[v] = converged ? 1 : 0;
+            if (A.solves) {
+                A.solves[v] = solves;
+            }
+        }
+        // The next voxel overwrites the vectors, the list and the triangle.
+        csd_wide_sync<GLOBAL_SYS>();
     }
 }
 

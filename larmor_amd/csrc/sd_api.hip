@@ -899,8 +899,8 @@ int lsd_csd_create(const double* initial, int64_t c0, int64_t m, const double* f
         return fail("max_iter must be at least 0");
     }
     if (!csd_supported_coefficients(c)) {
-        return fail("CSD on the GPU takes 6, 15, 28 or 45 coefficients (lmax 2 to 8), got " +
-                    std::to_string(c));
+        return fail("CSD on the GPU takes 1 to " + std::to_string(CSD_MAX_COEFFICIENTS) +
+                    " coefficients (lmax up to 30), got " + std::to_string(c));
     }
     if (csd_direction_slots(k) < 0) {
         return fail("CSD on the GPU takes at most 512 constraint directions, got " +

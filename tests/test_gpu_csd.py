@@ -124,3 +124,56 @@ def test_solve_csd_keeps_device_tensors_on_device():
     ref = oracle.fit_csd(signals, p.initial, p.forward, p.constraint, p.norm_weight,
                          p.threshold, p.max_iter)[0]
     np.testing.assert_array_equal(fods.cpu().numpy(), ref)
+
+
+def _high_b_workload(shape, n_dirs=150, bval=3000.0, seed=5):
+    ph = synthetic.phantom(shape=shape, n_b0=2, shells=[(bval, n_dirs)], seed=seed, snr=30.0,
+                           mask_share=None, tissues=False)
+    on_shell = ph.bvals > 50
+    signals = ph.data.reshape(-1, len(ph.bvals))[:, on_shell].astype(np.float64)
+    return np.ascontiguousarray(np.maximum(signals, 0.0)), ph.bvecs[on_shell]
+
+
+# lmax 10-18: the wide form with the system's triangle in LDS (66-190 coefficients); lmax 20:
+# its 231-coefficient triangle (208 KiB) exceeds a workgroup's LDS and lives in global memory.
+@pytest.mark.parametrize("lmax,shape", [(10, (8, 8, 4)), (12, (8, 8, 2)), (14, (6, 6, 2)),
+                                        (16, (6, 4, 2)), (18, (4, 4, 2)), (20, (4, 4, 2))])
+def test_csd_wide_lmax_bit_identical_to_oracle(lmax, shape):
+    signals, bvecs = _high_b_workload(shape)
+    zonal = synthetic.tensor_zonal(3000.0, lmax=lmax)
+    p = csd_problem(bvecs, zonal, lmax, CSDSolver())
+    print(f"csd wide lmax {lmax}: {signals.shape}, {p.forward.shape[1]} coefficients", flush=True)
+    ref, got = _both(signals, p)
+    assert ref[1].mean() > 0.9 and ref[2].max() >= 2
+    _assert_same(ref, got)
+
+
+def test_csd_wide_any_coefficient_count():
+    # A random well-posed problem whose coefficient count (50) has no register kernel, with
+    # constraint directions in every slot class.
+    rng = np.random.default_rng(11)
+    m, c, c0 = 70, 50, 20
+    forward = rng.normal(size=(m, c))
+    initial = rng.normal(size=(c0, m)) * 0.05
+    signals = np.ascontiguousarray(rng.normal(size=(300, m)))
+    for k in (100, 300, 500):
+        constraint = np.ascontiguousarray(rng.normal(size=(k, c)))
+        print(f"csd wide c {c} k {k}", flush=True)
+        ref = oracle.fit_csd(signals, initial, forward, constraint, 0.5, 0.0, 12,
+                             return_solves=True)
+        got = _core.fit_csd(signals, initial, forward, constraint, 0.5, 0.0, 12, 1,
+                            return_solves=True)
+        _assert_same(ref, got)
+
+
+def test_csd_wide_batch_invariance_and_limit():
+    signals, bvecs = _high_b_workload((4, 4, 2))
+    p = csd_problem(bvecs, synthetic.tensor_zonal(3000.0, lmax=12), 12, CSDSolver())
+    alone = _core.fit_csd(np.ascontiguousarray(signals[3:5]), p.initial, p.forward,
+                          p.constraint, p.norm_weight, p.threshold, p.max_iter, 1)[0]
+    mixed = _core.fit_csd(signals, p.initial, p.forward, p.constraint, p.norm_weight,
+                          p.threshold, p.max_iter, 1)[0]
+    np.testing.assert_array_equal(alone, mixed[3:5])
+    with pytest.raises(ValueError, match="CSD on the GPU takes 1 to 496 coefficients"):
+        _core.fit_csd(signals[:2], np.zeros((0, signals.shape[1])),
+                      np.zeros((signals.shape[1], 497)), np.zeros((10, 497)), 1.0, 0.0, 5, 1)
