@@ -403,7 +403,7 @@ int lsd_sh_climb(const double* coeffs, int64_t n, int64_t n_coeffs, int64_t lmax
  * A least-squares problem with inequality constraints, min ||a x - s||^2 + ridge^2 ||x||^2
  * subject to g x >= 0, prepared for many data vectors (host arrays a (m x n), g (k x g_cols)):
  * the column-pivoted QR of [a; ridge I] and the least-distance form E = g P R^-1 (unit rows),
- * E E^T, made on the host and kept on the device.  n <= 63, k <= 512.
+ * E E^T, made on the host and kept on the device.  n <= 255 (MSMT-CSD up to WM lmax 20), k <= 512.
  * Replaces: LsiDesign(a, g, ridge)  cpp/src/python/sd.cpp:361-367,
  *           make_lsi_design cpp/src/sd/lsi.cpp:19-69.
  * Errors: "the constraints have C columns but the design has N", "the ridge must be at least 0",

@@ -15,10 +15,13 @@
 //    gradient and the tolerance's column sums run over the rows in order, one column per lane;
 //  - x = P R^-1 (scale (E^T u) / (1 - f.u) + r), the sums over the non-zero u_c in order.
 //
-// Layout: constraint (NNLS column) c in lane c % 64, slot c / 64 (NKS slots); parameter i in
-// lane i (n <= 64).  The factor lives in a per-wave global slab, column-major with stride
-// KCAP + 1; H has rank at most n + 1 <= 65, so a factor of KCAP = 64 columns only overflows where
-// a pivot would be refused anyway, and both go to the rerun kernel.
+// Layout: constraint (NNLS column) c in lane c % 64, slot c / 64 (NKS slots); parameter i and
+// factor position p in lane i % 64 (p % 64), slot i / 64 (PS slots: n <= 63, 127 or 255 for PS =
+// 1, 2, 4).  The factor lives in a per-wave global slab, column-major with stride KCAP + 1; H has
+// rank at most n + 1 <= 64 PS, so a factor of KCAP = 64 PS columns only overflows where a pivot
+// would be refused anyway, and both go to the rerun kernel.  Every lane reads and writes only the
+// factor rows of its own positions, except the row shift of a removal, whose slots run one after
+// another with the wave's memory operations completed in between.
 #pragma once
 
 #include "qr_wave.hpp"
@@ -37,10 +40,17 @@ struct LsiDev {
     const double* abs_sums; // (k)
 };
 
-constexpr int LSI_KCAP = 64;
-constexpr int LSI_QR_ROWS = 1;    // row slots of [E^T; f^T]: n + 1 <= 64
 constexpr int LSI_GRADIENT_BATCH = 2; // rows of E E^T loaded together by the gradient
-constexpr int LSI_DIAG_DOUBLES = 2 * LSI_KCAP; // per-wave LDS table of the factor diagonal
+constexpr int LSI_MAX_PARAMS = 4 * WAVE - 1;
+
+// Parameter / position slots for n parameters: 1, 2 or 4 (0: too many).
+inline int lsi_position_slots(long n) { return n < WAVE ? 1 : (n < 2 * WAVE ? 2 : (n < 4 * WAVE ? 4 : 0)); }
+// The factor's columns, per-wave global slab doubles (stride KCAP + 1) and LDS diagonal table.
+constexpr int lsi_kcap(int ps) { return WAVE * ps; }
+constexpr long lsi_factor_slab(int ps) { return static_cast<long>(lsi_kcap(ps) + 1) * lsi_kcap(ps); }
+constexpr int lsi_diag_doubles(int ps) { return 2 * lsi_kcap(ps); }
+// The QR path's column slab: 64 PS rows (n + 1 <= 64 PS) by 64 PS columns.
+constexpr long lsi_qr_slab(int ps) { return static_cast<long>(WAVE * ps) * WAVE * ps; }
 
 struct LsiArgs {
     const double* signals;      // (n_vox, m)
@@ -69,17 +79,47 @@ __device__ __forceinline__ double uniform_at(const double (&a)[NKS], int c) {
     return readlane(v, c % WAVE);
 }
 
-// r = Q^T s (lane i), f's raw values -(E r) per constraint slot and their largest magnitude.
-// Returns scale; NaN when any raw value is NaN.
-template <int NKS>
-__device__ __forceinline__ double lsi_rotate(const LsiDev& d, const double* s, double& rot,
+// The value at uniform position p of a position-spread vector (p in lane p % 64, slot p / 64).
+template <int PS, class T>
+__device__ __forceinline__ T at_position(const T (&v)[PS], int p) {
+    T x = v[0];
+#pragma unroll
+    for (int s = 1; s < PS; ++s) {
+        x = (p / WAVE) == s ? v[s] : x;
+    }
+    return readlane(x, p % WAVE);
+}
+
+// The value at per-lane position q of a position-spread vector.
+template <int PS>
+__device__ __forceinline__ double gather_position(const double (&v)[PS], int q) {
+    double x = 0.0;
+#pragma unroll
+    for (int s = 0; s < PS; ++s) {
+        const double z = gather(v[s], q % WAVE);
+        x = (q / WAVE) == s ? z : x;
+    }
+    return x;
+}
+
+// r = Q^T s (parameter-spread), f's raw values -(E r) per constraint slot and their largest
+// magnitude.  Returns scale; NaN when any raw value is NaN.
+template <int NKS, int PS>
+__device__ __forceinline__ double lsi_rotate(const LsiDev& d, const double* s, double (&rot)[PS],
                                              double (&raw)[NKS]) {
     const int l = lane_id();
-    rot = 0.0;
+#pragma unroll
+    for (int q = 0; q < PS; ++q) {
+        rot[q] = 0.0;
+    }
     for (int r = 0; r < d.m; ++r) {
         const double sr = s[r];
-        if (l < d.n) {
-            rot = rot + d.q_t[static_cast<long>(l) * d.m + r] * sr;
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            const int i = l + WAVE * q;
+            if (i < d.n) {
+                rot[q] = rot[q] + d.q_t[static_cast<long>(i) * d.m + r] * sr;
+            }
         }
     }
     double mx = 0.0;
@@ -89,7 +129,7 @@ __device__ __forceinline__ double lsi_rotate(const LsiDev& d, const double* s, d
         raw[t] = 0.0;
     }
     for (int i = 0; i < d.n; ++i) {
-        const double ri = readlane(rot, i);
+        const double ri = at_position<PS>(rot, i);
 #pragma unroll
         for (int t = 0; t < NKS; ++t) {
             const int c = l + WAVE * t;
@@ -114,17 +154,24 @@ __device__ __forceinline__ double lsi_rotate(const LsiDev& d, const double* s, d
     return wave_max(mx);
 }
 
-// x = P R^-1 z for z in lane i (i < n); returns x in lane i.
-__device__ __forceinline__ double lsi_back(const LsiDev& d, double z) {
+// x = P R^-1 z, both parameter-spread.
+template <int PS>
+__device__ __forceinline__ void lsi_back(const LsiDev& d, const double (&z)[PS], double (&x)[PS]) {
     const int l = lane_id();
-    double x = 0.0;
+#pragma unroll
+    for (int q = 0; q < PS; ++q) {
+        x[q] = 0.0;
+    }
     for (int j = 0; j < d.n; ++j) {
-        const double zj = readlane(z, j);
-        if (l < d.n) {
-            x = x + d.back[static_cast<long>(l) * d.n + j] * zj;
+        const double zj = at_position<PS>(z, j);
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            const int i = l + WAVE * q;
+            if (i < d.n) {
+                x[q] = x[q] + d.back[static_cast<long>(i) * d.n + j] * zj;
+            }
         }
     }
-    return x;
 }
 
 // f.u over the non-zero u_c in column order (uniform).
@@ -144,12 +191,17 @@ __device__ __forceinline__ double lsi_dot(const double (&f)[NKS], const double (
 }
 
 // The solution from the multipliers: x = P R^-1 (scale (E^T u) / (1 - f.u) + r).
-template <int NKS>
-__device__ __forceinline__ double lsi_solution(const LsiDev& d, const double (&f)[NKS],
-                                               const double (&u)[NKS], double scale, double rot) {
+template <int NKS, int PS>
+__device__ __forceinline__ void lsi_solution(const LsiDev& d, const double (&f)[NKS],
+                                             const double (&u)[NKS], double scale,
+                                             const double (&rot)[PS], double (&x)[PS]) {
     const int l = lane_id();
     const double denom = 1.0 - lsi_dot<NKS>(f, u);
-    double sum = 0.0;
+    double sum[PS];
+#pragma unroll
+    for (int q = 0; q < PS; ++q) {
+        sum[q] = 0.0;
+    }
 #pragma unroll
     for (int t = 0; t < NKS; ++t) {
         unsigned long long bits = ballot(u[t] != 0.0);
@@ -158,23 +210,45 @@ __device__ __forceinline__ double lsi_solution(const LsiDev& d, const double (&f
             bits &= bits - 1;
             const int c = WAVE * t + b;
             const double uc = readlane(u[t], b);
-            if (l < d.n) {
-                sum = sum + d.e[static_cast<long>(c) * d.n + l] * uc;
+#pragma unroll
+            for (int q = 0; q < PS; ++q) {
+                const int i = l + WAVE * q;
+                if (i < d.n) {
+                    sum[q] = sum[q] + d.e[static_cast<long>(c) * d.n + i] * uc;
+                }
             }
         }
     }
-    const double z = (scale * sum) / denom + rot;
-    return lsi_back(d, z);
+    double z[PS];
+#pragma unroll
+    for (int q = 0; q < PS; ++q) {
+        z[q] = (scale * sum[q]) / denom + rot[q];
+    }
+    lsi_back<PS>(d, z, x);
+}
+
+// Writes the parameter-spread solution of voxel v.
+template <int PS>
+__device__ __forceinline__ void lsi_store(const LsiDev& d, double* x_out, long v, const double (&x)[PS]) {
+    const int l = lane_id();
+#pragma unroll
+    for (int q = 0; q < PS; ++q) {
+        const int i = l + WAVE * q;
+        if (i < d.n) {
+            x_out[v * d.n + i] = x[q];
+        }
+    }
 }
 
 // Lawson-Hanson on H = E E^T + f f^T with a growing Cholesky factor (ldp_multipliers).  As in
 // the ridge solver: the factor's diagonal and its division reciprocals sit in a per-wave LDS table
 // (the split division of wave.hpp where every diagonal and the dividend are in range), and
 // y = L^-1 f_A is kept across appends (the new row gives y_k by one sequential dot, the order of
-// the column-oriented solve) and recomputed from a removed position on.
-template <int NKS>
+// the column-oriented solve) and recomputed from a removed position on.  Factor positions are
+// spread over PS lane slots (position p in lane p % 64, slot p / 64).
+template <int NKS, int PS>
 struct LdpWave {
-    static constexpr int KCAP = LSI_KCAP;
+    static constexpr int KCAP = lsi_kcap(PS);
     static constexpr int S = KCAP + 1;
 
     const LsiDev& d;
@@ -187,18 +261,20 @@ struct LdpWave {
     double cand[NKS];
     bool act[NKS];
     int pos[NKS];
-    int fcol;    // column at factor position lane_id() (lane < kf)
-    int kf;      // factor size (uniform)
-    double y;    // y = L^-1 f_A at the lane's position, valid on positions [0, y_valid)
+    int fcol[PS]; // column at the lane's factor positions (position < kf)
+    int kf;       // factor size (uniform)
+    double y[PS]; // y = L^-1 f_A at the lane's positions, valid on positions [0, y_valid)
     int y_valid;
 
     __device__ LdpWave(const LsiDev& d_, const double (&f_)[NKS], double* L_, double* diag_)
         : d(d_), f(f_), L(L_), diag(diag_) {}
 
     __device__ __forceinline__ int col(int t) const { return lane_id() + WAVE * t; }
+    __device__ __forceinline__ int position(int q) const { return lane_id() + WAVE * q; }
     __device__ __forceinline__ double& Lat(int i, int j) { return L[j * S + i]; }
+    // L_ij at uniform (i, j): lane i % 64 reads its own row i.
     __device__ __forceinline__ double Luni(int i, int j) const {
-        return readlane(L[j * S + lane_id()], i);
+        return readlane(L[j * S + WAVE * (i / WAVE) + lane_id()], i % WAVE);
     }
     __device__ __forceinline__ double2 diag_at(int j) const {
         return *reinterpret_cast<const double2*>(diag + 2 * j);
@@ -228,32 +304,49 @@ struct LdpWave {
     // L w = b on positions [p0, kk) when positions [0, p0) of b already hold their final values
     // (p0 = 0: the whole solve), column-oriented; returns the sum of the w_j^2 in order (for
     // p0 = 0).  The factor lives in global memory: column j + 1 is loaded during step j, so the
-    // load latency is off the chain of dependent steps.
+    // load latency is off the chain of dependent steps.  Slots past kk load nothing.
     template <bool FAST>
-    __device__ __forceinline__ double forward_impl(double& b, int p0, int kk) {
+    __device__ __forceinline__ double forward_impl(double (&b)[PS], int p0, int kk) {
         const int l = lane_id();
         double sq = 0.0;
-        double col_next = L[l];
+        double col_next[PS];
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            col_next[q] = (q == 0 || WAVE * q < kk) ? L[WAVE * q + l] : 0.0;
+        }
         for (int j = 0; j < kk; ++j) {
-            const double colj = col_next;
+            double colj[PS];
+#pragma unroll
+            for (int q = 0; q < PS; ++q) {
+                colj[q] = col_next[q];
+            }
             const int jn = j + 1 < kk ? j + 1 : j;
-            col_next = L[jn * S + l];
+#pragma unroll
+            for (int q = 0; q < PS; ++q) {
+                if (q == 0 || WAVE * q < kk) {
+                    col_next[q] = L[jn * S + WAVE * q + l];
+                }
+            }
             const double2 dd = diag_at(j);
-            const double bj = readlane(b, j);
+            const double bj = at_position<PS>(b, j);
             const double wj = j >= p0 ? divide_by_diag<FAST>(bj, dd) : bj;
             sq = sq + wj * wj;
-            const double updated = b - colj * wj;
-            b = l == j ? wj : ((l > j && l < kk && l >= p0) ? updated : b);
+#pragma unroll
+            for (int q = 0; q < PS; ++q) {
+                const int p = position(q);
+                const double updated = b[q] - colj[q] * wj;
+                b[q] = p == j ? wj : ((p > j && p < kk && p >= p0) ? updated : b[q]);
+            }
         }
         return sq;
     }
 
-    __device__ __forceinline__ double forward(double& b, int kk_) {
+    __device__ __forceinline__ double forward(double (&b)[PS], int kk_) {
         const int kk = __builtin_amdgcn_readfirstlane(kk_);
         return diag_ok ? forward_impl<true>(b, 0, kk) : forward_impl<false>(b, 0, kk);
     }
 
-    __device__ __forceinline__ void forward_from(double& b, int p0_, int kk_) {
+    __device__ __forceinline__ void forward_from(double (&b)[PS], int p0_, int kk_) {
         const int p0 = __builtin_amdgcn_readfirstlane(p0_);
         const int kk = __builtin_amdgcn_readfirstlane(kk_);
         if (p0 >= kk) {
@@ -266,27 +359,42 @@ struct LdpWave {
         }
     }
 
-    // L^T z = w on positions [0, kk), column-oriented: step j reads row j (L_{j,l} for the
-    // lanes l < j, at stride S), loaded during step j + 1.
+    // L^T z = w on positions [0, kk), column-oriented: step j reads row j (L_{j,p} for the
+    // positions p < j, at stride S), loaded during step j + 1.
     template <bool FAST>
-    __device__ __forceinline__ void backward_impl(double& b, int kk) {
-        const int l = lane_id();
+    __device__ __forceinline__ void backward_impl(double (&b)[PS], int kk) {
         const int top = kk > 0 ? kk - 1 : 0;
-        const int lr = l < KCAP ? l : 0;
-        double row_next = L[lr * S + top];
+        double row_next[PS];
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            row_next[q] = (q == 0 || WAVE * q < kk) ? L[position(q) * S + top] : 0.0;
+        }
         for (int j = kk - 1; j >= 0; --j) {
-            const double rowj = row_next;
+            double rowj[PS];
+#pragma unroll
+            for (int q = 0; q < PS; ++q) {
+                rowj[q] = row_next[q];
+            }
             const int jn = j > 0 ? j - 1 : 0;
-            row_next = L[lr * S + jn];
+#pragma unroll
+            for (int q = 0; q < PS; ++q) {
+                if (q == 0 || WAVE * q < kk) {
+                    row_next[q] = L[position(q) * S + jn];
+                }
+            }
             const double2 dd = diag_at(j);
-            const double wj = readlane(b, j);
+            const double wj = at_position<PS>(b, j);
             const double zj = divide_by_diag<FAST>(wj, dd);
-            const double updated = b - rowj * zj;
-            b = l == j ? zj : (l < j ? updated : b);
+#pragma unroll
+            for (int q = 0; q < PS; ++q) {
+                const int p = position(q);
+                const double updated = b[q] - rowj[q] * zj;
+                b[q] = p == j ? zj : (p < j ? updated : b[q]);
+            }
         }
     }
 
-    __device__ __forceinline__ void backward(double& b, int kk_) {
+    __device__ __forceinline__ void backward(double (&b)[PS], int kk_) {
         const int kk = __builtin_amdgcn_readfirstlane(kk_);
         if (diag_ok) {
             backward_impl<true>(b, kk);
@@ -298,15 +406,19 @@ struct LdpWave {
     // Appends column c; false when its pivot is refused or the factor is full.  With y valid on
     // every position, also extends y to the new one.
     __device__ __forceinline__ bool append(int c) {
-        const int l = lane_id();
         if (kf >= KCAP) {
             return false;
         }
         const double fc = uniform_at<NKS>(f, c);
-        const double f_row = f_at(l < kf ? fcol : 0);
-        double b = 0.0;
-        if (l < kf) {
-            b = d.gram[static_cast<long>(fcol) * d.k + c] + f_row * fc;
+        double b[PS];
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            const int p = position(q);
+            const double f_row = f_at(p < kf ? fcol[q] : 0);
+            b[q] = 0.0;
+            if (p < kf) {
+                b[q] = d.gram[static_cast<long>(fcol[q]) * d.k + c] + f_row * fc;
+            }
         }
         const double sq = forward(b, kf);
         const double hcc = d.gram[static_cast<long>(c) * d.k + c] + fc * fc;
@@ -316,25 +428,36 @@ struct LdpWave {
         }
         const double lkk = sqrt(pivot);
         const double rkk = division_reciprocal(lkk);
-        if (l < kf) {
-            Lat(kf, l) = b;
-        }
-        if (l == kf) {
-            Lat(kf, kf) = lkk;
-            fcol = c;
-            diag[2 * kf] = lkk;
-            diag[2 * kf + 1] = rkk;
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            const int p = position(q);
+            if (p < kf) {
+                Lat(kf, p) = b[q];
+            }
+            if (p == kf) {
+                Lat(kf, kf) = lkk;
+                fcol[q] = c;
+                diag[2 * kf] = lkk;
+                diag[2 * kf + 1] = rkk;
+            }
         }
         const bool lkk_ok = divisor_in_range(lkk);
         if (y_valid == kf) {
             // y_k = (f_c - L_k0 y_0 - L_k1 y_1 - ...) / L_kk, subtracted in order.
-            const double prod = b * y;
+            double prod[PS];
+#pragma unroll
+            for (int q = 0; q < PS; ++q) {
+                prod[q] = b[q] * y[q];
+            }
             double v = fc;
             for (int j = 0; j < kf; ++j) {
-                v = v - readlane(prod, j);
+                v = v - at_position<PS>(prod, j);
             }
             const double yk = divide_nonzero_by(v, lkk, rkk, lkk_ok);
-            y = l == kf ? yk : y;
+#pragma unroll
+            for (int q = 0; q < PS; ++q) {
+                y[q] = position(q) == kf ? yk : y[q];
+            }
             y_valid = kf + 1;
         }
 #pragma unroll
@@ -349,11 +472,21 @@ struct LdpWave {
     // Removes factor position p (remove_cholesky_column).
     __device__ __forceinline__ void remove(int p) {
         const int l = lane_id();
-        for (int c = 0; c < kf; ++c) {
-            const bool move = l >= p && l < kf - 1 && c <= l + 1;
-            const double v = move ? Lat(l + 1, c) : 0.0;
-            if (move) {
-                Lat(l, c) = v;
+        // Rows p + 1 .. kf - 1 move up one.  Position 64 q + 63 reads row 64 (q + 1), which slot
+        // q + 1 overwrites, so the slots run in order with the wave's memory operations complete
+        // in between.
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            if (q > 0) {
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            }
+            const int r = position(q);
+            for (int c = 0; c < kf; ++c) {
+                const bool move = r >= p && r < kf - 1 && c <= r + 1;
+                const double v = move ? Lat(r + 1, c) : 0.0;
+                if (move) {
+                    Lat(r, c) = v;
+                }
             }
         }
         for (int column = p; column < kf - 1; ++column) {
@@ -363,27 +496,49 @@ struct LdpWave {
             const double rl = division_reciprocal(length);
             const double cosine = divide_by(diagonal, length, rl);
             const double sine = divide_by(above, length, rl);
-            if (l >= column && l < kf - 1) {
-                const double left = Lat(l, column);
-                const double right = Lat(l, column + 1);
-                Lat(l, column) = cosine * left + sine * right;
-                Lat(l, column + 1) = cosine * right - sine * left;
+#pragma unroll
+            for (int q = 0; q < PS; ++q) {
+                const int r = position(q);
+                if (r >= column && r < kf - 1) {
+                    const double left = Lat(r, column);
+                    const double right = Lat(r, column + 1);
+                    Lat(r, column) = cosine * left + sine * right;
+                    Lat(r, column + 1) = cosine * right - sine * left;
+                }
             }
         }
-        // The rotated diagonal and its reciprocals (each lane its own position).
+        // The rotated diagonal and its reciprocals (each lane its own positions).
         bool out_of_range = false;
-        if (l < kf - 1) {
-            double dl = diag[2 * l];
-            if (l >= p) {
-                dl = Lat(l, l);
-                diag[2 * l] = dl;
-                diag[2 * l + 1] = division_reciprocal(dl);
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            const int r = position(q);
+            if (r < kf - 1) {
+                double dl = diag[2 * r];
+                if (r >= p) {
+                    dl = Lat(r, r);
+                    diag[2 * r] = dl;
+                    diag[2 * r + 1] = division_reciprocal(dl);
+                }
+                out_of_range = out_of_range || !divisor_in_range(dl);
             }
-            out_of_range = !divisor_in_range(dl);
         }
         diag_ok = ballot(out_of_range) == 0ull;
-        const int from = gather(fcol, (l + 1) % WAVE);
-        fcol = (l >= p && l < kf - 1) ? from : fcol;
+        // Position r takes position r + 1's column (lane 63 of a slot from lane 0 of the next).
+        int from[PS];
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            from[q] = gather(fcol[q], (l + 1) % WAVE);
+        }
+#pragma unroll
+        for (int q = 0; q + 1 < PS; ++q) {
+            const int carry = readlane(fcol[q + 1], 0);
+            from[q] = l == WAVE - 1 ? carry : from[q];
+        }
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            const int r = position(q);
+            fcol[q] = (r >= p && r < kf - 1) ? from[q] : fcol[q];
+        }
 #pragma unroll
         for (int t = 0; t < NKS; ++t) {
             pos[t] = pos[t] == p ? -1 : (pos[t] > p ? pos[t] - 1 : pos[t]);
@@ -394,46 +549,62 @@ struct LdpWave {
 
     // Drops the positions whose column is inactive, last to first (GrowingCholesky::keep).
     __device__ __forceinline__ void keep() {
-        const int l = lane_id();
         int actbits = 0;
 #pragma unroll
         for (int t = 0; t < NKS; ++t) {
             actbits |= act[t] ? (1 << t) : 0;
         }
-        const int c = l < kf ? fcol : 0;
-        const int bits = gather(actbits, c % WAVE);
-        const bool leaves = l < kf && !((bits >> (c / WAVE)) & 1);
-        unsigned long long mask = ballot(leaves);
-        while (mask) {
-            const int b = 63 - __clzll(static_cast<long long>(mask));
-            mask &= ~(1ull << b);
-            remove(b);
+        unsigned long long mask[PS];
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            const int r = position(q);
+            const int c = r < kf ? fcol[q] : 0;
+            const int bits = gather(actbits, c % WAVE);
+            const bool leaves = r < kf && !((bits >> (c / WAVE)) & 1);
+            mask[q] = ballot(leaves);
+        }
+#pragma unroll
+        for (int q = PS - 1; q >= 0; --q) {
+            while (mask[q]) {
+                const int b = 63 - __clzll(static_cast<long long>(mask[q]));
+                mask[q] &= ~(1ull << b);
+                remove(WAVE * q + b);
+            }
         }
     }
 
     // cand = the solution of H_AA z = f_A, scattered to the columns.
     __device__ __forceinline__ void solve() {
-        const int l = lane_id();
-        const double fv = f_at(l < kf ? fcol : 0);
-        double b = l < kf ? (l < y_valid ? y : fv) : 0.0;
+        double b[PS];
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            const int p = position(q);
+            const double fv = f_at(p < kf ? fcol[q] : 0);
+            b[q] = p < kf ? (p < y_valid ? y[q] : fv) : 0.0;
+        }
         forward_from(b, y_valid, kf);
-        y = b;
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            y[q] = b[q];
+        }
         y_valid = kf;
         backward(b, kf);
 #pragma unroll
         for (int t = 0; t < NKS; ++t) {
             const int p = pos[t] < 0 ? 0 : pos[t];
-            const double z = gather(b, p);
+            const double z = gather_position<PS>(b, p);
             cand[t] = pos[t] >= 0 ? z : 0.0;
         }
     }
 
     // Returns true when the multipliers are found (u), false when the QR path must take over.
     __device__ __forceinline__ bool run() {
-        const int l = lane_id();
         kf = 0;
-        fcol = 0;
-        y = 0.0;
+#pragma unroll
+        for (int q = 0; q < PS; ++q) {
+            fcol[q] = 0;
+            y[q] = 0.0;
+        }
         y_valid = 0;
         diag_ok = true;
         double norm1 = 0.0;
@@ -556,7 +727,6 @@ struct LdpWave {
                 u[t] = cand[t];
             }
         }
-        (void)l;
     }
 };
 
@@ -787,23 +957,24 @@ __device__ __forceinline__ bool ldp_by_qr(const LsiDev& d, const double (&f)[NKS
 }
 
 // The fast path over all voxels: the Cholesky LDP; failures go to fallback_list.
-template <int NKS>
+template <int NKS, int PS>
 __global__ void __launch_bounds__(WAVE * 4) lsi_kernel(LsiArgs A) {
     extern __shared__ double lsi_diag[];
     const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
     const long gw = static_cast<long>(blockIdx.x) * (blockDim.x / WAVE) + w;
     const long n_waves = static_cast<long>(gridDim.x) * (blockDim.x / WAVE);
     double* slab = A.scratch + gw * A.per_wave;
-    double* diag = lsi_diag + w * LSI_DIAG_DOUBLES;
+    double* diag = lsi_diag + w * lsi_diag_doubles(PS);
     const int l = lane_id();
     const LsiDev& d = A.d;
     for (long v = gw; v < A.n_vox; v += n_waves) {
-        double rot;
+        double rot[PS];
         double raw[NKS];
-        const double scale = lsi_rotate<NKS>(d, A.signals + v * d.m, rot, raw);
+        const double scale = lsi_rotate<NKS, PS>(d, A.signals + v * d.m, rot, raw);
         if (!(scale > 0.0)) {
-            const double x = lsi_back(d, rot);
-            if (l < d.n) A.x_out[v * d.n + l] = x;
+            double x[PS];
+            lsi_back<PS>(d, rot, x);
+            lsi_store<PS>(d, A.x_out, v, x);
             if (l == 0) {
                 A.converged[v] = 1;
                 A.status[v] = 0;
@@ -815,15 +986,16 @@ __global__ void __launch_bounds__(WAVE * 4) lsi_kernel(LsiArgs A) {
         for (int t = 0; t < NKS; ++t) {
             f[t] = raw[t] / scale;
         }
-        LdpWave<NKS> ldp(d, f, slab, diag);
+        LdpWave<NKS, PS> ldp(d, f, slab, diag);
         if (!ldp.run()) {
             if (l == 0) {
                 A.fallback_list[atomicAdd(A.fallback_count, 1)] = static_cast<int>(v);
             }
             continue;
         }
-        const double x = lsi_solution<NKS>(d, f, ldp.u, scale, rot);
-        if (l < d.n) A.x_out[v * d.n + l] = x;
+        double x[PS];
+        lsi_solution<NKS, PS>(d, f, ldp.u, scale, rot, x);
+        lsi_store<PS>(d, A.x_out, v, x);
         if (l == 0) {
             A.converged[v] = 1;
             A.status[v] = 0;
@@ -831,28 +1003,28 @@ __global__ void __launch_bounds__(WAVE * 4) lsi_kernel(LsiArgs A) {
     }
 }
 
-// Doubles of the QR path's per-wave column slab: 64 rows (n + 1 <= 64) by 64 columns.
-constexpr long LSI_QR_SLAB = static_cast<long>(LSI_QR_ROWS) * WAVE * LSI_QR_ROWS * WAVE;
-
 // The QR path over the listed voxels (those the fast path handed over), one wave per block with
-// its column slab (32 KB) in the block's dynamic LDS.
-template <int NKS>
+// its column slab (lsi_qr_slab(PS) doubles): in the block's dynamic LDS for PS = 1 (32 KB), else
+// in the block's slab of A.scratch (A.per_wave doubles each).  Every lane reads and writes only
+// its own rows of the slab.
+template <int NKS, int PS>
 __global__ void __launch_bounds__(WAVE) lsi_qr_kernel(LsiArgs A) {
     extern __shared__ double smem[];
     const long gw = blockIdx.x;
     const long n_waves = gridDim.x;
-    double* slab = smem;
+    double* slab = PS == 1 ? smem : A.scratch + gw * A.per_wave;
     const int l = lane_id();
     const LsiDev& d = A.d;
     const long count = A.list ? static_cast<long>(*A.list_count) : A.n_vox;
     for (long it = gw; it < count; it += n_waves) {
         const long v = A.list ? A.list[it] : it;
-        double rot;
+        double rot[PS];
         double raw[NKS];
-        const double scale = lsi_rotate<NKS>(d, A.signals + v * d.m, rot, raw);
+        const double scale = lsi_rotate<NKS, PS>(d, A.signals + v * d.m, rot, raw);
         if (!(scale > 0.0)) {
-            const double x = lsi_back(d, rot);
-            if (l < d.n) A.x_out[v * d.n + l] = x;
+            double x[PS];
+            lsi_back<PS>(d, rot, x);
+            lsi_store<PS>(d, A.x_out, v, x);
             if (l == 0) {
                 A.converged[v] = 1;
                 A.status[v] = ST_QR_FALLBACK;
@@ -865,9 +1037,10 @@ __global__ void __launch_bounds__(WAVE) lsi_qr_kernel(LsiArgs A) {
             f[t] = raw[t] / scale;
         }
         double u[NKS];
-        const bool conv = ldp_by_qr<NKS, LSI_QR_ROWS>(d, f, u, slab);
-        const double x = lsi_solution<NKS>(d, f, u, scale, rot);
-        if (l < d.n) A.x_out[v * d.n + l] = x;
+        const bool conv = ldp_by_qr<NKS, PS>(d, f, u, slab);
+        double x[PS];
+        lsi_solution<NKS, PS>(d, f, u, scale, rot, x);
+        lsi_store<PS>(d, A.x_out, v, x);
         if (l == 0) {
             A.converged[v] = conv ? 1 : 0;
             A.status[v] = ST_QR_FALLBACK;

@@ -67,11 +67,15 @@ int launch_sh_project(const double* amps, long n, int k, const double* p, int c,
 
 // LSI launchers (inst_lsi.hip).
 int lsi_constraint_slots(int64_t k);
-int launch_lsi(int nks, const LsiArgs& fast, const LsiArgs& qr, int waves, int qr_waves,
+int launch_lsi(int nks, int ps, const LsiArgs& fast, const LsiArgs& qr, int waves, int qr_waves,
                hipStream_t stream);
-// Waves (one per block) of the LSI QR path; each keeps its column slab in LDS.
+// Waves (one per block) of the LSI QR path; with up to 63 parameters each keeps its column slab
+// in LDS, with more in a global slab (128 KB or 512 KB), so fewer run.
 constexpr int LSI_QR_WAVES = 2048;
-constexpr long LSI_FACTOR_SLAB = static_cast<long>(LSI_KCAP + 1) * LSI_KCAP;
+constexpr int LSI_QR_WAVES_WIDE = 512;
+// Waves of the fast LSI path with more than 63 parameters (each with a factor slab of 132 KB or
+// 526 KB).
+constexpr int LSI_WAVES_WIDE = 2048;
 
 // Peak, derivative and climb launchers (inst_peaks.hip).
 bool sh_supported(int lmax, bool full);
@@ -1134,8 +1138,9 @@ int lsd_lsi_create(const double* a, int64_t m, int64_t n, const double* g, int64
     if (!err.empty()) {
         return fail(err);
     }
-    if (n > WAVE - 1) {
-        return fail("the GPU LSI solve takes at most 63 parameters, got " + std::to_string(n));
+    if (n > LSI_MAX_PARAMS) {
+        return fail("the GPU LSI solve takes at most " + std::to_string(LSI_MAX_PARAMS) +
+                    " parameters, got " + std::to_string(n));
     }
     if (lsi_constraint_slots(k) < 0) {
         return fail("the GPU LSI solve takes at most 512 constraints, got " + std::to_string(k));
@@ -1195,15 +1200,18 @@ static int fit_lsi_impl(const double* signals, int64_t n_voxels, int64_t m, cons
     if (n_voxels == 0) {
         return 0;
     }
-    const int waves = main_waves(n_voxels);
-    const int qr_waves = std::min<int>(LSI_QR_WAVES, waves);
+    const int ps = lsi_position_slots(p->n);
+    const int waves = ps == 1 ? main_waves(n_voxels) : std::min(main_waves(n_voxels), LSI_WAVES_WIDE);
+    const int qr_waves = std::min<int>(ps == 1 ? LSI_QR_WAVES : LSI_QR_WAVES_WIDE, waves);
     Workspace mem(stream);
     int* counter = nullptr;
     int* list = nullptr;
     double* factor = nullptr;
+    double* qr_slabs = nullptr;
     int rc = mem.get(&counter, 1);
     if (!rc) rc = mem.get(&list, n_voxels);
-    if (!rc) rc = mem.get(&factor, LSI_FACTOR_SLAB * waves);
+    if (!rc) rc = mem.get(&factor, lsi_factor_slab(ps) * waves);
+    if (!rc && ps > 1) rc = mem.get(&qr_slabs, lsi_qr_slab(ps) * qr_waves);
 
     if (!rc && status == nullptr) rc = mem.get(&status, n_voxels);
     if (rc) return rc;
@@ -1219,17 +1227,17 @@ static int fit_lsi_impl(const double* signals, int64_t n_voxels, int64_t m, cons
     a.fallback_list = list;
     a.fallback_count = counter;
     a.scratch = factor;
-    a.per_wave = LSI_FACTOR_SLAB;
+    a.per_wave = lsi_factor_slab(ps);
     LsiArgs q = a;
     q.list = only_qr ? nullptr : list;
     q.list_count = only_qr ? nullptr : counter;
     q.fallback_list = nullptr;
     q.fallback_count = nullptr;
-    q.scratch = nullptr; // the QR path's slabs are in LDS
-    q.per_wave = 0;
+    q.scratch = qr_slabs; // null for one parameter slot: the QR path's slabs are in LDS
+    q.per_wave = ps > 1 ? lsi_qr_slab(ps) : 0;
     // QR only: the fast kernel gets no voxels, the QR kernel all of them.
     if (only_qr) a.n_vox = 0;
-    return launch_lsi(lsi_constraint_slots(p->k), a, q, waves, qr_waves, stream);
+    return launch_lsi(lsi_constraint_slots(p->k), ps, a, q, waves, qr_waves, stream);
 }
 
 int lsd_sh_project(const double* amplitudes, int64_t n, int64_t k, const double* projection,

@@ -127,3 +127,92 @@ def test_msmt_model_recovers_wm_csf_mixture():  # tests/sd/test_msmt.py:69-88
     fit = MSMTCSDModel([rfs[1], rfs[0]]).fit(volume, gtab, n_jobs=1)
     np.testing.assert_allclose(fit.tissue_fractions[0, 0, 0], (0.7, 0.3), atol=1e-6)
     np.testing.assert_allclose(fit.fods[0][0, 0, 0], fod_true, atol=1e-6)
+
+
+SHELLS_WIDE = np.array([0.0, 1000.0, 2000.0, 3000.0])
+
+
+def _wide_msmt(wm_lmax, n_vox, seed=1, n_dirs=90, full=False):
+    """MSMT with a WM response up to wm_lmax and a CSF response: 67 (lmax 10) to 232 (lmax 20)
+    parameters, on 2 b0 + 3 x n_dirs measurements; signals 300 +- 200 (noise), whose active sets
+    reach n - 6 constraints, past 64 and 128 factor positions."""
+    from larmor_amd import synthetic
+    from larmor_amd.sd.rf import ZonalSHRF
+    from larmor_amd.sd.sh import build_basis_matrix
+
+    nodes, weights = np.polynomial.legendre.leggauss(64)
+    theta = np.arccos(nodes)
+    basis = build_basis_matrix(theta, np.zeros_like(theta), wm_lmax)
+    zonal = [l * (l + 1) // 2 for l in range(0, wm_lmax + 1, 2)]
+    rows = [2 * np.pi * (weights * 1000.0 * np.exp(-b * (0.2e-3 + 1.5e-3 * nodes**2))) @ basis[:, zonal]
+            for b in SHELLS_WIDE]
+    wm = ZonalSHRF(bvals=SHELLS_WIDE, coefficients=np.array(rows))
+    csf = ZonalSHRF(bvals=SHELLS_WIDE,
+                    coefficients=np.sqrt(4 * np.pi) * (2000.0 * np.exp(-SHELLS_WIDE * 3e-3))[:, None])
+    dirs = synthetic.spiral_directions(n_dirs)
+    bvecs = np.vstack([np.zeros((2, 3))] + [dirs] * 3)
+    bvals = np.concatenate([[0.0, 0.0], np.repeat(SHELLS_WIDE[1:], n_dirs)])
+    forward = forward_matrix([wm, csf], [wm_lmax, 0], bvals, bvecs)
+    g = constraint_matrix([wm_lmax, 0], DIRS_300)
+    signals = 300.0 + 200.0 * np.random.default_rng(seed).standard_normal((n_vox, len(bvals)))
+    if full:
+        return forward, g, msmt_ridge(forward), signals, (wm, csf), (bvals, bvecs)
+    return forward, g, msmt_ridge(forward), signals
+
+
+# 67, 92 and 121 parameters take two lane slots; 154 and 232 take four.
+@pytest.mark.parametrize("wm_lmax,n_vox", [(10, 160), (12, 96), (14, 64), (16, 48), (20, 32)])
+def test_lsi_wide_msmt_bit_identical(wm_lmax, n_vox):
+    forward, g, ridge, signals = _wide_msmt(wm_lmax, n_vox)
+    print(f"lsi wide wm lmax {wm_lmax}: {forward.shape[1]} parameters, {signals.shape}", flush=True)
+    (rx, rc, tr), (x, c, status) = _both(signals, forward, g, ridge)
+    if wm_lmax >= 12:  # the factor outgrows the first (and, from 154 parameters, the second) slot
+        assert tr[:, 3].max() > 64 * (1 if forward.shape[1] < 128 else 2)
+    np.testing.assert_array_equal(c, rc)
+    np.testing.assert_array_equal(status != 0, tr[:, 0] != 0)
+    np.testing.assert_array_equal(x, rx)
+    # Feasible to the LDP's tolerance, which grows with the constraint count and the scale.
+    assert (g @ x.T).min() > -1e-7
+
+
+@pytest.mark.parametrize("wm_lmax", [10, 16])
+def test_lsi_wide_qr_path_matches_oracle(wm_lmax):
+    forward, g, ridge, signals = _wide_msmt(wm_lmax, 3, seed=4)
+    print(f"lsi wide qr wm lmax {wm_lmax}: {forward.shape[1]} parameters", flush=True)
+    d = _core.LsiDesign(forward, g, ridge)
+    xq, cq = _core.fit_lsi(signals, d, 1, only_qr=True)
+    fast, _ = _core.fit_lsi(signals, d, 1)
+    for v in range(len(signals)):
+        rx, rcv = oracle.solve_lsi_by_qr(signals[v], forward, g, ridge)
+        assert cq[v] == rcv
+        np.testing.assert_array_equal(xq[v], rx)
+    # The two algorithms agree to the conditioning of a 67- or 154-parameter LDP.
+    scale = np.abs(fast).max()
+    assert np.abs(xq - fast).max() < 1e-6 * scale
+
+
+def test_lsi_wide_edge_cases_and_limit():
+    forward, g, ridge, signals = _wide_msmt(12, 6, seed=9)
+    edge = np.vstack([np.zeros(signals.shape[1]), np.full(signals.shape[1], np.nan), signals])
+    (rx, rc, _), (x, c, _) = _both(edge, forward, g, ridge)
+    np.testing.assert_array_equal(c, rc)
+    np.testing.assert_array_equal(x, rx)
+    d = _core.LsiDesign(forward, g, ridge)
+    alone, _ = _core.fit_lsi(np.ascontiguousarray(signals[2:4]), d, 1)
+    np.testing.assert_array_equal(alone, x[4:6])
+    rng = np.random.default_rng(2)
+    with pytest.raises(ValueError, match="at most 255 parameters, got 256"):
+        _core.LsiDesign(rng.normal(size=(300, 256)), np.eye(256), 0.1)
+
+
+def test_msmt_model_recovers_wm_csf_mixture_at_wm_lmax_10():
+    # tests/sd/test_msmt.py:69-88 at WM lmax 10 (67 parameters).
+    from larmor_amd.types import GradientTable
+
+    forward, _, _, _, (wm, csf), (bvals, bvecs) = _wide_msmt(10, 1, full=True)
+    fod_true = fit_sh(0.7 * 9 / (4 * np.pi) * (DIRS_300 @ FIBRE) ** 8, DIRS_300, 10)
+    signal = forward @ np.concatenate([fod_true, [0.3 / np.sqrt(4 * np.pi)]])
+    volume = Volume(np.tile(signal, (2, 1, 1, 1)), np.eye(4))
+    fit = MSMTCSDModel([csf, wm], lmax=10).fit(volume, GradientTable(bvals, bvecs), n_jobs=1)
+    np.testing.assert_allclose(fit.tissue_fractions[0, 0, 0], (0.7, 0.3), atol=1e-6)
+    np.testing.assert_allclose(fit.fods[0][0, 0, 0], fod_true, atol=1e-6)
