@@ -367,7 +367,7 @@ void lsd_sphere_destroy(lsd_sphere* sphere);
  * NULL to compute them from the sphere's basis.  Outputs (device): directions (n_voxels x
  * n_peaks x 3) and amplitudes (n_voxels x n_peaks), largest first, NaN-padded; overflow (device
  * int32, may be NULL) counts voxels with more than 128 distinct peaks (the rest are dropped).
- * n_peaks <= 64; lmax <= 12 (symmetric) or 8 (full).
+ * n_peaks <= 64; lmax <= 20 (above 12 symmetric or 8 full, one kernel takes the lmax at run time).
  * Replaces: peaks_chunk(coeffs, lmax, on_sphere, vertices, neighbours, n_peaks,
  *           relative_threshold, threshold, n_threads, full)  cpp/src/python/sh.cpp:112-175,
  *           voxel_peaks cpp/src/sh/peaks.cpp:79-133, climb cpp/src/sh/peaks.cpp:33-77.
@@ -529,7 +529,8 @@ int lsd_track_calibrate(int64_t lmax, int full, double max_angle_rad, double* gr
  * LSD_END_* of the row's start and end), followed (n x 3, the direction given or found).
  * stats (host, may be NULL) receives the sampling counts; it makes the call wait for the
  * stream.  A streamline's points do not depend on the batch, the grid or the other seeds.
- * GPU limits: lmax <= 12 (symmetric) or 8 (full).
+ * GPU limits: lmax <= 20 (above 12 symmetric or 8 full, the series lives in the lane's private
+ * memory).
  * Replaces: track_batch(fod, fod_inv_affine, mask, seeds, directions, indices, run_seed,
  *           step_size, max_steps, bidirectional, algorithm, positions, n_threads, full)
  *           cpp/src/python/tracking.cpp:264-404; track_batch, track_seed, track_one_way

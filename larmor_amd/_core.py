@@ -681,8 +681,8 @@ def fit_csd(signals, initial, forward, constraint, norm_weight: float, threshold
 
 
 # The largest lmax of the reference's compiled SH code (cpp/src/sh/harmonics.hpp MAX_LMAX).  The
-# GPU kernels hold lmax <= 12 for a symmetric series and <= 8 for a full one, and say so when
-# asked for more.
+# GPU kernels take every lmax up to it: exact instantiations to 12 (symmetric) and 8 (full), one
+# lmax-20 instantiation per layout for the rest, with the lmax read at run time.
 MAX_LMAX = 20
 
 

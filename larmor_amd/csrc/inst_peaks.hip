@@ -1,5 +1,6 @@
-// Instantiations and launchers of the peak search, derivative and climb kernels: symmetric
-// series of even lmax 0-12 and full series of lmax 1-8.
+// Instantiations and launchers of the peak search, derivative and climb kernels: exact ones for
+// symmetric series of even lmax 0-12 and full series of lmax 1-8, and the lmax-20 ones, which take
+// every higher lmax (up to 20) at run time.
 #include "kernels.hpp"
 #include "peaks_wave.hpp"
 
@@ -66,42 +67,52 @@ int launch_climb_t(const double* coeffs, long n, int nc, const double* starts, i
 } // namespace
 
 bool sh_supported(int lmax, bool full) {
-#define LSD_TRY(L, F)                                                                          \
-    if (lmax == L && full == F) return true;
-    LSD_FOR_EACH_SH(LSD_TRY)
-#undef LSD_TRY
-    return false;
+    return lmax >= 0 && lmax <= SH_WIDE_LMAX && (full || lmax % 2 == 0);
 }
 
-int launch_peaks(int lmax, bool full, const PeaksArgs& a, int waves, hipStream_t stream) {
+// Each launcher: the exact instantiation when there is one, else the lmax-20 one with the
+// series' lmax in the constants.
+int launch_peaks(int lmax, bool full, const PeaksArgs& a_in, int waves, hipStream_t stream) {
+    PeaksArgs a = a_in;
+    a.k.lmax = lmax;
 #define LSD_TRY(L, F)                                                                          \
     if (lmax == L && full == F) return launch_peaks_t<L, F>(a, waves, stream);
     LSD_FOR_EACH_SH(LSD_TRY)
 #undef LSD_TRY
-    return fail("no peak kernel for lmax " + std::to_string(lmax));
+    if (!sh_supported(lmax, full)) return fail("no peak kernel for lmax " + std::to_string(lmax));
+    return full ? launch_peaks_t<SH_WIDE_LMAX, true>(a, waves, stream)
+                : launch_peaks_t<SH_WIDE_LMAX, false>(a, waves, stream);
 }
 
 int launch_sh_derivatives(int lmax, bool full, const double* coeffs, long n, int nc,
                           const double* theta, const double* phi, double* out, ShConstantsDev k,
                           hipStream_t stream) {
+    k.lmax = lmax;
 #define LSD_TRY(L, F)                                                                          \
     if (lmax == L && full == F)                                                                \
         return launch_derivs_t<L, F>(coeffs, n, nc, theta, phi, out, k, stream);
     LSD_FOR_EACH_SH(LSD_TRY)
 #undef LSD_TRY
-    return fail("no derivative kernel for lmax " + std::to_string(lmax));
+    if (!sh_supported(lmax, full)) return fail("no derivative kernel for lmax " + std::to_string(lmax));
+    return full ? launch_derivs_t<SH_WIDE_LMAX, true>(coeffs, n, nc, theta, phi, out, k, stream)
+                : launch_derivs_t<SH_WIDE_LMAX, false>(coeffs, n, nc, theta, phi, out, k, stream);
 }
 
 int launch_sh_climb(int lmax, bool full, const double* coeffs, long n, int nc,
                     const double* starts, int max_iters, double tol, double max_step,
                     double* dirs, double* amps, ShConstantsDev k, hipStream_t stream) {
+    k.lmax = lmax;
 #define LSD_TRY(L, F)                                                                          \
     if (lmax == L && full == F)                                                                \
         return launch_climb_t<L, F>(coeffs, n, nc, starts, max_iters, tol, max_step, dirs,     \
                                     amps, k, stream);
     LSD_FOR_EACH_SH(LSD_TRY)
 #undef LSD_TRY
-    return fail("no climb kernel for lmax " + std::to_string(lmax));
+    if (!sh_supported(lmax, full)) return fail("no climb kernel for lmax " + std::to_string(lmax));
+    return full ? launch_climb_t<SH_WIDE_LMAX, true>(coeffs, n, nc, starts, max_iters, tol,
+                                                     max_step, dirs, amps, k, stream)
+                : launch_climb_t<SH_WIDE_LMAX, false>(coeffs, n, nc, starts, max_iters, tol,
+                                                      max_step, dirs, amps, k, stream);
 }
 
 } // namespace lsd

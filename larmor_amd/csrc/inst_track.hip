@@ -1,5 +1,6 @@
 // Instantiations and launcher of the streamline tracking kernel: det and prob for symmetric
-// series of even lmax 0-12 and full series of lmax 1-8, and the fixed-rule test algorithm.
+// series of even lmax 0-12 and full series of lmax 1-8, the lmax-20 instantiations that take
+// every higher lmax at run time, and the fixed-rule test algorithm.
 #include "launch.hpp"
 #include "track_wave.hpp"
 
@@ -36,11 +37,14 @@ int launch_track_t(const TrackArgs& a, int lanes, hipStream_t stream) {
 } // namespace
 
 bool track_supported(int lmax, bool full) {
-    return full ? (lmax >= 1 && lmax <= 8) : (lmax >= 0 && lmax <= 12 && lmax % 2 == 0);
+    return full ? (lmax >= 1 && lmax <= SH_WIDE_LMAX)
+                : (lmax >= 0 && lmax <= SH_WIDE_LMAX && lmax % 2 == 0);
 }
 
-int launch_track(int algo, int lmax, bool full, const TrackArgs& a, int lanes,
+int launch_track(int algo, int lmax, bool full, const TrackArgs& a_in, int lanes,
                  hipStream_t stream) {
+    TrackArgs a = a_in;
+    a.k.lmax = lmax;
     if (algo == TRACK_TEST) {
         return launch_track_t<0, false, TRACK_TEST>(a, lanes, stream);
     }
@@ -54,7 +58,15 @@ int launch_track(int algo, int lmax, bool full, const TrackArgs& a, int lanes,
     LSD_TRY(1, true) LSD_TRY(2, true) LSD_TRY(3, true) LSD_TRY(4, true) LSD_TRY(5, true)
     LSD_TRY(6, true) LSD_TRY(7, true) LSD_TRY(8, true)
 #undef LSD_TRY
-    return fail("no tracking kernel for lmax " + std::to_string(lmax));
+    if (!track_supported(lmax, full)) {
+        return fail("no tracking kernel for lmax " + std::to_string(lmax));
+    }
+    if (full) {
+        return algo == TRACK_DET ? launch_track_t<SH_WIDE_LMAX, true, TRACK_DET>(a, lanes, stream)
+                                 : launch_track_t<SH_WIDE_LMAX, true, TRACK_PROB>(a, lanes, stream);
+    }
+    return algo == TRACK_DET ? launch_track_t<SH_WIDE_LMAX, false, TRACK_DET>(a, lanes, stream)
+                             : launch_track_t<SH_WIDE_LMAX, false, TRACK_PROB>(a, lanes, stream);
 }
 
 } // namespace lsd

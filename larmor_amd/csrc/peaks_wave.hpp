@@ -13,9 +13,12 @@
 //    it is appended; then the largest peaks, ties in the order kept, are cut at the absolute and
 //    relative thresholds and symmetric peaks are turned to z >= 0.
 //
-// The Legendre table of a climb lives in registers: LMAX and FULL are template parameters and
-// every index is a compile-time constant.  The transcendental functions (acos, atan2, sin, cos)
-// are the device library's, so the results match the CPU oracle to rounding, not to the bit.
+// A climb's Legendre values live in registers, one degree at a time: LMAX and FULL are template
+// parameters and every index is a compile-time constant.  The kernels instantiated for lmax 20
+// take any lower lmax at run time (from the constants), so every lmax up to 20 is served, the
+// common ones (12 and below symmetric, 8 and below full) by exact instantiations.  The
+// transcendental functions (acos, atan2, sin, cos) are the device library's, so the results match
+// the CPU oracle to rounding, not to the bit.
 #pragma once
 
 #include "wave.hpp"
@@ -33,7 +36,17 @@ struct ShConstantsDev {
     const double* up;       // (21, 21): sqrt((l - m)(l + m + 1)), m <= l
     const double* down;     // (21, 21): sqrt((l + m)(l - m + 1)), m <= l
     const double* offdiag;  // (21): sqrt(2 m + 3)
+    int lmax;               // the series' lmax (read by the lmax-20 instantiations)
 };
+
+// The lmax of the instantiation that serves every lmax above the exact ones.
+constexpr int SH_WIDE_LMAX = 20;
+
+// The series' lmax: LMAX, or for the wide instantiation the run-time value.
+template <int LMAX>
+__device__ __forceinline__ int series_lmax(const ShConstantsDev& k) {
+    return LMAX == SH_WIDE_LMAX ? k.lmax : LMAX;
+}
 
 constexpr int sh_index(int l, int m, bool full) {
     return full ? l * l + l + m : l * (l + 1) / 2 + m;
@@ -54,102 +67,95 @@ __device__ __forceinline__ double power_int_dev(double base, int exponent) {
     return result;
 }
 
+// Degree l of the normalised associated Legendre values, row[m] = P(l, m) for m <= l, from the
+// two degrees before it: fill_legendre's recurrences (P(m, m) from the diagonal constant, P(m + 1,
+// m) from P(m, m), then in l), the same expression for every value, so the same values as a table
+// filled order by order.
+template <int LMAX>
+__device__ __forceinline__ void legendre_row(int l, double cos_t, double sin_t,
+                                             const double (&prev1)[LMAX + 1],
+                                             const double (&prev2)[LMAX + 1],
+                                             double (&row)[LMAX + 1], const ShConstantsDev& c) {
+#pragma unroll
+    for (int m = 0; m <= LMAX; ++m) {
+        if (m > l) {
+            break;
+        }
+        if (m == l) {
+            row[m] = c.diagonal[m] * power_int_dev(sin_t, m);
+        } else if (m == l - 1) {
+            row[m] = c.offdiag[m] * cos_t * prev1[m];
+        } else {
+            row[m] = c.factor[l * SH_TABLE + m] *
+                     (cos_t * prev1[m] - prev2[m] / c.factor[(l - 1) * SH_TABLE + m]);
+        }
+    }
+}
+
+__device__ __forceinline__ double ladder_up_dev(const ShConstantsDev& k, int l, int m) {
+    return k.up[l * SH_TABLE + m];
+}
+__device__ __forceinline__ double ladder_down_dev(const ShConstantsDev& k, int l, int m) {
+    return k.down[l * SH_TABLE + m];
+}
+
+// d/dtheta of P(l, m) from the degree's row (legendre_d_theta).
+template <int LMAX>
+__device__ __forceinline__ double legendre_d_theta_row(const ShConstantsDev& k, int l, int m,
+                                                       const double (&row)[LMAX + 1]) {
+    if (l == 0) return 0.0;
+    if (m == 0) return ladder_up_dev(k, l, 0) * row[1];
+    const double above = m < l ? ladder_up_dev(k, l, m) * row[m < LMAX ? m + 1 : m] : 0.0;
+    return (above - ladder_down_dev(k, l, m) * row[m - 1]) / 2.0;
+}
+
+// d2/dtheta2 of P(l, m) from the degree's row (legendre_d2_theta).  At l = 1 the m = 0 form's
+// P(l, 2) term has a zero ladder factor; it is taken as zero.
+template <int LMAX>
+__device__ __forceinline__ double legendre_d2_theta_row(const ShConstantsDev& k, int l, int m,
+                                                        const double (&row)[LMAX + 1]) {
+    if (l == 0) return 0.0;
+    if (m == 0) {
+        const double u0 = ladder_up_dev(k, l, 0);
+        const double p2 = l >= 2 ? row[2 <= LMAX ? 2 : 0] : 0.0;
+        return (u0 * ladder_up_dev(k, l, 1) * p2 - u0 * u0 * row[0]) / 2.0;
+    }
+    const double uh = ladder_up_dev(k, l, m);
+    const double dh = ladder_down_dev(k, l, m);
+    double total = -(uh * uh + dh * dh) * row[m];
+    if (m == 1) {
+        total = total - ladder_down_dev(k, l, 1) * ladder_down_dev(k, l, 0) * row[1];
+    } else {
+        total = total + ladder_down_dev(k, l, m) * ladder_down_dev(k, l, m - 1) * row[m - 2];
+    }
+    if (m + 2 <= l) {
+        total = total + ladder_up_dev(k, l, m) * ladder_up_dev(k, l, m + 1) *
+                            row[m + 2 <= LMAX ? m + 2 : m];
+    }
+    return total / 4.0;
+}
 
 struct ShDerivs {
     double value, d_theta, d_phi, d_theta_theta, d_theta_phi, d_phi_phi;
 };
 
-// The table of normalised associated Legendre values needed by the derivatives: every order of
-// the degrees the series uses (all degrees if FULL, the even ones otherwise).
-template <int LMAX, bool FULL>
-struct LegendreRegs {
-    static constexpr bool used(int l) { return FULL || l % 2 == 0; }
-    // Position of (l, m) among the used degrees.
-    static constexpr int pos(int l, int m) {
-        int p = 0;
-        for (int q = 0; q < l; ++q) {
-            p += used(q) ? q + 1 : 0;
-        }
-        return p + m;
-    }
-    static constexpr int SIZE = pos(LMAX + 1, 0);
-    double v[SIZE > 0 ? SIZE : 1];
-    const ShConstantsDev& k;
-
-    __device__ explicit LegendreRegs(const ShConstantsDev& k_) : k(k_) {}
-    __device__ __forceinline__ double ladder_up_dev(int l, int m) const { return k.up[l * SH_TABLE + m]; }
-    __device__ __forceinline__ double ladder_down_dev(int l, int m) const {
-        return k.down[l * SH_TABLE + m];
-    }
-
-    __device__ __forceinline__ double at(int l, int m) const { return v[pos(l, m)]; }
-
-    // fill_legendre: per order the diagonal, the first off-diagonal, then the recurrence in l.
-    __device__ __forceinline__ void fill(double cos_t, double sin_t, const ShConstantsDev& c) {
-#pragma unroll
-        for (int m = 0; m <= LMAX; ++m) {
-            double lm2 = 0.0;
-            double lm1 = c.diagonal[m] * power_int_dev(sin_t, m);
-            if (used(m)) v[pos(m, m)] = lm1;
-            if (m < LMAX) {
-                const double next = c.offdiag[m] * cos_t * lm1;
-                lm2 = lm1;
-                lm1 = next;
-                if (used(m + 1)) v[pos(m + 1, m)] = lm1;
-            }
-#pragma unroll
-            for (int l = m + 2; l <= LMAX; ++l) {
-                const double next =
-                    c.factor[l * SH_TABLE + m] *
-                    (cos_t * lm1 - lm2 / c.factor[(l - 1) * SH_TABLE + m]);
-                lm2 = lm1;
-                lm1 = next;
-                if (used(l)) v[pos(l, m)] = lm1;
-            }
-        }
-    }
-
-    __device__ __forceinline__ double d_theta(int l, int m) const {
-        if (l == 0) return 0.0;
-        if (m == 0) return ladder_up_dev(l, 0) * at(l, 1);
-        const double above = m < l ? ladder_up_dev(l, m) * at(l, m + 1) : 0.0;
-        return (above - ladder_down_dev(l, m) * at(l, m - 1)) / 2.0;
-    }
-
-    __device__ __forceinline__ double d2_theta(int l, int m) const {
-        if (l == 0) return 0.0;
-        if (m == 0) {
-            const double u0 = ladder_up_dev(l, 0);
-            return (u0 * ladder_up_dev(l, 1) * at(l, 2) - u0 * u0 * at(l, 0)) / 2.0;
-        }
-        const double uh = ladder_up_dev(l, m);
-        const double dh = ladder_down_dev(l, m);
-        double total = -(uh * uh + dh * dh) * at(l, m);
-        if (m == 1) {
-            total = total - ladder_down_dev(l, 1) * ladder_down_dev(l, 0) * at(l, 1);
-        } else {
-            total = total + ladder_down_dev(l, m) * ladder_down_dev(l, m - 1) * at(l, m - 2);
-        }
-        if (m + 2 <= l) {
-            total = total + ladder_up_dev(l, m) * ladder_up_dev(l, m + 1) * at(l, m + 2);
-        }
-        return total / 4.0;
-    }
-};
-
 // derivatives (harmonics.cpp:139-185) of the series c at (theta, phi), per lane, given
 // sin theta, cos theta, sin phi and cos phi (the values of sin(m phi), cos(m phi) at m = 1).
+// The sums run over the degrees in increasing l and within a degree over m = 0..l; the Legendre
+// values come one degree at a time (legendre_row), so two degrees are live at once.
 template <int LMAX, bool FULL>
 __device__ __forceinline__ ShDerivs sh_derivs(const double* c, double sin_theta,
                                               double cos_theta, double phi, double sin_phi,
                                               double cos_phi, const ShConstantsDev& k) {
-    LegendreRegs<LMAX, FULL> P(k);
-    P.fill(cos_theta, sin_theta, k);
+    const int lmax = series_lmax<LMAX>(k);
     const bool at_pole = sin_theta < 1e-4;
     double s_m[LMAX + 1];
     double c_m[LMAX + 1];
 #pragma unroll
     for (int m = 1; m <= LMAX; ++m) {
+        if (m > lmax) {
+            break;
+        }
         if (m == 1) {
             s_m[m] = sin_phi;
             c_m[m] = cos_phi;
@@ -158,31 +164,48 @@ __device__ __forceinline__ ShDerivs sh_derivs(const double* c, double sin_theta,
         }
     }
     ShDerivs r{0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double prev2[LMAX + 1];
+    double prev1[LMAX + 1];
 #pragma unroll
-    for (int l = 0; l <= LMAX; l += FULL ? 1 : 2) {
-        const double zonal = c[sh_index(l, 0, FULL)];
-        r.value = r.value + zonal * P.at(l, 0);
-        r.d_theta = r.d_theta + zonal * P.d_theta(l, 0);
-        r.d_theta_theta = r.d_theta_theta + zonal * P.d2_theta(l, 0);
+    for (int l = 0; l <= LMAX; ++l) {
+        if (l > lmax) {
+            break;
+        }
+        double row[LMAX + 1];
+        legendre_row<LMAX>(l, cos_theta, sin_theta, prev1, prev2, row, k);
+        if (FULL || l % 2 == 0) {
+            const double zonal = c[sh_index(l, 0, FULL)];
+            r.value = r.value + zonal * row[0];
+            r.d_theta = r.d_theta + zonal * legendre_d_theta_row<LMAX>(k, l, 0, row);
+            r.d_theta_theta = r.d_theta_theta + zonal * legendre_d2_theta_row<LMAX>(k, l, 0, row);
 #pragma unroll
-        for (int m = 1; m <= l; ++m) {
-            const double mf = static_cast<double>(m);
-            const double cc = c[sh_index(l, m, FULL)];
-            const double sc = c[sh_index(l, -m, FULL)];
-            const double angular = M_SQRT2 * (cc * c_m[m] + sc * s_m[m]);
-            const double angular_d_phi = M_SQRT2 * (sc * c_m[m] - cc * s_m[m]);
-            const double lv = P.at(l, m);
-            const double ldt = P.d_theta(l, m);
-            r.value = r.value + angular * lv;
-            r.d_theta = r.d_theta + angular * ldt;
-            r.d_theta_theta = r.d_theta_theta + angular * P.d2_theta(l, m);
-            if (at_pole) {
-                r.d_phi = r.d_phi + angular_d_phi * ldt;
-            } else {
-                r.d_phi = r.d_phi + mf * angular_d_phi * lv;
-                r.d_theta_phi = r.d_theta_phi + mf * angular_d_phi * ldt;
-                r.d_phi_phi = r.d_phi_phi - mf * mf * angular * lv;
+            for (int m = 1; m <= l; ++m) {
+                const double mf = static_cast<double>(m);
+                const double cc = c[sh_index(l, m, FULL)];
+                const double sc = c[sh_index(l, -m, FULL)];
+                const double angular = M_SQRT2 * (cc * c_m[m] + sc * s_m[m]);
+                const double angular_d_phi = M_SQRT2 * (sc * c_m[m] - cc * s_m[m]);
+                const double lv = row[m];
+                const double ldt = legendre_d_theta_row<LMAX>(k, l, m, row);
+                r.value = r.value + angular * lv;
+                r.d_theta = r.d_theta + angular * ldt;
+                r.d_theta_theta =
+                    r.d_theta_theta + angular * legendre_d2_theta_row<LMAX>(k, l, m, row);
+                if (at_pole) {
+                    r.d_phi = r.d_phi + angular_d_phi * ldt;
+                } else {
+                    r.d_phi = r.d_phi + mf * angular_d_phi * lv;
+                    r.d_theta_phi = r.d_theta_phi + mf * angular_d_phi * ldt;
+                    r.d_phi_phi = r.d_phi_phi - mf * mf * angular * lv;
+                }
             }
+        }
+#pragma unroll
+        for (int m = 0; m <= l; ++m) {
+            if (m < l) {
+                prev2[m] = prev1[m];
+            }
+            prev1[m] = row[m];
         }
     }
     if (!at_pole) {

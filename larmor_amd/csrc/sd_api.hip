@@ -152,11 +152,6 @@ int check_sh_layout(int64_t lmax, bool full, int64_t n_coeffs) {
         return fail("lmax " + std::to_string(lmax) + " has " + std::to_string(expected) +
                     " coefficients, got " + std::to_string(n_coeffs));
     }
-    if (!sh_supported(static_cast<int>(lmax), full)) {
-        return fail("lmax " + std::to_string(lmax) +
-                    " is above what the GPU peak search holds (12 for a symmetric series, 8 "
-                    "for a full one)");
-    }
     return 0;
 }
 

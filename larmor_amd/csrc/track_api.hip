@@ -318,8 +318,7 @@ int lsd_track_batch(const float* fod, int64_t nx, int64_t ny, int64_t nz, int64_
         int rc = lmax_of(nc, full, &lmax);
         if (rc) return rc;
         if (!track_supported(lmax, full)) {
-            return fail("the GPU tracking kernels take lmax <= 12 (<= 8 with all degrees), got " +
-                        std::to_string(lmax));
+            return fail("the GPU tracking kernels take lmax <= 20, got " + std::to_string(lmax));
         }
     }
     if (nx > INT32_MAX || ny > INT32_MAX || nz > INT32_MAX || mx > INT32_MAX || my > INT32_MAX ||

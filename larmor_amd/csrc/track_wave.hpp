@@ -20,8 +20,9 @@
 // whose streamline has ended takes the next seed from the queue in the same pass (one atomic per
 // wave per pass, a ballot numbering the lanes that need one).  A wave exits when the queue is
 // empty and all its lanes have finished.  The coefficients interpolated at a position live in
-// registers (LMAX and FULL are template parameters), as does the Legendre table of a climb or an
-// amplitude.  The transcendental functions (acos, atan2, sin, cos) are the device library's, so
+// registers (LMAX and FULL are template parameters; the lmax-20 instantiations, which serve every
+// lmax above 12, or 8 with all degrees, keep them in the lane's private memory), and the Legendre
+// values of a climb or an amplitude are made a degree or an order at a time.  The transcendental functions (acos, atan2, sin, cos) are the device library's, so
 // det and prob match the CPU oracle to rounding; the step loop and trilinear interpolation are
 // the same arithmetic as the oracle's.
 #pragma once
@@ -177,10 +178,12 @@ __device__ __forceinline__ int clamped_index(double index, int last) {
     return static_cast<int>(index);
 }
 
-// Field::trilinear (grid.cpp:66-95) into c[0..NC).
-template <int NC>
+// Field::trilinear (grid.cpp:66-95) into c[0..nc): nc = NC, or for the wide instantiation
+// (WIDE) the image's channel count, at most NC.
+template <int NC, bool WIDE>
 __device__ __forceinline__ void trilinear(const TrackArgs& A, const double (&p)[3],
                                           double (&c)[NC]) {
+    const int nc = WIDE ? A.nc : NC;
     double v[3];
     to_voxel(A.fod_map, p, v);
     const int n[3] = {A.nx, A.ny, A.nz};
@@ -193,8 +196,12 @@ __device__ __forceinline__ void trilinear(const TrackArgs& A, const double (&p)[
         lo[a] = clamped_index(f, n[a] - 1);
         hi[a] = clamped_index(f + 1.0, n[a] - 1);
     }
+    if constexpr (WIDE) {
+        for (int q = 0; q < nc; ++q) c[q] = 0.0;
+    } else {
 #pragma unroll
-    for (int q = 0; q < NC; ++q) c[q] = 0.0;
+        for (int q = 0; q < NC; ++q) c[q] = 0.0;
+    }
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
 #pragma unroll
@@ -208,10 +215,16 @@ __device__ __forceinline__ void trilinear(const TrackArgs& A, const double (&p)[
                 const long x = i ? hi[0] : lo[0];
                 const long y = j ? hi[1] : lo[1];
                 const long z = kk ? hi[2] : lo[2];
-                const float* corner = A.fod + ((x * A.ny + y) * A.nz + z) * NC;
+                const float* corner = A.fod + ((x * A.ny + y) * A.nz + z) * nc;
+                if constexpr (WIDE) {
+                    for (int q = 0; q < nc; ++q) {
+                        c[q] = c[q] + weight * static_cast<double>(corner[q]);
+                    }
+                } else {
 #pragma unroll
-                for (int q = 0; q < NC; ++q) {
-                    c[q] = c[q] + weight * static_cast<double>(corner[q]);
+                    for (int q = 0; q < NC; ++q) {
+                        c[q] = c[q] + weight * static_cast<double>(corner[q]);
+                    }
                 }
             }
         }
@@ -247,13 +260,16 @@ __device__ __forceinline__ void rotate_to(const double (&r)[3], const double (&d
     out[2] = d[2] + along_z * (r[2] - 1.0) + along_h * (t2 - 0.0);
 }
 
-// amplitude (harmonics.cpp:187-219) of the series c along the unit vector d.
+// amplitude (harmonics.cpp:187-219) of the series c along the unit vector d: the zonal terms in
+// increasing l, then order by order (m = 1..lmax) the terms in increasing l, each order's
+// Legendre values made by its recurrence in l as the sum needs them (fill_legendre's expressions,
+// so the same values as a table).
 template <int LMAX, bool FULL, int NC>
 __device__ __forceinline__ double sh_amplitude_lane(const double (&c)[NC], const double (&d)[3],
                                                     const ShConstantsDev& k) {
+    const int lmax = series_lmax<LMAX>(k);
     const double xy = sqrt(d[0] * d[0] + d[1] * d[1]);
-    LegendreRegs<LMAX, FULL> P(k);
-    P.fill(d[2], xy, k);
+    const double cos_t = d[2];
     double cos_phi = 1.0;
     double sin_phi = 0.0;
     if (xy > 0.0) {
@@ -261,22 +277,48 @@ __device__ __forceinline__ double sh_amplitude_lane(const double (&c)[NC], const
         sin_phi = d[1] / xy;
     }
     double value = 0.0;
-#pragma unroll
-    for (int l = 0; l <= LMAX; l += FULL ? 1 : 2) {
-        value = value + c[sh_index(l, 0, FULL)] * P.at(l, 0);
-    }
     double cm = 1.0;
     double sm = 0.0;
+    auto add = [&](int l, int m, double p) {
+        if (FULL || l % 2 == 0) {
+            if (m == 0) {
+                value = value + c[sh_index(l, 0, FULL)] * p;
+            } else {
+                value = value + M_SQRT2 * p *
+                                    (c[sh_index(l, m, FULL)] * cm + c[sh_index(l, -m, FULL)] * sm);
+            }
+        }
+    };
 #pragma unroll
-    for (int m = 1; m <= LMAX; ++m) {
-        const double next_c = cm * cos_phi - sm * sin_phi;
-        const double next_s = sm * cos_phi + cm * sin_phi;
-        cm = next_c;
-        sm = next_s;
+    for (int m = 0; m <= LMAX; ++m) {
+        if (m > lmax) {
+            break;
+        }
+        if (m > 0) {
+            const double next_c = cm * cos_phi - sm * sin_phi;
+            const double next_s = sm * cos_phi + cm * sin_phi;
+            cm = next_c;
+            sm = next_s;
+        }
+        double lm2 = 0.0;
+        double lm1 = k.diagonal[m] * power_int_dev(xy, m);
+        add(m, m, lm1);
+        if (m < lmax) {
+            const double next = k.offdiag[m] * cos_t * lm1;
+            lm2 = lm1;
+            lm1 = next;
+            add(m + 1, m, lm1);
+        }
 #pragma unroll
-        for (int l = (FULL || m % 2 == 0) ? m : m + 1; l <= LMAX; l += FULL ? 1 : 2) {
-            value = value + M_SQRT2 * P.at(l, m) *
-                                (c[sh_index(l, m, FULL)] * cm + c[sh_index(l, -m, FULL)] * sm);
+        for (int l = m + 2; l <= LMAX; ++l) {
+            if (l > lmax) {
+                break;
+            }
+            const double next = k.factor[l * SH_TABLE + m] *
+                                (cos_t * lm1 - lm2 / k.factor[(l - 1) * SH_TABLE + m]);
+            lm2 = lm1;
+            lm1 = next;
+            add(l, m, lm1);
         }
     }
     return value;
@@ -300,7 +342,7 @@ __device__ __forceinline__ bool first_direction(const TrackArgs& A, const double
     } else {
         constexpr int NC = FULL ? (LMAX + 1) * (LMAX + 1) : (LMAX + 1) * (LMAX + 2) / 2;
         double c[NC];
-        trilinear<NC>(A, pos, c);
+        trilinear<NC, LMAX == SH_WIDE_LMAX>(A, pos, c);
         if constexpr (ALGO == TRACK_DET) {
             for (int it = 0; it < 20; ++it) {
                 double d[3];
@@ -357,7 +399,7 @@ __device__ __forceinline__ int next_direction(const TrackArgs& A, const double (
     } else {
         constexpr int NC = FULL ? (LMAX + 1) * (LMAX + 1) : (LMAX + 1) * (LMAX + 2) / 2;
         double c[NC];
-        trilinear<NC>(A, pos, c);
+        trilinear<NC, LMAX == SH_WIDE_LMAX>(A, pos, c);
         if constexpr (ALGO == TRACK_DET) {
             double d[3] = {dir[0], dir[1], dir[2]};
             double amp = 0.0;

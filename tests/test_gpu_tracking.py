@@ -186,7 +186,9 @@ def test_prob_matches_the_oracle_on_the_fixture():
 
 
 @pytest.mark.parametrize("kind", ["det", "prob"])
-@pytest.mark.parametrize("lmax,full", [(8, False), (4, False), (10, False), (6, True)])
+# lmax 14 and 20 symmetric and 10 full run the lmax-20 instantiations (series in private memory).
+@pytest.mark.parametrize("lmax,full", [(8, False), (4, False), (10, False), (6, True),
+                                       (14, False), (20, False), (10, True)])
 def test_synthetic_field_matches_the_oracle(kind, lmax, full):
     fod, affine = fibre_field(lmax=lmax, full=full)
     seeds = random_seeds(affine, fod.shape[:3], 600, seed=lmax + 10 * full)
@@ -252,8 +254,8 @@ def test_gpu_limits_raise():
     with pytest.raises(ValueError, match="lmax must be at most 20, and even unless full"):
         _core.track_batch(fod, np.eye(4), None, seeds, None, np.zeros(1, np.int64), 1, 1.0, 10,
                           True, _core.DetParams(0.1, 30.0), pos, 1)
-    fod = np.zeros((4, 4, 4, 120), dtype=np.float32)  # lmax 14
-    with pytest.raises(ValueError, match="the GPU tracking kernels take lmax <= 12"):
+    fod = np.zeros((4, 4, 4, 276), dtype=np.float32)  # lmax 22
+    with pytest.raises(ValueError, match="lmax must be at most 20"):
         _core.track_batch(fod, np.eye(4), None, seeds, None, np.zeros(1, np.int64), 1, 1.0, 10,
                           True, _core.DetParams(0.1, 30.0), pos, 1)
 
