@@ -125,6 +125,7 @@ void lsd_tissue_kernel_destroy(lsd_tissue_kernel* kernel);
  * Replaces: fit_single_tissue(signals, solver, n_threads)  cpp/src/python/sd.cpp:148-169,
  *           L2NnlsSolver::solve_block cpp/src/sd/solver.cpp:14-21,
  *           solve_ridge_nnls cpp/src/sd/nnls.cpp:294-350.
+ * m <= 1024 for ridge NNLS solvers (512 for the elastic-net and Richardson-Lucy ones).
  * Errors: "signals (r, c) do not have the solver's N measurements".
  */
 int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
@@ -141,8 +142,9 @@ int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
  *           fraction_tol, final_tol_x, n_threads)  cpp/src/python/sd.cpp:310-358,
  *           fit_multi_tissue_block cpp/src/sd/multi_tissue.cpp:28-100,
  *           compute_fractions cpp/src/sd/fractions.cpp:45-76.
+ * m <= 1024 with ridge NNLS solvers (512 with the others).
  * Errors: the reference's shape and kernel/solver mismatch messages; "max_outer must be at
- * least 0".
+ * least 0", "problem size exceeds 1024 measurements or 640 directions".
  */
 int lsd_fit_multi_tissue(const double* signals, int64_t n_voxels, int64_t m,
                          const double* weights, const lsd_tissue_kernel* lr,
@@ -273,7 +275,7 @@ int lsd_fit_multi_tissue_batched(const lsd_batch_job* jobs, int64_t n_jobs, int6
  * solution only when its gradient exceeds tol_x; has_tol 0 takes 10 eps norm(c, 1) max(m, n),
  * each column's absolute sum a wave sum.  Lawson-Hanson with a pivoted-QR least-squares fit per
  * active set and at most 3 n inner steps; status (nullable, device int32) receives the status
- * bits (1: the inner-step limit stopped it, the last candidate kept).  m <= 512, n <= 640.
+ * bits (1: the inner-step limit stopped it, the last candidate kept).  m <= 1024, n <= 640.
  * Replaces: lsqnonneg        cpp/src/python/sd.cpp:209-229 (module.def 471-479),
  *           solve_nnls        cpp/src/sd/nnls.cpp:258-262,
  *           solve_nnls_with_limit nnls.cpp:264-278.

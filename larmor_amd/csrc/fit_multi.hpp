@@ -146,9 +146,19 @@ inline int fit_multi_tissue_in(Workspace& mem, const real* signals, int64_t n_vo
     }
     const int ncpl_lr = ncpl_for(lr->ca);
     const int ncpl_hr = ncpl_for(hr->ca);
+#ifdef LSD_FP32
     const int rpl = rpl_for(m);
+    const bool wide_rows = false;
+#else
+    // The FP64 form with ridge NNLS solvers also takes 513 to 1,024 measurements; the mixed form,
+    // the allocation-free entry point and the other solvers stop at 512.
+    const bool wide_rows = !form.mixed && !mem.fixed() && !lr_solver->drl && !lr_solver->lasso &&
+                           !hr_solver->drl && !hr_solver->lasso;
+    const int rpl = wide_rows ? rpl_for_wide(m) : rpl_for(m);
+#endif
     if (ncpl_lr < 0 || ncpl_hr < 0 || rpl < 0) {
-        return fail("problem size exceeds 512 measurements or 640 directions");
+        return fail(std::string("problem size exceeds ") + (wide_rows ? "1024" : "512") +
+                    " measurements or 640 directions");
     }
     if (n_voxels == 0) {
         return 0;

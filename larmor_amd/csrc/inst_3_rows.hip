@@ -1,0 +1,10 @@
+// Instantiations of the FOD and fraction kernels with 3 column slot(s) per lane and 12 or 16 row
+// slots (the plain FP64 fits of 513 to 1,024 measurements).
+#include "kernels.hpp"
+
+namespace lsd {
+#ifndef LSD_FAST_BUILD
+LSD_LAUNCHER_DEF(3, 12)
+LSD_LAUNCHER_DEF(3, 16)
+#endif
+} // namespace lsd

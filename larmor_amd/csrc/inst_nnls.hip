@@ -133,22 +133,21 @@ int launch_single(const SingleNnlsArgs& args, hipStream_t stream) {
     return rc;
 }
 
-// The instantiation for (n, m): columns in 1, 5 or 10 slots per lane, rows in 4 or 8.
+// The instantiation for (n, m): columns in 1, 5 or 10 slots per lane, rows in 4, 8 or 16.
+template <int NCPL, bool RESID>
+int dispatch_rows(const SingleNnlsArgs& args, hipStream_t stream) {
+    const int m = args.design.m;
+    if (m <= 4 * WAVE) return launch_single<NCPL, 4, RESID>(args, stream);
+    if (m <= 8 * WAVE) return launch_single<NCPL, 8, RESID>(args, stream);
+    return launch_single<NCPL, 16, RESID>(args, stream);
+}
+
 template <bool RESID>
 int dispatch_single(const SingleNnlsArgs& args, hipStream_t stream) {
     const int n = args.design.n;
-    const int m = args.design.m;
-    const bool short_rows = m <= 4 * WAVE;
-    if (n <= WAVE) {
-        return short_rows ? launch_single<1, 4, RESID>(args, stream)
-                          : launch_single<1, 8, RESID>(args, stream);
-    }
-    if (n <= 5 * WAVE) {
-        return short_rows ? launch_single<5, 4, RESID>(args, stream)
-                          : launch_single<5, 8, RESID>(args, stream);
-    }
-    return short_rows ? launch_single<10, 4, RESID>(args, stream)
-                      : launch_single<10, 8, RESID>(args, stream);
+    if (n <= WAVE) return dispatch_rows<1, RESID>(args, stream);
+    if (n <= 5 * WAVE) return dispatch_rows<5, RESID>(args, stream);
+    return dispatch_rows<10, RESID>(args, stream);
 }
 
 int check_problem(const double* c, int64_t m, int64_t n, int64_t d_len, const double* d,
@@ -157,9 +156,9 @@ int check_problem(const double* c, int64_t m, int64_t n, int64_t d_len, const do
         return fail("c must be (m, n) and d (m,): got " + shape_tuple(m, n) + " and " +
                     std::to_string(d_len));
     }
-    if (m > 8 * WAVE || n > 10 * WAVE) {
+    if (m > 16 * WAVE || n > 10 * WAVE) {
         return fail("problem size " + shape_tuple(m, n) +
-                    " exceeds 512 measurements or 640 columns");
+                    " exceeds 1024 measurements or 640 columns");
     }
     if ((m * n > 0 && c == nullptr) || (m > 0 && d == nullptr) || (n > 0 && x == nullptr)) {
         return fail("c, d and x must be device arrays of the given sizes");

@@ -259,8 +259,10 @@ struct FodArgs {
 #define LSD_FOD_LR_WAVES 8
 #endif
 constexpr int lr_waves_for(int kcap) { return kcap <= 20 ? LSD_FOD_LR_WAVES : (kcap <= 24 ? 7 : 6); }
-constexpr int fod_waves_per_eu(int ncpl, int kcap, bool global_l) {
-    return (!global_l && ncpl == 1) ? lr_waves_for(kcap) : 1;
+// More than 8 row slots (above 512 measurements) hold the signal in 12 or 16 registers a lane,
+// so those solves take two waves per SIMD and the registers that gives.
+constexpr int fod_waves_per_eu(int ncpl, int kcap, bool global_l, int rpl) {
+    return (!global_l && ncpl == 1) ? (rpl > 8 ? 2 : lr_waves_for(kcap)) : 1;
 }
 
 // WARM (the mixed-precision form's low-resolution solves, NCPL 1): each voxel's solve starts from
@@ -269,7 +271,7 @@ constexpr int fod_waves_per_eu(int ncpl, int kcap, bool global_l) {
 // design; precomputed correlations are indexed by batch voxel number.
 template <int NCPL, int RPL, int KCAP, bool GLOBAL_L, bool WARM = false, bool BATCHED = false>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
-    __attribute__((amdgpu_waves_per_eu(fod_waves_per_eu(NCPL, KCAP, GLOBAL_L))))
+    __attribute__((amdgpu_waves_per_eu(fod_waves_per_eu(NCPL, KCAP, GLOBAL_L, RPL))))
     fod_kernel(FodArgs A) {
     constexpr int RPLQ = (RPL * WAVE + KCAP + WAVE - 1) / WAVE;
     extern __shared__ real smem[];
@@ -576,7 +578,8 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     const int slab = NORMAL ? 0 : qcols * RPL * WAVE;
     real* work = smem + w * slab;
     // The block's copy of the isotropic columns, after the waves' QR slabs (BATCHED: the wave's).
-    real* iso_lds = smem + WAVES_PER_BLOCK * slab + (BATCHED ? w * A.kernel.ci * RPL * WAVE : 0);
+    real* iso_lds = smem + static_cast<int>(blockDim.x / WAVE) * slab +
+                    (BATCHED ? w * A.kernel.ci * RPL * WAVE : 0);
     long count = 0;
     JobSpace js{};
     if constexpr (BATCHED) {
@@ -866,6 +869,33 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
     return launch(rerun_name, Family<NCPL, RPL, KG, true>::kernel, rerun, rerun_waves, 0, stream);
 }
 
+// A fraction launch: four waves to a block, each with its QR slab in the block's LDS, or for 12
+// and 16 row slots (more than 512 measurements) one wave to a block, so that a block's LDS stays
+// within 64 KB.
+template <int NC, int RP, bool FINAL>
+int launch_fraction_rows(const char* name, const FractionArgs& args, int waves, hipStream_t stream) {
+    const int qcols = (args.final_fit ? args.kernel.n_blocks : 1) + args.kernel.ci;
+    const int per_block = RP <= 8 ? WAVES_PER_BLOCK : 1;
+    const size_t lds = sizeof(real) * (qcols * per_block + args.kernel.ci) * RP * WAVE;
+    auto kern = fraction_kernel<NC, RP, FINAL>;
+    if (per_block == WAVES_PER_BLOCK) {
+        return launch_fill(name, kern, args, waves, lds, stream);
+    }
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, LAUNCH_WAVE, lds) == hipSuccess &&
+        per_cu > 0) {
+        int dev = 0;
+        int cus = 0;
+        LSD_HIP(hipGetDevice(&dev));
+        LSD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const long resident = static_cast<long>(per_cu) * cus;
+        if (resident > 0 && resident < waves) {
+            waves = static_cast<int>(resident);
+        }
+    }
+    return launch(name, kern, args, waves, lds, stream, 1);
+}
+
 // Launchers of one (NCPL, RPL) instantiation, defined in its own translation unit.
 // Names label the launches in the timing records (string literals: they are kept).
 #define LSD_LAUNCHER_DECL(NC, RP)                                                              \
@@ -883,11 +913,8 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
     }                                                                                          \
     int launch_fraction_##NC##_##RP(const char* name, const FractionArgs& args, int waves,      \
                                     hipStream_t stream) {                                      \
-        const int qcols = (args.final_fit ? args.kernel.n_blocks : 1) + args.kernel.ci;        \
-        const size_t lds = sizeof(real) * (qcols * WAVES_PER_BLOCK + args.kernel.ci) * RP * WAVE; \
-        return args.final_fit                                                                  \
-                   ? launch_fill(name, fraction_kernel<NC, RP, true>, args, waves, lds, stream)   \
-                   : launch_fill(name, fraction_kernel<NC, RP, false>, args, waves, lds, stream); \
+        return args.final_fit ? launch_fraction_rows<NC, RP, true>(name, args, waves, stream)  \
+                              : launch_fraction_rows<NC, RP, false>(name, args, waves, stream); \
     }
 
 // Every instantiated (NCPL, RPL): X(NC, RP).
@@ -897,7 +924,7 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
 #elif defined(LSD_FAST_BUILD)
 #define LSD_FOR_EACH_LAUNCHER(X) X(1, 4) X(5, 4)
 #else
-#define LSD_FOR_EACH_RPL(X, NC) X(NC, 4) X(NC, 6) X(NC, 8)
+#define LSD_FOR_EACH_RPL(X, NC) X(NC, 4) X(NC, 6) X(NC, 8) X(NC, 12) X(NC, 16)
 #define LSD_FOR_EACH_LAUNCHER(X)                                                               \
     LSD_FOR_EACH_RPL(X, 1) LSD_FOR_EACH_RPL(X, 3) LSD_FOR_EACH_RPL(X, 5) LSD_FOR_EACH_RPL(X, 10)
 #endif

@@ -145,6 +145,16 @@ inline int rpl_for(int64_t m) {
     if (m <= 512) return 8;
     return -1;
 }
+// Row slots of the plain FP64 fits, which also take up to 768 and 1,024 measurements (12 and 16
+// slots); the other forms stop at 512.
+constexpr int64_t MAX_MEASUREMENTS_WIDE = 1024;
+inline int rpl_for_wide(int64_t m) {
+    const int r = rpl_for(m);
+    if (r > 0) return r;
+    if (m <= 768) return 12;
+    if (m <= MAX_MEASUREMENTS_WIDE) return 16;
+    return -1;
+}
 
 // Grid of the main launch: enough waves to fill the chip, never more than voxels.
 inline int main_waves(int64_t n_vox) {

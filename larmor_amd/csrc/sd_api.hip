@@ -690,10 +690,10 @@ int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
         return run_lasso("lasso_single", solver, la, n_voxels, stream);
     }
     const int ncpl = ncpl_for(solver->n);
-    const int rpl = rpl_for(m);
+    const int rpl = rpl_for_wide(m);
     if (ncpl < 0 || rpl < 0) {
         return fail("problem size " + shape_tuple(m, solver->n) +
-                    " exceeds 512 measurements or 640 directions");
+                    " exceeds 1024 measurements or 640 directions");
     }
     if (n_voxels == 0) {
         return 0;
@@ -874,7 +874,7 @@ int64_t lsd_tuned_lr_capacity(int64_t m) {
 }
 
 int64_t lsd_fit_scratch_bytes(int64_t m, int64_t n_lr, int64_t n_hr) {
-    const int rpl = rpl_for(m);
+    const int rpl = rpl_for_wide(m);
     const int64_t sizes[2] = {n_lr, n_hr};
     int64_t total = 0;
     for (const int64_t n : sizes) {

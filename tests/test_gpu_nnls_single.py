@@ -76,8 +76,8 @@ def test_lsqnonneg_dtype_and_layout():
         _core.lsqnonneg(np.ones((32, 3), dtype=np.float32), np.ones(32))
     with pytest.raises(ValueError, match="C-contiguous"):
         _core.lsqnonneg(np.asfortranarray(np.ones((32, 3))), np.ones(32))
-    with pytest.raises(ValueError, match="exceeds"):
-        _core.lsqnonneg(np.ones((513, 3)), np.ones(513))
+    with pytest.raises(ValueError, match="exceeds 1024 measurements"):
+        _core.lsqnonneg(np.ones((1025, 3)), np.ones(1025))
 
 
 def test_lsqnonneg_empty_problems():

@@ -153,14 +153,14 @@ def test_request_errors():
         with pytest.raises(ValueError, match="lmax must be even"):
             server.submit(job.signals, gtab, lmax=7)
         assert server.submit(job.signals, gtab).result(timeout=60).n_voxels == 100
-        # A request the kernels refuse (more than 512 measurements) fails alone: its future
+        # A request the kernels refuse (more than 1024 measurements) fails alone: its future
         # holds the library's error and the next request is served.
         rng = np.random.default_rng(62)
-        big_b = np.concatenate([np.zeros(10), np.full(510, 1000.0)])
-        big_v = np.vstack([np.zeros((10, 3)), synthetic.spiral_directions(510)])
+        big_b = np.concatenate([np.zeros(10), np.full(1020, 1000.0)])
+        big_v = np.vstack([np.zeros((10, 3)), synthetic.spiral_directions(1020)])
         big = synthetic.tissue_signals(20, big_b, big_v, rng, snr=25.0)
         refused = server.submit(big, GradientTable(big_b, big_v))
-        with pytest.raises(ValueError, match="512 measurements"):
+        with pytest.raises(ValueError, match="1024 measurements"):
             refused.result(timeout=60)
         assert server.submit(job.signals, gtab).result(timeout=60).n_voxels == 100
     with pytest.raises(RuntimeError, match="closed"):
