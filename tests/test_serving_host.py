@@ -125,7 +125,8 @@ def test_request_bytes_covers_the_library_rerun_scratch():
     assert request_bytes(_model(), 0, 202, 8, False) >= scratch
     assert request_bytes(_model(multi=False), 0, 202, 8, False) >= lib.lsd_fit_scratch_bytes(
         202, 0, 300)
-    assert lib.lsd_fit_scratch_bytes(600, 44, 300) == -1
+    assert lib.lsd_fit_scratch_bytes(600, 44, 300) > scratch  # 12 row slots: larger slabs
+    assert lib.lsd_fit_scratch_bytes(1100, 44, 300) == -1
 
 
 class _Fut:
