@@ -562,6 +562,61 @@ def rows_report() -> int:
                                                n_threads=threads), len(psample), threads)
     report("peaks (a11)", f"{n_vox} lmax-8 FODs, 3 peaks, 642-vertex sphere", n_vox, t, cpu, kt)
 
+    # a9 past lmax 8: CSD at lmax 12 (91 coefficients, the system's triangle in LDS) on a b = 3000
+    # shell of 90 directions.
+    hph = synthetic.phantom(shape=(64, 64, 40), n_b0=5, shells=[(3000.0, 90)], seed=1, snr=20.0,
+                            mask_share=None, tissues=False)
+    h_on = hph.bvals > 50
+    hs = np.ascontiguousarray(
+        np.maximum(hph.data.reshape(-1, len(hph.bvals))[:, h_on], 0.0), dtype=np.float64)
+    hp = csd_problem(hph.bvecs[h_on], synthetic.tensor_zonal(3000.0, lmax=12), 12, CSDSolver())
+    hdev = torch.from_numpy(hs).to("cuda")
+    t, _, kt = profiled(lambda: _core.fit_csd(hdev, hp.initial, hp.forward, hp.constraint,
+                                              hp.norm_weight, hp.threshold, hp.max_iter, 1))
+    hsample = np.ascontiguousarray(hs[rng.choice(len(hs), 1024, replace=False)])
+    cpu = _cpu_rate(lambda: oracle.fit_csd(hsample, hp.initial, hp.forward, hp.constraint,
+                                           hp.norm_weight, hp.threshold, hp.max_iter,
+                                           n_threads=threads), len(hsample), threads)
+    report("CSD lmax 12 (a9, wide kernel)", f"{len(hs)} voxels x {hs.shape[1]} measurements, "
+           "lmax 12 (91 coefficients), 300 constraints", len(hs), t, cpu, kt)
+
+    # a10 past 63 parameters: MSMT-CSD with WM lmax 10 (66 + 2 parameters, two lane slots).
+    wrfs = [ZonalSHRF(bvals=shells, coefficients=np.array(
+                [synthetic.tensor_zonal(b, 1.7e-3, 0.3e-3, lmax=10) for b in shells])),
+            zonal_iso(0.8e-3), zonal_iso(3.0e-3)]
+    wforward = forward_matrix(wrfs, [10, 0, 0], ph.bvals, ph.bvecs)
+    wg = constraint_matrix([10, 0, 0], DIRS_300)
+    wdesign = _core.LsiDesign(np.ascontiguousarray(wforward), np.ascontiguousarray(wg),
+                              msmt_ridge(wforward))
+    t, _, kt = profiled(lambda: _core.fit_lsi(dev, wdesign, 1))
+    wsample = sample[:512]
+    cpu = _cpu_rate(lambda: oracle.fit_lsi(wsample, wforward, wg, msmt_ridge(wforward),
+                                           n_threads=threads), len(wsample), threads)
+    report("MSMT-CSD WM lmax 10 (a10, 68 parameters)", vol + ", 68 parameters, 302 constraints",
+           n_vox, t, cpu, kt)
+
+    # a5 past 512 measurements: the multi-tissue fit at 10 b0 + 3 x 200 (12 row slots).
+    mph = synthetic.phantom(shape=(64, 64, 40), n_b0=10,
+                            shells=[(1000.0, 200), (2000.0, 200), (3000.0, 200)], seed=4,
+                            snr=25.0, mask_share=0.40, tissues=True)
+    msig = synthetic.normalized_signals(mph.data, mph.bvals, mph.mask)
+    mw = np.ones(len(mph.bvals))
+    mhr = Kernel(rfs, mph.bvals, mph.bvecs, dirs=DIRS_300, weights=mw)
+    mlr = Kernel(rfs, mph.bvals, mph.bvecs, dirs=low_resolution_dirs(rfs, len(mph.bvals)),
+                 weights=mw)
+    mlr_s = _core.FodSolver.l2nnls(np.ascontiguousarray(mlr.w_k_aniso), 0.0, 1e-6)
+    mhr_s = _core.FodSolver.l2nnls(np.ascontiguousarray(mhr.w_k_aniso), 0.1, 1e-6)
+    mdev = torch.from_numpy(msig).to("cuda")
+    mwt = torch.from_numpy(mw).to("cuda")
+    t, _, kt = profiled(lambda: _core.fit_multi_tissue(
+        mdev, mwt, _deconv_kernel(mlr), mlr_s, _deconv_kernel(mhr), mhr_s, 50, 1e-7, 1e-2, 1))
+    msample = np.ascontiguousarray(msig[rng.choice(len(msig), 1024, replace=False)])
+    cpu = _cpu_rate(lambda: oracle.fit_multi_tissue(msample, mw, mlr, mhr, 0.0, 0.1, 1e-6,
+                                                    n_threads=threads), len(msample), threads)
+    report("mFOD at 610 measurements (a5, 12 row slots)",
+           f"{len(msig)} voxels x {msig.shape[1]} measurements, 44 LR / 300 HR directions",
+           len(msig), t, cpu, kt)
+
     # Response estimation (estimate_csd_rf): the CSD phantom above, every voxel searched. The
     # CPU side runs the same host code with the oracle's CSD fits and peak searches.
     from larmor_amd.dwi.inputs import signal_rows

@@ -210,29 +210,23 @@ __global__ void __launch_bounds__(WAVE * 4) csd_kernel(CsdArgs A) {
     }
 }
 
-// The wide form: any number of coefficients N, one voxel per single-wave workgroup.  The packed
-// lower triangle of the system (column-major, csd_lds_doubles(N) doubles) lives in the
+// The wide form: any number of coefficients N, one voxel per workgroup of CSD_WIDE_WAVES waves.
+// The packed lower triangle of the system (column-major, csd_lds_doubles(N) doubles) lives in the
 // workgroup's LDS, or, when it does not fit there (GLOBAL_SYS), in the workgroup's own slab of
 // global memory; the FOD, F^T s, the solve's right-hand side and the constrained directions'
-// list live in LDS.  Each step distributes rows over the lanes and gives every entry the same
-// terms in the same order as csd_kernel and the oracle:
+// list live in LDS.  Each step gives every entry the same terms in the same order as csd_kernel
+// and the oracle:
 //  - system: entry (i, j), i >= j, is G_ij plus c_d[i] c_d[j] over the constrained d in
-//    increasing d;
+//    increasing d (columns j over the waves, rows i over the lanes);
 //  - Cholesky, right-looking: at step j the column below the diagonal is divided by
-//    sqrt(A_jj), then every trailing entry (i, q) loses L_ij L_qj, so entry (i, q) receives its
-//    terms in increasing j, as the left-looking oracle subtracts them;
-//  - forward and backward solves column-oriented (earlier, then later positions first).
-// Between steps the wave synchronises its memory: LDS at wavefront scope, the global slab at
-// workgroup scope (the workgroup is the one wave).
-template <bool GLOBAL_SYS>
-__device__ __forceinline__ void csd_wide_sync() {
-    if constexpr (GLOBAL_SYS) {
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-    } else {
-        wave_lds_sync();
-    }
-}
+//    sqrt(A_jj), then every trailing entry (i, q) loses L_ij L_qj (columns q over the waves), so
+//    entry (i, q) receives its terms in increasing j, as the left-looking oracle subtracts them;
+//  - forward and backward solves column-oriented (earlier, then later positions first), on the
+//    first wave.
+// Every wave evaluates the constraint amplitudes and the constrained set itself (the same
+// arithmetic, so the same decisions); the workgroup synchronises between steps (its barrier also
+// orders the global slab's accesses at workgroup scope).
+constexpr int CSD_WIDE_WAVES = 4;
 
 // LDS bytes of a csd_wide_kernel workgroup: the triangle (unless in global memory), three
 // vectors of N doubles and k direction indices.
@@ -242,7 +236,7 @@ inline size_t csd_wide_lds_bytes(int n, int k, bool global_sys) {
 }
 
 template <int NK, bool GLOBAL_SYS>
-__global__ void __launch_bounds__(WAVE) csd_wide_kernel(CsdArgs A) {
+__global__ void __launch_bounds__(WAVE * CSD_WIDE_WAVES) csd_wide_kernel(CsdArgs A) {
     extern __shared__ double smem[];
     const int n = A.n;
     const long tri = static_cast<long>(n) * (n + 1) / 2;
@@ -252,11 +246,15 @@ __global__ void __launch_bounds__(WAVE) csd_wide_kernel(CsdArgs A) {
     double* b = proj + n;
     int* list = reinterpret_cast<int*>(b + n);
     const int l = lane_id();
+    const int w = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) / WAVE);
+    const int n_waves = static_cast<int>(blockDim.x) / WAVE;
+    const int tid = static_cast<int>(threadIdx.x);
+    const int n_threads = static_cast<int>(blockDim.x);
     auto column = [n](int i) { return static_cast<long>(i) * n - static_cast<long>(i) * (i - 1) / 2; };
 
     for (long v = blockIdx.x; v < A.n_vox; v += gridDim.x) {
         const double* s = A.signals + v * A.m;
-        for (int i = l; i < n; i += WAVE) {
+        for (int i = tid; i < n; i += n_threads) {
             double p = 0.0;
             double x = 0.0;
             for (int r = 0; r < A.m; ++r) {
@@ -269,7 +267,7 @@ __global__ void __launch_bounds__(WAVE) csd_wide_kernel(CsdArgs A) {
             proj[i] = p;
             f[i] = x;
         }
-        wave_lds_sync();
+        __syncthreads();
         bool first = true;
         bool converged = A.max_iter == 0;
         int solves = 0;
@@ -303,7 +301,7 @@ __global__ void __launch_bounds__(WAVE) csd_wide_kernel(CsdArgs A) {
                 const bool on = d < A.k && a[t] < A.threshold;
                 mask[t] = ballot(on);
                 same = same && mask[t] == prev[t];
-                if (on) {
+                if (on && w == 0) {
                     list[n_c + __popcll(mask[t] & ((1ull << l) - 1ull))] = d;
                 }
                 n_c += __popcll(mask[t]);
@@ -313,9 +311,9 @@ __global__ void __launch_bounds__(WAVE) csd_wide_kernel(CsdArgs A) {
                 break;
             }
             first = false;
-            wave_lds_sync();
+            __syncthreads();
 
-            for (int j = 0; j < n; ++j) {
+            for (int j = w; j < n; j += n_waves) {
                 const long cj = column(j);
                 for (int i = j + l; i < n; i += WAVE) {
                     double acc = A.gram[static_cast<long>(i) * n + j];
@@ -326,78 +324,81 @@ __global__ void __launch_bounds__(WAVE) csd_wide_kernel(CsdArgs A) {
                     sys[cj + (i - j)] = acc;
                 }
             }
-            csd_wide_sync<GLOBAL_SYS>();
+            __syncthreads();
 
             for (int j = 0; j < n; ++j) {
                 const long cj = column(j);
                 const double ljj = sqrt(sys[cj]);
-                for (int i = j + 1 + l; i < n; i += WAVE) {
+                __syncthreads();
+                for (int i = j + 1 + tid; i < n; i += n_threads) {
                     sys[cj + (i - j)] = sys[cj + (i - j)] / ljj;
                 }
-                if (l == 0) {
+                if (tid == 0) {
                     sys[cj] = ljj;
                 }
-                csd_wide_sync<GLOBAL_SYS>();
-                for (int q = j + 1; q < n; ++q) {
+                __syncthreads();
+                for (int q = j + 1 + w; q < n; q += n_waves) {
                     const double lqj = sys[cj + (q - j)];
                     const long cq = column(q);
                     for (int i = q + l; i < n; i += WAVE) {
                         sys[cq + (i - q)] = sys[cq + (i - q)] - sys[cj + (i - j)] * lqj;
                     }
                 }
-                csd_wide_sync<GLOBAL_SYS>();
+                __syncthreads();
             }
 
-            // L y = F^T s, column-oriented.
-            for (int i = l; i < n; i += WAVE) {
-                b[i] = proj[i];
-            }
-            wave_lds_sync();
-            for (int j = 0; j < n; ++j) {
-                const long cj = column(j);
-                const double bj = b[j];
-                const double yj = bj != 0.0 ? bj / sys[cj] : bj;
-                for (int i = j + 1 + l; i < n; i += WAVE) {
-                    b[i] = b[i] - sys[cj + (i - j)] * yj;
-                }
-                if (l == 0) {
-                    b[j] = yj;
+            if (w == 0) {
+                // L y = F^T s, column-oriented.
+                for (int i = l; i < n; i += WAVE) {
+                    b[i] = proj[i];
                 }
                 wave_lds_sync();
-            }
-            // L^T f = y, column-oriented from the last position.
-            for (int j = n - 1; j >= 0; --j) {
-                const double yj = b[j];
-                const double xj = yj != 0.0 ? yj / sys[column(j)] : yj;
-                for (int i = l; i < j; i += WAVE) {
-                    b[i] = b[i] - sys[column(i) + (j - i)] * xj;
+                for (int j = 0; j < n; ++j) {
+                    const long cj = column(j);
+                    const double bj = b[j];
+                    const double yj = bj != 0.0 ? bj / sys[cj] : bj;
+                    for (int i = j + 1 + l; i < n; i += WAVE) {
+                        b[i] = b[i] - sys[cj + (i - j)] * yj;
+                    }
+                    if (l == 0) {
+                        b[j] = yj;
+                    }
+                    wave_lds_sync();
                 }
-                if (l == 0) {
-                    b[j] = xj;
+                // L^T f = y, column-oriented from the last position.
+                for (int j = n - 1; j >= 0; --j) {
+                    const double yj = b[j];
+                    const double xj = yj != 0.0 ? yj / sys[column(j)] : yj;
+                    for (int i = l; i < j; i += WAVE) {
+                        b[i] = b[i] - sys[column(i) + (j - i)] * xj;
+                    }
+                    if (l == 0) {
+                        b[j] = xj;
+                    }
+                    wave_lds_sync();
                 }
-                wave_lds_sync();
+                for (int i = l; i < n; i += WAVE) {
+                    f[i] = b[i];
+                }
             }
-            for (int i = l; i < n; i += WAVE) {
-                f[i] = b[i];
-            }
-            wave_lds_sync();
+            __syncthreads();
             solves += 1;
 #pragma unroll
             for (int t = 0; t < NK; ++t) {
                 prev[t] = mask[t];
             }
         }
-        for (int i = l; i < n; i += WAVE) {
+        for (int i = tid; i < n; i += n_threads) {
             A.fods[v * n + i] = f[i];
         }
-        if (l == 0) {
+        if (tid == 0) {
             A.converged[v] = converged ? 1 : 0;
             if (A.solves) {
                 A.solves[v] = solves;
             }
         }
         // The next voxel overwrites the vectors, the list and the triangle.
-        csd_wide_sync<GLOBAL_SYS>();
+        __syncthreads();
     }
 }
 

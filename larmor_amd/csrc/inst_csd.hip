@@ -25,8 +25,9 @@ int launch_csd_k(int nk, const CsdArgs& args, int waves, hipStream_t stream) {
     }
 }
 
-// The wide form's grid: a workgroup per voxel, at most what the device keeps resident (and, with
-// the system in global memory, at most 4 per CU, each with its own slab).
+// The wide form's grid: a workgroup of CSD_WIDE_WAVES waves per voxel, at most what the device
+// keeps resident (and, with the system in global memory, at most 4 per CU, each with its own
+// slab).
 template <int NK, bool GLOBAL_SYS>
 int launch_csd_wide_g(const CsdArgs& args, hipStream_t stream) {
     const size_t lds = csd_wide_lds_bytes(args.n, args.k, GLOBAL_SYS);
@@ -36,7 +37,8 @@ int launch_csd_wide_g(const CsdArgs& args, hipStream_t stream) {
     LSD_HIP(hipGetDevice(&dev));
     LSD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     int per_cu = 0;
-    LSD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, LAUNCH_WAVE, lds));
+    LSD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, LAUNCH_WAVE * CSD_WIDE_WAVES,
+                                                         lds));
     if (per_cu <= 0) {
         return fail("the CSD system of " + std::to_string(args.n) + " coefficients does not fit a workgroup");
     }
@@ -53,7 +55,8 @@ int launch_csd_wide_g(const CsdArgs& args, hipStream_t stream) {
         if (rc) return rc;
         a.sys_slabs = static_cast<double*>(slabs);
     }
-    int rc = launch("csd_wide", kern, a, static_cast<int>(blocks), lds, stream, 1);
+    int rc = launch("csd_wide", kern, a, static_cast<int>(blocks) * CSD_WIDE_WAVES, lds, stream,
+                    CSD_WIDE_WAVES);
     if (GLOBAL_SYS) {
         const hipError_t e = hipFreeAsync(a.sys_slabs, stream);
         if (!rc && e != hipSuccess) rc = hip_fail(e, "hipFreeAsync");
