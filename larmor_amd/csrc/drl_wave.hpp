@@ -9,7 +9,8 @@
 //              1e-6),
 // stopping once the FOD changed in total by no more than the threshold.  Every sum (the mean and
 // sum of squares of sd, each K^T s and f K^T K entry, the change) runs in index order, one lane
-// per voxel, so the result is the oracle's to the bit.
+// per voxel, the K^T s and f K^T K entries by fused multiply-adds as the oracle accumulates them,
+// so the result is the oracle's to the bit.
 //
 // The per-voxel vectors live in HBM by list position, position-minor ((index, ld) layouts), so
 // a wave's 64 lanes read and write 512 contiguous bytes per index; the design rows are
@@ -144,7 +145,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) drl_kernel(DrlArgs A) 
                 const ConstRow krow = (ConstRow)(A.kernel + static_cast<long>(r) * A.c + j0);
 #pragma unroll
                 for (int jj = 0; jj < NCB; ++jj) {
-                    acc[jj] = acc[jj] + ar * krow[jj];
+                    acc[jj] = __fma_rn(ar, krow[jj], acc[jj]);
                 }
             }
             if (on) {
@@ -189,7 +190,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) drl_kernel(DrlArgs A) 
                             (ConstRow)(A.gram + static_cast<long>(k + u) * A.c + j0);
 #pragma unroll
                         for (int jj = 0; jj < NCB; ++jj) {
-                            acc[jj] = acc[jj] + fq[u] * grow[jj];
+                            acc[jj] = __fma_rn(fq[u], grow[jj], acc[jj]);
                         }
                     }
                 }
@@ -198,7 +199,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) drl_kernel(DrlArgs A) 
                     const ConstRow grow = (ConstRow)(A.gram + static_cast<long>(k) * A.c + j0);
 #pragma unroll
                     for (int jj = 0; jj < NCB; ++jj) {
-                        acc[jj] = acc[jj] + fk * grow[jj];
+                        acc[jj] = __fma_rn(fk, grow[jj], acc[jj]);
                     }
                 }
                 if (active) {

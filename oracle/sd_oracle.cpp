@@ -841,10 +841,13 @@ Vec drl_voxel(const DrlSolver& d, const Vec& signal, Index* iterations, bool* co
     }
     const double threshold = DRL_CONVERGENCE_SHARE * largest;
     const double nu8 = power_8(d.nu);
+    // K^T s and f K^T K: each entry accumulated in index order with fused multiply-adds (one
+    // rounding per term; the restatement's choice -- Eigen's products in the reference block
+    // their sums their own way).
     Vec ks(static_cast<size_t>(c), 0.0);
     for (Index j = 0; j < c; ++j) {
         double acc = 0.0;
-        for (Index i = 0; i < m; ++i) acc += signal[i] * d.kernel_t(j, i);
+        for (Index i = 0; i < m; ++i) acc = std::fma(signal[i], d.kernel_t(j, i), acc);
         ks[j] = acc;
     }
     Vec f(static_cast<size_t>(c), d.init);
@@ -855,7 +858,7 @@ Vec drl_voxel(const DrlSolver& d, const Vec& signal, Index* iterations, bool* co
     for (; it < d.n_iter && !done; ++it) {
         for (Index j = 0; j < c; ++j) {
             double acc = 0.0;
-            for (Index k = 0; k < c; ++k) acc += f[k] * d.gram(k, j);
+            for (Index k = 0; k < c; ++k) acc = std::fma(f[k], d.gram(k, j), acc);
             predicted[j] = acc;
         }
         double change = 0.0;
