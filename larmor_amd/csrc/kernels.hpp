@@ -55,7 +55,7 @@ constexpr int lds_factor_doubles(int kcap, int /*rpl*/) {
 // Device helpers
 // ---------------------------------------------------------------------------------------------
 
-// corr_c = sum over rows, in order, of K(r, c) a_r, minus l1 (nnls.cpp:302).
+// corr_c = sum over rows, in order, of K(r, c) a_r by fused multiply-adds, minus l1 (nnls.cpp:302).
 template <int NCPL, int RPL>
 __device__ __forceinline__ void correlations(const RidgeDesignDev& D, const RowVec<RPL>& a, real l1,
                              real (&corr)[NCPL]) {
@@ -76,7 +76,7 @@ __device__ __forceinline__ void correlations(const RidgeDesignDev& D, const RowV
             for (int t = 0; t < NCPL; ++t) {
                 const int c = l + WAVE * t;
                 if (c < D.n) {
-                    corr[t] = corr[t] + krow[c] * ar;
+                    corr[t] = fma(krow[c], ar, corr[t]);
                 }
             }
         }
@@ -507,7 +507,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
                     // Every lane reads the same kernel entries: LDS broadcasts.
 #pragma unroll
                     for (int j = 0; j < NCB; ++j) {
-                        acc[j] = acc[j] + ktile[rr][j] * ar;
+                        acc[j] = fma(ktile[rr][j], ar, acc[j]);
                     }
                 }
                 __syncthreads();

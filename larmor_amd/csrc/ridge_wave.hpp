@@ -175,11 +175,11 @@ struct RidgeNnlsWave {
             }
             const real bj = b.get(j);
             const real wj = j >= p0 ? divide_by_diag<FAST>(bj, d) : bj;
-            sq = sq + wj * wj;
+            sq = fma(wj, wj, sq);
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
                 const int i = l + WAVE * u;
-                const real updated = b.v[u] - lc[u] * wj;
+                const real updated = fma(-lc[u], wj, b.v[u]);
                 const bool below = i > j && i < kk && i >= p0;
                 b.v[u] = i == j ? wj : (below ? updated : b.v[u]);
             }
@@ -250,7 +250,7 @@ struct RidgeNnlsWave {
 #pragma unroll
             for (int u = 0; u < NSK; ++u) {
                 const int i = l + WAVE * u;
-                const real updated = b.v[u] - lc[u] * zj;
+                const real updated = fma(-lc[u], zj, b.v[u]);
                 b.v[u] = i == j ? zj : (i < j ? updated : b.v[u]);
             }
         }
@@ -321,16 +321,12 @@ struct RidgeNnlsWave {
         }
         const bool lkk_ok = divisor_in_range(lkk);
         if (y_valid == k) {
-            // y_k = (corr_c - L_k0 y_0 - L_k1 y_1 - ...) / L_kk, subtracted in order.
+            // y_k = (corr_c - L_k0 y_0 - L_k1 y_1 - ...) / L_kk, each term a fused
+            // multiply-subtract in order, as the column-oriented solve applies them.
             const real cc = uniform_col(corr, c);
-            LaneVec<NSK> prod;
-#pragma unroll
-            for (int u = 0; u < NSK; ++u) {
-                prod.v[u] = b.v[u] * y.v[u];
-            }
             real v = cc;
             for (int j = 0; j < k; ++j) {
-                v = v - prod.get(j);
+                v = fma(-b.get(j), y.get(j), v);
             }
             y.set(k, divide_nonzero_by(v, lkk, rkk, lkk_ok));
             y_valid = k + 1;
@@ -785,7 +781,7 @@ struct RidgeNnlsWave {
 #pragma unroll
                 for (int t = 0; t < NCPL; ++t) {
                     if (col(t) < D.n) {
-                        g[t] = g[t] - xc * grow[col(t)];
+                        g[t] = fma(-xc, grow[col(t)], g[t]);
                     }
                 }
             }

@@ -550,6 +550,9 @@ void remove_cholesky_column(Mat& factor, Index n_columns, Index position) {
     }
 }
 
+// The ridge solve's growing factor: its triangular solves and the new row's squared norm
+// accumulate by fused multiply-adds in order (one rounding per term), as do the correlations
+// and the gradient of solve_ridge_nnls.
 class ActiveCholesky {
 public:
     explicit ActiveCholesky(const Mat& g)
@@ -585,7 +588,7 @@ public:
                 w[j] = w[j] / factor(j, j);
             }
             for (Index i = 0; i < j; ++i) {
-                w[i] -= factor(j, i) * w[j];
+                w[i] = std::fma(-factor(j, i), w[j], w[i]);
             }
         }
         Vec full(rhs.size(), 0.0);
@@ -605,7 +608,7 @@ private:
                 w[j] = w[j] / factor(j, j);
             }
             for (Index i = j + 1; i < k; ++i) {
-                w[i] -= factor(i, j) * w[j];
+                w[i] = std::fma(-factor(i, j), w[j], w[i]);
             }
         }
     }
@@ -619,7 +622,7 @@ private:
         forward(cross, pos);
         double sq = 0.0;
         for (Index r = 0; r < pos; ++r) {
-            sq += cross[r] * cross[r];
+            sq = std::fma(cross[r], cross[r], sq);
         }
         const double pivot = gram(column, column) - sq;
         if (!(pivot > MIN_RELATIVE_PIVOT * gram(column, column))) {
@@ -651,7 +654,7 @@ Vec solve_ridge_nnls(const RidgeDesign& design, const Vec& signal, double l1, do
     for (Index c = 0; c < n; ++c) {
         double s = 0.0;
         for (Index i = 0; i < m; ++i) {
-            s += design.kernel(i, c) * signal[i];
+            s = std::fma(design.kernel(i, c), signal[i], s);
         }
         correlations[c] = s - l1;
     }
@@ -661,7 +664,7 @@ Vec solve_ridge_nnls(const RidgeDesign& design, const Vec& signal, double l1, do
             if (x[c] != 0.0) {
                 const double xc = x[c];
                 for (Index i = 0; i < n; ++i) {
-                    g[i] -= xc * design.gram(i, c);
+                    g[i] = std::fma(-xc, design.gram(i, c), g[i]);
                 }
             }
         }
