@@ -111,8 +111,8 @@ int lsd_solver_normal_equations(const lsd_solver* solver);
  * the anisotropic blocks as (start, stop) pairs (n_blocks x 2).
  * Replaces: DeconvKernel(k_aniso, k_iso, w_k_iso, aniso_blocks)  cpp/src/python/sd.cpp:261-294
  *           (construct_kernel), TissueKernel cpp/src/sd/fractions.hpp:27-38.
- * Errors: "aniso_blocks must be non-empty ranges within 0..ca, got [...]"; at most 4 fraction
- * columns (blocks + isotropic tissues).
+ * Errors: "aniso_blocks must be non-empty ranges within 0..ca, got [...]"; at most 3 anisotropic
+ * blocks and 6 fraction columns (blocks + isotropic tissues).
  */
 int lsd_tissue_kernel_create(const double* k_aniso, const double* k_iso, const double* w_k_iso,
                              int64_t m, int64_t ca, int64_t ci, const int64_t* blocks,

@@ -344,8 +344,8 @@ inline int fit_multi_tissue_in(Workspace& mem, const real* signals, int64_t n_vo
             } else
 #endif
             {
-                rc = launch_fill("fod_lr_corr", correlation_kernel<CORR_COLUMNS>, ca,
-                                 static_cast<int>((nv + WAVE - 1) / WAVE), 0, stream);
+                rc = launch_correlations<CORR_COLUMNS>("fod_lr_corr", ca,
+                                                       static_cast<int>((nv + WAVE - 1) / WAVE), stream);
             }
             if (rc) break;
             fod.corr = corr_lr;
@@ -450,8 +450,8 @@ inline int fit_multi_tissue_in(Workspace& mem, const real* signals, int64_t n_vo
                 ca.n = fod.design.n;
                 ca.n_vox = nv;
                 ca.corr = corr_hr;
-                rc = launch_fill("fod_hr_corr", correlation_kernel<CORR_COLUMNS>, ca,
-                                 static_cast<int>((nv + WAVE - 1) / WAVE), 0, stream);
+                rc = launch_correlations<CORR_COLUMNS>("fod_hr_corr", ca,
+                                                       static_cast<int>((nv + WAVE - 1) / WAVE), stream);
             }
             if (!rc) {
                 fod.corr = corr_hr;

@@ -121,7 +121,7 @@ struct FractionDesign {
     }
 };
 
-constexpr int MAX_GROUPS = 2; // anisotropic blocks of a per-block fraction fit
+constexpr int MAX_GROUPS = 3; // anisotropic blocks of a per-block fraction fit
 constexpr int GROUP_BATCH = 4; // kernel rows loaded together for a group signal
 
 // compute_fractions of one voxel: fod in column slots, s the unweighted signal.  lumped takes

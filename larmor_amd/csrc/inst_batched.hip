@@ -77,10 +77,14 @@ int fod_pair_batched(const char* name, const char* rerun_name, const FodArgs& fa
 template <int NCPL, int RPL, bool FINAL>
 int fraction_batched(const char* name, const FractionArgs& a, int waves, hipStream_t stream) {
     const int qcols = (FINAL ? a.kernel.n_blocks : 1) + a.kernel.ci;
-    const size_t lds =
-        sizeof(real) * static_cast<size_t>((qcols + a.kernel.ci) * WAVES_PER_BLOCK) * RPL * WAVE;
-    return launch_fill(name, fraction_kernel<NCPL, RPL, FINAL, false, true>, a, waves, lds,
-                       stream);
+    // Each wave keeps its QR slab and its own copy of its job's isotropic columns.
+    const size_t slab = sizeof(real) * static_cast<size_t>(RPL) * WAVE;
+    if (fraction_columns_small(a.kernel.n_blocks, a.kernel.ci)) {
+        return launch_fraction_blocks(name, fraction_kernel<NCPL, RPL, FINAL, false, true, NQ_SMALL>,
+                                      a, waves, (qcols + a.kernel.ci) * slab, 0, stream);
+    }
+    return launch_fraction_blocks(name, fraction_kernel<NCPL, RPL, FINAL, false, true>, a, waves,
+                                  (qcols + a.kernel.ci) * slab, 0, stream);
 }
 
 #define LSD_BATCHED_RPL(X, NC) X(NC, 4) X(NC, 6) X(NC, 8)
@@ -316,8 +320,7 @@ int lsd_fit_multi_tissue_batched(const lsd_batch_job* jobs, int64_t n_jobs, int6
         ca.list = list;
         ca.corr = corr_lr;
         ca.jobs = JobTable{d_lr, J, counts, 0};
-        rc = launch_fill("batch_lr_corr", correlation_kernel<CORR_COLUMNS, true>, ca, corr_waves,
-                         0, stream);
+        rc = launch_correlations<CORR_COLUMNS, true>("batch_lr_corr", ca, corr_waves, stream);
         if (rc) break;
         fod.x_out = x_lr;
         fod.corr = corr_lr;
@@ -346,8 +349,7 @@ int lsd_fit_multi_tissue_batched(const lsd_batch_job* jobs, int64_t n_jobs, int6
         ca.list = nullptr;
         ca.corr = corr_hr;
         ca.jobs = JobTable{d_hr, J, nullptr, 0};
-        rc = launch_fill("batch_hr_corr", correlation_kernel<CORR_COLUMNS, true>, ca, corr_waves,
-                         0, stream);
+        rc = launch_correlations<CORR_COLUMNS, true>("batch_hr_corr", ca, corr_waves, stream);
         if (!rc) {
             fod.x_out = fods;
             fod.corr = corr_hr;

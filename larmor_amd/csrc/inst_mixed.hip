@@ -32,16 +32,23 @@ int dispatch_fraction_normal(const char* name, int rpl, const FractionArgs& args
                              hipStream_t stream) {
     // The loop fit only (one lumped group): no QR slab, the block's isotropic columns in LDS.
     const size_t lds_per_row_slot = sizeof(double) * args.kernel.ci * WAVE;
+    const bool small = fraction_columns_small(args.kernel.n_blocks, args.kernel.ci);
     switch (rpl) {
     case 4:
-        return launch_fill(name, fraction_kernel<1, 4, false, true>, args, waves,
-                           lds_per_row_slot * 4, stream);
+        return small ? launch_fill(name, fraction_kernel<1, 4, false, true, false, NQ_SMALL>, args,
+                                   waves, lds_per_row_slot * 4, stream)
+                     : launch_fill(name, fraction_kernel<1, 4, false, true>, args, waves,
+                                   lds_per_row_slot * 4, stream);
     case 6:
-        return launch_fill(name, fraction_kernel<1, 6, false, true>, args, waves,
-                           lds_per_row_slot * 6, stream);
+        return small ? launch_fill(name, fraction_kernel<1, 6, false, true, false, NQ_SMALL>, args,
+                                   waves, lds_per_row_slot * 6, stream)
+                     : launch_fill(name, fraction_kernel<1, 6, false, true>, args, waves,
+                                   lds_per_row_slot * 6, stream);
     case 8:
-        return launch_fill(name, fraction_kernel<1, 8, false, true>, args, waves,
-                           lds_per_row_slot * 8, stream);
+        return small ? launch_fill(name, fraction_kernel<1, 8, false, true, false, NQ_SMALL>, args,
+                                   waves, lds_per_row_slot * 8, stream)
+                     : launch_fill(name, fraction_kernel<1, 8, false, true>, args, waves,
+                                   lds_per_row_slot * 8, stream);
     default:
         return fail("no normal-equation fraction kernel for " + std::to_string(rpl) +
                     " row slots");

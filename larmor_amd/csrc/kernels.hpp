@@ -35,7 +35,14 @@ namespace LSD_NS {
 using namespace lsd; // the shared launch helpers
 #endif
 
-constexpr int NQ = 4;             // most fraction columns (groups + isotropic tissues)
+constexpr int NQ = 6;             // most fraction columns (groups + isotropic tissues)
+// The fraction and correlation kernels are also built for at most 4 columns and 2 blocks (the
+// usual tissue models), which keeps their column arrays in fewer registers.
+constexpr int NQ_SMALL = 4;
+constexpr int GROUPS_SMALL = 2;
+inline bool fraction_columns_small(int n_blocks, int ci) {
+    return n_blocks + ci <= NQ_SMALL && 1 + ci <= NQ_SMALL && n_blocks <= GROUPS_SMALL;
+}
 // LDS factor capacity (active columns) of the fast FOD kernels: 20 for the low-resolution solve
 // (its active sets stay below 20 in all but about 0.1% of voxels, which take the rerun kernel),
 // so that 8 waves per SIMD fit in LDS; 32 for the others.
@@ -409,7 +416,7 @@ constexpr int CORR_ROWS = 16;    // signal rows staged per LDS tile
 // staged once per block, shared by its waves.  BATCHED: a block's 256 positions are those of one
 // job (each job's list is cut into blocks of its own), whose signals, measurements and design it
 // reads.
-template <int NCB, bool BATCHED = false>
+template <int NCB, bool BATCHED = false, int FQ = NQ>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(CorrArgs A) {
     __shared__ real tile[WAVES_PER_BLOCK][CORR_ROWS][WAVE + 1];
     __shared__ real ktile[CORR_ROWS][NCB];
@@ -457,9 +464,9 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
         const bool on = it < count;
         const long v = on ? (list ? list[it] : first + it) : first;
         const long row = v - first;
-        real p[NQ];
+        real p[FQ];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
+        for (int q = 0; q < FQ; ++q) {
             p[q] = (on && A.multi && q < A.ci) ? A.iso[v * A.ci + q] : real(0.0);
         }
         for (int c0 = 0; c0 < A.n; c0 += NCB) {
@@ -497,7 +504,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK) correlation_kernel(Cor
                         const real ws = sr * weights[r];
                         real wp = real(0.0);
 #pragma unroll
-                        for (int q = 0; q < NQ; ++q) {
+                        for (int q = 0; q < FQ; ++q) {
                             if (q < A.ci) {
                                 wp = wp + w_iso[static_cast<long>(r) * A.ci + q] * p[q];
                             }
@@ -565,7 +572,7 @@ constexpr int fraction_waves_per_eu(int ncpl, int rpl, bool normal) {
 // NORMAL: the mixed-precision form's loop fits through the normal equations (no QR slab).
 // BATCHED: the voxels of a batch's jobs, each wave keeping its own copy of its current job's
 // isotropic columns and appending to the job's next list.
-template <int NCPL, int RPL, bool FINAL, bool NORMAL = false, bool BATCHED = false>
+template <int NCPL, int RPL, bool FINAL, bool NORMAL = false, bool BATCHED = false, int FQ = NQ>
 __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
     __attribute__((amdgpu_waves_per_eu(fraction_waves_per_eu(NCPL, RPL, NORMAL)))) fraction_kernel(FractionArgs A) {
     extern __shared__ real smem[];
@@ -676,12 +683,12 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
             const int c = l + WAVE * t;
             fod[t] = c < ca ? A.x[v * ca + c] : real(0.0);
         }
-        real fr[NQ];
+        real fr[FQ];
         // The final fit's per-block groups, or the loop's one lumped group.
         const bool ok = FINAL
-                            ? fractions_wave<NCPL, RPL, NQ, MAX_GROUPS>(
+                            ? fractions_wave<NCPL, RPL, FQ, (FQ == NQ_SMALL ? GROUPS_SMALL : MAX_GROUPS)>(
                                   K, fod, s, false, true, A.final_tol, fr, work, iso_lds)
-                            : fractions_wave<NCPL, RPL, NQ, 1, NORMAL>(K, fod, s, true, false,
+                            : fractions_wave<NCPL, RPL, FQ, 1, NORMAL>(K, fod, s, true, false,
                                                                        real(0.0), fr, work, iso_lds);
         if (!ok && l == 0) {
             A.status[v] |= ST_NOT_CONVERGED;
@@ -690,14 +697,14 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
             const int nf = A.kernel.n_blocks + ci;
             real total = real(0.0);
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
+            for (int q = 0; q < FQ; ++q) {
                 if (q < nf) {
                     total = total + fr[q];
                 }
             }
             total = total + FRACTION_SUM_OFFSET;
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
+            for (int q = 0; q < FQ; ++q) {
                 if (l == q && q < nf) {
                     A.fractions[v * nf + q] = fr[q] / total;
                 }
@@ -706,7 +713,7 @@ __global__ void __launch_bounds__(WAVE * WAVES_PER_BLOCK)
         }
         real mine = real(0.0);
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
+        for (int q = 0; q < FQ; ++q) {
             mine = l == q ? fr[q] : mine;
         }
         const bool converged = ballot(l < nlf && !(fabs(mine - prev) < A.fraction_tol)) == 0ull;
@@ -869,31 +876,64 @@ int run_pair(const char* name, const char* rerun_name, const Args& fast, Args re
     return launch(rerun_name, Family<NCPL, RPL, KG, true>::kernel, rerun, rerun_waves, 0, stream);
 }
 
-// A fraction launch: four waves to a block, each with its QR slab in the block's LDS, or for 12
-// and 16 row slots (more than 512 measurements) one wave to a block, so that a block's LDS stays
-// within 64 KB.
-template <int NC, int RP, bool FINAL>
-int launch_fraction_rows(const char* name, const FractionArgs& args, int waves, hipStream_t stream) {
-    const int qcols = (args.final_fit ? args.kernel.n_blocks : 1) + args.kernel.ci;
-    const int per_block = RP <= 8 ? WAVES_PER_BLOCK : 1;
-    const size_t lds = sizeof(real) * (qcols * per_block + args.kernel.ci) * RP * WAVE;
-    auto kern = fraction_kernel<NC, RP, FINAL>;
+// A fraction launch: each wave has its QR slab (per_wave bytes) in the block's LDS beside the
+// block's shared bytes; four waves to a block, or two or one where four waves' slabs would take
+// more than 64 KB (many fraction columns, more than 512 measurements).  A single wave past the
+// device's LDS per block is refused.
+template <class Kern, class Args>
+int launch_fraction_blocks(const char* name, Kern kern, const Args& args, int waves,
+                           size_t per_wave, size_t shared, hipStream_t stream) {
+    constexpr size_t BLOCK_LDS = 64 * 1024;
+    int per_block = WAVES_PER_BLOCK;
+    while (per_block > 1 && per_wave * per_block + shared > BLOCK_LDS) {
+        per_block /= 2;
+    }
+    const size_t lds = per_wave * per_block + shared;
+    int dev = 0;
+    int max_lds = 0;
+    LSD_HIP(hipGetDevice(&dev));
+    LSD_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if (lds > static_cast<size_t>(max_lds)) {
+        return fail("the fraction fit needs " + std::to_string(lds) + " bytes of LDS per wave, more "
+                    "than the device's " + std::to_string(max_lds) +
+                    "; fewer tissues or measurements fit");
+    }
     if (per_block == WAVES_PER_BLOCK) {
         return launch_fill(name, kern, args, waves, lds, stream);
     }
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, LAUNCH_WAVE, lds) == hipSuccess &&
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, LAUNCH_WAVE * per_block, lds) ==
+            hipSuccess &&
         per_cu > 0) {
-        int dev = 0;
         int cus = 0;
-        LSD_HIP(hipGetDevice(&dev));
         LSD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        const long resident = static_cast<long>(per_cu) * cus;
+        const long resident = static_cast<long>(per_cu) * cus * per_block;
         if (resident > 0 && resident < waves) {
             waves = static_cast<int>(resident);
         }
     }
-    return launch(name, kern, args, waves, lds, stream, 1);
+    return launch(name, kern, args, waves, lds, stream, per_block);
+}
+
+template <int NC, int RP, bool FINAL>
+int launch_fraction_rows(const char* name, const FractionArgs& args, int waves, hipStream_t stream) {
+    const int qcols = (args.final_fit ? args.kernel.n_blocks : 1) + args.kernel.ci;
+    const size_t slab = sizeof(real) * static_cast<size_t>(RP) * WAVE;
+    if (fraction_columns_small(args.kernel.n_blocks, args.kernel.ci)) {
+        return launch_fraction_blocks(name, fraction_kernel<NC, RP, FINAL, false, false, NQ_SMALL>,
+                                      args, waves, qcols * slab, args.kernel.ci * slab, stream);
+    }
+    return launch_fraction_blocks(name, fraction_kernel<NC, RP, FINAL>, args, waves, qcols * slab,
+                                  args.kernel.ci * slab, stream);
+}
+
+// The correlation pass with the isotropic fractions of at most NQ_SMALL - 1 tissues, or NQ - 1.
+template <int NCB, bool BATCHED = false>
+int launch_correlations(const char* name, const CorrArgs& args, int waves, hipStream_t stream) {
+    if (args.ci + 1 <= NQ_SMALL) {
+        return launch_fill(name, correlation_kernel<NCB, BATCHED, NQ_SMALL>, args, waves, 0, stream);
+    }
+    return launch_fill(name, correlation_kernel<NCB, BATCHED>, args, waves, 0, stream);
 }
 
 // Launchers of one (NCPL, RPL) instantiation, defined in its own translation unit.

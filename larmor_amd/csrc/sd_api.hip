@@ -734,8 +734,8 @@ int lsd_fit_single_tissue(const double* signals, int64_t n_voxels, int64_t m,
         ca.n = fa.design.n;
         ca.n_vox = n_voxels;
         ca.corr = corr;
-        rc = launch_fill("fod_single_corr", correlation_kernel<CORR_COLUMNS>, ca,
-                         static_cast<int>((n_voxels + WAVE - 1) / WAVE), 0, stream);
+        rc = launch_correlations<CORR_COLUMNS>("fod_single_corr", ca,
+                                               static_cast<int>((n_voxels + WAVE - 1) / WAVE), stream);
         if (rc) return rc;
         fa.corr = corr;
     }
@@ -809,8 +809,8 @@ int lsd_correlations(const double* signals, int64_t n_voxels, int64_t m, const d
     ca.n = static_cast<int>(solver->n);
     ca.n_vox = n_voxels;
     ca.corr = corr;
-    return launch_fill("correlations", correlation_kernel<CORR_COLUMNS>, ca,
-                       static_cast<int>((n_voxels + WAVE - 1) / WAVE), 0, stream);
+    return launch_correlations<CORR_COLUMNS>("correlations", ca,
+                                             static_cast<int>((n_voxels + WAVE - 1) / WAVE), stream);
 }
 
 int lsd_fit_multi_tissue_mixed(const double* signals, int64_t n_voxels, int64_t m,

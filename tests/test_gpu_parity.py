@@ -209,3 +209,33 @@ def test_multi_tissue_many_measurements(shells):
     np.testing.assert_array_equal(rounds, tr[:, 0])
     np.testing.assert_array_equal(fods, ref_fods)
     np.testing.assert_array_equal(fr, ref_fr)
+
+
+# Up to 3 anisotropic blocks and 6 fraction columns: two anisotropic tissues (300 high-resolution
+# directions each) with four isotropic ones, and three (200 each) with three.
+@pytest.mark.parametrize("n_aniso,n_hr,isos", [(2, 300, [0.3e-3, 0.8e-3, 1.5e-3, 3e-3]),
+                                               (3, 200, [0.8e-3, 1.5e-3, 3e-3])])
+def test_six_fraction_columns_bit_identical_to_oracle(n_aniso, n_hr, isos):
+    cfg = dict(synthetic.MFOD_CONFIG)
+    cfg["shape"] = (8, 8, 6)
+    cfg["seed"] = 13
+    ph = synthetic.phantom(**cfg)
+    signals = np.ascontiguousarray(synthetic.normalized_signals(ph.data, ph.bvals, ph.mask)[:160])
+    tensors = [[1.7e-3, 0.3e-3, 0.3e-3], [1.4e-3, 0.5e-3, 0.5e-3], [2.0e-3, 0.2e-3, 0.2e-3]]
+    rfs = [AnisotropicRF(t, 0.0) for t in tensors[:n_aniso]] + [IsotropicRF(d) for d in isos]
+    w = np.ones(len(ph.bvals))
+    hr_dirs = get_dirs(n_hr + 1)[:-1] if n_hr != 300 else DIRS_300
+    hr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=[hr_dirs] * n_aniso, weights=w)
+    lr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=[get_dirs(45)[:-1]] * n_aniso, weights=w)
+    print(f"{n_aniso} anisotropic blocks, {len(isos)} isotropic tissues: {signals.shape}",
+          flush=True)
+    ref_fods, ref_fr, tr = oracle.fit_multi_tissue(signals, w, lr, hr, 0.0, 0.1, 1e-6, trace=True)
+    lr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(lr.w_k_aniso), 0.0, 1e-6)
+    hr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(hr.w_k_aniso), 0.1, 1e-6)
+    fods, fr, rounds, status = _core.fit_multi_tissue(
+        signals, w, _deconv(lr), lr_solver, _deconv(hr), hr_solver, 50, 1e-7, 1e-2, 1,
+        return_diagnostics=True)
+    assert fr.shape == (len(signals), n_aniso + len(isos))
+    np.testing.assert_array_equal(rounds, tr[:, 0])
+    np.testing.assert_array_equal(fods, ref_fods)
+    np.testing.assert_array_equal(fr, ref_fr)
