@@ -72,9 +72,11 @@ def single_shell(n_dirs=60, seed=1, n_vox=200, snr=20.0):
     return bvecs, np.sqrt(re * re + im * im)
 
 
-@pytest.mark.parametrize("lmax", [4, 6, 8])
-def test_converged_fod_is_the_fixed_point(lmax):
-    bvecs, signals = single_shell()
+# lmax 12 and 16 (91 and 153 coefficients, past the response's degree 8: the constraint alone
+# sets the higher degrees) pin the oracle where the GPU's wide CSD kernel is compared with it.
+@pytest.mark.parametrize("lmax,n_vox", [(4, 200), (6, 200), (8, 200), (12, 40), (16, 12)])
+def test_converged_fod_is_the_fixed_point(lmax, n_vox):
+    bvecs, signals = single_shell(n_vox=n_vox)
     p = csd_problem(bvecs, RESPONSE, lmax, CSDSolver())
     fods, conv, solves = oracle.fit_csd(signals, p.initial, p.forward, p.constraint,
                                         p.norm_weight, p.threshold, p.max_iter,

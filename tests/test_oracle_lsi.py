@@ -197,3 +197,39 @@ def test_refuses_more_isotropic_tissues_than_b_values_separate():  # test_msmt.p
            isotropic_response(2000.0, 3e-3)]
     with pytest.raises(ValueError, match="determine only"):
         prepare_msmt(rfs, [8, 0, 0], gtab.bvals[one_shell], gtab.bvecs[one_shell], DIRS_300)
+
+
+def _wide_msmt_problem(wm_lmax):
+    """An MSMT problem with a WM response up to wm_lmax: 67 (lmax 10) or 121 (lmax 14)
+    parameters, past the 63 of one lane slot."""
+    gtab = gradients()
+    nodes, weights = np.polynomial.legendre.leggauss(64)
+    theta = np.arccos(nodes)
+    basis = build_basis_matrix(theta, np.zeros_like(theta), wm_lmax)
+    zonal = [l * (l + 1) // 2 for l in range(0, wm_lmax + 1, 2)]
+    rows = [2 * np.pi * (weights * 1000.0 * np.exp(-b * (0.2e-3 + 1.5e-3 * nodes**2))) @ basis[:, zonal]
+            for b in SHELLS]
+    wm = ZonalSHRF(bvals=SHELLS, coefficients=np.array(rows))
+    rfs = [wm, isotropic_response(2000.0, 3e-3)]
+    forward = forward_matrix(rfs, [wm_lmax, 0], gtab.bvals, gtab.bvecs)
+    return gtab, forward, constraint_matrix([wm_lmax, 0], DIRS_300), msmt_ridge(forward)
+
+
+@pytest.mark.parametrize("wm_lmax", [10, 14])
+def test_wide_problems_are_non_negative_and_meet_kkt(wm_lmax):
+    """The KKT conditions of the LSI at 67 and 121 parameters (the GPU's two-slot kernel is compared
+    with the oracle there): feasible, and the gradient of the objective a non-negative combination
+    of the active constraints' rows."""
+    gtab, forward, g, ridge = _wide_msmt_problem(wm_lmax)
+    rng = np.random.default_rng(3)
+    noisy = 300.0 + 200.0 * rng.standard_normal((6, len(gtab.bvals)))
+    x, conv, tr = oracle.fit_lsi(noisy, forward, g, ridge, trace=True)
+    assert conv.all() and tr[:, 3].max() > 40
+    ridge_squared = NORM_REG * np.max(np.sum(forward**2, axis=0))
+    for signal, xv in zip(noisy, x):
+        values = g @ xv
+        assert values.min() > -1e-7 * np.abs(values).max()
+        active = values < 1e-7 * np.abs(values).max()
+        gradient = forward.T @ (forward @ xv - signal) + ridge_squared * xv
+        _, residual = nnls(g[active].T, gradient)
+        assert residual < 1e-7 * np.linalg.norm(forward.T @ signal)

@@ -250,3 +250,22 @@ def test_least_squares_drops_dependent_columns():
     full = rng.normal(size=(15, 5))
     np.testing.assert_allclose(oracle.least_squares(full, data[:15]),
                                np.linalg.lstsq(full, data[:15], rcond=None)[0], atol=1e-12)
+
+
+@pytest.mark.parametrize("m,n", [(610, 44), (1010, 300)])
+def test_ridge_objective_matches_scipy_with_many_measurements(m, n):
+    """The ridge solve at 610 and 1,010 measurements (where the GPU's 12- and 16-row-slot fits are
+    compared with the oracle) minimises ||[K; rI] x - [s; 0]|| as scipy's NNLS does, to rel 1e-10."""
+    from scipy.optimize import nnls as scipy_nnls
+
+    rng = np.random.default_rng(m)
+    kernel = np.exp(-rng.uniform(0, 3, size=(m, n)))
+    signal = kernel @ np.maximum(rng.normal(size=n), 0) + 0.01 * rng.normal(size=m)
+    ridge = 0.1
+    got = oracle.ridge_nnls(kernel, signal, ridge, tol=1e-12)
+    stacked = np.vstack([kernel, ridge * np.eye(n)])
+    data = np.concatenate([signal, np.zeros(n)])
+    ref, _ = scipy_nnls(stacked, data, maxiter=50 * n)
+    obj = lambda x: np.sum((stacked @ x - data) ** 2)
+    assert (got >= 0).all()
+    assert obj(got) <= obj(ref) * (1 + 1e-10)
