@@ -498,12 +498,13 @@ typedef struct lsd_track_stats {
 
 /*
  * Tuning (no counterpart in the reference): the low-resolution solve's LDS factor capacity for
- * the row class of m measurements (<= 256, <= 384, <= 512): 16, 20, 24 or 32 columns, or 0 for
- * the default (20), process-wide, used by every later plain and batched fit of that class.  A
+ * the row class of m measurements (<= 256, <= 384, <= 512, <= 768, <= 1024): 16, 20, 24 or 32
+ * columns, or 0 for the default (20), process-wide, used by every later plain and batched fit of
+ * that class (the batched form stops at 512 measurements).  A
  * voxel whose active set outgrows the capacity finishes in the global-memory solve with the same
  * arithmetic, so the choice changes speed only, never a bit of the results; the autotuner of
  * larmor_amd/sd/tuning.py times the capacities on a sample of a workload and keeps the fastest.
- * lsd_tuned_lr_capacity returns the capacity in force for m (-1 beyond 512).
+ * lsd_tuned_lr_capacity returns the capacity in force for m (-1 beyond 1024).
  * Errors: "the low-resolution factor capacity is 16, 20, 24 or 32 columns (0: the default), got N".
  */
 int lsd_tune_lr_capacity(int64_t m, int64_t capacity);

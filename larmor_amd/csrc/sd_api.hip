@@ -22,10 +22,20 @@ namespace lsd {
 thread_local std::string g_error;
 bool g_profiling = false;
 
-// The tuned low-resolution factor capacity per row class (RPL 4, 6, 8); 0 means the default.
-static std::atomic<int> g_lr_capacity[3] = {0, 0, 0};
+// The tuned low-resolution factor capacity per row class (RPL 4, 6, 8, 12, 16); 0 means the
+// default.
+static std::atomic<int> g_lr_capacity[5] = {0, 0, 0, 0, 0};
 
-static int rpl_slot(int rpl) { return rpl == 4 ? 0 : (rpl == 6 ? 1 : (rpl == 8 ? 2 : -1)); }
+static int rpl_slot(int rpl) {
+    switch (rpl) {
+    case 4: return 0;
+    case 6: return 1;
+    case 8: return 2;
+    case 12: return 3;
+    case 16: return 4;
+    default: return -1;
+    }
+}
 
 int lr_capacity(int rpl) {
     const int slot = rpl_slot(rpl);
@@ -856,9 +866,9 @@ int lsd_fit_multi_tissue_noalloc(const double* signals, int64_t n_voxels, int64_
 int64_t lsd_noalloc_workspace_bytes(void) { return static_cast<int64_t>(NOALLOC_WORKSPACE_BYTES); }
 
 int lsd_tune_lr_capacity(int64_t m, int64_t capacity) {
-    const int rpl = rpl_for(m);
+    const int rpl = rpl_for_wide(m);
     if (rpl < 0) {
-        return fail("problem size exceeds 512 measurements");
+        return fail("problem size exceeds 1024 measurements");
     }
     if (capacity != 0 && capacity != 16 && capacity != 20 && capacity != 24 && capacity != 32) {
         return fail("the low-resolution factor capacity is 16, 20, 24 or 32 columns (0: the "
@@ -869,7 +879,7 @@ int lsd_tune_lr_capacity(int64_t m, int64_t capacity) {
 }
 
 int64_t lsd_tuned_lr_capacity(int64_t m) {
-    const int rpl = rpl_for(m);
+    const int rpl = rpl_for_wide(m);
     return rpl < 0 ? -1 : lr_capacity(rpl);
 }
 

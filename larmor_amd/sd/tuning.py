@@ -6,7 +6,7 @@ a larger factor fits more voxels' active sets in LDS but leaves fewer waves per 
 and more registers to spill at the lower end; a voxel whose active set outgrows it finishes in
 the global-memory solve with the same arithmetic, so every capacity gives the same bits and the
 choice is speed only.  Which one is fastest depends on the bucket -- the row class of the
-acquisition (m <= 256, 384, 512 measurements) and the number of low-resolution directions
+acquisition (m <= 256, 384, 512, 768, 1024 measurements) and the number of low-resolution directions
 (27, 44 or 65 for lmax 6, 8, 10), which set the active-set sizes -- so `autotune` times each on a
 sample of the bucket's own voxels and keeps the fastest; `use` puts a bucket's choice in force
 before its fits (the library reads it when it enqueues a fit's launches).
@@ -24,14 +24,11 @@ DEFAULT_CAPACITY = 20
 
 
 def row_class(m: int) -> int:
-    """The measurement-count class of the solve kernels (rows per lane 4, 6, 8)."""
-    if m <= 256:
-        return 256
-    if m <= 384:
-        return 384
-    if m <= 512:
-        return 512
-    raise ValueError(f"problem size exceeds 512 measurements, got {m}")
+    """The measurement-count class of the solve kernels (rows per lane 4, 6, 8, 12, 16)."""
+    for bound in (256, 384, 512, 768, 1024):
+        if m <= bound:
+            return bound
+    raise ValueError(f"problem size exceeds 1024 measurements, got {m}")
 
 
 @dataclass

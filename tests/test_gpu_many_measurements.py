@@ -76,3 +76,37 @@ def test_limits_beyond_1024_and_for_the_other_forms():
     c = np.zeros((1030, 10))
     with pytest.raises(ValueError, match="1024 measurements"):
         _core.lsqnonneg(c, np.zeros(1030), 1e-10)
+
+
+def test_every_tuned_capacity_gives_the_same_bits_at_12_and_16_row_slots():
+    """The low-resolution factor capacity of the 768- and 1,024-measurement row classes is tunable
+    like the smaller ones, and every capacity gives the same bits."""
+    from larmor_amd.sd import tuning
+
+    for shells, seed in (([(1000.0, 200), (2000.0, 200), (3000.0, 200)], 11),
+                         ([(700.0, 250), (1000.0, 250), (2000.0, 250), (3000.0, 250)], 12)):
+        signals, w, lr, hr = _setup(shells, seed=seed, shape=(5, 5, 3))
+        m = signals.shape[1]
+        print(f"tuned capacities at {m} measurements", flush=True)
+        lr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(lr.w_k_aniso), 0.0, 1e-6)
+        hr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(hr.w_k_aniso), 0.1, 1e-6)
+        assert tuning.row_class(m) == (768 if m <= 768 else 1024)
+        ref = None
+        try:
+            for cap in tuning.CAPACITIES:
+                tuning.set_capacity(m, cap)
+                assert tuning.capacity_in_force(m) == cap
+                got = _core.fit_multi_tissue(signals, w, _deconv(lr), lr_solver, _deconv(hr),
+                                             hr_solver, 50, 1e-7, 1e-2, 1,
+                                             return_diagnostics=True)
+                got = [np.asarray(x) for x in got]
+                if ref is None:
+                    ref = got
+                for a, b in zip(got, ref):
+                    np.testing.assert_array_equal(a, b)
+        finally:
+            tuning.set_capacity(m, 0)
+        assert tuning.capacity_in_force(m) == tuning.DEFAULT_CAPACITY
+    assert tuning.capacity_in_force(1030) == -1
+    with pytest.raises(ValueError, match="1024 measurements"):
+        tuning.row_class(1030)

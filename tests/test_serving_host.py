@@ -235,3 +235,14 @@ def test_cancelled_requests_are_skipped_and_running_ones_cannot_be_cancelled(mon
     _settle(head.future, 2)
     _settle(head.future, exc=RuntimeError("late"))
     assert head.future.result() == 2
+
+
+def test_tuning_row_classes_and_buckets():
+    from larmor_amd.sd import tuning
+
+    assert [tuning.row_class(m) for m in (1, 256, 257, 384, 385, 512, 513, 768, 769, 1024)] == \
+        [256, 256, 384, 384, 512, 512, 768, 768, 1024, 1024]
+    with pytest.raises(ValueError, match="1024 measurements"):
+        tuning.row_class(1025)
+    assert tuning.TuningTable().bucket(610, 44) == (768, 44)
+    assert tuning.DEFAULT_CAPACITY in tuning.CAPACITIES
