@@ -206,7 +206,8 @@ int lsd_fit_multi_tissue_fp32(const float* signals, int64_t n_voxels, int64_t m,
  * the low-resolution FODs and correlations, the loop state, the overflow lists and the
  * global-memory rerun slabs) is carved from a static device workspace of
  * lsd_noalloc_workspace_bytes() (64 MiB), one per device, held until the process ends.  fods and
- * fractions must be given; rounds and status may be NULL.  Ridge NNLS solvers only.  The
+ * fractions must be given; rounds and status may be NULL.  Ridge NNLS solvers and at most 512
+ * measurements only ("problem size exceeds 512 measurements or 640 directions").  The
  * high-resolution correlations are computed inside the FOD kernel (the same sums, so the results
  * are bit-identical to lsd_fit_multi_tissue's), and the rerun slabs fill what the per-voxel state
  * leaves.  A call whose buffers do not fit fails with LSD_ERR_INVALID ("... pass fewer voxels per

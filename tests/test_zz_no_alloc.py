@@ -98,6 +98,19 @@ def test_noalloc_refuses_what_does_not_fit_and_keeps_working():
     dev = torch.from_numpy(signals).to("cuda")
     with pytest.raises(ValueError, match="ridge NNLS"):
         _fit("lsd_fit_multi_tissue_noalloc", dev, weights, lr_k, lr_solver, hr_k, drl)
+    # 610 measurements: the allocating fit takes them (12 row slots), the allocation-free one stops
+    # at 512
+    rng = np.random.default_rng(13)
+    bvals, bvecs = synthetic.acquisition(10, [(1000.0, 200), (2000.0, 200), (3000.0, 200)], rng)
+    wide = Kernel(RFS, bvals, bvecs, dirs=DIRS_300, weights=np.ones(len(bvals)))
+    wide_k = _deconv_kernel(wide)
+    wide_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(wide.w_k_aniso), 0.1, 1e-6)
+    wide_rows = torch.ones((4, len(bvals)), dtype=torch.float64, device="cuda")
+    wide_w = torch.ones(len(bvals), dtype=torch.float64, device="cuda")
+    print("no-alloc fit: 610 measurements", flush=True)
+    with pytest.raises(ValueError, match="exceeds 512 measurements"):
+        _fit("lsd_fit_multi_tissue_noalloc", wide_rows, wide_w, wide_k, wide_solver, wide_k,
+             wide_solver)
     print("no-alloc fit: 200 voxels after the refusals", flush=True)
     plain = _fit("lsd_fit_multi_tissue", dev, weights, lr_k, lr_solver, hr_k, hr_solver)
     free = _fit("lsd_fit_multi_tissue_noalloc", dev, weights, lr_k, lr_solver, hr_k, hr_solver)
