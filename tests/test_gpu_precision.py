@@ -85,8 +85,8 @@ def test_fp32_form_against_fp64(workload):
 
 def test_mixed_form_against_fp64(workload):
     """The mixed-precision form: BF16 matrix-core correlations, then FP64 refinement rounds, with
-    FP64 factors, gradients and KKT tests.  The north star's bars: FOD coefficients within 1e-4
-    relative L2 of FP64, and the oracle's active set on at least 99.9% of voxels."""
+    FP64 factors, gradients and KKT tests.  The accuracy bars of the mixed form: FOD coefficients
+    within 1e-4 relative L2 of FP64, and the oracle's active set on at least 99.9% of voxels."""
     name, _, s64, w, lr, hr, (ref_fods, ref_fr, trace) = workload
     lr_k, lr_s, hr_k, hr_s = _solvers(lr, hr)
     fods, fr, rounds, status = _core.fit_multi_tissue(
