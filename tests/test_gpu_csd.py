@@ -166,6 +166,22 @@ def test_csd_wide_any_coefficient_count():
         _assert_same(ref, got)
 
 
+def test_csd_wide_at_the_coefficient_limit():
+    # 496 coefficients (lmax 30), the largest system the kernels take: its 964 KiB triangle in
+    # global memory, with constraint directions in the largest slot class.
+    rng = np.random.default_rng(12)
+    m, c, c0, k = 520, 496, 120, 500
+    forward = rng.normal(size=(m, c))
+    initial = rng.normal(size=(c0, m)) * 0.05
+    constraint = np.ascontiguousarray(rng.normal(size=(k, c)))
+    signals = np.ascontiguousarray(rng.normal(size=(6, m)))
+    print(f"csd wide c {c} k {k}", flush=True)
+    ref = oracle.fit_csd(signals, initial, forward, constraint, 0.5, 0.0, 6, return_solves=True)
+    got = _core.fit_csd(signals, initial, forward, constraint, 0.5, 0.0, 6, 1, return_solves=True)
+    assert ref[2].max() >= 2
+    _assert_same(ref, got)
+
+
 def test_csd_wide_batch_invariance_and_limit():
     signals, bvecs = _high_b_workload((4, 4, 2))
     p = csd_problem(bvecs, synthetic.tensor_zonal(3000.0, lmax=12), 12, CSDSolver())

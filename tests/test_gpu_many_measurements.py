@@ -31,9 +31,10 @@ def _deconv(k):
                               [(b.start, b.stop) for b in k.aniso_blocks])
 
 
-# 610 measurements: 12 row slots; 1,010: 16.
+# 610 measurements: 12 row slots; 1,010: 16; 1,024: the limit, every row slot of every lane.
 @pytest.mark.parametrize("shells", [[(1000.0, 200), (2000.0, 200), (3000.0, 200)],
-                                    [(700.0, 250), (1000.0, 250), (2000.0, 250), (3000.0, 250)]])
+                                    [(700.0, 250), (1000.0, 250), (2000.0, 250), (3000.0, 250)],
+                                    [(700.0, 253), (1000.0, 253), (2000.0, 254), (3000.0, 254)]])
 def test_multi_and_single_tissue_fits_bit_identical(shells):
     signals, w, lr, hr = _setup(shells, seed=len(shells))
     m = signals.shape[1]
