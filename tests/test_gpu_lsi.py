@@ -191,6 +191,22 @@ def test_lsi_wide_qr_path_matches_oracle(wm_lmax):
     assert np.abs(xq - fast).max() < 1e-6 * scale
 
 
+def test_lsi_at_the_parameter_limit():
+    # 255 parameters, the most the kernels take (four lane slots), on a random full-rank design
+    # with 302 random constraints (x = 0 is always feasible); the factor grows past 128 positions.
+    rng = np.random.default_rng(21)
+    n, m, k = 255, 300, 302
+    a = rng.normal(size=(m, n))
+    g = rng.normal(size=(k, n))
+    signals = rng.normal(size=(6, m))
+    print(f"lsi at the limit: {n} parameters, {k} constraints", flush=True)
+    (rx, rc, tr), (x, c, status) = _both(signals, a, g, 0.1)
+    assert tr[:, 3].max() > 128
+    np.testing.assert_array_equal(c, rc)
+    np.testing.assert_array_equal(status != 0, tr[:, 0] != 0)
+    np.testing.assert_array_equal(x, rx)
+
+
 def test_lsi_wide_edge_cases_and_limit():
     forward, g, ridge, signals = _wide_msmt(12, 6, seed=9)
     edge = np.vstack([np.zeros(signals.shape[1]), np.full(signals.shape[1], np.nan), signals])
