@@ -239,3 +239,32 @@ def test_six_fraction_columns_bit_identical_to_oracle(n_aniso, n_hr, isos):
     np.testing.assert_array_equal(rounds, tr[:, 0])
     np.testing.assert_array_equal(fods, ref_fods)
     np.testing.assert_array_equal(fr, ref_fr)
+
+
+def test_640_directions_the_column_limit():
+    """640 FOD directions, the most the solve kernels take (ten column slots of every lane): the
+    single-tissue fit (ridge 0.1 and the QR-only 1e-7) and the multi-tissue fit."""
+    cfg = dict(synthetic.MFOD_CONFIG)
+    cfg["shape"] = (6, 6, 4)
+    cfg["seed"] = 7
+    ph = synthetic.phantom(**cfg)
+    signals = np.ascontiguousarray(synthetic.normalized_signals(ph.data, ph.bvals, ph.mask))
+    rfs = [AnisotropicRF([1.7e-3, 0.3e-3, 0.3e-3], 0.0), IsotropicRF(0.8e-3), IsotropicRF(3e-3)]
+    w = np.ones(len(ph.bvals))
+    hr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=[get_dirs(640)], weights=w)
+    lr = Kernel(rfs, ph.bvals, ph.bvecs, dirs=[get_dirs(45)[:-1]], weights=w)
+    k = np.ascontiguousarray(hr.w_k_aniso)
+    assert k.shape[1] == 640
+    print(f"640 directions: {signals.shape}", flush=True)
+    for ridge in (0.1, 1e-7):
+        ref = oracle.fit_single_tissue(signals, k, ridge, 1e-6)
+        got = _core.fit_single_tissue(signals, _core.FodSolver.l2nnls(k, ridge, 1e-6), 1)
+        np.testing.assert_array_equal(got, ref)
+    ref_fods, ref_fr, tr = oracle.fit_multi_tissue(signals, w, lr, hr, 0.0, 0.1, 1e-6, trace=True)
+    lr_solver = _core.FodSolver.l2nnls(np.ascontiguousarray(lr.w_k_aniso), 0.0, 1e-6)
+    fods, fr, rounds, status = _core.fit_multi_tissue(
+        signals, w, _deconv(lr), lr_solver, _deconv(hr), _core.FodSolver.l2nnls(k, 0.1, 1e-6),
+        50, 1e-7, 1e-2, 1, return_diagnostics=True)
+    np.testing.assert_array_equal(rounds, tr[:, 0])
+    np.testing.assert_array_equal(fods, ref_fods)
+    np.testing.assert_array_equal(fr, ref_fr)
